@@ -1,0 +1,177 @@
+/*
+ * rz.h -- C ABI of the MI355X coverage rasterizer (librz.so).
+ *
+ * This is the boundary a foreign caller binds (Rust `extern "C"`, ctypes, ...).
+ * Plain pointers and sizes only. Each entry point names the reference
+ * interface (aslpavel/rasterize v0.6.7) it replaces.
+ *
+ * Conventions
+ *   - Every function returns an rz_status; rz_last_error(ctx) describes the
+ *     last failure on that context.
+ *   - The caller owns all host buffers. The context owns device scratch, its
+ *     HIP stream and its tuned launch parameters. A context is not
+ *     thread-safe: use one per host thread (the reference's rasterizers are
+ *     `&self` and single-threaded).
+ *   - Blocking entry points take host buffers and return when the result is
+ *     in them. `_device` entry points take device pointers, are asynchronous
+ *     on the context's stream and report errors at the next rz_sync().
+ *   - Coordinates and transforms are f64 as in the reference (Scalar = f64).
+ *     Masks are produced in f32 and widened where the reference's image is
+ *     f64. Colours are premultiplied linear RGBA f32 (LinColor).
+ */
+#ifndef RZ_H
+#define RZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rz_status {
+  RZ_OK = 0,
+  RZ_E_NAN = 1,      /* "cannot flatten segment with NaN" (src/path.rs:765-767 panics) */
+  RZ_E_ARG = 2,      /* invalid argument (empty subpath, flatness <= 0, bad shape, ...) */
+  RZ_E_HIP = 3,      /* HIP runtime error */
+  RZ_E_DEPTH = 4,    /* subdivision deeper than 62 levels (non-finite or astronomically large input) */
+  RZ_E_NOMEM = 5,    /* device allocation failed */
+  RZ_E_CAPACITY = 6  /* an async call outgrew the context's buffers; re-run (buffers were grown) */
+} rz_status;
+
+/* FillRule (src/path.rs:18-47) */
+enum { RZ_NONZERO = 0, RZ_EVENODD = 1 };
+
+/* Right-edge semantics of the signed-difference accumulation.
+ *  RZ_EDGE_MASK: SignedDifferenceRasterizer::mask -- clip at x = W-1
+ *                (src/rasterize.rs:298-311, 370-387).
+ *  RZ_EDGE_ITER: SignedDifferenceRasterizer::mask_iter / fill -- (W+1)-wide
+ *                scratch, clip at x = W, column W dropped (src/rasterize.rs:313-355).
+ *  RZ_EDGE_NONE: ActiveEdgeRasterizer -- no right clip, columns >= W never
+ *                produced (src/rasterize.rs:517-575, 722-727). */
+enum { RZ_EDGE_MASK = 0, RZ_EDGE_ITER = 1, RZ_EDGE_NONE = 2 };
+
+/* Segment kinds of rz_path.kinds (src/curve.rs:905-909) */
+enum { RZ_LINE = 0, RZ_QUAD = 1, RZ_CUBIC = 2 };
+
+/* Transform [m00, m01, m02, m10, m11, m12] (src/geometry.rs:309-317) */
+typedef struct rz_transform {
+  double m[6];
+} rz_transform;
+
+/* Path {segments, subpaths, closed} (src/path.rs:228-233) as a host SoA view.
+ * Segment k has kinds[k]+2 points starting at point index point_offsets[k]
+ * (points holds x,y pairs). Subpath i is segments [subpaths[i], subpaths[i+1]).
+ * Elliptic arcs are expanded to cubics at build time, as PathBuilder::arc_to
+ * does (src/path.rs:945-972). */
+typedef struct rz_path {
+  uint32_t n_segments;
+  uint32_t n_subpaths;
+  const uint8_t* kinds;           /* [n_segments] */
+  const uint32_t* point_offsets;  /* [n_segments + 1] */
+  const double* points;           /* [2 * point_offsets[n_segments]] */
+  const uint32_t* subpaths;       /* [n_subpaths + 1] */
+  const uint8_t* closed;          /* [n_subpaths] */
+} rz_path;
+
+/* Shape (src/image.rs:6-17): element offsets of a strided image view */
+typedef struct rz_shape {
+  size_t start;
+  size_t width;
+  size_t height;
+  size_t row_stride;
+  size_t col_stride;
+} rz_shape;
+
+/* Paint (src/rasterize.rs:185-198): LinColor (src/color.rs:271),
+ * GradLinear (src/grad.rs:150-211) or GradRadial (src/grad.rs:307-411).
+ * Stop colours are given as the user passes them to GradStop (linear
+ * premultiplied); the sRGB conversion of non-linear_colors gradients
+ * (src/grad.rs:177-179) is applied by the library. */
+enum { RZ_PAINT_SOLID = 0, RZ_PAINT_LINEAR = 1, RZ_PAINT_RADIAL = 2 };
+enum { RZ_UNITS_USER_SPACE = 0, RZ_UNITS_BBOX = 1 };
+enum { RZ_SPREAD_PAD = 0, RZ_SPREAD_REPEAT = 1, RZ_SPREAD_REFLECT = 2 };
+typedef struct rz_paint {
+  int32_t kind;
+  float color[4];        /* solid */
+  int32_t units;
+  int32_t linear_colors;
+  int32_t spread;
+  double tr[6];          /* paint transform */
+  double start[2];       /* linear */
+  double end[2];
+  double center[2];      /* radial */
+  double radius;
+  double fcenter[2];
+  double fradius;
+  uint32_t n_stops;
+  const double* stop_positions; /* [n_stops] */
+  const float* stop_colors;     /* [4 * n_stops] */
+} rz_paint;
+
+typedef struct rz_ctx rz_ctx;
+typedef struct rz_dpath rz_dpath;
+
+/* ---- context ---------------------------------------------------------- */
+rz_status rz_ctx_create(rz_ctx** out, int device);
+rz_status rz_ctx_destroy(rz_ctx* ctx);
+const char* rz_last_error(const rz_ctx* ctx);
+/* Wait for the context's stream; returns the first asynchronous error. */
+rz_status rz_sync(rz_ctx* ctx);
+/* The context's hipStream_t (as void*), for callers that order their own work. */
+void* rz_stream(rz_ctx* ctx);
+
+/* ---- stage hooks (bit-exact parity surfaces) ------------------------- */
+
+/* Path::flatten(tr, flatness, close) (src/path.rs:418-425, 734-796): lines as
+ * (x0, y0, x1, y1) f64 quadruples in the reference's order. If cap is too
+ * small nothing is written, *n_lines gets the count and RZ_E_CAPACITY is
+ * returned. */
+rz_status rz_flatten(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                     int close, double* lines, size_t cap, size_t* n_lines);
+
+/* Binning stage of the mask pipeline for a width x height image: per 16x16
+ * tile its record count (tile_counts[tiles]), the records themselves (4 f32
+ * each, tile-major, records of one tile in unspecified order), the band
+ * backdrop winding per tile (backdrop[tiles], exclusive prefix along each
+ * tile row) and the free-endpoint events (4 x int32/f32 each). Sizes are
+ * returned through n_records / n_events; pass NULL buffers to query them.
+ * No reference counterpart: this is the GPU decomposition of
+ * signed_difference_line (src/rasterize.rs:365-470) exposed for checking. */
+rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                 int edge_mode, size_t width, size_t height, uint32_t* tile_counts,
+                 float* records, size_t records_cap, int32_t* backdrop, float* events,
+                 size_t events_cap, size_t* n_records, size_t* n_events);
+
+/* ---- Rasterizer::mask / mask_iter (drop-in, blocking) --------------------- */
+
+/* Rasterizer::mask (src/rasterize.rs:50-56): SignedDifferenceRasterizer
+ * (edge_mode RZ_EDGE_MASK, src/rasterize.rs:298-311) or ActiveEdgeRasterizer
+ * (RZ_EDGE_NONE, src/rasterize.rs:537-556) into the caller's f64 image
+ * data[data_len] viewed through `shape`. Every pixel of the view is written.
+ * flatness: the rasterizer's tolerance (DEFAULT_FLATNESS = 0.05). */
+rz_status rz_mask(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                  int fill_rule, int edge_mode, double* data, size_t data_len, rz_shape shape);
+
+/* Same computation into a dense row-major f32 buffer out[height * row_stride]. */
+rz_status rz_mask_f32(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                      int fill_rule, int edge_mode, size_t width, size_t height, float* out,
+                      size_t row_stride);
+
+/* ---- device-resident path (inputs already in HBM) ------------------------ */
+
+/* Upload a path once; the handle is reused across calls. */
+rz_status rz_path_upload(rz_ctx* ctx, const rz_path* path, rz_dpath** out);
+rz_status rz_path_free(rz_ctx* ctx, rz_dpath* dpath);
+
+/* Asynchronous Rasterizer::mask of a resident path into device memory
+ * d_out[height * row_stride] (f32). Errors (NaN, capacity) surface at rz_sync. */
+rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, double flatness,
+                         int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
+                         size_t row_stride);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RZ_H */

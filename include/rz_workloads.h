@@ -1,0 +1,54 @@
+/*
+ * rz_workloads.h -- seeded synthetic inputs for the benchmark configurations.
+ *
+ * Not part of the reference's API: these build the inputs the benchmark and
+ * the parity tests share, from the reference's own LCG `Rnd`
+ * (benches/scene_bench.rs:51-112), so every consumer sees identical paths.
+ * Paths are built with PathBuilder semantics (src/path.rs:799-1056), arcs
+ * expanded to cubics as PathBuilder::arc_to does.
+ */
+#ifndef RZ_WORKLOADS_H
+#define RZ_WORKLOADS_H
+
+#include "rz.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rzw_set rzw_set;
+
+/* One closed subpath of 64 cubic Beziers with control points uniform in
+ * [0, size)^2, the last endpoint back at the first point (self-intersecting). */
+rzw_set* rzw_cubics64(uint32_t seed, double size);
+
+/* One path of n_segments commands: 40% lines, 30% quads, 25% cubics, 5%
+ * elliptic arcs (radii U(1,200), rotation U(0,360) degrees, random flags);
+ * endpoints a random walk with N(0, 40) steps clamped to [0, canvas)^2,
+ * a new closed subpath every 500 commands. The set also carries the random
+ * affine transform: scale U(0.5, 2), rotation U[0, 2pi), centroid moved to the
+ * canvas centre. */
+rzw_set* rzw_random_walk(uint32_t seed, uint32_t n_segments, double canvas);
+
+/* Number of paths in the set and a view of path i (valid while the set lives). */
+size_t rzw_count(const rzw_set* set);
+void rzw_path(const rzw_set* set, size_t i, rz_path* out);
+/* The set's transform (identity unless the generator draws one). */
+void rzw_transform(const rzw_set* set, rz_transform* out);
+void rzw_free(rzw_set* set);
+
+/* The reference's Rnd: first `n` values of uniform() for the given seed
+ * (used to pin the generator against the LCG's definition). */
+void rzw_rnd_uniform(uint32_t seed, size_t n, double* out);
+
+/* PathBuilder::arc_to from `src` (src/path.rs:945-972): writes the cubics
+ * (8 doubles each, at most 4) and returns their count, or -1 when the arc
+ * degenerates into line_to. */
+int rzw_arc_to(const double* src, const double* radii, double x_axis_rot, int large, int sweep,
+               const double* dst, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RZ_WORKLOADS_H */

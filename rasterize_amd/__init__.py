@@ -1,0 +1,426 @@
+"""rasterize_amd -- MI355X coverage rasterizer with the API of aslpavel/rasterize.
+
+Python mirror of the crate's host API over the C ABI (include/rz.h):
+Transform, Path / PathBuilder, FillRule, SignedDifferenceRasterizer and
+ActiveEdgeRasterizer (`mask`, `mask_iter`), plus the flatten/bin stage hooks.
+All rasterization runs in the HIP kernels of librz.so; nothing here computes
+coverage on the CPU.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi
+from ._abi import lib
+
+DEFAULT_FLATNESS = 0.05  # src/path.rs:16
+EPSILON = 2.220446049250313e-16
+
+
+_libm = C.CDLL("libm.so.6")
+_libm.sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+_libm.sincos.restype = None
+
+
+def sin_cos(a):
+    """f64::sin_cos: one glibc sincos() call, as LLVM lowers it on x86_64-linux-gnu."""
+    s, c = C.c_double(), C.c_double()
+    _libm.sincos(float(a), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+class RzError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_abi.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class FillRule:
+    NonZero = _abi.RZ_NONZERO
+    EvenOdd = _abi.RZ_EVENODD
+
+
+class Transform:
+    """2D affine [m00, m01, m02, m10, m11, m12] (src/geometry.rs:316-539)."""
+
+    __slots__ = ("m",)
+
+    def __init__(self, m00=1.0, m01=0.0, m02=0.0, m10=0.0, m11=1.0, m12=0.0):
+        self.m = (float(m00), float(m01), float(m02), float(m10), float(m11), float(m12))
+
+    @staticmethod
+    def identity():
+        return Transform()
+
+    @staticmethod
+    def new_translate(tx, ty):
+        return Transform(1.0, 0.0, tx, 0.0, 1.0, ty)
+
+    @staticmethod
+    def new_scale(sx, sy):
+        return Transform(sx, 0.0, 0.0, 0.0, sy, 0.0)
+
+    @staticmethod
+    def new_rotate(a):
+        s, c = sin_cos(a)
+        return Transform(c, -s, 0.0, s, c, 0.0)
+
+    def __mul__(self, o):
+        s, q = self.m, o.m
+        return Transform(s[0] * q[0] + s[1] * q[3], s[0] * q[1] + s[1] * q[4],
+                         s[0] * q[2] + s[1] * q[5] + s[2], s[3] * q[0] + s[4] * q[3],
+                         s[3] * q[1] + s[4] * q[4], s[3] * q[2] + s[4] * q[5] + s[5])
+
+    def pre_translate(self, tx, ty):
+        return self * Transform.new_translate(tx, ty)
+
+    def pre_scale(self, sx, sy):
+        return self * Transform.new_scale(sx, sy)
+
+    def pre_rotate(self, a):
+        return self * Transform.new_rotate(a)
+
+    def apply(self, p):
+        m = self.m
+        return (p[0] * m[0] + p[1] * m[1] + m[2], p[0] * m[3] + p[1] * m[4] + m[5])
+
+    def abi(self):
+        t = _abi.rz_transform()
+        for i, v in enumerate(self.m):
+            t.m[i] = v
+        return t
+
+
+def _is_close(a, b):
+    return abs(a[0] - b[0]) < EPSILON and abs(a[1] - b[1]) < EPSILON
+
+
+class Path:
+    """Path {segments, subpaths, closed} (src/path.rs:228-233) as SoA arrays."""
+
+    def __init__(self, kinds, point_offsets, points, subpaths, closed):
+        self.kinds = np.ascontiguousarray(kinds, dtype=np.uint8)
+        self.point_offsets = np.ascontiguousarray(point_offsets, dtype=np.uint32)
+        self.points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1)
+        self.subpaths = np.ascontiguousarray(subpaths, dtype=np.uint32)
+        self.closed = np.ascontiguousarray(closed, dtype=np.uint8)
+        self._abi = None
+
+    @staticmethod
+    def builder():
+        return PathBuilder()
+
+    @staticmethod
+    def from_abi(p):
+        """Copy an rz_path view into a Path."""
+        ns, nsub = p.n_segments, p.n_subpaths
+        kinds = np.ctypeslib.as_array(p.kinds, (ns,)).copy() if ns else np.zeros(0, np.uint8)
+        offs = np.ctypeslib.as_array(p.point_offsets, (ns + 1,)).copy()
+        npts = int(offs[-1])
+        pts = np.ctypeslib.as_array(p.points, (2 * npts,)).copy() if npts else np.zeros(0)
+        subs = np.ctypeslib.as_array(p.subpaths, (nsub + 1,)).copy() if nsub else np.zeros(0, np.uint32)
+        closed = np.ctypeslib.as_array(p.closed, (nsub,)).copy() if nsub else np.zeros(0, np.uint8)
+        return Path(kinds, offs, pts, subs, closed)
+
+    def segments_count(self):
+        return int(self.kinds.size)
+
+    def __len__(self):
+        return max(0, int(self.subpaths.size) - 1)
+
+    def abi(self):
+        if self._abi is None:
+            p = _abi.rz_path()
+            p.n_segments = self.kinds.size
+            p.n_subpaths = max(0, self.subpaths.size - 1)
+            p.kinds = self.kinds.ctypes.data_as(C.POINTER(C.c_uint8))
+            p.point_offsets = self.point_offsets.ctypes.data_as(C.POINTER(C.c_uint32))
+            p.points = self.points.ctypes.data_as(C.POINTER(C.c_double))
+            p.subpaths = self.subpaths.ctypes.data_as(C.POINTER(C.c_uint32))
+            p.closed = self.closed.ctypes.data_as(C.POINTER(C.c_uint8))
+            self._abi = p
+        return self._abi
+
+    def transform_points(self, tr):
+        """Path::transform (src/path.rs:359-363) on a copy."""
+        m = tr.m
+        xs, ys = self.points[0::2], self.points[1::2]
+        out = np.empty_like(self.points)
+        out[0::2] = xs * m[0] + ys * m[1] + m[2]
+        out[1::2] = xs * m[3] + ys * m[4] + m[5]
+        return Path(self.kinds, self.point_offsets, out, self.subpaths, self.closed)
+
+    def mask(self, rasterizer, tr, fill_rule, img):
+        """Path::mask (src/path.rs:513-520)."""
+        rasterizer.mask(self, tr, img, fill_rule)
+        return img
+
+
+class PathBuilder:
+    """PathBuilder (src/path.rs:799-1056)."""
+
+    def __init__(self):
+        self.position = (0.0, 0.0)
+        self._kinds, self._pts, self._subpaths, self._closed = [], [], [], []
+
+    def _push(self, kind, pts):
+        self._kinds.append(kind)
+        self._pts.append(tuple(float(v) for p in pts for v in p))
+
+    def _finish(self, close):
+        n = len(self._kinds)
+        if n == 0 or (self._subpaths and self._subpaths[-1] == n):
+            return
+        if not self._subpaths:
+            self._subpaths.append(0)
+        if close:
+            first = self._pts[self._subpaths[-1]]
+            self.position = (first[0], first[1])
+        self._subpaths.append(n)
+        self._closed.append(1 if close else 0)
+
+    def move_to(self, p):
+        self._finish(False)
+        self.position = (float(p[0]), float(p[1]))
+        return self
+
+    def close(self):
+        self._finish(True)
+        return self
+
+    def line_to(self, p):
+        p = (float(p[0]), float(p[1]))
+        if not _is_close(self.position, p):
+            self._push(_abi.RZ_LINE, [self.position, p])
+            self.position = p
+        return self
+
+    def quad_to(self, p1, p2):
+        self._push(_abi.RZ_QUAD, [self.position, p1, p2])
+        self.position = (float(p2[0]), float(p2[1]))
+        return self
+
+    def cubic_to(self, p1, p2, p3):
+        self._push(_abi.RZ_CUBIC, [self.position, p1, p2, p3])
+        self.position = (float(p3[0]), float(p3[1]))
+        return self
+
+    def arc_to(self, radii, x_axis_rot, large, sweep, p):
+        out = (C.c_double * 32)()
+        src = (C.c_double * 2)(*self.position)
+        rad = (C.c_double * 2)(float(radii[0]), float(radii[1]))
+        dst = (C.c_double * 2)(float(p[0]), float(p[1]))
+        n = lib().rzw_arc_to(src, rad, float(x_axis_rot), int(bool(large)), int(bool(sweep)), dst, out)
+        if n < 0:
+            return self.line_to(p)
+        for i in range(n):
+            o = out[8 * i: 8 * i + 8]
+            self._push(_abi.RZ_CUBIC, [(o[0], o[1]), (o[2], o[3]), (o[4], o[5]), (o[6], o[7])])
+        self.position = (float(p[0]), float(p[1]))
+        return self
+
+    def circle(self, radius):
+        off = 0.5522847498307935 * radius
+        cx, cy = self.position
+        p0, p1 = (cx - radius, cy), (cx, cy - radius)
+        p2, p3 = (cx + radius, cy), (cx, cy + radius)
+        return (self.move_to(p0)
+                .cubic_to((p0[0], p0[1] - off), (p1[0] - off, p1[1]), p1)
+                .cubic_to((p1[0] + off, p1[1]), (p2[0], p2[1] - off), p2)
+                .cubic_to((p2[0], p2[1] + off), (p3[0] + off, p3[1]), p3)
+                .cubic_to((p3[0] - off, p3[1]), (p0[0], p0[1] + off), p0)
+                .close()
+                .move_to((cx, cy)))
+
+    def build(self):
+        self._finish(False)
+        kinds = np.array(self._kinds, dtype=np.uint8)
+        offs = np.zeros(len(kinds) + 1, dtype=np.uint32)
+        offs[1:] = np.cumsum(kinds.astype(np.uint32) + 2)
+        pts = np.array([v for seg in self._pts for v in seg], dtype=np.float64)
+        path = Path(kinds, offs, pts, np.array(self._subpaths, np.uint32),
+                    np.array(self._closed, np.uint8))
+        self.__init__()
+        return path
+
+
+class Context:
+    """One rz_ctx (device scratch, stream). Not thread-safe."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        s = lib().rz_ctx_create(C.byref(h), device)
+        if s != _abi.RZ_OK:
+            raise RzError(s, "rz_ctx_create failed (no usable GPU?)")
+        self.h = h
+
+    def check(self, s):
+        if s != _abi.RZ_OK:
+            raise RzError(s, lib().rz_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            lib().rz_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _as_f64_view(img):
+    """(data array, rz_shape) for a numpy f64 2D array or view of one."""
+    if img.dtype != np.float64 or img.ndim != 2:
+        raise TypeError("mask image must be a 2D float64 array")
+    base = img
+    while base.base is not None and isinstance(base.base, np.ndarray):
+        base = base.base
+    item = img.itemsize
+    start = (img.__array_interface__["data"][0] - base.__array_interface__["data"][0]) // item
+    shape = _abi.rz_shape(start, img.shape[1], img.shape[0], img.strides[0] // item,
+                          img.strides[1] // item)
+    return base, shape
+
+
+class _RasterizerBase:
+    edge_mode = None
+
+    def __init__(self, flatness=DEFAULT_FLATNESS, ctx=None):
+        self.flatness = float(flatness)
+        self.ctx = ctx
+
+    def _ctx(self):
+        return self.ctx or default_context()
+
+    def mask(self, path, tr, img, fill_rule):
+        """Rasterizer::mask into a 2D float64 array (or strided view of one)."""
+        ctx = self._ctx()
+        base, shape = _as_f64_view(img)
+        if not base.flags.c_contiguous:
+            raise TypeError("the image's base buffer must be contiguous")
+        data = base.ctypes.data_as(C.POINTER(C.c_double))
+        ctx.check(lib().rz_mask(ctx.h, C.byref(path.abi()), tr.abi(), self.flatness, fill_rule,
+                                self.edge_mode, data, base.size, shape))
+        return img
+
+    def mask_f32(self, path, tr, width, height, fill_rule, edge_mode=None):
+        ctx = self._ctx()
+        out = np.zeros((height, width), dtype=np.float32)
+        mode = self.edge_mode if edge_mode is None else edge_mode
+        ctx.check(lib().rz_mask_f32(ctx.h, C.byref(path.abi()), tr.abi(), self.flatness, fill_rule,
+                                    mode, width, height,
+                                    out.ctypes.data_as(C.POINTER(C.c_float)), width))
+        return out
+
+    def mask_iter(self, path, tr, size, fill_rule):
+        """Rasterizer::mask_iter: list of (x, y, alpha) in row-major order."""
+        width, height = size
+        dense = self.mask_f32(path, tr, width, height, fill_rule, self.iter_mode)
+        ys, xs = np.nonzero(np.abs(dense) >= 1e-6)
+        return [(int(x), int(y), float(dense[y, x])) for y, x in zip(ys, xs)]
+
+
+class SignedDifferenceRasterizer(_RasterizerBase):
+    """SignedDifferenceRasterizer (src/rasterize.rs:278-360)."""
+
+    edge_mode = _abi.RZ_EDGE_MASK
+    iter_mode = _abi.RZ_EDGE_ITER
+
+    def name(self):
+        return "signed-difference"
+
+
+class ActiveEdgeRasterizer(_RasterizerBase):
+    """ActiveEdgeRasterizer (src/rasterize.rs:517-575); same coverage, no right clip."""
+
+    edge_mode = _abi.RZ_EDGE_NONE
+    iter_mode = _abi.RZ_EDGE_NONE
+
+    def name(self):
+        return "active-edge"
+
+
+def flatten(path, tr, flatness=DEFAULT_FLATNESS, close=True, ctx=None):
+    """Path::flatten on the GPU: (n, 4) float64 array of lines (x0, y0, x1, y1)."""
+    ctx = ctx or default_context()
+    n = C.c_size_t()
+    ctx.check(lib().rz_flatten(ctx.h, C.byref(path.abi()), tr.abi(), flatness, int(close), None, 0,
+                               C.byref(n)))
+    out = np.zeros((max(n.value, 1), 4), dtype=np.float64)
+    ctx.check(lib().rz_flatten(ctx.h, C.byref(path.abi()), tr.abi(), flatness, int(close),
+                               out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n)))
+    return out[: n.value]
+
+
+def bin_stage(path, tr, width, height, edge_mode=_abi.RZ_EDGE_MASK, flatness=DEFAULT_FLATNESS,
+              ctx=None):
+    """Binning stage hook: (tile_counts, records[n,4] f32, backdrop, events[n,4])."""
+    ctx = ctx or default_context()
+    nr, ne = C.c_size_t(), C.c_size_t()
+    ctx.check(lib().rz_bin(ctx.h, C.byref(path.abi()), tr.abi(), flatness, edge_mode, width, height,
+                           None, None, 0, None, None, 0, C.byref(nr), C.byref(ne)))
+    tx, ty = (width + 15) // 16, (height + 15) // 16
+    counts = np.zeros(tx * ty, np.uint32)
+    recs = np.zeros((max(nr.value, 1), 4), np.float32)
+    bd = np.zeros(tx * ty, np.int32)
+    ev = np.zeros((max(ne.value, 1), 4), np.float32)
+    ctx.check(lib().rz_bin(ctx.h, C.byref(path.abi()), tr.abi(), flatness, edge_mode, width, height,
+                           counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
+                           bd.ctypes.data_as(C.POINTER(C.c_int32)),
+                           ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0],
+                           C.byref(nr), C.byref(ne)))
+    return counts, recs[: nr.value], bd, ev[: ne.value]
+
+
+# ---- workloads ------------------------------------------------------------
+
+class WorkloadSet:
+    """Owned set of generated paths (include/rz_workloads.h)."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("workload generation failed")
+        self.h = handle
+        L = lib()
+        self.paths = []
+        for i in range(L.rzw_count(handle)):
+            p = _abi.rz_path()
+            L.rzw_path(handle, i, C.byref(p))
+            self.paths.append(Path.from_abi(p))
+        t = _abi.rz_transform()
+        L.rzw_transform(handle, C.byref(t))
+        self.transform = Transform(*t.m)
+
+    def __del__(self):
+        try:
+            lib().rzw_free(self.h)
+        except Exception:
+            pass
+
+
+def workload_cubics64(seed=1, size=512.0):
+    return WorkloadSet(lib().rzw_cubics64(seed, size))
+
+
+def workload_random_walk(seed=2, n_segments=200_000, canvas=4096.0):
+    return WorkloadSet(lib().rzw_random_walk(seed, n_segments, canvas))
+
+
+def rnd_uniform(seed, n):
+    out = np.zeros(n, np.float64)
+    lib().rzw_rnd_uniform(seed, n, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out

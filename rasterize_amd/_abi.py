@@ -1,0 +1,136 @@
+"""ctypes bindings of librz.so (include/rz.h, include/rz_workloads.h).
+
+The library is loaded from this package directory. There is no fallback: if
+librz.so is missing the import fails, and GPU entry points fail loudly when no
+device is present.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librz.so")
+
+RZ_OK, RZ_E_NAN, RZ_E_ARG, RZ_E_HIP, RZ_E_DEPTH, RZ_E_NOMEM, RZ_E_CAPACITY = range(7)
+STATUS_NAMES = {
+    RZ_OK: "RZ_OK", RZ_E_NAN: "RZ_E_NAN", RZ_E_ARG: "RZ_E_ARG", RZ_E_HIP: "RZ_E_HIP",
+    RZ_E_DEPTH: "RZ_E_DEPTH", RZ_E_NOMEM: "RZ_E_NOMEM", RZ_E_CAPACITY: "RZ_E_CAPACITY",
+}
+RZ_NONZERO, RZ_EVENODD = 0, 1
+RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
+RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
+RZ_PAINT_SOLID, RZ_PAINT_LINEAR, RZ_PAINT_RADIAL = 0, 1, 2
+
+
+class rz_transform(C.Structure):
+    _fields_ = [("m", C.c_double * 6)]
+
+
+class rz_path(C.Structure):
+    _fields_ = [
+        ("n_segments", C.c_uint32),
+        ("n_subpaths", C.c_uint32),
+        ("kinds", C.POINTER(C.c_uint8)),
+        ("point_offsets", C.POINTER(C.c_uint32)),
+        ("points", C.POINTER(C.c_double)),
+        ("subpaths", C.POINTER(C.c_uint32)),
+        ("closed", C.POINTER(C.c_uint8)),
+    ]
+
+
+class rz_shape(C.Structure):
+    _fields_ = [
+        ("start", C.c_size_t),
+        ("width", C.c_size_t),
+        ("height", C.c_size_t),
+        ("row_stride", C.c_size_t),
+        ("col_stride", C.c_size_t),
+    ]
+
+
+class rz_paint(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("color", C.c_float * 4),
+        ("units", C.c_int32),
+        ("linear_colors", C.c_int32),
+        ("spread", C.c_int32),
+        ("tr", C.c_double * 6),
+        ("start", C.c_double * 2),
+        ("end", C.c_double * 2),
+        ("center", C.c_double * 2),
+        ("radius", C.c_double),
+        ("fcenter", C.c_double * 2),
+        ("fradius", C.c_double),
+        ("n_stops", C.c_uint32),
+        ("stop_positions", C.POINTER(C.c_double)),
+        ("stop_colors", C.POINTER(C.c_float)),
+    ]
+
+
+P = C.POINTER
+c_ctx = C.c_void_p
+c_dpath = C.c_void_p
+c_set = C.c_void_p
+
+# (name, restype, argtypes) of every function include/rz.h and
+# include/rz_workloads.h declare
+PROTOTYPES = [
+    ("rz_ctx_create", C.c_int, [P(c_ctx), C.c_int]),
+    ("rz_ctx_destroy", C.c_int, [c_ctx]),
+    ("rz_last_error", C.c_char_p, [c_ctx]),
+    ("rz_sync", C.c_int, [c_ctx]),
+    ("rz_stream", C.c_void_p, [c_ctx]),
+    ("rz_flatten", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int,
+                             P(C.c_double), C.c_size_t, P(C.c_size_t)]),
+    ("rz_bin", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_size_t,
+                         C.c_size_t, P(C.c_uint32), P(C.c_float), C.c_size_t, P(C.c_int32),
+                         P(C.c_float), C.c_size_t, P(C.c_size_t), P(C.c_size_t)]),
+    ("rz_mask", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
+                          P(C.c_double), C.c_size_t, rz_shape]),
+    ("rz_mask_f32", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
+                              C.c_size_t, C.c_size_t, P(C.c_float), C.c_size_t]),
+    ("rz_path_upload", C.c_int, [c_ctx, P(rz_path), P(c_dpath)]),
+    ("rz_path_free", C.c_int, [c_ctx, c_dpath]),
+    ("rz_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
+                                 C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    # workloads
+    ("rzw_cubics64", c_set, [C.c_uint32, C.c_double]),
+    ("rzw_random_walk", c_set, [C.c_uint32, C.c_uint32, C.c_double]),
+    ("rzw_count", C.c_size_t, [c_set]),
+    ("rzw_path", None, [c_set, C.c_size_t, P(rz_path)]),
+    ("rzw_transform", None, [c_set, P(rz_transform)]),
+    ("rzw_free", None, [c_set]),
+    ("rzw_rnd_uniform", None, [C.c_uint32, C.c_size_t, P(C.c_double)]),
+    ("rzw_arc_to", C.c_int, [P(C.c_double), P(C.c_double), C.c_double, C.c_int, C.c_int,
+                             P(C.c_double), P(C.c_double)]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load librz.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} not found: build it with `python -m rasterize_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in PROTOTYPES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of the functions declared in include/*.h (parsed from the headers)."""
+    import re
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    names = set()
+    for fn in ("rz.h", "rz_workloads.h"):
+        src = open(os.path.join(inc, fn)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"\b(rzw?_[a-z0-9_]+)\s*\(", src):
+            names.add(m.group(1))
+    return names

@@ -1,0 +1,374 @@
+// Geometry shared by the HIP kernels (device) and the host side: f64 transform
+// and segment arithmetic for flattening, and the binning geometry that turns a
+// flattened line into per-tile pieces.
+//
+// Flattening must reproduce the reference's line list bit for bit, so every
+// expression below keeps the reference's operation order and the library is
+// compiled with -ffp-contract=off (Rust never fuses multiply-add).
+#pragma once
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define RZ_HD __host__ __device__ inline
+#else
+#define RZ_HD inline
+#endif
+
+namespace rzg {
+
+// ---------------------------------------------------------------------------
+// Constants (src/geometry.rs:14-16)
+// ---------------------------------------------------------------------------
+static constexpr double kEpsilon = 2.220446049250313e-16;  // f64::EPSILON
+static constexpr int kTile = 16;                            // tile edge in pixels
+static constexpr int kFixShift = 20;                        // LDS fixed point: 2^-20
+static constexpr int kMaxDepth = 62;                        // subdivision depth limit
+
+enum SegKind { KIND_LINE = 0, KIND_QUAD = 1, KIND_CUBIC = 2 };
+enum EdgeMode { EDGE_MASK = 0, EDGE_ITER = 1, EDGE_NONE = 2 };
+enum Status { ST_OK = 0, ST_NAN = 1, ST_DEPTH = 2 };
+
+struct Pt {
+  double x, y;
+};
+RZ_HD Pt mkpt(double x, double y) {
+  Pt p;
+  p.x = x;
+  p.y = y;
+  return p;
+}
+
+// Transform::apply (src/geometry.rs:363-367): x*m00 + y*m01 + m02
+RZ_HD Pt tr_apply(const double* m, Pt p) {
+  return mkpt(p.x * m[0] + p.y * m[1] + m[2], p.x * m[3] + p.y * m[4] + m[5]);
+}
+
+// A Bezier node: K+1 control points (K = 1 line, 2 quad, 3 cubic).
+template <int NP>
+struct Node {
+  Pt p[NP];
+};
+
+// Segment::flatness (src/curve.rs:233-235, 413-417, 692-697)
+RZ_HD double flatness(const Node<2>&) { return 0.0; }
+RZ_HD double flatness(const Node<3>& n) {
+  double x = 2.0 * n.p[1].x - n.p[0].x - n.p[2].x;
+  double y = 2.0 * n.p[1].y - n.p[0].y - n.p[2].y;
+  return x * x + y * y;
+}
+RZ_HD double flatness(const Node<4>& n) {
+  double ux = 3.0 * n.p[1].x - 2.0 * n.p[0].x - n.p[3].x;
+  double uy = 3.0 * n.p[1].y - 2.0 * n.p[0].y - n.p[3].y;
+  double vx = 3.0 * n.p[2].x - n.p[0].x - 2.0 * n.p[3].x;
+  double vy = 3.0 * n.p[2].y - n.p[0].y - 2.0 * n.p[3].y;
+  return fmax(ux * ux, vx * vx) + fmax(uy * uy, vy * vy);
+}
+
+template <int NP>
+RZ_HD bool has_nans(const Node<NP>& n) {
+  bool r = false;
+  for (int i = 0; i < NP; ++i) r = r || isnan(n.p[i].x) || isnan(n.p[i].y);
+  return r;
+}
+
+// Curve::split_at(0.5), the split PathFlattenIter uses (src/curve.rs:43-45,
+// 1186-1201). With t = 0.5 every coefficient the reference computes
+// (1-t, t*t, 2*t*(1-t), 3*t*(1-t)^2, ...) is an exact power-of-two multiple,
+// so the literals below are bit-identical to the reference's products; the
+// additions keep its left-to-right order (src/curve.rs:458-468, 749-769).
+RZ_HD void child(const Node<3>& n, int right, Node<3>* c) {
+  Pt mid = mkpt(0.25 * n.p[0].x + 0.5 * n.p[1].x + 0.25 * n.p[2].x,
+                0.25 * n.p[0].y + 0.5 * n.p[1].y + 0.25 * n.p[2].y);
+  if (!right) {
+    c->p[0] = n.p[0];
+    c->p[1] = mkpt(0.5 * n.p[0].x + 0.5 * n.p[1].x, 0.5 * n.p[0].y + 0.5 * n.p[1].y);
+    c->p[2] = mid;
+  } else {
+    c->p[0] = mid;
+    c->p[1] = mkpt(0.5 * n.p[1].x + 0.5 * n.p[2].x, 0.5 * n.p[1].y + 0.5 * n.p[2].y);
+    c->p[2] = n.p[2];
+  }
+}
+RZ_HD void child(const Node<4>& n, int right, Node<4>* c) {
+  Pt mid = mkpt(0.125 * n.p[0].x + 0.375 * n.p[1].x + 0.375 * n.p[2].x + 0.125 * n.p[3].x,
+                0.125 * n.p[0].y + 0.375 * n.p[1].y + 0.375 * n.p[2].y + 0.125 * n.p[3].y);
+  if (!right) {
+    c->p[0] = n.p[0];
+    c->p[1] = mkpt(0.5 * n.p[0].x + 0.5 * n.p[1].x, 0.5 * n.p[0].y + 0.5 * n.p[1].y);
+    c->p[2] = mkpt(0.25 * n.p[0].x + 0.5 * n.p[1].x + 0.25 * n.p[2].x,
+                   0.25 * n.p[0].y + 0.5 * n.p[1].y + 0.25 * n.p[2].y);
+    c->p[3] = mid;
+  } else {
+    c->p[0] = mid;
+    c->p[1] = mkpt(0.25 * n.p[1].x + 0.5 * n.p[2].x + 0.25 * n.p[3].x,
+                   0.25 * n.p[1].y + 0.5 * n.p[2].y + 0.25 * n.p[3].y);
+    c->p[2] = mkpt(0.5 * n.p[2].x + 0.5 * n.p[3].x, 0.5 * n.p[2].y + 0.5 * n.p[3].y);
+    c->p[3] = n.p[3];
+  }
+}
+RZ_HD void child(const Node<2>& n, int right, Node<2>* c) { *c = n; }
+
+// In-order traversal of one segment's subdivision tree (PathFlattenIter,
+// src/path.rs:758-796): a node whose flatness is below `fl` (= 16*tol^2) is a
+// leaf and emits Line(start, end); otherwise it is split at t = 0.5 and its
+// left child is visited first.
+//
+// The reference keeps an explicit stack. Here the pending right siblings live
+// in a 4-entry register shift stack; when that underflows the next node is
+// recomputed from the root along its path bits (depth splits from the root are
+// the same operations the recursive split performs, so the recomputed node is
+// bit-identical). This keeps the traversal in registers with no scratch
+// memory; recomputation only happens above the deepest four levels.
+template <int NP, typename Leaf>
+RZ_HD int traverse(const Node<NP>& root, double fl, Leaf&& leaf) {
+  Node<NP> cur = root, s0, s1, s2, s3;
+  int nst = 0, depth = 0;
+  uint64_t bits = 0;
+  while (true) {
+    if (has_nans(cur)) return ST_NAN;
+    if (flatness(cur) < fl) {
+      leaf(cur.p[0], cur.p[NP - 1]);
+      while (depth > 0 && (bits & 1)) {
+        bits >>= 1;
+        --depth;
+      }
+      if (depth == 0) return ST_OK;
+      bits |= 1;
+      if (nst > 0) {
+        cur = s0;
+        s0 = s1;
+        s1 = s2;
+        s2 = s3;
+        --nst;
+      } else {
+        Node<NP> n = root;
+        for (int k = depth - 1; k >= 0; --k) {
+          Node<NP> c;
+          child(n, (int)((bits >> k) & 1), &c);
+          n = c;
+        }
+        cur = n;
+      }
+    } else {
+      if (depth >= kMaxDepth) return ST_DEPTH;
+      Node<NP> l, r;
+      child(cur, 0, &l);
+      child(cur, 1, &r);
+      s3 = s2;
+      s2 = s1;
+      s1 = s0;
+      s0 = r;
+      nst = nst < 4 ? nst + 1 : 4;
+      cur = l;
+      ++depth;
+      bits <<= 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binning geometry
+// ---------------------------------------------------------------------------
+struct BinGrid {
+  double wc;      // right clip abscissa: W-1 (mask), W (mask_iter/fill); unused for EDGE_NONE
+  int mode;       // EdgeMode
+  int W, H;       // image size in pixels
+  int tiles_x, tiles_y;
+};
+
+// x of the line (a, b) at ordinate Y, y of it at abscissa X. Both use the
+// reference's lerp form (1-t)*p0 + t*p1 (src/rasterize.rs:378), which returns
+// the endpoint exactly at t = 0 and t = 1, so pieces that meet on a tile or
+// band boundary share their endpoint bit for bit.
+RZ_HD Pt at_y(Pt a, Pt b, double Y) {
+  double t = (Y - a.y) / (b.y - a.y);
+  return mkpt((1.0 - t) * a.x + t * b.x, Y);
+}
+RZ_HD Pt at_x(Pt a, Pt b, double X) {
+  double t = (X - a.x) / (b.x - a.x);
+  return mkpt(X, (1.0 - t) * a.y + t * b.y);
+}
+
+// The reference's per-line preprocessing inside signed_difference_line: right
+// clip at x = wc (src/rasterize.rs:370-387), then the left split at x = 0
+// (src/rasterize.rs:389-393, 922-937) whose x<0 part becomes a vertical line on
+// x = 0. Produces up to two pieces in `out` (returns the count) and reports
+// which endpoints of the effective polyline were created by the right clip.
+//
+// Deliberate deviation: for a line starting exactly on x = wc and ending to
+// its right the reference's `p0.x() < width` test keeps the part beyond the
+// clip line (and then indexes past the row); here that part is dropped, like
+// every other part beyond the clip.
+struct EffLine {
+  Pt p[3];   // polyline vertices p[0] -> ... -> p[n]
+  int n;     // number of pieces (0, 1 or 2)
+  bool clip0, clip1;  // start / end vertex created by the right clip
+};
+
+RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {
+  EffLine e;
+  e.clip0 = e.clip1 = false;
+  e.n = 1;
+  if (g.mode != EDGE_NONE) {
+    const double width = g.wc;
+    if (p0.x > width || p1.x > width) {
+      if (p0.x > width && p1.x > width) {
+        p0 = mkpt(width - 0.001, p0.y);
+        p1 = mkpt(width - 0.001, p1.y);
+        e.clip0 = e.clip1 = true;
+      } else {
+        double t = (p0.x - width) / (p0.x - p1.x);
+        Pt mid = mkpt(width, (1.0 - t) * p0.y + t * p1.y);
+        if (p0.x <= width) {
+          p1 = mid;
+          e.clip1 = true;
+        } else {
+          p0 = mid;
+          e.clip0 = true;
+        }
+      }
+    }
+  }
+  if (p0.x >= 0.0 && p1.x >= 0.0) {
+    e.p[0] = p0;
+    e.p[1] = p1;
+  } else if (p0.x <= 0.0 && p1.x <= 0.0) {
+    e.p[0] = mkpt(0.0, p0.y);
+    e.p[1] = mkpt(0.0, p1.y);
+  } else {
+    double t = p0.x / (p0.x - p1.x);
+    Pt mid = mkpt((1.0 - t) * p0.x + t * p1.x, (1.0 - t) * p0.y + t * p1.y);
+    if (p0.x < 0.0) {
+      e.p[0] = mkpt(0.0, p0.y);
+      e.p[1] = mid;
+      e.p[2] = p1;
+    } else {
+      e.p[0] = p0;
+      e.p[1] = mid;
+      e.p[2] = mkpt(0.0, p1.y);
+    }
+    e.n = 2;
+  }
+  return e;
+}
+
+RZ_HD int tile_col(double x, const BinGrid& g) {
+  double f = floor(x / (double)kTile);
+  if (!(f >= 0.0)) return 0;
+  if (f > (double)(g.tiles_x - 1)) return g.tiles_x - 1;
+  return (int)f;
+}
+
+// One tile record: a piece of a line clipped to a 16x16 cell, in coordinates
+// relative to the cell's top-left corner. The sign bit of x0 (x1) marks that
+// the piece starts (ends) where its band piece crosses the cell's left edge
+// from the left (to the left); the fine stage then adds +V(y0) (-V(y1)) to the
+// cell's rows, V(y) being the overlap of each pixel row with [0, y].
+struct Rec {
+  float x0, y0, x1, y1;
+};
+
+// Free-endpoint event: a chain endpoint that is not shared with another line
+// (a gap between segments, or a vertex created by the right clip). Adds
+// sign * V(yrel) to every tile of band `band` right of column `col`.
+struct Event {
+  int band;
+  int col;
+  float yrel;
+  float sign;
+};
+
+// Enumerate the records, band-boundary crossings and free-endpoint events of
+// one effective piece a -> b.
+//   rec(tile_index, Rec)        -- one cell record
+//   bd(band, col, dir)          -- crossing of the band's bottom edge at column col
+template <typename RecF, typename BdF>
+RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
+  const double H = (double)g.H;
+  double ylo = fmin(a.y, b.y), yhi = fmax(a.y, b.y);
+  if (!(yhi >= 0.0) || !(ylo <= H)) return;  // entirely above or below the image
+  const int dir = b.y > a.y ? 1 : -1;
+  // bands whose closed y-range meets [ylo, yhi]
+  int ty0 = ylo <= 0.0 ? 0 : (int)ceil(ylo / (double)kTile) - 1;
+  if (ty0 < 0) ty0 = 0;
+  int ty1 = (int)floor(fmin(yhi, H) / (double)kTile);
+  if (ty1 > g.tiles_y - 1) ty1 = g.tiles_y - 1;
+  for (int ty = ty0; ty <= ty1; ++ty) {
+    const double r0 = (double)(ty * kTile);
+    const double rb = fmin(r0 + (double)kTile, H);
+    // band-bottom crossing (a point on y = rb counts as inside the band)
+    if (ylo <= rb && rb < yhi) {
+      Pt c = at_y(a, b, rb);
+      int col = tile_col(c.x, g) + 1;
+      if (c.x < 0.0) col = 0;  // left of every tile start (x < 0 cannot occur after the left split)
+      if (col < g.tiles_x) bd(ty, col, dir);
+    }
+    // piece of a -> b inside r0 <= y <= rb, direction preserved
+    Pt s = a, e = b;
+    if (a.y != b.y) {
+      if (a.y < b.y) {
+        if (s.y < r0) s = at_y(a, b, r0);
+        if (e.y > rb) e = at_y(a, b, rb);
+      } else {
+        if (s.y > rb) s = at_y(a, b, rb);
+        if (e.y < r0) e = at_y(a, b, r0);
+      }
+    } else if (a.y < r0 || a.y > rb) {
+      continue;
+    }
+    if (s.x == e.x && s.y == e.y) continue;  // touches the band in one point only
+    const double xlo = fmin(s.x, e.x), xhi = fmax(s.x, e.x);
+    int tx0 = tile_col(xlo, g), tx1 = tile_col(xhi, g);
+    for (int tx = tx0; tx <= tx1; ++tx) {
+      const double c0 = (double)(tx * kTile), c1 = c0 + (double)kTile;
+      if (xhi < c0 || xlo > c1) continue;
+      // clip the band piece to c0 <= x <= c1
+      Pt u = s, v = e;
+      if (s.x != e.x) {
+        if (s.x < e.x) {
+          if (u.x < c0) u = at_x(s, e, c0);
+          if (v.x > c1) v = at_x(s, e, c1);
+        } else {
+          if (u.x > c1) u = at_x(s, e, c1);
+          if (v.x < c0) v = at_x(s, e, c0);
+        }
+      }
+      // crossings of the cell's left edge x = c0 (inside means x < c0)
+      const bool in_s = s.x < c0, in_e = e.x < c0;
+      bool flag0 = false, flag1 = false;
+      if (in_s && !in_e) flag0 = u.y != r0;  // exits the left region: +V(u.y)
+      if (!in_s && in_e) flag1 = v.y != r0;  // enters the left region: -V(v.y)
+      // no vertical extent and no edge crossing: nothing to deposit
+      if (u.y == v.y && !flag0 && !flag1) continue;
+      Rec r;
+      r.x0 = (float)(u.x - c0);
+      r.y0 = (float)(u.y - r0);
+      r.x1 = (float)(v.x - c0);
+      r.y1 = (float)(v.y - r0);
+      if (flag0) r.x0 = -0.0f;
+      if (flag1) r.x1 = -0.0f;
+      rec(ty * g.tiles_x + tx, r);
+    }
+  }
+}
+
+// Free-endpoint event for effective vertex p (sign +1 for a chain end, -1 for
+// a chain start). Returns false when no tile lies right of it in its band or
+// the point is outside the image rows (then the event has no effect).
+RZ_HD bool make_event(Pt p, float sign, const BinGrid& g, Event* ev) {
+  const double H = (double)g.H;
+  if (!(p.y > 0.0) || !(p.y <= H)) return false;
+  int band = (int)ceil(p.y / (double)kTile) - 1;
+  if (band < 0) band = 0;
+  if (band > g.tiles_y - 1) return false;
+  int col = tile_col(p.x, g);
+  if (col >= g.tiles_x - 1) return false;
+  ev->band = band;
+  ev->col = col;
+  ev->yrel = (float)(p.y - (double)(band * kTile));
+  ev->sign = sign;
+  return true;
+}
+
+}  // namespace rzg

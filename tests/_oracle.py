@@ -1,0 +1,185 @@
+"""ctypes loader for the oracle (oracle/_build/liboracle.so).
+
+The oracle is the CPU restatement of the reference used as the checker; only
+tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+
+class orc_path(C.Structure):
+    _fields_ = [
+        ("n_segments", C.c_uint32), ("n_subpaths", C.c_uint32),
+        ("kinds", C.POINTER(C.c_uint8)), ("point_offsets", C.POINTER(C.c_uint32)),
+        ("points", C.POINTER(C.c_double)), ("subpaths", C.POINTER(C.c_uint32)),
+        ("closed", C.POINTER(C.c_uint8)),
+    ]
+
+
+class orc_paint(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("color", C.c_float * 4), ("units", C.c_int32),
+        ("linear_colors", C.c_int32), ("spread", C.c_int32), ("tr", C.c_double * 6),
+        ("start", C.c_double * 2), ("end", C.c_double * 2), ("center", C.c_double * 2),
+        ("radius", C.c_double), ("fcenter", C.c_double * 2), ("fradius", C.c_double),
+        ("n_stops", C.c_uint32), ("stop_positions", C.POINTER(C.c_double)),
+        ("stop_colors", C.POINTER(C.c_float)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run([os.path.join(ROOT, "oracle", "build.sh")], check=True)
+        _lib = C.CDLL(ORACLE_SO)
+        P = C.POINTER
+        d, sz, i = C.c_double, C.c_size_t, C.c_int
+        protos = {
+            "orc_flatten": (i, [P(orc_path), P(d), d, i, P(d), sz, P(sz)]),
+            "orc_signed_difference_line": (i, [P(d), sz, sz, P(d)]),
+            "orc_sd_mask": (i, [P(orc_path), P(d), d, i, P(d), sz, sz, sz, sz, sz, sz]),
+            "orc_sd_mask_iter": (i, [P(orc_path), P(d), d, i, sz, sz, P(d)]),
+            "orc_ae_mask": (i, [P(orc_path), P(d), d, i, sz, sz, P(d)]),
+            "orc_ae_lines": (i, [P(d), sz, sz, sz, i, P(d)]),
+            "orc_exact_winding_area": (i, [P(orc_path), P(d), d, sz, sz, P(d)]),
+            "orc_exact_fill_coverage": (i, [P(orc_path), P(d), d, i, sz, sz, P(d)]),
+            "orc_paint_at": (i, [P(orc_paint), d, d, P(C.c_float)]),
+            "orc_radial_offset": (i, [P(orc_paint), d, d, P(d)]),
+            "orc_l2s": (i, [P(C.c_float), P(C.c_float)]),
+            "orc_s2l": (i, [P(C.c_float), P(C.c_float)]),
+            "orc_linear_to_srgb": (C.c_float, [C.c_float]),
+            "orc_srgb_to_linear": (C.c_float, [C.c_float]),
+            "orc_rgba_to_lin": (i, [P(C.c_uint8), P(C.c_float)]),
+            "orc_lin_to_rgba": (i, [P(C.c_float), P(C.c_uint8)]),
+            "orc_spread_at": (d, [i, d]),
+            "orc_quadratic_solve": (i, [d, d, d, P(d)]),
+            "orc_alpha_from_winding": (d, [i, d]),
+            "orc_split": (i, [i, P(d), i, P(d)]),
+            "orc_flatness": (d, [i, P(d)]),
+            "orc_arc_to": (i, [P(d), P(d), d, i, i, P(d), P(d)]),
+            "orc_path_bbox": (i, [P(orc_path), P(d), P(d)]),
+            "orc_time_sd_mask": (d, [P(orc_path), P(d), d, i, sz, sz, i, i]),
+            "orc_time_flatten": (d, [P(orc_path), P(d), d, i, P(sz)]),
+        }
+        for name, (res, args) in protos.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def opath(path):
+    """orc_path view of a rasterize_amd.Path (same layout as rz_path)."""
+    p = orc_path()
+    p.n_segments = path.kinds.size
+    p.n_subpaths = max(0, path.subpaths.size - 1)
+    p.kinds = path.kinds.ctypes.data_as(C.POINTER(C.c_uint8))
+    p.point_offsets = path.point_offsets.ctypes.data_as(C.POINTER(C.c_uint32))
+    p.points = path.points.ctypes.data_as(C.POINTER(C.c_double))
+    p.subpaths = path.subpaths.ctypes.data_as(C.POINTER(C.c_uint32))
+    p.closed = path.closed.ctypes.data_as(C.POINTER(C.c_uint8))
+    return p
+
+
+def _tr(tr):
+    return np.array(tr.m if hasattr(tr, "m") else tr, dtype=np.float64)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc == 1:
+        raise OracleError("cannot flatten segment with NaN")
+    if rc != 0:
+        raise OracleError(f"oracle error {rc}")
+
+
+def flatten(path, tr, flatness=0.05, close=True):
+    p, t = opath(path), _tr(tr)
+    n = C.c_size_t()
+    rc = lib().orc_flatten(C.byref(p), _dp(t), flatness, int(close), None, 0, C.byref(n))
+    if rc == 1:
+        raise OracleError("cannot flatten segment with NaN")
+    out = np.zeros((max(n.value, 1), 4))
+    _check(lib().orc_flatten(C.byref(p), _dp(t), flatness, int(close), _dp(out), n.value, C.byref(n)))
+    return out[: n.value]
+
+
+def sd_mask(path, tr, W, H, rule, flatness=0.05):
+    """SignedDifferenceRasterizer::mask into a zeroed W x H f64 image. rule=-1: raw accumulation."""
+    img = np.zeros((H, W))
+    p, t = opath(path), _tr(tr)
+    _check(lib().orc_sd_mask(C.byref(p), _dp(t), flatness, rule, _dp(img), img.size, W, H, 0, W, 1))
+    return img
+
+
+def sd_mask_iter(path, tr, W, H, rule, flatness=0.05):
+    out = np.zeros((H, W))
+    p, t = opath(path), _tr(tr)
+    _check(lib().orc_sd_mask_iter(C.byref(p), _dp(t), flatness, rule, W, H, _dp(out)))
+    return out
+
+
+def ae_mask(path, tr, W, H, rule, flatness=0.05):
+    out = np.zeros((H, W))
+    p, t = opath(path), _tr(tr)
+    _check(lib().orc_ae_mask(C.byref(p), _dp(t), flatness, rule, W, H, _dp(out)))
+    return out
+
+
+def ae_lines(lines, W, H, rule):
+    lines = np.ascontiguousarray(lines, dtype=np.float64)
+    out = np.zeros((H, W))
+    _check(lib().orc_ae_lines(_dp(lines), len(lines), W, H, rule, _dp(out)))
+    return out
+
+
+def signed_difference_line(W, H, line):
+    img = np.zeros((H, W))
+    l = np.array(line, dtype=np.float64)
+    lib().orc_signed_difference_line(_dp(img), W, H, _dp(l))
+    return img
+
+
+def exact_winding_area(path, tr, W, H, flatness=0.05):
+    out = np.zeros((H, W))
+    p, t = opath(path), _tr(tr)
+    _check(lib().orc_exact_winding_area(C.byref(p), _dp(t), flatness, W, H, _dp(out)))
+    return out
+
+
+def exact_fill_coverage(path, tr, W, H, rule, flatness=0.05):
+    out = np.zeros((H, W))
+    p, t = opath(path), _tr(tr)
+    _check(lib().orc_exact_fill_coverage(C.byref(p), _dp(t), flatness, rule, W, H, _dp(out)))
+    return out
+
+
+def path_bbox(path, tr):
+    out = np.zeros(4)
+    p, t = opath(path), _tr(tr)
+    rc = lib().orc_path_bbox(C.byref(p), _dp(t), _dp(out))
+    return None if rc else out
+
+
+def arc_to(src, radii, rot, large, sweep, dst):
+    out = np.zeros(32)
+    n = lib().orc_arc_to(_dp(np.array(src, float)), _dp(np.array(radii, float)), rot, int(large),
+                         int(sweep), _dp(np.array(dst, float)), _dp(out))
+    return n, out

@@ -1,0 +1,271 @@
+"""GPU parity of the mask pipeline (flatten -> bin -> fine) against the oracle.
+
+Tolerance for coverage (f32 output vs the reference's f64): max |d| <= 2e-3
+and mean |d| <= 1e-5 per pixel. Flattened lines, tile records, backdrops and
+events are compared bit for bit.
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _paths import OCTAGON_EVENODD, fill_rule_shapes, octagon, path_size, rect, squirrel
+import rasterize_amd as rz
+from rasterize_amd import (ActiveEdgeRasterizer, FillRule, PathBuilder, SignedDifferenceRasterizer,
+                           Transform, workload_cubics64, workload_random_walk)
+
+pytestmark = pytest.mark.gpu
+
+MAX_ABS = 2e-3
+MEAN_ABS = 1e-5
+I = Transform.identity()
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def check_close(gpu, ref, what):
+    d = np.abs(gpu.astype(np.float64) - ref)
+    mx, mean = float(d.max()) if d.size else 0.0, float(d.mean()) if d.size else 0.0
+    _say(f"{what}: max|d|={mx:.3e} mean|d|={mean:.3e}")
+    assert mx <= MAX_ABS, f"{what}: max |d| {mx}"
+    assert mean <= MEAN_ABS, f"{what}: mean |d| {mean}"
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return workload_cubics64(1, 512.0).paths[0]
+
+
+@pytest.fixture(scope="module")
+def c2():
+    ws = workload_random_walk(2, 200_000, 4096.0)
+    return ws.paths[0], ws.transform
+
+
+# ---- flatten: bit-exact line lists -------------------------------------------
+def test_flatten_bit_exact_c1(c1):
+    _say("flatten c1")
+    g = rz.flatten(c1, I, 0.05, True)
+    r = O.flatten(c1, I, 0.05, True)
+    assert g.shape == r.shape and np.array_equal(g, r)
+
+
+def test_flatten_bit_exact_squirrel_rotated():
+    p = squirrel()
+    tr = Transform.new_rotate(math.pi / 3.0).pre_translate(-10.0, -20.0)
+    for close in (True, False):
+        g = rz.flatten(p, tr, 0.05, close)
+        r = O.flatten(p, tr, 0.05, close)
+        assert np.array_equal(g, r)
+    # a much finer tolerance deepens every tree
+    g = rz.flatten(p, tr.pre_scale(40.0, 40.0), 0.001, True)
+    r = O.flatten(p, tr.pre_scale(40.0, 40.0), 0.001, True)
+    assert np.array_equal(g, r)
+
+
+def test_flatten_bit_exact_c2(c2):
+    path, tr = c2
+    _say("flatten c2", path.segments_count())
+    g = rz.flatten(path, tr, 0.05, True)
+    r = O.flatten(path, tr, 0.05, True)
+    _say("lines", len(g))
+    assert g.shape == r.shape and np.array_equal(g, r)
+
+
+def test_flatten_nan_is_error():
+    p = PathBuilder().move_to((0, 0)).cubic_to((1, 1), (float("nan"), 2), (3, 0)).close().build()
+    with pytest.raises(rz.RzError) as e:
+        rz.flatten(p, I)
+    assert e.value.status == rz._abi.RZ_E_NAN
+
+
+# ---- binning: bit-exact against the sequential replay -------------------------
+def _replay(path, tr, W, H, mode):
+    p, t = O.opath(path), O._tr(tr)
+    nr, ne = C.c_size_t(), C.c_size_t()
+    L = O.lib()
+    L.orc_bin_replay.restype = C.c_int
+    L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H, None, None, 0, None, None,
+                     0, C.byref(nr), C.byref(ne))
+    tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    counts = np.zeros(tiles, np.uint32)
+    recs = np.zeros((max(nr.value, 1), 4), np.float32)
+    bd = np.zeros(tiles, np.int32)
+    ev = np.zeros((max(ne.value, 1), 4), np.float32)
+    rc = L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H,
+                          counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                          recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
+                          bd.ctypes.data_as(C.POINTER(C.c_int32)),
+                          ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0], C.byref(nr),
+                          C.byref(ne))
+    assert rc == 0
+    return counts, recs[: nr.value], bd, ev[: ne.value]
+
+
+def _sort_tiles(counts, recs):
+    """Sort records within each tile by their bit patterns."""
+    bits = recs.view(np.uint32)
+    tile = np.repeat(np.arange(counts.size), counts)
+    order = np.lexsort((bits[:, 3], bits[:, 2], bits[:, 1], bits[:, 0], tile))
+    return bits[order]
+
+
+def _sort_events(ev):
+    b = ev.view(np.uint32)
+    return b[np.lexsort((b[:, 3], b[:, 2], b[:, 1], b[:, 0]))]
+
+
+@pytest.mark.parametrize("case", ["c1", "squirrel", "edges", "c2"])
+@pytest.mark.parametrize("mode", [rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_ITER])
+def test_bin_bit_exact(case, mode, c1, c2):
+    if case == "c1":
+        path, tr, W, H = c1, I, 512, 512
+    elif case == "squirrel":
+        path, tr, W, H = squirrel(), Transform.new_scale(20.0, 20.0), 330, 310
+    elif case == "edges":
+        # crosses all four image edges, W-1 a multiple of 16 (1-column last tile)
+        path, tr, W, H = workload_cubics64(5, 120.0).paths[0], Transform.new_translate(-20, -20), 81, 70
+    else:
+        path, tr = c2
+        W = H = 4096
+    _say(f"bin {case} mode={mode}")
+    gc, gr, gb, ge = rz.bin_stage(path, tr, W, H, mode)
+    rc, rr, rb, re = _replay(path, tr, W, H, mode)
+    assert np.array_equal(gc, rc), "per-tile record counts differ"
+    assert np.array_equal(gb, rb), "band backdrops differ"
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr)), "records differ"
+    assert np.array_equal(_sort_events(ge), _sort_events(re)), "events differ"
+
+
+# ---- masks: reference known answers on the GPU --------------------------------
+@pytest.mark.parametrize("R", [SignedDifferenceRasterizer, ActiveEdgeRasterizer])
+def test_rasterizer_octagon_evenodd(R):
+    img = np.zeros((4, 9))
+    R().mask(octagon(), I, img, FillRule.EvenOdd)
+    assert np.max(np.abs(img - OCTAGON_EVENODD)) < 1e-6
+
+
+@pytest.mark.parametrize("R", [SignedDifferenceRasterizer, ActiveEdgeRasterizer])
+def test_fill_rule_areas(R):
+    path = fill_rule_shapes()
+    w, h, tr = path_size(path, I, O.path_bbox(path, I))
+    img = np.zeros((h, w))
+    R().mask(path, tr, img, FillRule.EvenOdd)
+    y, x0, x1, x2 = 50, 50, 150, 250
+    assert abs(img[y, x0]) < 1e-6 and abs(img[y, x1]) < 1e-6 and abs(img[y, x2]) < 1e-6
+    assert abs(img.sum() - 13130.0) < 1.0
+    img = np.zeros((h, w))
+    R().mask(path, tr, img, FillRule.NonZero)
+    assert abs(img[y, x0] - 1.0) < 1e-6 and abs(img[y, x1] - 1.0) < 1e-6 and abs(img[y, x2]) < 1e-6
+    assert abs(img.sum() - 16492.5) < 1.0
+
+
+# ---- masks: parity with the oracle --------------------------------------------
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_mask_c1(rule, c1):
+    g = SignedDifferenceRasterizer().mask_f32(c1, I, 512, 512, rule)
+    r = O.sd_mask(c1, I, 512, 512, rule)
+    check_close(g, r, f"c1 rule={rule}")
+
+
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_mask_c2(rule, c2):
+    path, tr = c2
+    g = SignedDifferenceRasterizer().mask_f32(path, tr, 4096, 4096, rule)
+    r = O.sd_mask(path, tr, 4096, 4096, rule)
+    check_close(g, r, f"c2 rule={rule}")
+
+
+@pytest.mark.parametrize("W,H", [(81, 70), (17, 33), (1, 5), (16, 16), (15, 47), (100, 1)])
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_mask_edges(W, H, rule):
+    """Paths crossing every image edge; image widths with W-1 = 16k (one-column
+    last tile), W < 16, single rows/columns."""
+    path = workload_cubics64(7, 1.6 * max(W, H)).paths[0]
+    tr = Transform.new_translate(-0.3 * W, -0.3 * H)
+    g = SignedDifferenceRasterizer().mask_f32(path, tr, W, H, rule)
+    r = O.sd_mask(path, tr, W, H, rule)
+    check_close(g, r, f"edges {W}x{H} rule={rule}")
+
+
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_mask_iter_matches_oracle(rule):
+    path = workload_cubics64(8, 200.0).paths[0]
+    tr = Transform.new_translate(-20.0, 10.0)
+    W, H = 160, 150
+    g = SignedDifferenceRasterizer().mask_f32(path, tr, W, H, rule, rz._abi.RZ_EDGE_ITER)
+    r = O.sd_mask_iter(path, tr, W, H, rule)
+    g = np.where(np.abs(g) >= 1e-6, g, 0.0)
+    check_close(g, r, f"mask_iter rule={rule}")
+
+
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_active_edge_matches_oracle(rule):
+    # crosses the right and bottom edges but not x = 0: the reference's
+    # ActiveEdgeIter shifts column-0 coverage into column 1 when a left-edge
+    # split point rounds to a tiny negative x (src/rasterize.rs:857-871),
+    # which the GPU does not reproduce
+    path = workload_cubics64(9, 300.0).paths[0]
+    tr = Transform.new_translate(2.0, -10.0)
+    W, H = 200, 230
+    g = ActiveEdgeRasterizer().mask_f32(path, tr, W, H, rule)
+    r = O.ae_mask(path, tr, W, H, rule)
+    check_close(g, r, f"active-edge rule={rule}")
+
+
+def test_strided_view_output():
+    """mask into a strided f64 view (ImageMut view_mut, src/image.rs:300-311)."""
+    path = workload_cubics64(10, 60.0).paths[0]
+    big = np.full((100, 120), -7.0)
+    view = big[10:80, 20:90]
+    SignedDifferenceRasterizer().mask(path, I, view, FillRule.NonZero)
+    r = O.sd_mask(path, I, 70, 70, 0)
+    check_close(view, r, "strided view")
+    assert np.all(big[:10] == -7.0) and np.all(big[:, :20] == -7.0) and np.all(big[80:] == -7.0)
+
+
+def test_empty_and_zero_size():
+    empty = PathBuilder().build()
+    img = np.zeros((8, 8))
+    SignedDifferenceRasterizer().mask(empty, I, img, FillRule.NonZero)
+    assert np.all(img == 0.0)
+    p = rect(1, 1, 5, 5)
+    z = np.zeros((0, 0))
+    SignedDifferenceRasterizer().mask(p, I, np.zeros((0, 4)), FillRule.NonZero)
+    assert SignedDifferenceRasterizer().mask_iter(p, I, (0, 0), FillRule.NonZero) == []
+    del z
+
+
+def test_mask_nan_is_error():
+    p = PathBuilder().move_to((0, 0)).line_to((float("nan"), 1)).line_to((2, 2)).close().build()
+    with pytest.raises(rz.RzError) as e:
+        SignedDifferenceRasterizer().mask(p, I, np.zeros((4, 4)), FillRule.NonZero)
+    assert e.value.status == rz._abi.RZ_E_NAN
+
+
+def test_deterministic(c1):
+    a = SignedDifferenceRasterizer().mask_f32(c1, I, 512, 512, FillRule.EvenOdd)
+    b = SignedDifferenceRasterizer().mask_f32(c1, I, 512, 512, FillRule.EvenOdd)
+    assert np.array_equal(a, b)
+
+
+def test_error_vs_exact_area(c1):
+    """Error of the GPU mask against the FP64 exact coverage of the flattened
+    path (reported, and bounded: box-filtering the winding before the fill
+    rule is the reference's method, so the reference has the same error)."""
+    path = workload_cubics64(12, 100.0).paths[0]
+    tr = Transform.new_translate(6.0, 6.0)
+    W = H = 112
+    for rule in (0, 1):
+        g = SignedDifferenceRasterizer().mask_f32(path, tr, W, H, rule).astype(np.float64)
+        r = O.sd_mask(path, tr, W, H, rule)
+        ex = O.exact_fill_coverage(path, tr, W, H, rule)
+        dg, dr = np.abs(g - ex), np.abs(r - ex)
+        _say(f"rule={rule}: gpu vs exact max {dg.max():.3e} mean {dg.mean():.3e}; "
+             f"reference vs exact max {dr.max():.3e} mean {dr.mean():.3e}")
+        assert abs(dg.mean() - dr.mean()) < 1e-5
