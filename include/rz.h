@@ -170,6 +170,19 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, do
                          int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
                          size_t row_stride);
 
+/* ---- kernel timing -------------------------------------------------------
+ * No reference counterpart (the reference only has `tracing` spans in its
+ * examples): with profiling on, every kernel the context launches is
+ * bracketed by HIP events on the context's stream; rz_profile_stats waits for
+ * the stream and returns per-kernel launch counts and total device time. */
+typedef struct rz_kernel_stat {
+  char name[40];
+  uint64_t calls;
+  double total_ms;
+} rz_kernel_stat;
+rz_status rz_profile(rz_ctx* ctx, int enable); /* also clears the statistics */
+rz_status rz_profile_stats(rz_ctx* ctx, rz_kernel_stat* out, size_t cap, size_t* n);
+
 #ifdef __cplusplus
 }
 #endif

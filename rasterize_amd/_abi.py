@@ -67,6 +67,10 @@ class rz_paint(C.Structure):
     ]
 
 
+class rz_kernel_stat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("calls", C.c_uint64), ("total_ms", C.c_double)]
+
+
 P = C.POINTER
 c_ctx = C.c_void_p
 c_dpath = C.c_void_p
@@ -93,6 +97,8 @@ PROTOTYPES = [
     ("rz_path_free", C.c_int, [c_ctx, c_dpath]),
     ("rz_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
                                  C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    ("rz_profile", C.c_int, [c_ctx, C.c_int]),
+    ("rz_profile_stats", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
     # workloads
     ("rzw_cubics64", c_set, [C.c_uint32, C.c_double]),
     ("rzw_random_walk", c_set, [C.c_uint32, C.c_uint32, C.c_double]),

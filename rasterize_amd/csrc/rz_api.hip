@@ -38,6 +38,15 @@ struct rz_ctx {
   DBuf counts;    // per-slot counts for rz_flatten
   uint32_t rec_cap = 0, ev_cap = 0;
   rzk::Status* host_status = nullptr;  // pinned
+  // kernel timing (rz_profile)
+  bool prof = false;
+  struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> pending;
+  std::vector<hipEvent_t> pool;
+  std::vector<rz_kernel_stat> stats;
 };
 
 struct rz_dpath {
@@ -86,6 +95,26 @@ void free_buf(DBuf& b) {
   b.cap = 0;
 }
 
+hipEvent_t take_event(rz_ctx* ctx);
+void push_prof(rz_ctx* ctx, const char* name, hipEvent_t a, hipEvent_t b);
+
+// Launch on the context's stream; with profiling on, bracket the kernel with
+// events so rz_profile_stats can report its device time.
+#define RZ_LAUNCH(ctx, name, kern, grid, block, ...)                              \
+  do {                                                                           \
+    hipEvent_t a_ = nullptr;                                                     \
+    if ((ctx)->prof) {                                                           \
+      a_ = take_event(ctx);                                                      \
+      (void)hipEventRecord(a_, (ctx)->stream);                                   \
+    }                                                                            \
+    hipLaunchKernelGGL(kern, grid, block, 0, (ctx)->stream, __VA_ARGS__);        \
+    if ((ctx)->prof) {                                                           \
+      hipEvent_t b_ = take_event(ctx);                                           \
+      (void)hipEventRecord(b_, (ctx)->stream);                                   \
+      push_prof(ctx, name, a_, b_);                                              \
+    }                                                                            \
+  } while (0)
+
 uint32_t blocks_for(size_t n) { return (uint32_t)((n + rzk::kBlock - 1) / rzk::kBlock); }
 
 rz_status check_launch(rz_ctx* ctx, const char* what) {
@@ -107,11 +136,11 @@ rz_status scan_u32(rz_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uin
   size_t nb = std::max<size_t>(1, (n + rzk::kScanTile - 1) / rzk::kScanTile);
   RZ_TRY(ensure(ctx, ctx->sums, nb * sizeof(uint32_t)));
   uint32_t* sums = (uint32_t*)ctx->sums.p;
-  hipLaunchKernelGGL(rzk::k_scan_tiles, dim3((uint32_t)nb), dim3(rzk::kBlock), 0, ctx->stream, in,
+  RZ_LAUNCH(ctx, "k_scan_tiles", rzk::k_scan_tiles, dim3((uint32_t)nb), dim3(rzk::kBlock), in,
                      out, n, sums);
-  hipLaunchKernelGGL(rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), 0, ctx->stream, sums, nb, out, n,
+  RZ_LAUNCH(ctx, "k_scan_sums", rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), sums, nb, out, n,
                      total);
-  hipLaunchKernelGGL(rzk::k_scan_add, dim3((uint32_t)nb), dim3(rzk::kBlock), 0, ctx->stream, out, n,
+  RZ_LAUNCH(ctx, "k_scan_add", rzk::k_scan_add, dim3((uint32_t)nb), dim3(rzk::kBlock), out, n,
                      (const uint32_t*)sums, cursor);
   return check_launch(ctx, "scan");
 }
@@ -324,8 +353,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.tile_off = (uint32_t*)ctx->tile_off.p;
   P.ev_off = (uint32_t*)ctx->ev_off.p;
 
-  hipLaunchKernelGGL(rzk::k_fill_u32, dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)),
-                     dim3(rzk::kBlock), 0, ctx->stream, meta, meta_u32, 0u);
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)), dim3(rzk::kBlock), meta, meta_u32, 0u);
   rzk::Xform x;
   for (int i = 0; i < 6; ++i) x.m[i] = tr.m[i];
   const double fl = 16.0 * flatness * flatness;
@@ -340,15 +368,15 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.st = P.st;
   const uint32_t nblk = d->n_order / rzk::kBlock;
   if (nblk)
-    hipLaunchKernelGGL(rzk::k_bin<0>, dim3(nblk), dim3(rzk::kBlock), 0, ctx->stream, d->view, x, fl,
+    RZ_LAUNCH(ctx, "k_bin<0>", rzk::k_bin<0>, dim3(nblk), dim3(rzk::kBlock), d->view, x, fl,
                        P.grid.g, b);
   RZ_TRY(check_launch(ctx, "k_bin<0>"));
   RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, P.tile_count, &P.st->n_records));
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
-  hipLaunchKernelGGL(rzk::k_bd_prefix, dim3(blocks_for(ty)), dim3(rzk::kBlock), 0, ctx->stream, P.bd,
+  RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3(blocks_for(ty)), dim3(rzk::kBlock), P.bd,
                      P.grid.g.tiles_x, P.grid.g.tiles_y);
   if (nblk)
-    hipLaunchKernelGGL(rzk::k_bin<1>, dim3(nblk), dim3(rzk::kBlock), 0, ctx->stream, d->view, x, fl,
+    RZ_LAUNCH(ctx, "k_bin<1>", rzk::k_bin<1>, dim3(nblk), dim3(rzk::kBlock), d->view, x, fl,
                        P.grid.g, b);
   return check_launch(ctx, "k_bin<1>");
 }
@@ -372,11 +400,9 @@ rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   a.tiles_x = P.grid.g.tiles_x;
   a.tiles_y = P.grid.g.tiles_y;
   if (rule == RZ_NONZERO)
-    hipLaunchKernelGGL(rzk::k_fine_mask<0>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), 0,
-                       ctx->stream, a);
+    RZ_LAUNCH(ctx, "k_fine_mask<0>", rzk::k_fine_mask<0>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), a);
   else
-    hipLaunchKernelGGL(rzk::k_fine_mask<1>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), 0,
-                       ctx->stream, a);
+    RZ_LAUNCH(ctx, "k_fine_mask<1>", rzk::k_fine_mask<1>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), a);
   RZ_TRY(check_launch(ctx, "k_fine_mask"));
   // the status block is read back at rz_sync
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, P.st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
@@ -418,10 +444,65 @@ rz_status mask_blocking(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, 
 
 }  // namespace
 
+namespace {
+hipEvent_t take_event(rz_ctx* ctx) {
+  if (!ctx->pool.empty()) {
+    hipEvent_t e = ctx->pool.back();
+    ctx->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+void push_prof(rz_ctx* ctx, const char* name, hipEvent_t a, hipEvent_t b) {
+  ctx->pending.push_back({name, a, b});
+}
+// fold finished event pairs into the per-kernel statistics
+rz_status collect_prof(rz_ctx* ctx) {
+  RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->pending) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, r.a, r.b);
+    rz_kernel_stat* st = nullptr;
+    for (auto& k : ctx->stats)
+      if (std::strncmp(k.name, r.name, sizeof(k.name)) == 0) st = &k;
+    if (!st) {
+      rz_kernel_stat k{};
+      std::strncpy(k.name, r.name, sizeof(k.name) - 1);
+      ctx->stats.push_back(k);
+      st = &ctx->stats.back();
+    }
+    st->calls += 1;
+    st->total_ms += ms;
+    ctx->pool.push_back(r.a);
+    ctx->pool.push_back(r.b);
+  }
+  ctx->pending.clear();
+  return RZ_OK;
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+rz_status rz_profile(rz_ctx* ctx, int enable) {
+  if (!ctx) return RZ_E_ARG;
+  RZ_TRY(collect_prof(ctx));
+  ctx->stats.clear();
+  ctx->prof = enable != 0;
+  return RZ_OK;
+}
+
+rz_status rz_profile_stats(rz_ctx* ctx, rz_kernel_stat* out, size_t cap, size_t* n) {
+  if (!ctx || !n) return RZ_E_ARG;
+  RZ_TRY(collect_prof(ctx));
+  *n = ctx->stats.size();
+  for (size_t i = 0; i < ctx->stats.size() && i < cap; ++i) out[i] = ctx->stats[i];
+  return RZ_OK;
+}
 
 rz_status rz_ctx_create(rz_ctx** out, int device) {
   if (!out) return RZ_E_ARG;
@@ -447,6 +528,11 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   DBuf* bufs[] = {&ctx->meta, &ctx->tile_off, &ctx->ev_off, &ctx->sums, &ctx->recs,
                   &ctx->events, &ctx->out, &ctx->lines, &ctx->counts};
   for (DBuf* b : bufs) free_buf(*b);
+  for (auto& r : ctx->pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->host_status) (void)hipHostFree(ctx->host_status);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -500,7 +586,7 @@ rz_status rz_flatten(rz_ctx* ctx, const rz_path* path, rz_transform tr, double f
   const double fl = 16.0 * flatness * flatness;
   const uint32_t nblk = d->n_order / rzk::kBlock;
   if (nblk)
-    hipLaunchKernelGGL(rzk::k_flatten_count, dim3(nblk), dim3(rzk::kBlock), 0, ctx->stream, d->view,
+    RZ_LAUNCH(ctx, "k_flatten_count", rzk::k_flatten_count, dim3(nblk), dim3(rzk::kBlock), d->view,
                        x, fl, counts, dst);
   RZ_TRY(check_launch(ctx, "k_flatten_count"));
   if (!close) {
@@ -523,10 +609,11 @@ rz_status rz_flatten(rz_ctx* ctx, const rz_path* path, rz_transform tr, double f
   if (hs.err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
   if (hs.err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
   *n_lines = total;
-  if (!lines || total > cap) return total > cap ? fail(ctx, RZ_E_CAPACITY, "line buffer too small") : RZ_OK;
+  if (!lines) return RZ_OK;
+  if (total > cap) return fail(ctx, RZ_E_CAPACITY, "line buffer too small");
   RZ_TRY(ensure(ctx, ctx->lines, (size_t)total * 32 + 32));
   if (nblk)
-    hipLaunchKernelGGL(rzk::k_flatten_emit, dim3(nblk), dim3(rzk::kBlock), 0, ctx->stream, d->view, x,
+    RZ_LAUNCH(ctx, "k_flatten_emit", rzk::k_flatten_emit, dim3(nblk), dim3(rzk::kBlock), d->view, x,
                        fl, (const uint32_t*)counts, (const uint32_t*)offs, (double*)ctx->lines.p);
   RZ_TRY(check_launch(ctx, "k_flatten_emit"));
   RZ_HIP(ctx, hipMemcpyAsync(lines, ctx->lines.p, (size_t)total * 32, hipMemcpyDeviceToHost, ctx->stream));

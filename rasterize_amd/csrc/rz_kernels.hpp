@@ -174,6 +174,12 @@ struct BinSink {
     }
   }
   __device__ void line(Pt p0, Pt p1) {
+    // infinite coordinates (the NaN case is caught by the traversal) would
+    // produce NaN cell coordinates; report them like NaN
+    if (!(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
+      set_err(b.st, rzg::ST_NAN);
+      return;
+    }
     rzg::EffLine e = rzg::effective_line(p0, p1, g);
     for (int k = 0; k < e.n; ++k)
       rzg::bin_piece(e.p[k], e.p[k + 1], g, [&](int t, const rzg::Rec& r) { rec(t, r); },
@@ -356,7 +362,7 @@ __device__ inline void add_v(int* rowd, float yc, float sign) {
 }
 
 __device__ inline void dep(int* acc, int row, int col, float v) {
-  if (col < 16) atomicAdd(&acc[row * 16 + col], fix(v));
+  if ((unsigned)col < 16u) atomicAdd(&acc[row * 16 + col], fix(v));
 }
 
 // signed_difference_line (src/rasterize.rs:395-469) for one cell record, in
@@ -364,6 +370,11 @@ __device__ inline void dep(int* acc, int row, int col, float v) {
 // and are accounted for by their backdrops.
 __device__ inline void fine_record(float4 r, int* acc, int* rowd) {
   float x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
+  // records are clipped to the cell; anything else is skipped rather than
+  // allowed to index outside the tile
+  if (!(fabsf(x0) <= 16.0f && fabsf(x1) <= 16.0f && y0 >= 0.0f && y0 <= 16.0f && y1 >= 0.0f &&
+        y1 <= 16.0f))
+    return;
   if (signbit(x0)) add_v(rowd, y0, 1.0f);
   if (signbit(x1)) add_v(rowd, y1, -1.0f);
   x0 = fabsf(x0);
