@@ -80,11 +80,11 @@ def main():
 
     # warmup (sizes the context's buffers; a capacity miss is re-run)
     for i in range(max(1, args.warmup)):
-        step()
-        s = lib.rz_sync(ctx.h)
-        if s == _abi.RZ_E_CAPACITY:
+        for attempt in range(4):
             step()
             s = lib.rz_sync(ctx.h)
+            if s != _abi.RZ_E_CAPACITY:
+                break
         ctx.check(s)
 
     stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))

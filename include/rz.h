@@ -133,15 +133,19 @@ rz_status rz_flatten(rz_ctx* ctx, const rz_path* path, rz_transform tr, double f
 /* Binning stage of the mask pipeline for a width x height image: per 16x16
  * tile its record count (tile_counts[tiles]), the records themselves (4 f32
  * each, tile-major, records of one tile in unspecified order), the band
- * backdrop winding per tile (backdrop[tiles], exclusive prefix along each
- * tile row) and the free-endpoint events (4 x int32/f32 each). Sizes are
- * returned through n_records / n_events; pass NULL buffers to query them.
+ * backdrop winding per tile (backdrop[tiles], the integer winding that
+ * crossings of the band's bottom edge left of the tile contribute), the
+ * free-endpoint events (4 x int32/f32 each) and the right-edge column
+ * (right_edge[height], per pixel row the 2^-20 fixed-point sum of the row
+ * overlaps of the lines that lie beyond the right clip). Sizes are returned
+ * through n_records / n_events; pass NULL buffers to query them.
  * No reference counterpart: this is the GPU decomposition of
  * signed_difference_line (src/rasterize.rs:365-470) exposed for checking. */
 rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
                  int edge_mode, size_t width, size_t height, uint32_t* tile_counts,
                  float* records, size_t records_cap, int32_t* backdrop, float* events,
-                 size_t events_cap, size_t* n_records, size_t* n_events);
+                 size_t events_cap, int32_t* right_edge, size_t* n_records,
+                 size_t* n_events);
 
 /* ---- Rasterizer::mask / mask_iter (drop-in, blocking) --------------------- */
 
@@ -169,6 +173,14 @@ rz_status rz_path_free(rz_ctx* ctx, rz_dpath* dpath);
 rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, double flatness,
                          int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
                          size_t row_stride);
+
+/* ---- launch parameters ----------------------------------------------------
+ * No reference counterpart. RZ_PARAM_SPLIT_QUAD / RZ_PARAM_SPLIT_CUBIC: each
+ * quad / cubic segment's subdivision tree is processed by 2^value threads
+ * (one per subtree at that depth), 0..8. Results do not depend on them. */
+enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1 };
+rz_status rz_set_param(rz_ctx* ctx, int param, int value);
+rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 
 /* ---- kernel timing -------------------------------------------------------
  * No reference counterpart (the reference only has `tracing` spans in its

@@ -24,9 +24,11 @@ struct BinReplay {
   std::vector<std::vector<rzg::Rec>> recs;  // per tile
   std::vector<int32_t> bd;                  // per tile, prefix along each band
   std::vector<rzg::Event> events;
+  std::vector<int32_t> redge;               // [H]: right-edge column per row
 };
 
 inline rzg::Pt rp(Point p) { return rzg::mkpt(p.x, p.y); }
+inline rzg::Pt tr_pt(const Transform& tr, Point p) { return rp(tr.apply(p)); }
 
 inline void bin_replay(const Path& path, const Transform& tr, double flatness, int mode, int W,
                        int H, BinReplay* out) {
@@ -43,6 +45,7 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
   out->recs.assign(tiles, {});
   out->bd.assign(tiles, 0);
   out->events.clear();
+  std::vector<int32_t> ed((size_t)H + 2, 0);
   const double fl = 16.0 * flatness * flatness;
   auto event = [&](rzg::Pt p, float sign) {
     rzg::Event ev;
@@ -50,6 +53,12 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
   };
   auto line = [&](Point a, Point b) {
     rzg::EffLine e = rzg::effective_line(rp(a), rp(b), g);
+    if (e.proj) {
+      auto add = [&](int k, int v) { ed[(size_t)k] += v; };
+      rzg::redge_endpoint(b.y, 1, g, add);
+      rzg::redge_endpoint(a.y, -1, g, add);
+      return;
+    }
     for (int k = 0; k < e.n; ++k)
       rzg::bin_piece(
           e.p[k], e.p[k + 1], g, [&](int t, const rzg::Rec& r) { out->recs[t].push_back(r); },
@@ -57,8 +66,21 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
     if (e.clip0) event(e.p[0], -1.0f);
     if (e.clip1) event(e.p[e.n], 1.0f);
   };
+  // gap between a (previous segment's end) and b (this start): +V(a), -V(b),
+  // dropped together when they land on the same band, column and f32 offset;
+  // ends beyond the right clip are reported by the clip
+  auto gap = [&](Point a0, Point b0) {
+    const bool clip = mode != rzg::EDGE_NONE;
+    rzg::Pt a = tr_pt(tr, a0), b = tr_pt(tr, b0);
+    rzg::Event va, vb;
+    bool ha = !(clip && a.x > g.wc) && rzg::make_event(rzg::mkpt(std::fmax(a.x, 0.0), a.y), 1.0f, g, &va);
+    bool hb = !(clip && b.x > g.wc) && rzg::make_event(rzg::mkpt(std::fmax(b.x, 0.0), b.y), -1.0f, g, &vb);
+    if (ha && hb && va.band == vb.band && va.col == vb.col && va.yrel == vb.yrel) return;
+    if (ha) out->events.push_back(va);
+    if (hb) out->events.push_back(vb);
+  };
   // one slot: flatten a transformed segment like PathFlattenIter does
-  auto slot = [&](const Segment& seg, bool gap_start, bool gap_end) {
+  auto slot = [&](const Segment& seg) {
     Segment root = seg.transform(tr);
     std::vector<Segment> stack{root};
     while (!stack.empty()) {
@@ -74,21 +96,19 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
       stack.push_back(s1);
       stack.push_back(s0);
     }
-    const bool clip = mode != rzg::EDGE_NONE;
-    Point st = root.start(), en = root.end();
-    if (gap_start && !(clip && st.x > g.wc)) event(rzg::mkpt(std::fmax(st.x, 0.0), st.y), -1.0f);
-    if (gap_end && !(clip && en.x > g.wc)) event(rzg::mkpt(std::fmax(en.x, 0.0), en.y), 1.0f);
   };
   for (size_t i = 0; i < path.nsubpaths(); ++i) {
     size_t b = path.subpaths[i], e = path.subpaths[i + 1];
     for (size_t k = b; k < e; ++k) {
-      bool gs = k > b && !(path.segments[k - 1].end() == path.segments[k].start());
-      bool ge = k + 1 < e && !(path.segments[k].end() == path.segments[k + 1].start());
-      slot(path.segments[k], gs, ge);
+      if (k > b && !(path.segments[k - 1].end() == path.segments[k].start()))
+        gap(path.segments[k - 1].end(), path.segments[k].start());
+      slot(path.segments[k]);
     }
-    Segment closing = Segment::line(path.segments[e - 1].end(), path.segments[b].start());
-    slot(closing, false, false);
+    slot(Segment::line(path.segments[e - 1].end(), path.segments[b].start()));
   }
+  out->redge.assign((size_t)H, 0);
+  int32_t run = 0;
+  for (int y = 0; y < H; ++y) out->redge[(size_t)y] = run += ed[(size_t)y];
   for (int band = 0; band < g.tiles_y; ++band) {
     int32_t acc = 0;
     for (int c = 0; c < g.tiles_x; ++c) {

@@ -410,7 +410,8 @@ extern "C" {
 // tile-major; within a tile, and for events, the order is the replay's own.
 int orc_bin_replay(const orc_path* p, const double* tr, double flatness, int mode, size_t W,
                    size_t H, uint32_t* counts, float* recs, size_t rec_cap, int32_t* bd,
-                   float* events, size_t ev_cap, size_t* n_recs, size_t* n_events) {
+                   float* events, size_t ev_cap, int32_t* redge, size_t* n_recs,
+                   size_t* n_events) {
   Path path = to_path(p);
   BinReplay r;
   try {
@@ -433,6 +434,8 @@ int orc_bin_replay(const orc_path* p, const double* tr, double flatness, int mod
     }
     bd[t] = r.bd[t];
   }
+  if (redge)
+    for (size_t y = 0; y < H; ++y) redge[y] = r.redge[y];
   for (size_t i = 0; i < r.events.size(); ++i) {
     const auto& e = r.events[i];
     std::memcpy(&events[4 * i], &e.band, 4);

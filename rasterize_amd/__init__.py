@@ -367,23 +367,26 @@ def flatten(path, tr, flatness=DEFAULT_FLATNESS, close=True, ctx=None):
 
 def bin_stage(path, tr, width, height, edge_mode=_abi.RZ_EDGE_MASK, flatness=DEFAULT_FLATNESS,
               ctx=None):
-    """Binning stage hook: (tile_counts, records[n,4] f32, backdrop, events[n,4])."""
+    """Binning stage hook: (tile_counts, records[n,4] f32, backdrop, events[n,4],
+    right_edge[height] int32)."""
     ctx = ctx or default_context()
     nr, ne = C.c_size_t(), C.c_size_t()
     ctx.check(lib().rz_bin(ctx.h, C.byref(path.abi()), tr.abi(), flatness, edge_mode, width, height,
-                           None, None, 0, None, None, 0, C.byref(nr), C.byref(ne)))
+                           None, None, 0, None, None, 0, None, C.byref(nr), C.byref(ne)))
     tx, ty = (width + 15) // 16, (height + 15) // 16
     counts = np.zeros(tx * ty, np.uint32)
     recs = np.zeros((max(nr.value, 1), 4), np.float32)
     bd = np.zeros(tx * ty, np.int32)
     ev = np.zeros((max(ne.value, 1), 4), np.float32)
+    redge = np.zeros(max(height, 1), np.int32)
     ctx.check(lib().rz_bin(ctx.h, C.byref(path.abi()), tr.abi(), flatness, edge_mode, width, height,
                            counts.ctypes.data_as(C.POINTER(C.c_uint32)),
                            recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
                            bd.ctypes.data_as(C.POINTER(C.c_int32)),
                            ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0],
+                           redge.ctypes.data_as(C.POINTER(C.c_int32)),
                            C.byref(nr), C.byref(ne)))
-    return counts, recs[: nr.value], bd, ev[: ne.value]
+    return counts, recs[: nr.value], bd, ev[: ne.value], redge[:height]
 
 
 # ---- workloads ------------------------------------------------------------

@@ -88,7 +88,7 @@ PROTOTYPES = [
                              P(C.c_double), C.c_size_t, P(C.c_size_t)]),
     ("rz_bin", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_size_t,
                          C.c_size_t, P(C.c_uint32), P(C.c_float), C.c_size_t, P(C.c_int32),
-                         P(C.c_float), C.c_size_t, P(C.c_size_t), P(C.c_size_t)]),
+                         P(C.c_float), C.c_size_t, P(C.c_int32), P(C.c_size_t), P(C.c_size_t)]),
     ("rz_mask", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
                           P(C.c_double), C.c_size_t, rz_shape]),
     ("rz_mask_f32", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
@@ -97,6 +97,8 @@ PROTOTYPES = [
     ("rz_path_free", C.c_int, [c_ctx, c_dpath]),
     ("rz_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
                                  C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    ("rz_set_param", C.c_int, [c_ctx, C.c_int, C.c_int]),
+    ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
     ("rz_profile", C.c_int, [c_ctx, C.c_int]),
     ("rz_profile_stats", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
     # workloads

@@ -38,6 +38,10 @@ struct rz_ctx {
   DBuf counts;    // per-slot counts for rz_flatten
   uint32_t rec_cap = 0, ev_cap = 0;
   rzk::Status* host_status = nullptr;  // pinned
+  // launch parameters (rz_set_param)
+  int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
+  uint32_t bin_blocks = 2048;          // grid of the per-line binning kernels
+  uint32_t line_cap = 0;
   // kernel timing (rz_profile)
   bool prof = false;
   struct ProfRec {
@@ -324,21 +328,28 @@ struct Pipe {
   uint32_t* band_ev;
   uint32_t* tile_off;
   uint32_t* ev_off;
+  int32_t* redge;  // [H + 2] right-edge row differences, then (scanned) per-row sums
 };
 
-// Enqueue count pass, scans and scatter pass. Records/events land in
-// ctx->recs / ctx->events (capacity-checked on the device).
+// Enqueue flatten, count pass, scans and scatter pass. Lines, records and
+// events land in ctx->lines / ctx->recs / ctx->events (capacity-checked on the
+// device; finish() grows them and asks for a re-run).
 rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
                       int mode, size_t W, size_t H, Pipe* pp) {
   Pipe& P = *pp;
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
-  const size_t meta_u32 = 4 + tiles + tiles + ty;
+  const size_t sw = rzk::kStatusWords;
+  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2);
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, (tiles + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, (ty + 1) * 4));
+  if (ctx->line_cap == 0) {
+    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
+  }
   if (ctx->rec_cap == 0) {
-    ctx->rec_cap = (uint32_t)std::min<size_t>(std::max<size_t>(d->n_slots * 8 + tiles, 1 << 16), 1u << 30);
+    ctx->rec_cap = (uint32_t)std::min<size_t>((size_t)ctx->line_cap + tiles, 1u << 30);
     RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
   }
   if (ctx->ev_cap == 0) {
@@ -347,16 +358,32 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   }
   uint32_t* meta = (uint32_t*)ctx->meta.p;
   P.st = (rzk::Status*)meta;
-  P.tile_count = meta + 4;
-  P.bd = (int32_t*)(meta + 4 + tiles);
-  P.band_ev = meta + 4 + 2 * tiles;
+  P.tile_count = meta + sw;
+  P.bd = (int32_t*)(meta + sw + tiles);
+  P.band_ev = meta + sw + 2 * tiles;
+  P.redge = (int32_t*)(meta + sw + 2 * tiles + ty);
   P.tile_off = (uint32_t*)ctx->tile_off.p;
   P.ev_off = (uint32_t*)ctx->ev_off.p;
 
-  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)), dim3(rzk::kBlock), meta, meta_u32, 0u);
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32,
+            dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)), dim3(rzk::kBlock), meta,
+            meta_u32, 0u);
   rzk::Xform x;
   for (int i = 0; i < 6; ++i) x.m[i] = tr.m[i];
   const double fl = 16.0 * flatness * flatness;
+  // flatten: quad and cubic slots are split into 2^dq / 2^dc virtual subtrees
+  const int dq = ctx->split_quad, dc = ctx->split_cubic;
+  const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
+                 bc = d->view.blocks_cubic << dc;
+  const uint32_t nblk = bl + bq + bc;
+  rzk::LineBufs lb;
+  lb.lines = (double4*)ctx->lines.p;
+  lb.cap = ctx->line_cap;
+  lb.count = &P.st->n_lines;
+  lb.st = P.st;
+  if (nblk)
+    RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines, dim3(nblk), dim3(rzk::kBlock), d->view, x,
+              fl, lb, bl, bq, dq, dc);
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
   b.bd_diff = P.bd;
@@ -365,20 +392,24 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.rec_cap = ctx->rec_cap;
   b.events = (int4*)ctx->events.p;
   b.ev_cap = ctx->ev_cap;
+  b.redge = P.redge;
   b.st = P.st;
-  const uint32_t nblk = d->n_order / rzk::kBlock;
-  if (nblk)
-    RZ_LAUNCH(ctx, "k_bin<0>", rzk::k_bin<0>, dim3(nblk), dim3(rzk::kBlock), d->view, x, fl,
-                       P.grid.g, b);
-  RZ_TRY(check_launch(ctx, "k_bin<0>"));
+  const uint32_t gb = ctx->bin_blocks;
+  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
+  RZ_LAUNCH(ctx, "k_bin_lines<0>", rzk::k_bin_lines<0>, dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.count, P.grid.g, b);
+  RZ_LAUNCH(ctx, "k_gap_events<0>", rzk::k_gap_events<0>, dim3(gs), dim3(rzk::kBlock), d->view, x,
+            P.grid.g, b);
+  RZ_TRY(check_launch(ctx, "bin pass 0"));
   RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, P.tile_count, &P.st->n_records));
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
-  RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3(blocks_for(ty)), dim3(rzk::kBlock), P.bd,
-                     P.grid.g.tiles_x, P.grid.g.tiles_y);
-  if (nblk)
-    RZ_LAUNCH(ctx, "k_bin<1>", rzk::k_bin<1>, dim3(nblk), dim3(rzk::kBlock), d->view, x, fl,
-                       P.grid.g, b);
-  return check_launch(ctx, "k_bin<1>");
+  RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
+            P.grid.g.tiles_x, P.redge, (int)(H + 2));
+  RZ_LAUNCH(ctx, "k_bin_lines<1>", rzk::k_bin_lines<1>, dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.count, P.grid.g, b);
+  RZ_LAUNCH(ctx, "k_gap_events<1>", rzk::k_gap_events<1>, dim3(gs), dim3(rzk::kBlock), d->view, x,
+            P.grid.g, b);
+  return check_launch(ctx, "bin pass 1");
 }
 
 rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
@@ -399,10 +430,20 @@ rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   a.H = (int)H;
   a.tiles_x = P.grid.g.tiles_x;
   a.tiles_y = P.grid.g.tiles_y;
+  a.redge = P.redge;
+  if (mode == RZ_EDGE_NONE) {
+    a.redge_col = INT32_MAX;
+    a.redge_w = 0.0;
+  } else {
+    const double X = rzg::redge_x(P.grid.g), fx = std::floor(X);
+    a.redge_col = (int)fx;
+    a.redge_w = 1.0 - (X - fx);
+  }
+  const uint32_t gf = (uint32_t)P.grid.tiles;
   if (rule == RZ_NONZERO)
-    RZ_LAUNCH(ctx, "k_fine_mask<0>", rzk::k_fine_mask<0>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), a);
+    RZ_LAUNCH(ctx, "k_fine_mask<0>", rzk::k_fine_mask<0>, dim3(gf), dim3(rzk::kFineWave), a);
   else
-    RZ_LAUNCH(ctx, "k_fine_mask<1>", rzk::k_fine_mask<1>, dim3((uint32_t)P.grid.tiles), dim3(rzk::kBlock), a);
+    RZ_LAUNCH(ctx, "k_fine_mask<1>", rzk::k_fine_mask<1>, dim3(gf), dim3(rzk::kFineWave), a);
   RZ_TRY(check_launch(ctx, "k_fine_mask"));
   // the status block is read back at rz_sync
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, P.st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
@@ -421,6 +462,16 @@ rz_status finish(rz_ctx* ctx) {
     if (s.n_records > ctx->rec_cap) {
       ctx->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_records * 5 / 4 + 1024, 0xffffffffu);
       RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
+    }
+    if (s.n_lines > ctx->line_cap) {
+      ctx->line_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_lines * 5 / 4 + 1024, 0xffffffffu);
+      RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
+      // binning was skipped, so its record count is unknown: most lines give
+      // one record, so size for that rather than re-running twice
+      if (ctx->rec_cap < ctx->line_cap) {
+        ctx->rec_cap = ctx->line_cap;
+        RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
+      }
     }
     if (s.n_events > ctx->ev_cap) {
       ctx->ev_cap = s.n_events * 5 / 4 + 1024;
@@ -487,6 +538,23 @@ rz_status collect_prof(rz_ctx* ctx) {
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
+  if (!ctx) return RZ_E_ARG;
+  if (value < 0 || value > 8) return fail(ctx, RZ_E_ARG, "parameter out of range 0..8");
+  if (param == RZ_PARAM_SPLIT_QUAD) ctx->split_quad = value;
+  else if (param == RZ_PARAM_SPLIT_CUBIC) ctx->split_cubic = value;
+  else return fail(ctx, RZ_E_ARG, "unknown parameter");
+  return RZ_OK;
+}
+
+rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
+  if (!ctx || !value) return RZ_E_ARG;
+  if (param == RZ_PARAM_SPLIT_QUAD) *value = ctx->split_quad;
+  else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
+  else return fail(ctx, RZ_E_ARG, "unknown parameter");
+  return RZ_OK;
+}
 
 rz_status rz_profile(rz_ctx* ctx, int enable) {
   if (!ctx) return RZ_E_ARG;
@@ -671,7 +739,7 @@ rz_status rz_mask(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flat
 rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness, int edge_mode,
                  size_t width, size_t height, uint32_t* tile_counts, float* records,
                  size_t records_cap, int32_t* backdrop, float* events, size_t events_cap,
-                 size_t* n_records, size_t* n_events) {
+                 int32_t* right_edge, size_t* n_records, size_t* n_events) {
   RZ_TRY(check_args(ctx, flatness, RZ_NONZERO, edge_mode, width, height));
   if (width == 0 || height == 0) {
     if (n_records) *n_records = 0;
@@ -705,6 +773,7 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
   if (tile_counts)
     for (size_t t = 0; t < tiles; ++t) tile_counts[t] = off[t + 1] - off[t];
   if (backdrop) RZ_HIP(ctx, hipMemcpy(backdrop, P.bd, tiles * 4, hipMemcpyDeviceToHost));
+  if (right_edge) RZ_HIP(ctx, hipMemcpy(right_edge, P.redge, height * 4, hipMemcpyDeviceToHost));
   if (records) {
     if (records_cap < nr) return fail(ctx, RZ_E_CAPACITY, "records buffer too small");
     RZ_HIP(ctx, hipMemcpy(records, ctx->recs.p, nr * 16, hipMemcpyDeviceToHost));

@@ -204,19 +204,26 @@ struct EffLine {
   Pt p[3];   // polyline vertices p[0] -> ... -> p[n]
   int n;     // number of pieces (0, 1 or 2)
   bool clip0, clip1;  // start / end vertex created by the right clip
+  bool proj;  // both ends beyond the clip: a vertical line on x = redge_x(g)
 };
+
+// Abscissa of the vertical line that replaces a line lying entirely beyond the
+// right clip (src/rasterize.rs:372-375).
+RZ_HD double redge_x(const BinGrid& g) { return g.wc - 0.001; }
 
 RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {
   EffLine e;
   e.clip0 = e.clip1 = false;
+  e.proj = false;
   e.n = 1;
   if (g.mode != EDGE_NONE) {
     const double width = g.wc;
     if (p0.x > width || p1.x > width) {
       if (p0.x > width && p1.x > width) {
-        p0 = mkpt(width - 0.001, p0.y);
-        p1 = mkpt(width - 0.001, p1.y);
+        p0 = mkpt(redge_x(g), p0.y);
+        p1 = mkpt(redge_x(g), p1.y);
         e.clip0 = e.clip1 = true;
+        e.proj = true;
       } else {
         double t = (p0.x - width) / (p0.x - p1.x);
         Pt mid = mkpt(width, (1.0 - t) * p0.y + t * p1.y);
@@ -271,7 +278,8 @@ struct Rec {
 
 // Free-endpoint event: a chain endpoint that is not shared with another line
 // (a gap between segments, or a vertex created by the right clip). Adds
-// sign * V(yrel) to every tile of band `band` right of column `col`.
+// sign * V(yrel) to every tile of band `band` right of column `col` (col = -1:
+// every tile of the band).
 struct Event {
   int band;
   int col;
@@ -300,8 +308,8 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
     // band-bottom crossing (a point on y = rb counts as inside the band)
     if (ylo <= rb && rb < yhi) {
       Pt c = at_y(a, b, rb);
-      int col = tile_col(c.x, g) + 1;
-      if (c.x < 0.0) col = 0;  // left of every tile start (x < 0 cannot occur after the left split)
+      // first tile whose left region holds c (tile 0's is x <= 0, see below)
+      int col = c.x <= 0.0 ? 0 : tile_col(c.x, g) + 1;
       if (col < g.tiles_x) bd(ty, col, dir);
     }
     // piece of a -> b inside r0 <= y <= rb, direction preserved
@@ -323,6 +331,12 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
     for (int tx = tx0; tx <= tx1; ++tx) {
       const double c0 = (double)(tx * kTile), c1 = c0 + (double)kTile;
       if (xhi < c0 || xlo > c1) continue;
+      // Tile 0's left region is x <= 0 rather than x < 0: a line on x = 0
+      // deposits its whole cover into pixel 0, exactly like a line left of
+      // the image, so the vertical lines made by the left split (every part of
+      // the path left of the image) need no records at all.
+      const bool first = tx == 0;
+      if (first && xhi <= 0.0) continue;
       // clip the band piece to c0 <= x <= c1
       Pt u = s, v = e;
       if (s.x != e.x) {
@@ -335,7 +349,8 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
         }
       }
       // crossings of the cell's left edge x = c0 (inside means x < c0)
-      const bool in_s = s.x < c0, in_e = e.x < c0;
+      const bool in_s = first ? s.x <= 0.0 : s.x < c0;
+      const bool in_e = first ? e.x <= 0.0 : e.x < c0;
       bool flag0 = false, flag1 = false;
       if (in_s && !in_e) flag0 = u.y != r0;  // exits the left region: +V(u.y)
       if (!in_s && in_e) flag1 = v.y != r0;  // enters the left region: -V(v.y)
@@ -353,6 +368,32 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
   }
 }
 
+// Right-edge column. A line beyond the right clip becomes a vertical line on
+// X = redge_x(g) (src/rasterize.rs:372-375). Per pixel row it deposits
+// d (1 - f) into pixel floor(X) and d f into the next one (f = X - floor(X),
+// src/rasterize.rs:437-444), d being the row's signed overlap V(y1) - V(y0),
+// V(y)(row) = clamp(y - row, 0, 1). All such lines share X, so only the per-row
+// sum R(row) of d matters: pixel floor(X) gets (1 - f) R and every pixel right
+// of it R. Such lines therefore make no records, crossings or events; each adds
+// its two endpoint terms to E, the 2^-20 fixed-point differences of V between
+// consecutive rows (integer sums, exact in any order), and the row prefix of E
+// is R.
+RZ_HD int fix_frac(double f) { return (int)rint(f * (double)(1 << kFixShift)); }
+
+// sign * V(y) as differences E[k] += sign (F - 1), E[k + 1] -= sign F with
+// k = floor(y), F = frac(y), y clamped to [0, H] (E has H + 2 entries; the
+// constant term of V cancels between a line's two ends).
+template <typename AddF>
+RZ_HD void redge_endpoint(double y, int sign, const BinGrid& g, AddF&& add) {
+  const double H = (double)g.H;
+  const double yc = y < 0.0 ? 0.0 : (y > H ? H : y);
+  const int k = (int)floor(yc);
+  const int F = fix_frac(yc - (double)k);
+  const int one = 1 << kFixShift;
+  if (F != one) add(k, sign * (F - one));
+  if (F != 0) add(k + 1, -sign * F);
+}
+
 // Free-endpoint event for effective vertex p (sign +1 for a chain end, -1 for
 // a chain start). Returns false when no tile lies right of it in its band or
 // the point is outside the image rows (then the event has no effect).
@@ -362,7 +403,8 @@ RZ_HD bool make_event(Pt p, float sign, const BinGrid& g, Event* ev) {
   int band = (int)ceil(p.y / (double)kTile) - 1;
   if (band < 0) band = 0;
   if (band > g.tiles_y - 1) return false;
-  int col = tile_col(p.x, g);
+  // an endpoint on x = 0 lies in tile 0's left region: it affects every tile
+  int col = p.x <= 0.0 ? -1 : tile_col(p.x, g);
   if (col >= g.tiles_x - 1) return false;
   ev->band = band;
   ev->col = col;

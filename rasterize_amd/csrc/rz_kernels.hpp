@@ -31,10 +31,13 @@ struct DPath {
 
 struct Status {
   uint32_t err;        // first error (rzg::Status), 0 if none
-  uint32_t overflow;   // record or event buffer overflow
+  uint32_t overflow;   // bit0 records, bit1 events, bit2 lines outgrew their buffer
   uint32_t n_records;  // total records (written by the scan)
   uint32_t n_events;
+  uint32_t n_lines;    // flattened lines (atomic counter of the flatten kernel)
+  uint32_t pad[3];
 };
+constexpr int kStatusWords = sizeof(Status) / 4;
 
 __device__ inline void set_err(Status* st, uint32_t e) { atomicCAS(&st->err, 0u, e); }
 
@@ -118,9 +121,115 @@ __global__ __launch_bounds__(kBlock) void k_flatten_emit(DPath p, Xform x, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Binning (flatten fused with cell binning). PASS 0 counts records per tile,
+// Flatten to lines for binning. Line slots take one thread each; every quad /
+// cubic slot is split into 2^D0 virtual subtrees (the subtree at depth D0
+// along path bits j), one thread each: the thread walks from the root along
+// j; a leaf met above depth D0 is processed only by the leftmost j under it.
+// Every leaf is produced exactly once, with the same arithmetic as the
+// slot-wide traversal, and a curve with hundreds of lines is spread over many
+// threads. The order of lines in the buffer does not matter for binning, so
+// each wave reserves its output with one atomic.
+// ---------------------------------------------------------------------------
+struct LineBufs {
+  double4* lines;
+  uint32_t cap;
+  uint32_t* count;  // total lines (device)
+  Status* st;
+};
+
+// Walk from the slot's root along j for D0 levels. Returns the subtree root
+// in *n and whether a traversal from it is needed; *leaf is set when a leaf
+// above D0 is owned by this j.
+template <int NP>
+__device__ inline bool walk_to_subtree(rzg::Node<NP>* n, uint32_t j, int D0, double fl, bool* leaf,
+                                       int* err) {
+  *leaf = false;
+  for (int k = 0; k < D0; ++k) {
+    if (rzg::has_nans(*n)) {
+      *err = rzg::ST_NAN;
+      return false;
+    }
+    if (rzg::flatness(*n) < fl) {
+      *leaf = (j & ((1u << (D0 - k)) - 1u)) == 0;
+      return false;
+    }
+    rzg::Node<NP> c;
+    rzg::child(*n, (int)((j >> (D0 - 1 - k)) & 1u), &c);
+    *n = c;
+  }
+  return true;
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  return v;
+}
+
+template <int NP>
+__device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                          const Xform& x, double fl, const LineBufs& lb) {
+  rzg::Node<NP> n;
+  bool walk = false, leaf = false;
+  int err = rzg::ST_OK;
+  if (active) {
+    n = load_node<NP>(p, slot, x.m);
+    walk = walk_to_subtree(&n, j, D0, fl, &leaf, &err);
+  }
+  uint32_t cnt = leaf ? 1u : 0u;
+  if (walk) {
+    int r = rzg::traverse(n, fl, [&](Pt, Pt) { ++cnt; });
+    if (r != rzg::ST_OK) err = r;
+  }
+  if (err != rzg::ST_OK) set_err(lb.st, (uint32_t)err);
+  // one atomic per wave reserves the wave's lines
+  const uint32_t incl = wave_incl_scan(cnt);
+  const uint32_t total = __shfl(incl, 63, 64);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63) == 63 && total) base = atomicAdd(lb.count, total);
+  base = __shfl(base, 63, 64);
+  uint32_t o = base + incl - cnt;
+  if (!cnt || err != rzg::ST_OK) return;
+  if (o + cnt > lb.cap) {
+    atomicOr(&lb.st->overflow, 4u);
+    return;
+  }
+  if (leaf) {
+    lb.lines[o] = make_double4(n.p[0].x, n.p[0].y, n.p[NP - 1].x, n.p[NP - 1].y);
+    return;
+  }
+  rzg::traverse(n, fl, [&](Pt a, Pt c) { lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y); });
+}
+
+// Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
+// cubic virtual items. Every lane of a wave reaches the wave-wide reservation.
+__global__ __launch_bounds__(kBlock) void k_flat_lines(DPath p, Xform x, double fl, LineBufs lb,
+                                                        uint32_t bl, uint32_t bq, int dq, int dc) {
+  const uint32_t blk = blockIdx.x, t = threadIdx.x;
+  if (blk < bl) {
+    const uint32_t slot = p.order[blk * kBlock + t];
+    flat_item<2>(p, slot, 0, 0, slot != ~0u, x, fl, lb);
+  } else if (blk < bl + bq) {
+    const uint32_t v = (blk - bl) * kBlock + t;
+    const uint32_t k = v >> dq;
+    const uint32_t slot = p.order[p.blocks_line * kBlock + k];
+    flat_item<3>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, lb);
+  } else {
+    const uint32_t v = (blk - bl - bq) * kBlock + t;
+    const uint32_t k = v >> dc;
+    const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + k];
+    flat_item<4>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, lb);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Binning of lines into cell records. PASS 0 counts records per tile,
 // accumulates band-bottom crossings and counts events per band; PASS 1 writes
-// records and events into the slots reserved by the scan.
+// records and events into the slots reserved by the scans.
 // ---------------------------------------------------------------------------
 struct BinBufs {
   uint32_t* tile_count;   // PASS 0: counts; PASS 1: cursor (starts at tile offsets)
@@ -130,97 +239,170 @@ struct BinBufs {
   uint32_t rec_cap;
   int4* events;           // (band, col, yrel bits, sign bits)
   uint32_t ev_cap;
+  int32_t* redge;         // [H + 2] right-edge row differences (PASS 0)
   Status* st;
 };
 
 template <int PASS>
-struct BinSink {
-  const rzg::BinGrid& g;
-  const BinBufs& b;
-  int run_tile;
-  uint32_t run_n;
-  __device__ BinSink(const rzg::BinGrid& g_, const BinBufs& b_) : g(g_), b(b_), run_tile(-1), run_n(0) {}
-  __device__ void flush() {
-    if (PASS == 0 && run_tile >= 0 && run_n) atomicAdd(&b.tile_count[run_tile], run_n);
-    run_tile = -1;
-    run_n = 0;
-  }
-  __device__ void rec(int tile, const rzg::Rec& r) {
-    if (PASS == 0) {
-      if (tile != run_tile) {
-        flush();
-        run_tile = tile;
-      }
-      ++run_n;
-    } else {
-      uint32_t slot = atomicAdd(&b.tile_count[tile], 1u);
-      if (slot < b.rec_cap) b.recs[slot] = make_float4(r.x0, r.y0, r.x1, r.y1);
-      else atomicOr(&b.st->overflow, 1u);
-    }
-  }
-  __device__ void bd(int band, int col, int dir) {
-    if (PASS == 0) atomicAdd(&b.bd_diff[band * g.tiles_x + col], dir);
-  }
-  __device__ void event(Pt p, float sign) {
-    rzg::Event ev;
-    if (!rzg::make_event(p, sign, g, &ev)) return;
-    if (PASS == 0) {
-      atomicAdd(&b.band_ev[ev.band], 1u);
-    } else {
-      uint32_t slot = atomicAdd(&b.band_ev[ev.band], 1u);
-      if (slot < b.ev_cap)
-        b.events[slot] = make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
-      else atomicOr(&b.st->overflow, 2u);
-    }
-  }
-  __device__ void line(Pt p0, Pt p1) {
-    // infinite coordinates (the NaN case is caught by the traversal) would
-    // produce NaN cell coordinates; report them like NaN
-    if (!(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
-      set_err(b.st, rzg::ST_NAN);
-      return;
-    }
-    rzg::EffLine e = rzg::effective_line(p0, p1, g);
-    for (int k = 0; k < e.n; ++k)
-      rzg::bin_piece(e.p[k], e.p[k + 1], g, [&](int t, const rzg::Rec& r) { rec(t, r); },
-                     [&](int band, int col, int dir) { bd(band, col, dir); });
-    if (e.clip0) event(e.p[0], -1.0f);
-    if (e.clip1) event(e.p[e.n], 1.0f);
-  }
-};
-
-template <int NP, int PASS>
-__device__ void bin_slot(const DPath& p, uint32_t slot, const Xform& x, double fl,
-                         const rzg::BinGrid& g, const BinBufs& b) {
-  rzg::Node<NP> root = load_node<NP>(p, slot, x.m);
-  BinSink<PASS> sink(g, b);
-  int r = rzg::traverse(root, fl, [&](Pt a, Pt c) { sink.line(a, c); });
-  sink.flush();
-  if (PASS == 0 && r != rzg::ST_OK) set_err(b.st, (uint32_t)r);
-  // gaps between consecutive segments: the segment's own start/end is a free
-  // chain endpoint unless the right clip already made it one
-  uint8_t f = p.slot_flags[slot];
-  if (f) {
-    const double wc = g.wc;
-    const bool clip = g.mode != rzg::EDGE_NONE;
-    Pt s = root.p[0], e = root.p[NP - 1];
-    if ((f & 1) && !(clip && s.x > wc)) sink.event(rzg::mkpt(fmax(s.x, 0.0), s.y), -1.0f);
-    if ((f & 2) && !(clip && e.x > wc)) sink.event(rzg::mkpt(fmax(e.x, 0.0), e.y), 1.0f);
+__device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, Pt p, float sign) {
+  rzg::Event ev;
+  if (!rzg::make_event(p, sign, g, &ev)) return;
+  if (PASS == 0) {
+    atomicAdd(&b.band_ev[ev.band], 1u);
+  } else {
+    uint32_t slot = atomicAdd(&b.band_ev[ev.band], 1u);
+    if (slot < b.ev_cap)
+      b.events[slot] = make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
+    else atomicOr(&b.st->overflow, 2u);
   }
 }
 
 template <int PASS>
-__global__ __launch_bounds__(kBlock) void k_bin(DPath p, Xform x, double fl, rzg::BinGrid g,
-                                                 BinBufs b) {
-  if (PASS == 1 && b.st->err) return;
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t slot = p.order[i];
-  if (slot == ~0u) return;
-  switch (block_kind(p, blockIdx.x)) {
-    case rzg::KIND_LINE: bin_slot<2, PASS>(p, slot, x, fl, g, b); break;
-    case rzg::KIND_QUAD: bin_slot<3, PASS>(p, slot, x, fl, g, b); break;
-    default: bin_slot<4, PASS>(p, slot, x, fl, g, b); break;
+__device__ inline void put_record(const BinBufs& b, int tile, const rzg::Rec& r) {
+  if (PASS == 0) {
+    atomicAdd(&b.tile_count[tile], 1u);
+  } else {
+    uint32_t slot = atomicAdd(&b.tile_count[tile], 1u);
+    if (slot < b.rec_cap) b.recs[slot] = make_float4(r.x0, r.y0, r.x1, r.y1);
+    else atomicOr(&b.st->overflow, 1u);
   }
+}
+
+// Records a lane keeps in registers for aggregation. A line gives two records
+// in one cell only when it is split at x = 0 inside the first column.
+constexpr int kRecBuf = 2;
+
+// One thread per line (grid-stride, lanes of a wave on consecutive lines).
+// A line whose records all fall in one tile (the common case: flattened lines
+// are a few pixels long) takes part in a wave-wide aggregation: lanes with the
+// same tile get consecutive slots from one atomic, and all of a wave's atomics
+// are in flight together. Other lines use one atomic per record. Lines beyond
+// the right clip only add to the right-edge column; two such lines on
+// neighbouring lanes that continue each other drop their shared endpoint
+// (its two terms cancel exactly).
+template <int PASS>
+__global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines, const uint32_t* n_lines,
+                                                       rzg::BinGrid g, BinBufs b) {
+  if (b.st->err || (b.st->overflow & 4u)) return;
+  const uint32_t n = *n_lines;
+  const uint32_t stride = gridDim.x * kBlock;
+  const int lane = threadIdx.x & 63;
+  // the loop bound is uniform per wave so every lane reaches the ballots
+  for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    int nrec = 0, tile0 = -1;
+    bool same = true;
+    rzg::Rec r0{}, r1{};  // separate registers: no dynamic indexing
+    int cnt_fast = 0;
+    bool have = i < n;
+    const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
+    const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
+    if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
+      set_err(b.st, rzg::ST_NAN);
+      have = false;
+    }
+    const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
+    if (PASS == 0) {
+      const double nx = __shfl_down(L.x, 1, 64), ny = __shfl_down(L.y, 1, 64);
+      const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
+      const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
+      const bool pproj = __shfl_up((int)proj, 1, 64) != 0;
+      if (proj) {
+        const bool skip_end = lane < 63 && nproj && nx == p1.x && ny == p1.y;
+        const bool skip_start = lane > 0 && pproj && px == p0.x && py == p0.y;
+        auto add = [&](int k, int v) { atomicAdd(&b.redge[k], v); };
+        if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
+        if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
+      }
+    }
+    if (have && !proj) {
+      const rzg::EffLine e = rzg::effective_line(p0, p1, g);
+      auto collect = [&](int t, const rzg::Rec& r) {
+        if (nrec == 0) tile0 = t;
+        else if (t != tile0) same = false;
+        if (nrec == 0) r0 = r;
+        else r1 = r;  // only read when there are exactly two
+        ++nrec;
+      };
+      auto bd = [&](int band, int col, int dir) {
+        if (PASS == 0) atomicAdd(&b.bd_diff[band * g.tiles_x + col], dir);
+      };
+      rzg::bin_piece(e.p[0], e.p[1], g, collect, bd);
+      if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, collect, bd);
+      if (e.clip0) put_event<PASS>(b, g, e.p[0], -1.0f);
+      if (e.clip1) put_event<PASS>(b, g, e.n == 2 ? e.p[2] : e.p[1], 1.0f);
+      const bool fast = nrec > 0 && same && nrec <= kRecBuf;
+      if (fast) {
+        cnt_fast = nrec;
+      } else if (nrec > 0) {
+        // records in several tiles or too many to keep: regenerate them
+        auto direct = [&](int t, const rzg::Rec& r) { put_record<PASS>(b, t, r); };
+        auto nobd = [&](int, int, int) {};
+        rzg::bin_piece(e.p[0], e.p[1], g, direct, nobd);
+        if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, direct, nobd);
+      }
+    }
+    // wave aggregation of the single-tile lines: group lanes by tile (the
+    // first pending lane leads), offsets from ballots (counts are 1 or 2)
+    const uint64_t below = (1ull << lane) - 1ull;
+    bool pending = cnt_fast > 0;
+    int my_leader = 0;
+    uint32_t my_off = 0, lead_total = 0;
+    while (true) {
+      const uint64_t m = __ballot(pending);
+      if (m == 0) break;
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      const int T = __builtin_amdgcn_readlane(tile0, leader);
+      const bool mine = pending && tile0 == T;
+      const uint64_t g1 = __ballot(mine), g2 = __ballot(mine && cnt_fast == 2);
+      if (lane == leader) lead_total = (uint32_t)(__popcll(g1) + __popcll(g2));
+      if (mine) {
+        my_leader = leader;
+        my_off = (uint32_t)(__popcll(g1 & below) + __popcll(g2 & below));
+        pending = false;
+      }
+    }
+    uint32_t s0 = 0;
+    if (lead_total) s0 = atomicAdd(&b.tile_count[tile0], lead_total);
+    if (PASS == 1) {
+      const uint32_t my_slot = (uint32_t)__shfl((int)s0, my_leader, 64) + my_off;
+      if (cnt_fast > 0) {
+        if (my_slot + (uint32_t)cnt_fast <= b.rec_cap) {
+          b.recs[my_slot] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
+          if (cnt_fast > 1) b.recs[my_slot + 1] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
+        } else {
+          atomicOr(&b.st->overflow, 1u);
+        }
+      }
+    }
+  }
+}
+
+// A gap between the previous segment's end `a` and this segment's start `b`
+// (arc endpoints are recomputed by sin/cos and miss their neighbours by
+// ~1e-13): the chain has a free end at a (+V) and a free start at b (-V).
+// Both events are made here; a pair that lands on the same band, column and
+// f32 row offset contributes exactly zero and is dropped. Ends created by the
+// right clip are reported by the clip itself. One thread per slot.
+template <int PASS>
+__global__ __launch_bounds__(kBlock) void k_gap_events(DPath p, Xform x, rzg::BinGrid g, BinBufs b) {
+  if (PASS == 1 && b.st->err) return;
+  const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+  if (slot >= p.n_slots || !(p.slot_flags[slot] & 1)) return;
+  const uint32_t k = p.slot_item[slot];  // a segment (closing lines never start a gap)
+  const double* qa = p.pts + 2 * (size_t)(p.pt_off[k] - 1);  // previous segment's last point
+  const double* qb = p.pts + 2 * (size_t)p.pt_off[k];
+  const Pt a = rzg::tr_apply(x.m, rzg::mkpt(qa[0], qa[1]));
+  const Pt bb = rzg::tr_apply(x.m, rzg::mkpt(qb[0], qb[1]));
+  const bool clip = g.mode != rzg::EDGE_NONE;
+  const bool ea = !(clip && a.x > g.wc), eb = !(clip && bb.x > g.wc);
+  rzg::Event va, vb;
+  const Pt ap = rzg::mkpt(fmax(a.x, 0.0), a.y), bp = rzg::mkpt(fmax(bb.x, 0.0), bb.y);
+  const bool ha = ea && rzg::make_event(ap, 1.0f, g, &va);
+  const bool hb = eb && rzg::make_event(bp, -1.0f, g, &vb);
+  if (ha && hb && va.band == vb.band && va.col == vb.col && va.yrel == vb.yrel) return;
+  if (ha) put_event<PASS>(b, g, ap, 1.0f);
+  if (hb) put_event<PASS>(b, g, bp, -1.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,15 +495,22 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, size_t n, co
   }
 }
 
-// Inclusive prefix of the band backdrop differences along each tile row.
-__global__ void k_bd_prefix(int32_t* bd, int tiles_x, int tiles_y) {
-  int band = blockIdx.x * blockDim.x + threadIdx.x;
-  if (band >= tiles_y) return;
-  int32_t* row = bd + (size_t)band * tiles_x;
-  int32_t acc = 0;
-  for (int i = 0; i < tiles_x; ++i) {
-    acc += row[i];
-    row[i] = acc;
+// Inclusive prefix of the band backdrop differences along each tile row, one
+// block per band, and of the right-edge row differences in the last block
+// (two's-complement sums, so the unsigned scan is exact).
+__global__ __launch_bounds__(kBlock) void k_bd_prefix(int32_t* bd, int tiles_x, int32_t* redge,
+                                                       int n_redge) {
+  const bool edge = blockIdx.x == gridDim.x - 1;
+  uint32_t* row = (uint32_t*)(edge ? redge : bd + (size_t)blockIdx.x * tiles_x);
+  const int n = edge ? n_redge : tiles_x;
+  uint32_t carry = 0;
+  for (int start = 0; start < n; start += kBlock) {
+    const int i = start + threadIdx.x;
+    const uint32_t v = i < n ? row[i] : 0u;
+    uint32_t tot;
+    const uint32_t pre = block_exclusive_scan(v, &tot);
+    if (i < n) row[i] = carry + pre + v;
+    carry += tot;
   }
 }
 
@@ -433,10 +622,13 @@ struct FineArgs {
   const int32_t* bd;
   const uint32_t* band_ev_off;
   const int4* events;
+  const int32_t* redge;  // [H] right-edge column per row (2^-20 fixed point)
   const Status* st;
   float* out;
   size_t row_stride;
   int W, H, tiles_x, tiles_y;
+  int redge_col;         // floor(redge_x): gets redge_w * R; pixels right of it get R
+  double redge_w;
 };
 
 // FillRule::alpha_from_winding (src/path.rs:32-46) on a 2^-20 fixed-point winding
@@ -456,32 +648,52 @@ __device__ inline float alpha_fix(int64_t w) {
   }
 }
 
+constexpr int kFineWave = 64;  // one wave per tile
+
+// One 16x16 tile per wave (one workgroup per tile, so the hardware balances
+// tiles of very different record counts): records and events into LDS, then
+// four passes of 16-lane DPP row scans (one pixel row per 16 lanes) and
+// coalesced stores.
 template <int RULE>
-__global__ __launch_bounds__(kBlock) void k_fine_mask(FineArgs a) {
+__global__ __launch_bounds__(kFineWave) void k_fine_mask(FineArgs a) {
   if (a.st->err || a.st->overflow) return;
-  const int tile = blockIdx.x;
-  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   __shared__ int acc[256];
   __shared__ int rowd[18];
-  const int tid = threadIdx.x;
-  acc[tid] = 0;
-  if (tid < 18) rowd[tid] = 0;
+  const int lane = threadIdx.x;
+  const uint32_t tile = blockIdx.x;
+  const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
+  if (lane < 18) rowd[lane] = 0;
   __syncthreads();
   const uint32_t rb = a.tile_off[tile], re = a.tile_off[tile + 1];
-  for (uint32_t i = rb + tid; i < re; i += kBlock) fine_record(a.recs[i], acc, rowd);
+  for (uint32_t i = rb + lane; i < re; i += kFineWave) fine_record(a.recs[i], acc, rowd);
   const uint32_t eb = a.band_ev_off[ty], ee = a.band_ev_off[ty + 1];
-  for (uint32_t i = eb + tid; i < ee; i += kBlock) {
-    int4 ev = a.events[i];
+  for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
+    const int4 ev = a.events[i];
     if (ev.y < tx) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
   }
   __syncthreads();
-  const int row = tid >> 4, col = tid & 15;
-  int v = row_scan16(acc[tid]);
-  int base = 0;
-  for (int k = 0; k <= row; ++k) base += rowd[k];
-  const int64_t w = ((int64_t)a.bd[tile] << rzg::kFixShift) + (int64_t)(int32_t)(base + v);
-  const int x = tx * 16 + col, y = ty * 16 + row;
-  if (x < a.W && y < a.H) a.out[(size_t)y * a.row_stride + x] = alpha_fix<RULE>(w);
+  // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
+  if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
+  __syncthreads();
+  const int64_t bd = (int64_t)a.bd[tile] << rzg::kFixShift;
+  const int col = lane & 15;
+  const int x = tx * 16 + col;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int row = 4 * k + (lane >> 4);
+    const int v = row_scan16(acc[row * 16 + col]);
+    int64_t w = bd + (int64_t)(int32_t)(rowd[row] + v);
+    const int y = ty * 16 + row;
+    if (x < a.W && y < a.H) {
+      if (x >= a.redge_col) {
+        const int64_t r = a.redge[y];
+        w += x == a.redge_col ? (int64_t)rint((double)r * a.redge_w) : r;
+      }
+      a.out[(size_t)y * a.row_stride + x] = alpha_fix<RULE>(w);
+    }
+  }
 }
 
 }  // namespace rzk

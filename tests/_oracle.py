@@ -70,6 +70,9 @@ def lib():
             "orc_path_bbox": (i, [P(orc_path), P(d), P(d)]),
             "orc_time_sd_mask": (d, [P(orc_path), P(d), d, i, sz, sz, i, i]),
             "orc_time_flatten": (d, [P(orc_path), P(d), d, i, P(sz)]),
+            "orc_bin_replay": (i, [P(orc_path), P(d), d, i, sz, sz, P(C.c_uint32), P(C.c_float),
+                                   sz, P(C.c_int32), P(C.c_float), sz, P(C.c_int32), P(sz),
+                                   P(sz)]),
         }
         for name, (res, args) in protos.items():
             f = getattr(_lib, name)
@@ -183,3 +186,26 @@ def arc_to(src, radii, rot, large, sweep, dst):
     n = lib().orc_arc_to(_dp(np.array(src, float)), _dp(np.array(radii, float)), rot, int(large),
                          int(sweep), _dp(np.array(dst, float)), _dp(out))
     return n, out
+
+
+def bin_replay(path, tr, W, H, mode, flatness=0.05):
+    """Sequential replay of the binning stage: (tile_counts, records[n,4] f32,
+    backdrop[tiles], events[n,4], right_edge[H] int32)."""
+    p, t = opath(path), _tr(tr)
+    nr, ne = C.c_size_t(), C.c_size_t()
+    L = lib()
+    _check(L.orc_bin_replay(C.byref(p), _dp(t), flatness, mode, W, H, None, None, 0, None, None,
+                            0, None, C.byref(nr), C.byref(ne)))
+    tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    counts = np.zeros(tiles, np.uint32)
+    recs = np.zeros((max(nr.value, 1), 4), np.float32)
+    bd = np.zeros(tiles, np.int32)
+    ev = np.zeros((max(ne.value, 1), 4), np.float32)
+    redge = np.zeros(max(H, 1), np.int32)
+    _check(L.orc_bin_replay(C.byref(p), _dp(t), flatness, mode, W, H,
+                            counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
+                            bd.ctypes.data_as(C.POINTER(C.c_int32)),
+                            ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0],
+                            redge.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nr), C.byref(ne)))
+    return counts, recs[: nr.value], bd, ev[: ne.value], redge[:H]

@@ -3,8 +3,6 @@ backdrops and events, accumulated the way the fine stage does (here in f64
 numpy), must reproduce the oracle's reference mask. This validates the
 decomposition itself (closed-chain backdrops, left-edge crossing terms,
 free-endpoint events, right clip) without a GPU."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
@@ -15,30 +13,11 @@ from rasterize_amd import PathBuilder, Transform, workload_cubics64, workload_ra
 I = Transform.identity()
 
 
-def replay(path, tr, W, H, mode):
-    p, t = O.opath(path), O._tr(tr)
-    nr, ne = C.c_size_t(), C.c_size_t()
-    L = O.lib()
-    L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H, None, None, 0, None, None,
-                     0, C.byref(nr), C.byref(ne))
-    tiles = ((W + 15) // 16) * ((H + 15) // 16)
-    counts = np.zeros(tiles, np.uint32)
-    recs = np.zeros((max(nr.value, 1), 4), np.float32)
-    bd = np.zeros(tiles, np.int32)
-    ev = np.zeros((max(ne.value, 1), 4), np.float32)
-    L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H,
-                     counts.ctypes.data_as(C.POINTER(C.c_uint32)),
-                     recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
-                     bd.ctypes.data_as(C.POINTER(C.c_int32)),
-                     ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0], C.byref(nr), C.byref(ne))
-    return counts, recs[: nr.value].astype(np.float64), bd, ev[: ne.value]
-
-
 def V(y):
     return np.clip(y - np.arange(16), 0.0, 1.0)
 
 
-def fine_emulation(counts, recs, bd, events, W, H, rule):
+def fine_emulation(counts, recs, bd, events, redge, W, H, rule, mode):
     tx_n, ty_n = (W + 15) // 16, (H + 15) // 16
     out = np.zeros((ty_n * 16, tx_n * 16))
     ev_band = events[:, 0].view(np.int32) if len(events) else np.zeros(0, np.int32)
@@ -101,6 +80,15 @@ def fine_emulation(counts, recs, bd, events, W, H, rule):
         for e in events[sel]:
             base += e[3] * V(e[2])
         w = bd[t] + base[:, None] + np.cumsum(acc[:, :16], axis=1)
+        if mode != 2:
+            # right-edge column: lines beyond the clip, as per-row sums R
+            X = (W - 1.0 if mode == 0 else float(W)) - 0.001
+            xr = int(np.floor(X))
+            rows = np.arange(ty * 16, ty * 16 + 16)
+            R = np.where(rows < H, redge[np.minimum(rows, H - 1)], 0) / float(1 << 20)
+            cols = tx * 16 + np.arange(16)
+            wcol = np.where(cols > xr, 1.0, np.where(cols == xr, 1.0 - (X - xr), 0.0))
+            w = w + R[:, None] * wcol[None, :]
         if rule == 0:
             a = np.abs(w)
             a = np.where(a >= 1.0, 1.0, np.where(a < 1e-6, 0.0, a))
@@ -131,8 +119,8 @@ def test_decomposition_reproduces_reference(name, mode, rule):
     else:
         mk, W, H = CASES[name]
         path, tr = mk()
-    counts, recs, bd, ev = replay(path, tr, W, H, mode)
-    got = fine_emulation(counts, recs, bd, ev, W, H, rule)
+    counts, recs, bd, ev, redge = O.bin_replay(path, tr, W, H, mode)
+    got = fine_emulation(counts, recs.astype(np.float64), bd, ev, redge, W, H, rule, mode)
     if mode == 0:
         ref = O.sd_mask(path, tr, W, H, rule)
     elif mode == 1:

@@ -85,28 +85,6 @@ def test_flatten_nan_is_error():
 
 
 # ---- binning: bit-exact against the sequential replay -------------------------
-def _replay(path, tr, W, H, mode):
-    p, t = O.opath(path), O._tr(tr)
-    nr, ne = C.c_size_t(), C.c_size_t()
-    L = O.lib()
-    L.orc_bin_replay.restype = C.c_int
-    L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H, None, None, 0, None, None,
-                     0, C.byref(nr), C.byref(ne))
-    tiles = ((W + 15) // 16) * ((H + 15) // 16)
-    counts = np.zeros(tiles, np.uint32)
-    recs = np.zeros((max(nr.value, 1), 4), np.float32)
-    bd = np.zeros(tiles, np.int32)
-    ev = np.zeros((max(ne.value, 1), 4), np.float32)
-    rc = L.orc_bin_replay(C.byref(p), O._dp(t), C.c_double(0.05), mode, W, H,
-                          counts.ctypes.data_as(C.POINTER(C.c_uint32)),
-                          recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
-                          bd.ctypes.data_as(C.POINTER(C.c_int32)),
-                          ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0], C.byref(nr),
-                          C.byref(ne))
-    assert rc == 0
-    return counts, recs[: nr.value], bd, ev[: ne.value]
-
-
 def _sort_tiles(counts, recs):
     """Sort records within each tile by their bit patterns."""
     bits = recs.view(np.uint32)
@@ -134,10 +112,11 @@ def test_bin_bit_exact(case, mode, c1, c2):
         path, tr = c2
         W = H = 4096
     _say(f"bin {case} mode={mode}")
-    gc, gr, gb, ge = rz.bin_stage(path, tr, W, H, mode)
-    rc, rr, rb, re = _replay(path, tr, W, H, mode)
+    gc, gr, gb, ge, gx = rz.bin_stage(path, tr, W, H, mode)
+    rc, rr, rb, re, rx = O.bin_replay(path, tr, W, H, mode)
     assert np.array_equal(gc, rc), "per-tile record counts differ"
     assert np.array_equal(gb, rb), "band backdrops differ"
+    assert np.array_equal(gx, rx), "right-edge column differs"
     assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr)), "records differ"
     assert np.array_equal(_sort_events(ge), _sort_events(re)), "events differ"
 
