@@ -174,6 +174,60 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, do
                          int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
                          size_t row_stride);
 
+/* ---- Rasterizer::fill ------------------------------------------------------ */
+
+/* Rasterizer::fill (src/rasterize.rs:69-115): the pixels mask_iter yields
+ * (alpha >= 1e-6; edge_mode RZ_EDGE_ITER for SignedDifferenceRasterizer,
+ * RZ_EDGE_NONE for ActiveEdgeRasterizer) are composited source-over,
+ * dst = dst.blend_over(paint.at(pixel centre).with_alpha(alpha)), into the
+ * caller's premultiplied linear RGBA f32 image (LinColor, 4 floats per pixel)
+ * rgba[rgba_len floats] viewed through `shape` (in pixels). Gradient paints
+ * use the pixel-to-paint transform (tr * paint.tr)^-1 (UserSpaceOnUse) or
+ * (tr * path.bbox(I).unit_transform() * paint.tr)^-1 (BoundingBox); a
+ * non-invertible one draws nothing, as in the reference. */
+rz_status rz_fill(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                  int fill_rule, int edge_mode, const rz_paint* paint, float* rgba,
+                  size_t rgba_len, rz_shape shape);
+
+/* Asynchronous fill of a resident path into a device RGBA f32 image
+ * d_rgba[height * row_stride pixels]. */
+rz_status rz_fill_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, double flatness,
+                         int fill_rule, int edge_mode, const rz_paint* paint, size_t width,
+                         size_t height, float* d_rgba, size_t row_stride);
+
+/* ---- Scene::render ---------------------------------------------------------- */
+
+/* A scene: Scene::group of n Scene::fill(path, paint, fill_rule) nodes
+ * (src/scene.rs:64-110), uploaded once and rendered many times. */
+typedef struct rz_dscene rz_dscene;
+rz_status rz_scene_upload(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
+                          const int32_t* fill_rules, size_t n, rz_dscene** out);
+rz_status rz_scene_free(rz_ctx* ctx, rz_dscene* scene);
+
+/* Scene::render(rasterizer, tr, Some(view), bg) (src/scene.rs:186-199,
+ * 384-459) into a device RGBA f32 layer d_rgba[lh * row_stride pixels], where
+ * the layer spans floor(view.min) .. ceil(view.max) (Layer::new,
+ * src/scene.rs:483-501); view = {x0, y0, x1, y1}; bg = LinColor or NULL
+ * (transparent). Each fill is the rasterizer's fill of its path under
+ * translate(-floor(bbox.min)) * tr into the sub-view its bbox covers, in path
+ * order. edge_mode selects the rasterizer (RZ_EDGE_ITER: SignedDifference,
+ * RZ_EDGE_NONE: ActiveEdge). batched = 1 renders every path in one pipeline
+ * (per-tile path lists, one store per pixel); batched = 0 runs the fill
+ * pipeline once per path (the plain form). Asynchronous; errors at rz_sync.
+ * The batched form has no reference counterpart. */
+rz_status rz_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr, double flatness,
+                           int edge_mode, const double* view, const float* bg, int batched,
+                           float* d_rgba, size_t row_stride);
+
+/* Blocking Scene::render of n fills into host memory. view = NULL renders the
+ * union of the fills' bboxes. layer_xywh receives the layer's origin and
+ * size ({0,0,0,0} when no fill is in view: the reference's empty layer); the
+ * pixels are written when rgba holds at least w * h pixels (cap_px). */
+rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
+                    const int32_t* fill_rules, size_t n, rz_transform tr, double flatness,
+                    int edge_mode, const double* view, const float* bg, int batched, float* rgba,
+                    size_t cap_px, int32_t* layer_xywh);
+
 /* ---- launch parameters ----------------------------------------------------
  * No reference counterpart. RZ_PARAM_SPLIT_QUAD / RZ_PARAM_SPLIT_CUBIC: each
  * quad / cubic segment's subdivision tree is processed by 2^value threads

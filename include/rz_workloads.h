@@ -30,6 +30,23 @@ rzw_set* rzw_cubics64(uint32_t seed, double size);
  * canvas centre. */
 rzw_set* rzw_random_walk(uint32_t seed, uint32_t n_segments, double canvas);
 
+/* A scene of n_paths closed star/blob paths (segment count exp(N(ln 60, 1.4))
+ * clamped to 4..4000, each edge a line, quad or cubic with equal odds),
+ * bbox side log-uniform in 8..1024 px, centre uniform in [0, canvas)^2.
+ * Paints: 60% solid, 25% linear, 15% radial (focus at the centre), 2-6
+ * sorted random stops, random sRGB colours with random alpha, UserSpaceOnUse,
+ * Pad, sRGB interpolation; fill rules 80% NonZero, 20% EvenOdd. */
+rzw_set* rzw_scene(uint32_t seed, uint32_t n_paths, double canvas);
+
+/* The reference's scene benchmark (benches/scene_bench.rs:6-47): `count`
+ * circles (PathBuilder::circle) of radius U(30, 40) centred uniformly in
+ * [0, size)^2, opaque random colours, Rnd seed 0. */
+rzw_set* rzw_circles(uint32_t count, double size);
+
+/* Paint and fill rule of path i of a scene set. */
+void rzw_paint(const rzw_set* set, size_t i, rz_paint* out);
+int32_t rzw_rule(const rzw_set* set, size_t i);
+
 /* Number of paths in the set and a view of path i (valid while the set lives). */
 size_t rzw_count(const rzw_set* set);
 void rzw_path(const rzw_set* set, size_t i, rz_path* out);

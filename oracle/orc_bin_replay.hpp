@@ -31,14 +31,16 @@ inline rzg::Pt rp(Point p) { return rzg::mkpt(p.x, p.y); }
 inline rzg::Pt tr_pt(const Transform& tr, Point p) { return rp(tr.apply(p)); }
 
 inline void bin_replay(const Path& path, const Transform& tr, double flatness, int mode, int W,
-                       int H, BinReplay* out) {
+                       int H, BinReplay* out, int phx = 0, int phy = 0) {
   rzg::BinGrid g;
   g.mode = mode;
   g.W = W;
   g.H = H;
-  g.tiles_x = (W + rzg::kTile - 1) / rzg::kTile;
-  g.tiles_y = (H + rzg::kTile - 1) / rzg::kTile;
+  g.tiles_x = (W + phx + rzg::kTile - 1) / rzg::kTile;
+  g.tiles_y = (H + phy + rzg::kTile - 1) / rzg::kTile;
   g.wc = mode == rzg::EDGE_MASK ? (double)W - 1.0 : (double)W;
+  g.phx = phx;
+  g.phy = phy;
   const size_t tiles = (size_t)g.tiles_x * g.tiles_y;
   out->tiles_x = g.tiles_x;
   out->tiles_y = g.tiles_y;

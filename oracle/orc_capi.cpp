@@ -411,11 +411,11 @@ extern "C" {
 int orc_bin_replay(const orc_path* p, const double* tr, double flatness, int mode, size_t W,
                    size_t H, uint32_t* counts, float* recs, size_t rec_cap, int32_t* bd,
                    float* events, size_t ev_cap, int32_t* redge, size_t* n_recs,
-                   size_t* n_events) {
+                   size_t* n_events, int phx, int phy) {
   Path path = to_path(p);
   BinReplay r;
   try {
-    bin_replay(path, to_tr(tr), flatness, mode, (int)W, (int)H, &r);
+    bin_replay(path, to_tr(tr), flatness, mode, (int)W, (int)H, &r, phx, phy);
   } catch (const std::runtime_error&) {
     return ORC_E_NAN;
   }

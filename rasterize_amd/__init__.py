@@ -295,6 +295,140 @@ def _as_f64_view(img):
     return base, shape
 
 
+# ---- paint (src/color.rs, src/grad.rs) --------------------------------------
+
+_libm.powf.argtypes = [C.c_float, C.c_float]
+_libm.powf.restype = C.c_float
+
+
+def _srgb_to_linear(v):
+    """Exact sRGB -> linear of one f32 channel (src/color.rs:479-486)."""
+    v = np.float32(v)
+    if v <= np.float32(0.04045):
+        return np.float32(v / np.float32(12.92))
+    return np.float32(_libm.powf(np.float32((v + np.float32(0.055)) / np.float32(1.055)), 2.4))
+
+
+class LinColor(tuple):
+    """Premultiplied linear RGBA colour (src/color.rs:271); also a solid Paint."""
+
+    def __new__(cls, r, g, b, a):
+        return tuple.__new__(cls, (float(np.float32(r)), float(np.float32(g)),
+                                   float(np.float32(b)), float(np.float32(a))))
+
+    @staticmethod
+    def from_rgba8(r, g, b, a):
+        """From<RGBA> (src/color.rs:394-406): sRGB 8-bit to premultiplied linear."""
+        al = np.float32(a) / np.float32(255.0)
+        ch = [_srgb_to_linear(np.float32(c) / np.float32(255.0)) * al for c in (r, g, b)]
+        return LinColor(ch[0], ch[1], ch[2], al)
+
+    def abi(self):
+        p = _abi.rz_paint()
+        p.kind = _abi.RZ_PAINT_SOLID
+        for i in range(4):
+            p.color[i] = self[i]
+        return p
+
+
+class Units:
+    UserSpaceOnUse = 0
+    BoundingBox = 1
+
+
+class GradSpread:
+    Pad = 0
+    Repeat = 1
+    Reflect = 2
+
+
+class GradStop:
+    def __init__(self, position, color):
+        self.position = float(position)
+        self.color = LinColor(*color)
+
+
+class _Grad:
+    kind = None
+
+    def _abi_common(self):
+        p = _abi.rz_paint()
+        p.kind = self.kind
+        p.units = self.units
+        p.linear_colors = int(bool(self.linear_colors))
+        p.spread = self.spread
+        for i, v in enumerate(self.tr.m):
+            p.tr[i] = v
+        pos = np.array([s.position for s in self.stops], dtype=np.float64)
+        cols = np.array([c for s in self.stops for c in s.color], dtype=np.float32)
+        p.n_stops = len(self.stops)
+        p.stop_positions = pos.ctypes.data_as(C.POINTER(C.c_double))
+        p.stop_colors = cols.ctypes.data_as(C.POINTER(C.c_float))
+        p._keep = (pos, cols)
+        return p
+
+
+class GradLinear(_Grad):
+    """GradLinear (src/grad.rs:150-211)."""
+
+    kind = _abi.RZ_PAINT_LINEAR
+
+    def __init__(self, stops, units=Units.UserSpaceOnUse, linear_colors=False,
+                 spread=GradSpread.Pad, tr=None, start=(0.0, 0.0), end=(1.0, 0.0)):
+        self.stops = [s if isinstance(s, GradStop) else GradStop(*s) for s in stops]
+        self.units, self.linear_colors, self.spread = units, linear_colors, spread
+        self.tr = tr or Transform.identity()
+        self.start, self.end = tuple(map(float, start)), tuple(map(float, end))
+
+    def abi(self):
+        p = self._abi_common()
+        p.start[0], p.start[1] = self.start
+        p.end[0], p.end[1] = self.end
+        return p
+
+
+class GradRadial(_Grad):
+    """GradRadial (src/grad.rs:307-411): two-circle gradient."""
+
+    kind = _abi.RZ_PAINT_RADIAL
+
+    def __init__(self, stops, units=Units.UserSpaceOnUse, linear_colors=False,
+                 spread=GradSpread.Pad, tr=None, center=(0.5, 0.5), radius=0.5,
+                 fcenter=None, fradius=0.0):
+        self.stops = [s if isinstance(s, GradStop) else GradStop(*s) for s in stops]
+        self.units, self.linear_colors, self.spread = units, linear_colors, spread
+        self.tr = tr or Transform.identity()
+        self.center = tuple(map(float, center))
+        self.radius = float(radius)
+        self.fcenter = tuple(map(float, fcenter if fcenter is not None else center))
+        self.fradius = float(fradius)
+
+    def abi(self):
+        p = self._abi_common()
+        p.center[0], p.center[1] = self.center
+        p.radius = self.radius
+        p.fcenter[0], p.fcenter[1] = self.fcenter
+        p.fradius = self.fradius
+        return p
+
+
+def _as_rgba_view(img):
+    """(base float32 array, rz_shape in pixels) for an (H, W, 4) float32 array or view."""
+    if img.dtype != np.float32 or img.ndim != 3 or img.shape[2] != 4 or img.strides[2] != 4:
+        raise TypeError("colour image must be an (H, W, 4) float32 array with packed pixels")
+    if img.strides[0] % 16 or img.strides[1] % 16:
+        raise TypeError("pixel strides must be whole pixels")
+    base = img
+    while base.base is not None and isinstance(base.base, np.ndarray):
+        base = base.base
+    if not base.flags.c_contiguous:
+        raise TypeError("the image's base buffer must be contiguous")
+    start = (img.__array_interface__["data"][0] - base.__array_interface__["data"][0]) // 16
+    shape = _abi.rz_shape(start, img.shape[1], img.shape[0], img.strides[0] // 16,
+                          img.strides[1] // 16)
+    return base, shape
+
+
 class _RasterizerBase:
     edge_mode = None
 
@@ -324,6 +458,17 @@ class _RasterizerBase:
                                     mode, width, height,
                                     out.ctypes.data_as(C.POINTER(C.c_float)), width))
         return out
+
+    def fill(self, path, tr, fill_rule, paint, img):
+        """Rasterizer::fill: composite `paint` through the path's coverage into an
+        (H, W, 4) float32 premultiplied linear RGBA image (or a view of one)."""
+        ctx = self._ctx()
+        base, shape = _as_rgba_view(img)
+        p = paint.abi()
+        ctx.check(lib().rz_fill(ctx.h, C.byref(path.abi()), tr.abi(), self.flatness, fill_rule,
+                                self.iter_mode, C.byref(p),
+                                base.ctypes.data_as(C.POINTER(C.c_float)), base.size, shape))
+        return img
 
     def mask_iter(self, path, tr, size, fill_rule):
         """Rasterizer::mask_iter: list of (x, y, alpha) in row-major order."""
@@ -391,22 +536,137 @@ def bin_stage(path, tr, width, height, edge_mode=_abi.RZ_EDGE_MASK, flatness=DEF
 
 # ---- workloads ------------------------------------------------------------
 
-class WorkloadSet:
-    """Owned set of generated paths (include/rz_workloads.h)."""
+# ---- scenes (src/scene.rs) ---------------------------------------------------
 
-    def __init__(self, handle):
+class RawPaint:
+    """A paint given directly as an rz_paint (stop arrays owned here)."""
+
+    def __init__(self, p):
+        self.p = _abi.rz_paint()
+        C.pointer(self.p)[0] = p
+        n = p.n_stops
+        self.pos = np.ctypeslib.as_array(p.stop_positions, (n,)).copy() if n else np.zeros(1)
+        self.col = np.ctypeslib.as_array(p.stop_colors, (4 * n,)).copy() if n else np.zeros(4, np.float32)
+        self.p.stop_positions = self.pos.ctypes.data_as(C.POINTER(C.c_double))
+        self.p.stop_colors = self.col.ctypes.data_as(C.POINTER(C.c_float))
+
+    def abi(self):
+        return self.p
+
+
+class Layer:
+    """Rendered layer (src/scene.rs:463-520): `image` (H, W, 4) float32
+    premultiplied linear RGBA whose pixel (0, 0) is at (x, y)."""
+
+    def __init__(self, image, x, y):
+        self.image, self.x, self.y = image, x, y
+
+
+class Scene:
+    """Scene::fill / Scene::group (src/scene.rs:64-110). Groups render their
+    children in order, so a tree of groups renders as its fills in order."""
+
+    def __init__(self, fills):
+        self._fills = fills
+
+    @staticmethod
+    def fill(path, paint, fill_rule=FillRule.NonZero):
+        return Scene([(path, paint, int(fill_rule))])
+
+    @staticmethod
+    def group(children):
+        return Scene([f for c in children for f in c._fills])
+
+    def fills(self):
+        return list(self._fills)
+
+    def _abi_arrays(self):
+        n = len(self._fills)
+        keep = [f[0].abi() for f in self._fills]
+        paths = (C.POINTER(_abi.rz_path) * max(n, 1))(*[C.pointer(p) for p in keep])
+        pk = [f[1].abi() for f in self._fills]
+        paints = (_abi.rz_paint * max(n, 1))(*pk)
+        rules = (C.c_int32 * max(n, 1))(*[f[2] for f in self._fills])
+        return paths, paints, rules, (keep, pk)
+
+    def render(self, rasterizer, tr, view=None, bg=None, batched=True):
+        """Scene::render (src/scene.rs:186-199) -> Layer. view = (x0, y0, x1, y1)."""
+        ctx = rasterizer._ctx()
+        paths, paints, rules, keep = self._abi_arrays()
+        n = len(self._fills)
+        v = (C.c_double * 4)(*view) if view is not None else None
+        b = (C.c_float * 4)(*bg) if bg is not None else None
+        xywh = (C.c_int32 * 4)()
+        L = lib()
+        ctx.check(L.rz_render(ctx.h, paths, paints, rules, n, tr.abi(), rasterizer.flatness,
+                              rasterizer.iter_mode, v, b, int(batched), None, 0, xywh))
+        x, y, w, h = list(xywh)
+        img = np.zeros((h, w, 4), np.float32)
+        if w and h:
+            ctx.check(L.rz_render(ctx.h, paths, paints, rules, n, tr.abi(), rasterizer.flatness,
+                                  rasterizer.iter_mode, v, b, int(batched),
+                                  img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh))
+        return Layer(img, x, y)
+
+
+class DeviceScene:
+    """A scene uploaded once (rz_scene_upload) for repeated renders into device
+    memory (rz_render_device)."""
+
+    def __init__(self, scene, ctx=None):
+        self.ctx = ctx or default_context()
+        paths, paints, rules, keep = scene._abi_arrays()
+        self.n = len(scene.fills())
+        h = C.c_void_p()
+        self.ctx.check(lib().rz_scene_upload(self.ctx.h, paths, paints, rules, self.n, C.byref(h)))
+        self.h = h
+
+    def render_device(self, tr, view, d_rgba, row_stride, bg=None, batched=True,
+                      flatness=DEFAULT_FLATNESS, edge_mode=_abi.RZ_EDGE_ITER):
+        v = (C.c_double * 4)(*view)
+        b = (C.c_float * 4)(*bg) if bg is not None else None
+        self.ctx.check(lib().rz_render_device(self.ctx.h, self.h, tr.abi(), flatness, edge_mode, v, b,
+                                              int(batched), C.c_void_p(d_rgba), row_stride))
+
+    def close(self):
+        if self.h:
+            lib().rz_scene_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WorkloadSet:
+    """Owned set of generated paths (include/rz_workloads.h); scene sets also
+    carry a paint and a fill rule per path."""
+
+    def __init__(self, handle, with_paints=False):
         if not handle:
             raise RuntimeError("workload generation failed")
         self.h = handle
         L = lib()
         self.paths = []
+        self.paints = []
+        self.rules = []
         for i in range(L.rzw_count(handle)):
             p = _abi.rz_path()
             L.rzw_path(handle, i, C.byref(p))
             self.paths.append(Path.from_abi(p))
+            if with_paints:
+                q = _abi.rz_paint()
+                L.rzw_paint(handle, i, C.byref(q))
+                self.paints.append(RawPaint(q))
+                self.rules.append(int(L.rzw_rule(handle, i)))
         t = _abi.rz_transform()
         L.rzw_transform(handle, C.byref(t))
         self.transform = Transform(*t.m)
+
+    def scene(self):
+        return Scene.group([Scene.fill(p, q, r) for p, q, r in zip(self.paths, self.paints, self.rules)])
 
     def __del__(self):
         try:
@@ -421,6 +681,16 @@ def workload_cubics64(seed=1, size=512.0):
 
 def workload_random_walk(seed=2, n_segments=200_000, canvas=4096.0):
     return WorkloadSet(lib().rzw_random_walk(seed, n_segments, canvas))
+
+
+def workload_scene(seed=3, n_paths=10_000, canvas=2048.0):
+    """Batched scene: star/blob paths with solid / linear / radial paints."""
+    return WorkloadSet(lib().rzw_scene(seed, n_paths, canvas), with_paints=True)
+
+
+def workload_circles(count=1024, size=1024.0):
+    """The reference's many-circles scene benchmark (benches/scene_bench.rs)."""
+    return WorkloadSet(lib().rzw_circles(count, size), with_paints=True)
 
 
 def rnd_uniform(seed, n):

@@ -97,6 +97,18 @@ PROTOTYPES = [
     ("rz_path_free", C.c_int, [c_ctx, c_dpath]),
     ("rz_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
                                  C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    ("rz_fill", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
+                          P(rz_paint), P(C.c_float), C.c_size_t, rz_shape]),
+    ("rz_fill_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
+                                 P(rz_paint), C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    ("rz_scene_upload", C.c_int, [c_ctx, P(P(rz_path)), P(rz_paint), P(C.c_int32), C.c_size_t,
+                                  P(C.c_void_p)]),
+    ("rz_scene_free", C.c_int, [c_ctx, C.c_void_p]),
+    ("rz_render_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,
+                                   P(C.c_double), P(C.c_float), C.c_int, C.c_void_p, C.c_size_t]),
+    ("rz_render", C.c_int, [c_ctx, P(P(rz_path)), P(rz_paint), P(C.c_int32), C.c_size_t,
+                            rz_transform, C.c_double, C.c_int, P(C.c_double), P(C.c_float), C.c_int,
+                            P(C.c_float), C.c_size_t, P(C.c_int32)]),
     ("rz_set_param", C.c_int, [c_ctx, C.c_int, C.c_int]),
     ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
     ("rz_profile", C.c_int, [c_ctx, C.c_int]),
@@ -104,6 +116,10 @@ PROTOTYPES = [
     # workloads
     ("rzw_cubics64", c_set, [C.c_uint32, C.c_double]),
     ("rzw_random_walk", c_set, [C.c_uint32, C.c_uint32, C.c_double]),
+    ("rzw_scene", c_set, [C.c_uint32, C.c_uint32, C.c_double]),
+    ("rzw_circles", c_set, [C.c_uint32, C.c_double]),
+    ("rzw_paint", None, [c_set, C.c_size_t, P(rz_paint)]),
+    ("rzw_rule", C.c_int32, [c_set, C.c_size_t]),
     ("rzw_count", C.c_size_t, [c_set]),
     ("rzw_path", None, [c_set, C.c_size_t, P(rz_path)]),
     ("rzw_transform", None, [c_set, P(rz_transform)]),

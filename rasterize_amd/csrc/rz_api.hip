@@ -10,7 +10,9 @@
 #include <vector>
 
 #include "../../include/rz.h"
+#include "rz_host.hpp"
 #include "rz_kernels.hpp"
+#include "rz_scene.hpp"
 
 namespace {
 
@@ -36,6 +38,14 @@ struct rz_ctx {
   DBuf out;       // dense f32 output for the blocking API
   DBuf lines;     // f64 lines for rz_flatten
   DBuf counts;    // per-slot counts for rz_flatten
+  DBuf stops;     // gradient stops of the current fill: f64 positions, then C4 colours
+  DBuf sticky;    // status folded over the runs of a plain-form scene render
+  rzk::Status* sticky_st = nullptr;  // set while such a render is enqueued
+  // batched scene render
+  DBuf line_path;  // path of each line
+  DBuf fl_counts;  // lines per flatten thread, then per-wave totals and offsets
+  DBuf sc_boxes, sc_has, sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
+  uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   uint32_t rec_cap = 0, ev_cap = 0;
   rzk::Status* host_status = nullptr;  // pinned
   // launch parameters (rz_set_param)
@@ -58,6 +68,8 @@ struct rz_dpath {
   std::vector<void*> allocs;
   uint32_t n_order = 0;  // padded length of `order`
   uint32_t n_slots = 0;
+  bool has_bbox = false;  // Path::bbox(identity), for bounding-box paints
+  rzh::BBox bbox{};
 };
 
 namespace {
@@ -285,6 +297,8 @@ static rz_status path_upload_impl(rz_ctx* ctx, const rz_path* p, rz_dpath** out)
     delete d;
     return fail(ctx, RZ_E_HIP, std::string("upload: ") + hipGetErrorString(e));
   }
+  const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
+  d->has_bbox = rzh::path_bbox(p, ident, &d->bbox);
   *out = d;
   return RZ_OK;
 }
@@ -307,6 +321,7 @@ Grid make_grid(int mode, size_t W, size_t H) {
   r.g.tiles_x = (int)((W + rzg::kTile - 1) / rzg::kTile);
   r.g.tiles_y = (int)((H + rzg::kTile - 1) / rzg::kTile);
   r.g.wc = mode == RZ_EDGE_MASK ? (double)W - 1.0 : (double)W;
+  r.g.phx = r.g.phy = 0;
   r.tiles = (size_t)r.g.tiles_x * r.g.tiles_y;
   return r;
 }
@@ -330,6 +345,47 @@ struct Pipe {
   uint32_t* ev_off;
   int32_t* redge;  // [H + 2] right-edge row differences, then (scanned) per-row sums
 };
+
+// Flatten every slot of `d` into ctx->lines (and, for a batched scene, the
+// path of each line into ctx->line_path): count pass, scan of the per-wave
+// totals, emit pass. The total goes to st->n_lines.
+rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, double fl,
+                          const rzk::PathInfo* pi, const uint32_t* slot_path, rzk::Status* st,
+                          rzk::LineBufs* lb) {
+  const int dq = ctx->split_quad, dc = ctx->split_cubic;
+  const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
+                 bc = d->view.blocks_cubic << dc;
+  const uint32_t nblk = bl + bq + bc;
+  lb->lines = (double4*)ctx->lines.p;
+  lb->line_path = pi ? (uint32_t*)ctx->line_path.p : nullptr;
+  lb->cap = ctx->line_cap;
+  lb->count = &st->n_lines;
+  lb->st = st;
+  if (!nblk) return RZ_OK;
+  const size_t nthreads = (size_t)nblk * rzk::kBlock, nwaves = nthreads / 64;
+  RZ_TRY(ensure(ctx, ctx->fl_counts, (nthreads + 2 * (nwaves + 1)) * 4));
+  uint32_t* counts = (uint32_t*)ctx->fl_counts.p;
+  uint32_t* wave_tot = counts + nthreads;
+  uint32_t* wave_off = wave_tot + nwaves + 1;
+  if (pi) {
+    RZ_LAUNCH(ctx, "k_flat_count", rzk::k_flat_count<true>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
+              fl, bl, bq, dq, dc, pi, slot_path, counts, wave_tot, st);
+  } else {
+    RZ_LAUNCH(ctx, "k_flat_count", rzk::k_flat_count<false>, dim3(nblk), dim3(rzk::kBlock), d->view,
+              x, fl, bl, bq, dq, dc, pi, slot_path, counts, wave_tot, st);
+  }
+  RZ_TRY(scan_u32(ctx, wave_tot, wave_off, nwaves, nullptr, &st->n_lines));
+  if (pi) {
+    RZ_LAUNCH(ctx, "k_flat_emit", rzk::k_flat_emit<true>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
+              fl, bl, bq, dq, dc, pi, slot_path, (const uint32_t*)counts, (const uint32_t*)wave_off,
+              *lb);
+  } else {
+    RZ_LAUNCH(ctx, "k_flat_emit", rzk::k_flat_emit<false>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
+              fl, bl, bq, dq, dc, pi, slot_path, (const uint32_t*)counts, (const uint32_t*)wave_off,
+              *lb);
+  }
+  return check_launch(ctx, "flatten");
+}
 
 // Enqueue flatten, count pass, scans and scatter pass. Lines, records and
 // events land in ctx->lines / ctx->recs / ctx->events (capacity-checked on the
@@ -372,18 +428,8 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   for (int i = 0; i < 6; ++i) x.m[i] = tr.m[i];
   const double fl = 16.0 * flatness * flatness;
   // flatten: quad and cubic slots are split into 2^dq / 2^dc virtual subtrees
-  const int dq = ctx->split_quad, dc = ctx->split_cubic;
-  const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
-                 bc = d->view.blocks_cubic << dc;
-  const uint32_t nblk = bl + bq + bc;
   rzk::LineBufs lb;
-  lb.lines = (double4*)ctx->lines.p;
-  lb.cap = ctx->line_cap;
-  lb.count = &P.st->n_lines;
-  lb.st = P.st;
-  if (nblk)
-    RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines, dim3(nblk), dim3(rzk::kBlock), d->view, x,
-              fl, lb, bl, bq, dq, dc);
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, &lb));
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
   b.bd_diff = P.bd;
@@ -396,24 +442,31 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.st = P.st;
   const uint32_t gb = ctx->bin_blocks;
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_LAUNCH(ctx, "k_bin_lines<0>", rzk::k_bin_lines<0>, dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.count, P.grid.g, b);
-  RZ_LAUNCH(ctx, "k_gap_events<0>", rzk::k_gap_events<0>, dim3(gs), dim3(rzk::kBlock), d->view, x,
-            P.grid.g, b);
+  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, false>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)nullptr, (const uint32_t*)lb.count, P.grid.g,
+            (const rzk::PathInfo*)nullptr, b);
+  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   RZ_TRY(check_launch(ctx, "bin pass 0"));
   RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, P.tile_count, &P.st->n_records));
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
   RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
             P.grid.g.tiles_x, P.redge, (int)(H + 2));
-  RZ_LAUNCH(ctx, "k_bin_lines<1>", rzk::k_bin_lines<1>, dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.count, P.grid.g, b);
-  RZ_LAUNCH(ctx, "k_gap_events<1>", rzk::k_gap_events<1>, dim3(gs), dim3(rzk::kBlock), d->view, x,
-            P.grid.g, b);
+  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, false>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)nullptr, (const uint32_t*)lb.count, P.grid.g,
+            (const rzk::PathInfo*)nullptr, b);
+  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, false>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   return check_launch(ctx, "bin pass 1");
 }
 
-rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
-                       int rule, int mode, size_t W, size_t H, float* d_out, size_t row_stride) {
+// Enqueue the whole pipeline with the fine stage writing through `o`
+// (MaskOut or FillOut). On capacity overflow the fine kernel writes nothing,
+// so a re-run after finish() grew the buffers is exact.
+template <class Out>
+rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
+                       int rule, int mode, size_t W, size_t H, const Out& o, const char* name0,
+                       const char* name1) {
   if (W == 0 || H == 0) return RZ_OK;
   Pipe P;
   RZ_TRY(enqueue_bin(ctx, d, tr, flatness, mode, W, H, &P));
@@ -424,8 +477,6 @@ rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   a.band_ev_off = P.ev_off;
   a.events = (const int4*)ctx->events.p;
   a.st = P.st;
-  a.out = d_out;
-  a.row_stride = row_stride;
   a.W = (int)W;
   a.H = (int)H;
   a.tiles_x = P.grid.g.tiles_x;
@@ -441,14 +492,61 @@ rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   }
   const uint32_t gf = (uint32_t)P.grid.tiles;
   if (rule == RZ_NONZERO)
-    RZ_LAUNCH(ctx, "k_fine_mask<0>", rzk::k_fine_mask<0>, dim3(gf), dim3(rzk::kFineWave), a);
+    RZ_LAUNCH(ctx, name0, (rzk::k_fine<0, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
   else
-    RZ_LAUNCH(ctx, "k_fine_mask<1>", rzk::k_fine_mask<1>, dim3(gf), dim3(rzk::kFineWave), a);
-  RZ_TRY(check_launch(ctx, "k_fine_mask"));
+    RZ_LAUNCH(ctx, name1, (rzk::k_fine<1, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
+  RZ_TRY(check_launch(ctx, name0));
+  if (ctx->sticky_st) {
+    RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), P.st, ctx->sticky_st);
+    return check_launch(ctx, "k_status_merge");
+  }
   // the status block is read back at rz_sync
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, P.st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
                              ctx->stream));
   return RZ_OK;
+}
+
+rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
+                       int rule, int mode, size_t W, size_t H, float* d_out, size_t row_stride) {
+  rzk::MaskOut o;
+  o.out = d_out;
+  o.row_stride = row_stride;
+  return enqueue_fine(ctx, d, tr, flatness, rule, mode, W, H, o, "k_fine_mask<0>", "k_fine_mask<1>");
+}
+
+// Resolve the paint on the host and stage its stops in ctx->stops.
+rz_status stage_paint(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, const rz_paint* paint,
+                      rzh::PaintPrep* pp) {
+  if (!paint) return fail(ctx, RZ_E_ARG, "paint is NULL");
+  if (!rzh::prepare_paint(*paint, rzh::tr_of(tr.m), d->has_bbox ? &d->bbox : nullptr, pp))
+    return fail(ctx, RZ_E_ARG, "invalid paint");
+  const size_t n = pp->pos.size();
+  if (n) {
+    const size_t pos_bytes = (n * sizeof(double) + 15) & ~(size_t)15;
+    RZ_TRY(ensure(ctx, ctx->stops, pos_bytes + n * sizeof(rzp::C4)));
+    char* base = (char*)ctx->stops.p;
+    // pageable sources: the copies are staged before the calls return
+    RZ_HIP(ctx, hipMemcpyAsync(base, pp->pos.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                               ctx->stream));
+    RZ_HIP(ctx, hipMemcpyAsync(base + pos_bytes, pp->col.data(), n * sizeof(rzp::C4),
+                               hipMemcpyHostToDevice, ctx->stream));
+    pp->p.stop_pos = (const double*)base;
+    pp->p.stop_col = (const rzp::C4*)(base + pos_bytes);
+  }
+  return RZ_OK;
+}
+
+rz_status enqueue_fill(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
+                       int rule, int mode, const rz_paint* paint, size_t W, size_t H, float* d_rgba,
+                       size_t row_stride) {
+  rzh::PaintPrep pp;
+  RZ_TRY(stage_paint(ctx, d, tr, paint, &pp));
+  if (!pp.draw || W == 0 || H == 0) return RZ_OK;
+  rzk::FillOut o;
+  o.img = (rzp::C4*)d_rgba;
+  o.row_stride = row_stride;
+  o.paint = pp.p;
+  return enqueue_fine(ctx, d, tr, flatness, rule, mode, W, H, o, "k_fine_fill<0>", "k_fine_fill<1>");
 }
 
 // Wait, then interpret the status block: errors, and capacity growth.
@@ -476,6 +574,15 @@ rz_status finish(rz_ctx* ctx) {
     if (s.n_events > ctx->ev_cap) {
       ctx->ev_cap = s.n_events * 5 / 4 + 1024;
       RZ_TRY(ensure(ctx, ctx->events, (size_t)ctx->ev_cap * 16));
+    }
+    if (s.overflow & 8u) {
+      // a batched scene's per-tile / per-band / per-row slices (totals in pad[])
+      auto grow = [](uint32_t need, uint32_t cap) {
+        return need > cap ? (uint32_t)std::min<uint64_t>((uint64_t)need * 5 / 4 + 4096, 0xffffffffu) : cap;
+      };
+      ctx->sc_cap_t = grow(s.pad[0], ctx->sc_cap_t);
+      ctx->sc_cap_b = grow(s.pad[1], ctx->sc_cap_b);
+      ctx->sc_cap_r = grow(s.pad[2], ctx->sc_cap_r);
     }
     return fail(ctx, RZ_E_CAPACITY, "pipeline buffers grown; re-run");
   }
@@ -537,6 +644,370 @@ rz_status collect_prof(rz_ctx* ctx) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Scenes: Scene::group(fills).render(rasterizer, tr, view, bg)
+// ---------------------------------------------------------------------------
+struct rz_dscene {
+  uint32_t n = 0;
+  rz_dpath* all = nullptr;             // every path's segments, for the batched form
+  std::vector<rz_dpath*> per;          // one resident path each, for the plain form
+  std::vector<void*> allocs;           // ScenePath table, slot -> path map, stops
+  const rzk::ScenePath* sp = nullptr;
+  const uint32_t* slot_path = nullptr;
+  std::vector<rzk::ScenePath> host_sp;  // same table on the host (plain form)
+  // host copies of the paths, for Path::bbox(tr) in the plain form
+  std::vector<std::vector<uint8_t>> kinds;
+  std::vector<std::vector<uint32_t>> offs, subs;
+  std::vector<std::vector<double>> pts;
+  std::vector<std::vector<uint8_t>> closed;
+  rz_path host_path(uint32_t i) const {
+    rz_path p;
+    p.n_segments = (uint32_t)kinds[i].size();
+    p.n_subpaths = subs[i].empty() ? 0 : (uint32_t)subs[i].size() - 1;
+    p.kinds = kinds[i].data();
+    p.point_offsets = offs[i].data();
+    p.points = pts[i].data();
+    p.subpaths = subs[i].data();
+    p.closed = closed[i].data();
+    return p;
+  }
+};
+
+namespace {
+
+void scene_release(rz_dscene* s) {
+  if (s->all) {
+    dpath_release(s->all);
+    delete s->all;
+  }
+  for (rz_dpath* d : s->per) {
+    dpath_release(d);
+    delete d;
+  }
+  for (void* p : s->allocs) (void)hipFree(p);
+  s->per.clear();
+  s->allocs.clear();
+  s->all = nullptr;
+}
+
+// The layer of a view (Layer::new, src/scene.rs:483-501).
+void layer_of(const double* v, int* lx, int* ly, int* lw, int* lh) {
+  const int x0 = (int)std::floor(v[0]), x1 = (int)std::ceil(v[2]);
+  const int y0 = (int)std::floor(v[1]), y1 = (int)std::ceil(v[3]);
+  *lx = x0;
+  *ly = y0;
+  *lw = std::max(0, x1 - x0);
+  *lh = std::max(0, y1 - y0);
+}
+
+rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
+                            const int32_t* rules, size_t n, rz_dscene** out) {
+  if (!ctx || !out) return RZ_E_ARG;
+  if (n && (!paths || !paints || !rules)) return fail(ctx, RZ_E_ARG, "null scene arrays");
+  if (n > (1u << 24)) return fail(ctx, RZ_E_ARG, "too many paths");
+  rz_dscene* s = new rz_dscene();
+  s->n = (uint32_t)n;
+  struct Guard {
+    rz_dscene* s;
+    bool keep = false;
+    ~Guard() {
+      if (!keep) {
+        scene_release(s);
+        delete s;
+      }
+    }
+  } guard{s};
+  // combined path, slot -> path map, host copies, paints
+  std::vector<uint8_t> kinds;
+  std::vector<uint32_t> offs{0}, subs{0}, slot_path;
+  std::vector<double> pts;
+  std::vector<uint8_t> closed;
+  std::vector<double> stop_pos;
+  std::vector<rzp::C4> stop_col;
+  std::vector<size_t> stop_first(n);
+  s->host_sp.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const rz_path* p = paths[i];
+    RZ_TRY(validate_path(ctx, p));
+    if (rules[i] != RZ_NONZERO && rules[i] != RZ_EVENODD) return fail(ctx, RZ_E_ARG, "bad fill rule");
+    const uint32_t seg0 = (uint32_t)kinds.size(), pt0 = (uint32_t)(pts.size() / 2);
+    const uint32_t ns = p->n_segments, nsub = p->n_subpaths, npts = ns ? p->point_offsets[ns] : 0;
+    s->kinds.emplace_back(p->kinds, p->kinds + ns);
+    s->offs.emplace_back(p->point_offsets, p->point_offsets + (ns ? ns + 1 : 0));
+    s->pts.emplace_back(p->points, p->points + 2 * (size_t)npts);
+    s->subs.emplace_back(p->subpaths, p->subpaths + (nsub ? nsub + 1 : 0));
+    s->closed.emplace_back(p->closed, p->closed + nsub);
+    for (uint32_t k = 0; k < ns; ++k) {
+      kinds.push_back(p->kinds[k]);
+      offs.push_back(pt0 + p->point_offsets[k + 1]);
+    }
+    pts.insert(pts.end(), p->points, p->points + 2 * (size_t)npts);
+    for (uint32_t j = 0; j < nsub; ++j) {
+      subs.push_back(seg0 + p->subpaths[j + 1]);
+      closed.push_back(p->closed[j]);
+      for (uint32_t k = p->subpaths[j]; k <= p->subpaths[j + 1]; ++k) slot_path.push_back((uint32_t)i);
+    }
+    rzh::PaintPrep pp;
+    if (!rzh::prepare_paint_static(paints[i], &pp)) return fail(ctx, RZ_E_ARG, "invalid paint");
+    rzk::ScenePath& S = s->host_sp[i];
+    S = rzk::ScenePath{};
+    S.seg_begin = seg0;
+    S.seg_end = seg0 + ns;
+    S.rule = rules[i];
+    const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
+    S.has_bbox_i = rzh::path_bbox(p, ident, &S.bbox_i) ? 1 : 0;
+    S.units = paints[i].units;
+    for (int k = 0; k < 6; ++k) S.paint_tr[k] = paints[i].tr[k];
+    S.paint = pp.p;
+    stop_first[i] = stop_pos.size();
+    stop_pos.insert(stop_pos.end(), pp.pos.begin(), pp.pos.end());
+    stop_col.insert(stop_col.end(), pp.col.begin(), pp.col.end());
+    rz_dpath* d = nullptr;
+    RZ_TRY(path_upload_impl(ctx, p, &d));
+    s->per.push_back(d);
+  }
+  if (kinds.empty()) offs.clear();
+  rz_path all;
+  all.n_segments = (uint32_t)kinds.size();
+  all.n_subpaths = (uint32_t)closed.size();
+  all.kinds = kinds.data();
+  all.point_offsets = offs.data();
+  all.points = pts.data();
+  all.subpaths = subs.data();
+  all.closed = closed.data();
+  RZ_TRY(path_upload_impl(ctx, &all, &s->all));
+  // stops, then the ScenePath table pointing into them
+  auto up = [&](const void* src, size_t bytes, void** dst) -> rz_status {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return fail(ctx, RZ_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->allocs.push_back(p);
+    if (bytes) RZ_HIP(ctx, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *dst = p;
+    return RZ_OK;
+  };
+  void *dpos = nullptr, *dcol = nullptr, *dsp = nullptr, *dslot = nullptr;
+  RZ_TRY(up(stop_pos.data(), stop_pos.size() * sizeof(double), &dpos));
+  RZ_TRY(up(stop_col.data(), stop_col.size() * sizeof(rzp::C4), &dcol));
+  for (size_t i = 0; i < n; ++i) {
+    rzk::ScenePath& S = s->host_sp[i];
+    S.paint.stop_pos = (const double*)dpos + stop_first[i];
+    S.paint.stop_col = (const rzp::C4*)dcol + stop_first[i];
+  }
+  RZ_TRY(up(s->host_sp.data(), n * sizeof(rzk::ScenePath), &dsp));
+  RZ_TRY(up(slot_path.data(), slot_path.size() * sizeof(uint32_t), &dslot));
+  s->sp = (const rzk::ScenePath*)dsp;
+  s->slot_path = (const uint32_t*)dslot;
+  guard.keep = true;
+  *out = s;
+  return RZ_OK;
+}
+
+// The batched form: one pipeline for every path of the scene.
+rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
+                                const double* view, rzp::C4 bg, int mode, double flatness,
+                                rzp::C4* d_out, size_t row_stride) {
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  const uint32_t n = s->n;
+  if (lw == 0 || lh == 0) return RZ_OK;
+  const int ctx_x = (lw + 15) / 16, cty = (lh + 15) / 16;
+  if (n == 0) {
+    RZ_LAUNCH(ctx, "k_fill_rgba", rzk::k_fill_rgba, dim3(1024), dim3(256), d_out, lw, lh, row_stride, bg);
+    return check_launch(ctx, "k_fill_rgba");
+  }
+  const rz_dpath* d = s->all;
+  // buffers
+  if (ctx->sc_cap_t == 0) {
+    ctx->sc_cap_t = (uint32_t)std::max<size_t>((size_t)ctx_x * cty * 16, 1 << 16);
+    ctx->sc_cap_b = (uint32_t)std::max<size_t>((size_t)cty * 4 + n * 4, 1 << 12);
+    ctx->sc_cap_r = (uint32_t)std::max<size_t>((size_t)lh * 4 + n * 66, 1 << 14);
+  }
+  if (ctx->line_cap == 0) {
+    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
+  }
+  if (ctx->rec_cap == 0) {
+    ctx->rec_cap = ctx->line_cap;
+    RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
+  }
+  if (ctx->ev_cap == 0) {
+    ctx->ev_cap = 1 << 14;
+    RZ_TRY(ensure(ctx, ctx->events, (size_t)ctx->ev_cap * 16));
+  }
+  const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b, cr = ctx->sc_cap_r;
+  const size_t sw = rzk::kStatusWords;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr;
+  RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
+  RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
+  RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
+  RZ_TRY(ensure(ctx, ctx->line_path, (size_t)ctx->line_cap * 4));
+  RZ_TRY(ensure(ctx, ctx->sc_boxes, (size_t)n * sizeof(double4)));
+  RZ_TRY(ensure(ctx, ctx->sc_has, (size_t)n * 4));
+  RZ_TRY(ensure(ctx, ctx->sc_pi, (size_t)n * sizeof(rzk::PathInfo)));
+  RZ_TRY(ensure(ctx, ctx->sc_counts, (size_t)n * 12));
+  RZ_TRY(ensure(ctx, ctx->sc_rects, (size_t)n * sizeof(int4)));
+  RZ_TRY(ensure(ctx, ctx->sc_rows, (size_t)cty * n * 4));
+  RZ_TRY(ensure(ctx, ctx->sc_rowcnt, (size_t)cty * 4));
+  uint32_t* meta = (uint32_t*)ctx->meta.p;
+  rzk::Status* st = (rzk::Status*)meta;
+  uint32_t* tile_count = meta + sw;
+  int32_t* bd = (int32_t*)(meta + sw + ct);
+  uint32_t* band_ev = meta + sw + 2 * (size_t)ct;
+  int32_t* redge = (int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  uint32_t* tile_off = (uint32_t*)ctx->tile_off.p;
+  uint32_t* ev_off = (uint32_t*)ctx->ev_off.p;
+  rzk::PathInfo* pi = (rzk::PathInfo*)ctx->sc_pi.p;
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32,
+            dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)), dim3(rzk::kBlock), meta,
+            meta_u32, 0u);
+  rzk::SceneView v;
+  v.vx0 = view[0];
+  v.vy0 = view[1];
+  v.vx1 = view[2];
+  v.vy1 = view[3];
+  v.lx = lx;
+  v.ly = ly;
+  v.lw = lw;
+  v.lh = lh;
+  for (int k = 0; k < 6; ++k) v.tr[k] = tr.m[k];
+  v.mode = mode;
+  RZ_LAUNCH(ctx, "k_scene_bbox", rzk::k_scene_bbox, dim3(n), dim3(64), d->view, s->sp, v,
+            (double4*)ctx->sc_boxes.p, (int*)ctx->sc_has.p);
+  RZ_LAUNCH(ctx, "k_scene_setup", rzk::k_scene_setup, dim3(blocks_for(n)), dim3(rzk::kBlock), s->sp,
+            n, v, (const double4*)ctx->sc_boxes.p, (const int*)ctx->sc_has.p, pi,
+            (uint32_t*)ctx->sc_counts.p, (int4*)ctx->sc_rects.p);
+  RZ_LAUNCH(ctx, "k_scene_offsets", rzk::k_scene_offsets, dim3(1), dim3(rzk::kBlock), n,
+            (const uint32_t*)ctx->sc_counts.p, pi, st, ct, cb, cr);
+  // flatten every path into its sub-view
+  rzk::Xform x{};
+  const double fl = 16.0 * flatness * flatness;
+  rzk::LineBufs lb;
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
+  rzk::BinBufs b;
+  b.tile_count = tile_count;
+  b.bd_diff = bd;
+  b.band_ev = band_ev;
+  b.recs = (float4*)ctx->recs.p;
+  b.rec_cap = ctx->rec_cap;
+  b.events = (int4*)ctx->events.p;
+  b.ev_cap = ctx->ev_cap;
+  b.redge = redge;
+  b.st = st;
+  rzg::BinGrid g0{};
+  const uint32_t gb = ctx->bin_blocks, gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
+  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, true>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
+            (const rzk::PathInfo*)pi, b);
+  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  RZ_TRY(check_launch(ctx, "scene pass 0"));
+  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, tile_count, &st->n_records));
+  RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
+  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
+            bd, redge, (const rzk::Status*)st);
+  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, true>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
+            (const rzk::PathInfo*)pi, b);
+  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
+            (const int4*)ctx->sc_rects.p, n, (uint32_t*)ctx->sc_rows.p, (uint32_t*)ctx->sc_rowcnt.p,
+            (const rzk::Status*)st);
+  rzk::SceneFineArgs a;
+  a.pi = pi;
+  a.rects = (const int4*)ctx->sc_rects.p;
+  a.rows = (const uint32_t*)ctx->sc_rows.p;
+  a.row_count = (const uint32_t*)ctx->sc_rowcnt.p;
+  a.n_paths = n;
+  a.tile_off = tile_off;
+  a.recs = (const float4*)ctx->recs.p;
+  a.bd = bd;
+  a.band_ev_off = ev_off;
+  a.events = (const int4*)ctx->events.p;
+  a.redge = redge;
+  a.st = st;
+  a.bg = bg;
+  a.out = d_out;
+  a.row_stride = row_stride;
+  a.lw = lw;
+  a.lh = lh;
+  a.tiles_x = ctx_x;
+  RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
+  RZ_TRY(check_launch(ctx, "scene pass 1"));
+  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  return RZ_OK;
+}
+
+// The plain form: Pipeline::render_rec's Fill arm (src/scene.rs:410-430) run
+// path by path, each through the single-path pipeline into its sub-view.
+rz_status enqueue_scene_plain(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
+                              const double* view, rzp::C4 bg, int mode, double flatness,
+                              rzp::C4* d_out, size_t row_stride) {
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  if (lw == 0 || lh == 0) return RZ_OK;
+  RZ_LAUNCH(ctx, "k_fill_rgba", rzk::k_fill_rgba, dim3(1024), dim3(256), d_out, lw, lh, row_stride, bg);
+  RZ_TRY(ensure(ctx, ctx->sticky, sizeof(rzk::Status)));
+  RZ_HIP(ctx, hipMemsetAsync(ctx->sticky.p, 0, sizeof(rzk::Status), ctx->stream));
+  ctx->sticky_st = (rzk::Status*)ctx->sticky.p;
+  struct Unset {
+    rz_ctx* c;
+    ~Unset() { c->sticky_st = nullptr; }
+  } unset{ctx};
+  const rzh::Tr T = rzh::tr_of(tr.m);
+  for (uint32_t i = 0; i < s->n; ++i) {
+    const rz_path hp = s->host_path(i);
+    rzh::BBox b;
+    if (!rzh::path_bbox(&hp, T, &b)) continue;
+    if (view[0] > b.mx.x || b.mn.x > view[2] || view[1] > b.mx.y || b.mn.y > view[3]) continue;
+    const double bx0 = std::fmax(view[0], b.mn.x), bx1 = std::fmin(view[2], b.mx.x);
+    const double by0 = std::fmax(view[1], b.mn.y), by1 = std::fmin(view[3], b.mx.y);
+    const double fx = std::floor(bx0), fy = std::floor(by0);
+    int col_min = (int)fx - lx, col_max = (int)std::ceil(bx1) - lx + 1;
+    int row_min = (int)fy - ly, row_max = (int)std::ceil(by1) - ly + 1;
+    col_min = std::min(std::max(col_min, 0), lw);
+    col_max = std::min(std::max(col_max, col_min), lw);
+    row_min = std::min(std::max(row_min, 0), lh);
+    row_max = std::min(std::max(row_max, row_min), lh);
+    const int W = col_max - col_min, H = row_max - row_min;
+    if (W <= 0 || H <= 0) continue;
+    const rzh::Tr align = {{1.0, 0.0, -fx, 0.0, 1.0, -fy}};
+    const rzh::Tr m = rzh::tr_mul(align, T);
+    const rzk::ScenePath& S = s->host_sp[i];
+    rzk::FillOut o;
+    o.img = d_out + (size_t)row_min * row_stride + col_min;
+    o.row_stride = row_stride;
+    o.paint = S.paint;
+    if (S.paint.kind != rzp::PAINT_SOLID &&
+        !rzh::paint_pixel_transform(S.units, S.paint_tr, m, S.has_bbox_i ? &S.bbox_i : nullptr,
+                                    o.paint.ptr))
+      continue;
+    rz_transform tm;
+    for (int k = 0; k < 6; ++k) tm.m[k] = m.m[k];
+    RZ_TRY(enqueue_fine(ctx, s->per[i], tm, flatness, S.rule, mode, (size_t)W, (size_t)H, o,
+                        "k_fine_fill<0>", "k_fine_fill<1>"));
+  }
+  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->sticky_st, sizeof(rzk::Status),
+                             hipMemcpyDeviceToHost, ctx->stream));
+  return RZ_OK;
+}
+
+rz_status check_scene_args(rz_ctx* ctx, double flatness, int mode, const double* view) {
+  if (!ctx) return RZ_E_ARG;
+  if (!(flatness > 0.0) || !std::isfinite(flatness)) return fail(ctx, RZ_E_ARG, "flatness must be > 0");
+  if (mode < RZ_EDGE_MASK || mode > RZ_EDGE_NONE) return fail(ctx, RZ_E_ARG, "bad edge mode");
+  if (!view) return fail(ctx, RZ_E_ARG, "view is NULL");
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(view[k])) return fail(ctx, RZ_E_ARG, "view must be finite");
+  if (view[2] - view[0] > (double)(1 << 24) || view[3] - view[1] > (double)(1 << 24))
+    return fail(ctx, RZ_E_ARG, "view too large");
+  return RZ_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
@@ -698,6 +1169,60 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double
   return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width, height, d_out, row_stride);
 }
 
+rz_status rz_fill_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
+                         int fill_rule, int edge_mode, const rz_paint* paint, size_t width,
+                         size_t height, float* d_rgba, size_t row_stride) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, width, height));
+  if (!d || (!d_rgba && width && height)) return fail(ctx, RZ_E_ARG, "null argument");
+  if (row_stride < width) return fail(ctx, RZ_E_ARG, "row_stride < width");
+  return enqueue_fill(ctx, d, tr, flatness, fill_rule, edge_mode, paint, width, height, d_rgba,
+                      row_stride);
+}
+
+rz_status rz_fill(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
+                  int fill_rule, int edge_mode, const rz_paint* paint, float* rgba,
+                  size_t rgba_len, rz_shape shape) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, shape.width, shape.height));
+  const size_t W = shape.width, H = shape.height;
+  if (W == 0 || H == 0) return RZ_OK;
+  if (!rgba) return fail(ctx, RZ_E_ARG, "rgba is NULL");
+  if (!paint) return fail(ctx, RZ_E_ARG, "paint is NULL");
+  const size_t last = shape.start + (H - 1) * shape.row_stride + (W - 1) * shape.col_stride;
+  if (4 * last + 4 > rgba_len) return fail(ctx, RZ_E_ARG, "shape exceeds data");
+  rz_dpath* d = nullptr;
+  RZ_TRY(path_upload_impl(ctx, path, &d));
+  struct Guard {
+    rz_ctx* c;
+    rz_dpath* d;
+    ~Guard() { rz_path_free(c, d); }
+  } guard{ctx, d};
+  // the destination view goes to a dense device image and back
+  const size_t bytes = W * H * 4 * sizeof(float);
+  std::vector<float> dense(W * H * 4);
+  for (size_t y = 0; y < H; ++y)
+    for (size_t x = 0; x < W; ++x) {
+      const size_t o = 4 * (shape.start + y * shape.row_stride + x * shape.col_stride);
+      std::memcpy(&dense[4 * (y * W + x)], rgba + o, 4 * sizeof(float));
+    }
+  RZ_TRY(ensure(ctx, ctx->out, bytes));
+  RZ_HIP(ctx, hipMemcpy(ctx->out.p, dense.data(), bytes, hipMemcpyHostToDevice));
+  rz_status st = RZ_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    st = enqueue_fill(ctx, d, tr, flatness, fill_rule, edge_mode, paint, W, H, (float*)ctx->out.p, W);
+    if (st != RZ_OK) return st;
+    st = finish(ctx);
+    if (st != RZ_E_CAPACITY) break;
+  }
+  if (st != RZ_OK) return st;
+  RZ_HIP(ctx, hipMemcpy(dense.data(), ctx->out.p, bytes, hipMemcpyDeviceToHost));
+  for (size_t y = 0; y < H; ++y)
+    for (size_t x = 0; x < W; ++x) {
+      const size_t o = 4 * (shape.start + y * shape.row_stride + x * shape.col_stride);
+      std::memcpy(rgba + o, &dense[4 * (y * W + x)], 4 * sizeof(float));
+    }
+  return RZ_OK;
+}
+
 rz_status rz_mask_f32(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatness,
                       int fill_rule, int edge_mode, size_t width, size_t height, float* out,
                       size_t row_stride) {
@@ -782,6 +1307,102 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
     if (events_cap < ne) return fail(ctx, RZ_E_CAPACITY, "events buffer too small");
     RZ_HIP(ctx, hipMemcpy(events, ctx->events.p, ne * 16, hipMemcpyDeviceToHost));
   }
+  return RZ_OK;
+}
+
+rz_status rz_scene_upload(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
+                          const int32_t* fill_rules, size_t n, rz_dscene** out) {
+  return scene_upload_impl(ctx, paths, paints, fill_rules, n, out);
+}
+
+rz_status rz_scene_free(rz_ctx* ctx, rz_dscene* scene) {
+  if (!scene) return RZ_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  scene_release(scene);
+  delete scene;
+  return RZ_OK;
+}
+
+rz_status rz_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr, double flatness,
+                           int edge_mode, const double* view, const float* bg, int batched,
+                           float* d_rgba, size_t row_stride) {
+  RZ_TRY(check_scene_args(ctx, flatness, edge_mode, view));
+  if (!scene) return fail(ctx, RZ_E_ARG, "scene is NULL");
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  if (lw && lh && !d_rgba) return fail(ctx, RZ_E_ARG, "d_rgba is NULL");
+  if (row_stride < (size_t)lw) return fail(ctx, RZ_E_ARG, "row_stride < layer width");
+  const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
+  if (batched)
+    return enqueue_scene_batched(ctx, scene, tr, view, c, edge_mode, flatness, (rzp::C4*)d_rgba,
+                                 row_stride);
+  return enqueue_scene_plain(ctx, scene, tr, view, c, edge_mode, flatness, (rzp::C4*)d_rgba,
+                             row_stride);
+}
+
+rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
+                    const int32_t* fill_rules, size_t n, rz_transform tr, double flatness,
+                    int edge_mode, const double* view, const float* bg, int batched, float* rgba,
+                    size_t cap_px, int32_t* layer_xywh) {
+  if (!ctx || !layer_xywh) return RZ_E_ARG;
+  // Pipeline::build (src/scene.rs:268-357): without a view the layer is the
+  // union of the fills' bboxes; no fill in view means an empty layer
+  double v[4];
+  bool any = false;
+  const rzh::Tr T = rzh::tr_of(tr.m);
+  for (size_t i = 0; i < n; ++i) {
+    RZ_TRY(validate_path(ctx, paths ? paths[i] : nullptr));
+    rzh::BBox b;
+    if (!rzh::path_bbox(paths[i], T, &b)) continue;
+    if (view) {
+      if (view[0] > b.mx.x || b.mn.x > view[2] || view[1] > b.mx.y || b.mn.y > view[3]) continue;
+      any = true;
+      continue;
+    }
+    if (!any) {
+      v[0] = b.mn.x;
+      v[1] = b.mn.y;
+      v[2] = b.mx.x;
+      v[3] = b.mx.y;
+    } else {
+      v[0] = std::fmin(v[0], b.mn.x);
+      v[1] = std::fmin(v[1], b.mn.y);
+      v[2] = std::fmax(v[2], b.mx.x);
+      v[3] = std::fmax(v[3], b.mx.y);
+    }
+    any = true;
+  }
+  for (int k = 0; k < 4; ++k) layer_xywh[k] = 0;
+  if (!any) return RZ_OK;
+  if (view)
+    for (int k = 0; k < 4; ++k) v[k] = view[k];
+  RZ_TRY(check_scene_args(ctx, flatness, edge_mode, v));
+  int lx, ly, lw, lh;
+  layer_of(v, &lx, &ly, &lw, &lh);
+  layer_xywh[0] = lx;
+  layer_xywh[1] = ly;
+  layer_xywh[2] = lw;
+  layer_xywh[3] = lh;
+  const size_t px = (size_t)lw * lh;
+  if (!rgba || cap_px < px) return RZ_OK;
+  rz_dscene* s = nullptr;
+  RZ_TRY(scene_upload_impl(ctx, paths, paints, fill_rules, n, &s));
+  struct Guard {
+    rz_ctx* c;
+    rz_dscene* s;
+    ~Guard() { rz_scene_free(c, s); }
+  } guard{ctx, s};
+  RZ_TRY(ensure(ctx, ctx->out, std::max<size_t>(px, 1) * 16));
+  rz_status st = RZ_OK;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    st = rz_render_device(ctx, s, tr, flatness, edge_mode, v, bg, batched, (float*)ctx->out.p,
+                          (size_t)lw);
+    if (st != RZ_OK) return st;
+    st = finish(ctx);
+    if (st != RZ_E_CAPACITY) break;
+  }
+  if (st != RZ_OK) return st;
+  RZ_HIP(ctx, hipMemcpy(rgba, ctx->out.p, px * 16, hipMemcpyDeviceToHost));
   return RZ_OK;
 }
 

@@ -175,6 +175,11 @@ struct BinGrid {
   int mode;       // EdgeMode
   int W, H;       // image size in pixels
   int tiles_x, tiles_y;
+  // Grid phase: tile column k spans x in [16k - phx, 16k + 16 - phx) and band
+  // k spans y in [16k - phy, ...). Zero for a whole image; a scene path drawn
+  // into a sub-view at (ox, oy) uses (ox mod 16, oy mod 16) so that its tiles
+  // are the canvas tiles.
+  int phx, phy;
 };
 
 // x of the line (a, b) at ordinate Y, y of it at abscissa X. Both use the
@@ -261,7 +266,7 @@ RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {
 }
 
 RZ_HD int tile_col(double x, const BinGrid& g) {
-  double f = floor(x / (double)kTile);
+  double f = floor((x + (double)g.phx) / (double)kTile);
   if (!(f >= 0.0)) return 0;
   if (f > (double)(g.tiles_x - 1)) return g.tiles_x - 1;
   return (int)f;
@@ -269,9 +274,10 @@ RZ_HD int tile_col(double x, const BinGrid& g) {
 
 // One tile record: a piece of a line clipped to a 16x16 cell, in coordinates
 // relative to the cell's top-left corner. The sign bit of x0 (x1) marks that
-// the piece starts (ends) where its band piece crosses the cell's left edge
-// from the left (to the left); the fine stage then adds +V(y0) (-V(y1)) to the
-// cell's rows, V(y) being the overlap of each pixel row with [0, y].
+// the piece starts (ends) where its band piece leaves (enters) the region left
+// of the cell (x < c0; x <= 0 for tile 0); the fine stage then adds +V(y0)
+// (-V(y1)) to the cell's rows, V(y) being the overlap of each pixel row with
+// [0, y].
 struct Rec {
   float x0, y0, x1, y1;
 };
@@ -298,12 +304,12 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
   if (!(yhi >= 0.0) || !(ylo <= H)) return;  // entirely above or below the image
   const int dir = b.y > a.y ? 1 : -1;
   // bands whose closed y-range meets [ylo, yhi]
-  int ty0 = ylo <= 0.0 ? 0 : (int)ceil(ylo / (double)kTile) - 1;
+  int ty0 = ylo <= 0.0 ? 0 : (int)ceil((ylo + (double)g.phy) / (double)kTile) - 1;
   if (ty0 < 0) ty0 = 0;
-  int ty1 = (int)floor(fmin(yhi, H) / (double)kTile);
+  int ty1 = (int)floor((fmin(yhi, H) + (double)g.phy) / (double)kTile);
   if (ty1 > g.tiles_y - 1) ty1 = g.tiles_y - 1;
   for (int ty = ty0; ty <= ty1; ++ty) {
-    const double r0 = (double)(ty * kTile);
+    const double r0 = (double)(ty * kTile - g.phy);
     const double rb = fmin(r0 + (double)kTile, H);
     // band-bottom crossing (a point on y = rb counts as inside the band)
     if (ylo <= rb && rb < yhi) {
@@ -329,7 +335,7 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
     const double xlo = fmin(s.x, e.x), xhi = fmax(s.x, e.x);
     int tx0 = tile_col(xlo, g), tx1 = tile_col(xhi, g);
     for (int tx = tx0; tx <= tx1; ++tx) {
-      const double c0 = (double)(tx * kTile), c1 = c0 + (double)kTile;
+      const double c0 = (double)(tx * kTile - g.phx), c1 = c0 + (double)kTile;
       if (xhi < c0 || xlo > c1) continue;
       // Tile 0's left region is x <= 0 rather than x < 0: a line on x = 0
       // deposits its whole cover into pixel 0, exactly like a line left of
@@ -361,8 +367,10 @@ RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
       r.y0 = (float)(u.y - r0);
       r.x1 = (float)(v.x - c0);
       r.y1 = (float)(v.y - r0);
-      if (flag0) r.x0 = -0.0f;
-      if (flag1) r.x1 = -0.0f;
+      // the flag is the sign bit; the magnitude stays the piece's x, which is 0
+      // except where tile 0's left region (x <= 0) ends inside the tile (phx > 0)
+      if (flag0) r.x0 = -r.x0;
+      if (flag1) r.x1 = -r.x1;
       rec(ty * g.tiles_x + tx, r);
     }
   }
@@ -400,7 +408,7 @@ RZ_HD void redge_endpoint(double y, int sign, const BinGrid& g, AddF&& add) {
 RZ_HD bool make_event(Pt p, float sign, const BinGrid& g, Event* ev) {
   const double H = (double)g.H;
   if (!(p.y > 0.0) || !(p.y <= H)) return false;
-  int band = (int)ceil(p.y / (double)kTile) - 1;
+  int band = (int)ceil((p.y + (double)g.phy) / (double)kTile) - 1;
   if (band < 0) band = 0;
   if (band > g.tiles_y - 1) return false;
   // an endpoint on x = 0 lies in tile 0's left region: it affects every tile
@@ -408,7 +416,7 @@ RZ_HD bool make_event(Pt p, float sign, const BinGrid& g, Event* ev) {
   if (col >= g.tiles_x - 1) return false;
   ev->band = band;
   ev->col = col;
-  ev->yrel = (float)(p.y - (double)(band * kTile));
+  ev->yrel = (float)(p.y - (double)(band * kTile - g.phy));
   ev->sign = sign;
   return true;
 }

@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "rz_geom.hpp"
+#include "rz_paint.hpp"
 
 namespace rzk {
 
@@ -66,6 +67,24 @@ __device__ inline rzg::Node<NP> load_node(const DPath& p, uint32_t slot, const d
 
 struct Xform {
   double m[6];
+};
+
+// One path of a batched scene as the device sees it in one render: its
+// transform into its sub-view (align * tr, src/scene.rs:425-429), the
+// sub-view's binning grid (phased onto the canvas tile grid), the offsets of
+// its slices of the per-tile, per-band and per-row arrays, and its paint
+// resolved for the sub-view.
+struct PathInfo {
+  double m[6];
+  rzg::BinGrid g;
+  int live;      // has a non-empty sub-view and a drawable paint
+  int rule;      // FillRule
+  int ox, oy;    // sub-view origin in layer pixels
+  int tx0, ty0;  // canvas tile holding the sub-view's tile (0, 0)
+  uint32_t tile_base, band_base, redge_base;
+  int redge_col;  // right-edge column (see k_fine)
+  double redge_w;
+  rzp::DPaint paint;
 };
 
 // ---------------------------------------------------------------------------
@@ -132,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_flatten_emit(DPath p, Xform x, doubl
 // ---------------------------------------------------------------------------
 struct LineBufs {
   double4* lines;
+  uint32_t* line_path;  // batched scenes: path of each line
   uint32_t cap;
   uint32_t* count;  // total lines (device)
   Status* st;
@@ -170,60 +190,122 @@ __device__ inline uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-template <int NP>
-__device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
-                          const Xform& x, double fl, const LineBufs& lb) {
-  rzg::Node<NP> n;
-  bool walk = false, leaf = false;
-  int err = rzg::ST_OK;
-  if (active) {
-    n = load_node<NP>(p, slot, x.m);
-    walk = walk_to_subtree(&n, j, D0, fl, &leaf, &err);
+// The subtree a thread of the flatten kernels handles. Returns the node to
+// traverse (walk = true) or a single leaf owned by this thread (leaf = true).
+template <int NP, bool MULTI>
+__device__ inline void flat_subtree(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                                    const Xform& x, double fl, const PathInfo* pi,
+                                    const uint32_t* slot_path, rzg::Node<NP>* n, bool* walk,
+                                    bool* leaf, int* err, uint32_t* pid) {
+  *walk = *leaf = false;
+  *err = rzg::ST_OK;
+  *pid = 0;
+  if (MULTI && active) {
+    *pid = slot_path[slot];
+    active = pi[*pid].live != 0;
   }
+  if (active) {
+    *n = load_node<NP>(p, slot, MULTI ? pi[*pid].m : x.m);
+    *walk = walk_to_subtree(n, j, D0, fl, leaf, err);
+  }
+}
+
+// Count pass: lines per thread, and per wave their total (no atomics: the
+// wave totals are scanned into the wave's output offsets).
+template <int NP, bool MULTI>
+__device__ void flat_count_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                                const Xform& x, double fl, const PathInfo* pi,
+                                const uint32_t* slot_path, uint32_t* counts, uint32_t* wave_tot,
+                                Status* st) {
+  rzg::Node<NP> n;
+  bool walk, leaf;
+  int err;
+  uint32_t pid;
+  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
   uint32_t cnt = leaf ? 1u : 0u;
   if (walk) {
-    int r = rzg::traverse(n, fl, [&](Pt, Pt) { ++cnt; });
+    const int r = rzg::traverse(n, fl, [&](Pt, Pt) { ++cnt; });
     if (r != rzg::ST_OK) err = r;
   }
-  if (err != rzg::ST_OK) set_err(lb.st, (uint32_t)err);
-  // one atomic per wave reserves the wave's lines
+  if (err != rzg::ST_OK) {
+    set_err(st, (uint32_t)err);
+    cnt = 0;
+  }
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  counts[gid] = cnt;
   const uint32_t incl = wave_incl_scan(cnt);
-  const uint32_t total = __shfl(incl, 63, 64);
-  uint32_t base = 0;
-  if ((threadIdx.x & 63) == 63 && total) base = atomicAdd(lb.count, total);
-  base = __shfl(base, 63, 64);
-  uint32_t o = base + incl - cnt;
-  if (!cnt || err != rzg::ST_OK) return;
+  if ((threadIdx.x & 63) == 63) wave_tot[gid >> 6] = incl;
+}
+
+// Emit pass: the thread's lines at its wave's offset plus its prefix in the wave.
+template <int NP, bool MULTI>
+__device__ void flat_emit_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                               const Xform& x, double fl, const PathInfo* pi,
+                               const uint32_t* slot_path, const uint32_t* counts,
+                               const uint32_t* wave_off, const LineBufs& lb) {
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t cnt = counts[gid];
+  const uint32_t incl = wave_incl_scan(cnt);
+  if (!cnt) return;
+  uint32_t o = wave_off[gid >> 6] + incl - cnt;
   if (o + cnt > lb.cap) {
     atomicOr(&lb.st->overflow, 4u);
     return;
   }
+  rzg::Node<NP> n;
+  bool walk, leaf;
+  int err;
+  uint32_t pid;
+  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
   if (leaf) {
     lb.lines[o] = make_double4(n.p[0].x, n.p[0].y, n.p[NP - 1].x, n.p[NP - 1].y);
+    if (MULTI) lb.line_path[o] = pid;
     return;
   }
-  rzg::traverse(n, fl, [&](Pt a, Pt c) { lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y); });
+  rzg::traverse(n, fl, [&](Pt a, Pt c) {
+    if (MULTI) lb.line_path[o] = pid;
+    lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y);
+  });
 }
 
 // Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
-// cubic virtual items. Every lane of a wave reaches the wave-wide reservation.
-__global__ __launch_bounds__(kBlock) void k_flat_lines(DPath p, Xform x, double fl, LineBufs lb,
-                                                        uint32_t bl, uint32_t bq, int dq, int dc) {
-  const uint32_t blk = blockIdx.x, t = threadIdx.x;
-  if (blk < bl) {
-    const uint32_t slot = p.order[blk * kBlock + t];
-    flat_item<2>(p, slot, 0, 0, slot != ~0u, x, fl, lb);
-  } else if (blk < bl + bq) {
-    const uint32_t v = (blk - bl) * kBlock + t;
-    const uint32_t k = v >> dq;
-    const uint32_t slot = p.order[p.blocks_line * kBlock + k];
-    flat_item<3>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, lb);
-  } else {
-    const uint32_t v = (blk - bl - bq) * kBlock + t;
-    const uint32_t k = v >> dc;
-    const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + k];
-    flat_item<4>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, lb);
+// cubic virtual items. Every lane of a wave reaches the wave scans.
+#define RZ_FLAT_DISPATCH(ITEM, ...)                                                        \
+  const uint32_t blk = blockIdx.x, t = threadIdx.x;                                        \
+  if (blk < bl) {                                                                          \
+    const uint32_t slot = p.order[blk * kBlock + t];                                       \
+    ITEM<2, MULTI>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);         \
+  } else if (blk < bl + bq) {                                                              \
+    const uint32_t v = (blk - bl) * kBlock + t;                                            \
+    const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];                     \
+    ITEM<3, MULTI>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi, slot_path,  \
+                   __VA_ARGS__);                                                           \
+  } else {                                                                                 \
+    const uint32_t v = (blk - bl - bq) * kBlock + t;                                       \
+    const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];   \
+    ITEM<4, MULTI>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi, slot_path,  \
+                   __VA_ARGS__);                                                           \
   }
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_flat_count(DPath p, Xform x, double fl, uint32_t bl,
+                                                        uint32_t bq, int dq, int dc,
+                                                        const PathInfo* pi,
+                                                        const uint32_t* slot_path,
+                                                        uint32_t* counts, uint32_t* wave_tot,
+                                                        Status* st) {
+  if (st->overflow) return;  // a batched scene whose slices did not fit
+  RZ_FLAT_DISPATCH(flat_count_item, counts, wave_tot, st)
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_flat_emit(DPath p, Xform x, double fl, uint32_t bl,
+                                                       uint32_t bq, int dq, int dc,
+                                                       const PathInfo* pi, const uint32_t* slot_path,
+                                                       const uint32_t* counts,
+                                                       const uint32_t* wave_off, LineBufs lb) {
+  if (lb.st->err || lb.st->overflow) return;
+  RZ_FLAT_DISPATCH(flat_emit_item, counts, wave_off, lb)
 }
 
 // ---------------------------------------------------------------------------
@@ -244,13 +326,14 @@ struct BinBufs {
 };
 
 template <int PASS>
-__device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, Pt p, float sign) {
+__device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, uint32_t band_base, Pt p,
+                                 float sign) {
   rzg::Event ev;
   if (!rzg::make_event(p, sign, g, &ev)) return;
   if (PASS == 0) {
-    atomicAdd(&b.band_ev[ev.band], 1u);
+    atomicAdd(&b.band_ev[band_base + ev.band], 1u);
   } else {
-    uint32_t slot = atomicAdd(&b.band_ev[ev.band], 1u);
+    uint32_t slot = atomicAdd(&b.band_ev[band_base + ev.band], 1u);
     if (slot < b.ev_cap)
       b.events[slot] = make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
     else atomicOr(&b.st->overflow, 2u);
@@ -268,34 +351,48 @@ __device__ inline void put_record(const BinBufs& b, int tile, const rzg::Rec& r)
   }
 }
 
-// Records a lane keeps in registers for aggregation. A line gives two records
-// in one cell only when it is split at x = 0 inside the first column.
-constexpr int kRecBuf = 2;
-
-// One thread per line (grid-stride, lanes of a wave on consecutive lines).
-// A line whose records all fall in one tile (the common case: flattened lines
-// are a few pixels long) takes part in a wave-wide aggregation: lanes with the
-// same tile get consecutive slots from one atomic, and all of a wave's atomics
-// are in flight together. Other lines use one atomic per record. Lines beyond
-// the right clip only add to the right-edge column; two such lines on
-// neighbouring lanes that continue each other drop their shared endpoint
-// (its two terms cancel exactly).
-template <int PASS>
-__global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines, const uint32_t* n_lines,
-                                                       rzg::BinGrid g, BinBufs b) {
+// One thread per line (grid-stride, lanes of a wave on consecutive lines; the
+// next line is loaded while this one is binned). A line with at most two
+// records (the common case: flattened lines are a few pixels long and cross
+// at most one cell edge) keeps them in registers and takes part in a
+// wave-wide aggregation: the records of the wave that go to one tile get
+// consecutive slots from a single atomic, and all of a wave's atomics are in
+// flight together. Longer lines use one atomic per record. Lines beyond the
+// right clip only add to the right-edge column; two such lines on neighbouring
+// lanes that continue each other drop their shared endpoint (its two terms
+// cancel exactly).
+template <int PASS, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
+                                                       const uint32_t* line_path,
+                                                       const uint32_t* n_lines, rzg::BinGrid g0,
+                                                       const PathInfo* pi, BinBufs b) {
   if (b.st->err || (b.st->overflow & 4u)) return;
   const uint32_t n = *n_lines;
   const uint32_t stride = gridDim.x * kBlock;
   const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const double4 zero = make_double4(0, 0, 0, 0);
+  uint32_t first = blockIdx.x * kBlock + threadIdx.x;
+  double4 Lnext = first < n ? lines[first] : zero;
   // the loop bound is uniform per wave so every lane reaches the ballots
   for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
-    int nrec = 0, tile0 = -1;
-    bool same = true;
+    const double4 L = Lnext;
+    Lnext = i + stride < n ? lines[i + stride] : zero;
+    int nrec = 0, t0 = -1, t1 = -1;
     rzg::Rec r0{}, r1{};  // separate registers: no dynamic indexing
-    int cnt_fast = 0;
     bool have = i < n;
-    const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
+    // this line's path: grid and slices of the per-tile / per-band / per-row arrays
+    uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
+    rzg::BinGrid g = g0;
+    if (MULTI && have) {
+      pid = line_path[i];
+      const PathInfo& P = pi[pid];
+      g = P.g;
+      tb = P.tile_base;
+      bb = P.band_base;
+      rb0 = P.redge_base;
+    }
     const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
     if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
       set_err(b.st, rzg::ST_NAN);
@@ -307,10 +404,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines, cons
       const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
       const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
       const bool pproj = __shfl_up((int)proj, 1, 64) != 0;
+      const uint32_t npid = __shfl_down(pid, 1, 64), ppid = __shfl_up(pid, 1, 64);
       if (proj) {
-        const bool skip_end = lane < 63 && nproj && nx == p1.x && ny == p1.y;
-        const bool skip_start = lane > 0 && pproj && px == p0.x && py == p0.y;
-        auto add = [&](int k, int v) { atomicAdd(&b.redge[k], v); };
+        const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
+        const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
+        auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
         if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
         if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
       }
@@ -318,61 +416,81 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines, cons
     if (have && !proj) {
       const rzg::EffLine e = rzg::effective_line(p0, p1, g);
       auto collect = [&](int t, const rzg::Rec& r) {
-        if (nrec == 0) tile0 = t;
-        else if (t != tile0) same = false;
-        if (nrec == 0) r0 = r;
-        else r1 = r;  // only read when there are exactly two
+        if (nrec == 0) {
+          t0 = (int)tb + t;
+          r0 = r;
+        } else {
+          t1 = (int)tb + t;  // only used when there are exactly two
+          r1 = r;
+        }
         ++nrec;
       };
       auto bd = [&](int band, int col, int dir) {
-        if (PASS == 0) atomicAdd(&b.bd_diff[band * g.tiles_x + col], dir);
+        if (PASS == 0) atomicAdd(&b.bd_diff[tb + band * g.tiles_x + col], dir);
       };
       rzg::bin_piece(e.p[0], e.p[1], g, collect, bd);
       if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, collect, bd);
-      if (e.clip0) put_event<PASS>(b, g, e.p[0], -1.0f);
-      if (e.clip1) put_event<PASS>(b, g, e.n == 2 ? e.p[2] : e.p[1], 1.0f);
-      const bool fast = nrec > 0 && same && nrec <= kRecBuf;
-      if (fast) {
-        cnt_fast = nrec;
-      } else if (nrec > 0) {
-        // records in several tiles or too many to keep: regenerate them
-        auto direct = [&](int t, const rzg::Rec& r) { put_record<PASS>(b, t, r); };
+      if (e.clip0) put_event<PASS>(b, g, bb, e.p[0], -1.0f);
+      if (e.clip1) put_event<PASS>(b, g, bb, e.n == 2 ? e.p[2] : e.p[1], 1.0f);
+      if (nrec > 2) {
+        // long line: regenerate its records with one atomic each
+        auto direct = [&](int t, const rzg::Rec& r) { put_record<PASS>(b, (int)tb + t, r); };
         auto nobd = [&](int, int, int) {};
         rzg::bin_piece(e.p[0], e.p[1], g, direct, nobd);
         if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, direct, nobd);
+        nrec = 0;
       }
     }
-    // wave aggregation of the single-tile lines: group lanes by tile (the
-    // first pending lane leads), offsets from ballots (counts are 1 or 2)
-    const uint64_t below = (1ull << lane) - 1ull;
-    bool pending = cnt_fast > 0;
-    int my_leader = 0;
-    uint32_t my_off = 0, lead_total = 0;
+    // Wave aggregation of the register records: each round takes the tile of
+    // the first pending record and groups every pending record of that tile;
+    // slots within a group are ordered by (lane, record).
+    bool pend0 = nrec >= 1, pend1 = nrec == 2;
+    uint32_t tot0 = 0, tot1 = 0;    // group sizes this lane leads (for t0 / t1)
+    int ldr0 = 0, ldr1 = 0;         // lane leading the group of record 0 / 1
+    bool sel0 = false, sel1 = false;  // ... through its t1 atomic
+    uint32_t off0 = 0, off1 = 0;
     while (true) {
-      const uint64_t m = __ballot(pending);
-      if (m == 0) break;
-      const int leader = __ffsll((unsigned long long)m) - 1;
-      const int T = __builtin_amdgcn_readlane(tile0, leader);
-      const bool mine = pending && tile0 == T;
-      const uint64_t g1 = __ballot(mine), g2 = __ballot(mine && cnt_fast == 2);
-      if (lane == leader) lead_total = (uint32_t)(__popcll(g1) + __popcll(g2));
-      if (mine) {
-        my_leader = leader;
-        my_off = (uint32_t)(__popcll(g1 & below) + __popcll(g2 & below));
-        pending = false;
+      const uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
+      if ((m0 | m1) == 0) break;
+      const bool use0 = m0 != 0;
+      const int leader = __ffsll((unsigned long long)(use0 ? m0 : m1)) - 1;
+      const int T = __builtin_amdgcn_readlane(use0 ? t0 : t1, leader);
+      const bool mine0 = pend0 && t0 == T, mine1 = pend1 && t1 == T;
+      const uint64_t g0 = __ballot(mine0), g1 = __ballot(mine1);
+      const uint32_t total = (uint32_t)(__popcll(g0) + __popcll(g1));
+      if (lane == leader) {
+        if (use0) tot0 = total;
+        else tot1 = total;
+      }
+      const uint32_t off = (uint32_t)(__popcll(g0 & below) + __popcll(g1 & below));
+      if (mine0) {
+        ldr0 = leader;
+        sel0 = !use0;
+        off0 = off;
+        pend0 = false;
+      }
+      if (mine1) {
+        ldr1 = leader;
+        sel1 = !use0;
+        off1 = off + (mine0 ? 1u : 0u);
+        pend1 = false;
       }
     }
-    uint32_t s0 = 0;
-    if (lead_total) s0 = atomicAdd(&b.tile_count[tile0], lead_total);
+    uint32_t s0 = 0, s1 = 0;
+    if (tot0) s0 = atomicAdd(&b.tile_count[t0], tot0);
+    if (tot1) s1 = atomicAdd(&b.tile_count[t1], tot1);
     if (PASS == 1) {
-      const uint32_t my_slot = (uint32_t)__shfl((int)s0, my_leader, 64) + my_off;
-      if (cnt_fast > 0) {
-        if (my_slot + (uint32_t)cnt_fast <= b.rec_cap) {
-          b.recs[my_slot] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
-          if (cnt_fast > 1) b.recs[my_slot + 1] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
-        } else {
-          atomicOr(&b.st->overflow, 1u);
-        }
+      const uint32_t a0 = (uint32_t)__shfl((int)s0, ldr0, 64), b0 = (uint32_t)__shfl((int)s1, ldr0, 64);
+      const uint32_t a1 = (uint32_t)__shfl((int)s0, ldr1, 64), b1 = (uint32_t)__shfl((int)s1, ldr1, 64);
+      if (nrec >= 1) {
+        const uint32_t slot = (sel0 ? b0 : a0) + off0;
+        if (slot < b.rec_cap) b.recs[slot] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
+        else atomicOr(&b.st->overflow, 1u);
+      }
+      if (nrec == 2) {
+        const uint32_t slot = (sel1 ? b1 : a1) + off1;
+        if (slot < b.rec_cap) b.recs[slot] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
+        else atomicOr(&b.st->overflow, 1u);
       }
     }
   }
@@ -384,16 +502,28 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines, cons
 // Both events are made here; a pair that lands on the same band, column and
 // f32 row offset contributes exactly zero and is dropped. Ends created by the
 // right clip are reported by the clip itself. One thread per slot.
-template <int PASS>
-__global__ __launch_bounds__(kBlock) void k_gap_events(DPath p, Xform x, rzg::BinGrid g, BinBufs b) {
-  if (PASS == 1 && b.st->err) return;
+template <int PASS, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_gap_events(DPath p, Xform x, rzg::BinGrid g0, BinBufs b,
+                                                        const PathInfo* pi,
+                                                        const uint32_t* slot_path) {
+  if (b.st->err || b.st->overflow) return;
   const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
   if (slot >= p.n_slots || !(p.slot_flags[slot] & 1)) return;
+  rzg::BinGrid g = g0;
+  const double* m = x.m;
+  uint32_t band_base = 0;
+  if (MULTI) {
+    const PathInfo& P = pi[slot_path[slot]];
+    if (!P.live) return;
+    g = P.g;
+    m = P.m;
+    band_base = P.band_base;
+  }
   const uint32_t k = p.slot_item[slot];  // a segment (closing lines never start a gap)
   const double* qa = p.pts + 2 * (size_t)(p.pt_off[k] - 1);  // previous segment's last point
   const double* qb = p.pts + 2 * (size_t)p.pt_off[k];
-  const Pt a = rzg::tr_apply(x.m, rzg::mkpt(qa[0], qa[1]));
-  const Pt bb = rzg::tr_apply(x.m, rzg::mkpt(qb[0], qb[1]));
+  const Pt a = rzg::tr_apply(m, rzg::mkpt(qa[0], qa[1]));
+  const Pt bb = rzg::tr_apply(m, rzg::mkpt(qb[0], qb[1]));
   const bool clip = g.mode != rzg::EDGE_NONE;
   const bool ea = !(clip && a.x > g.wc), eb = !(clip && bb.x > g.wc);
   rzg::Event va, vb;
@@ -401,8 +531,29 @@ __global__ __launch_bounds__(kBlock) void k_gap_events(DPath p, Xform x, rzg::Bi
   const bool ha = ea && rzg::make_event(ap, 1.0f, g, &va);
   const bool hb = eb && rzg::make_event(bp, -1.0f, g, &vb);
   if (ha && hb && va.band == vb.band && va.col == vb.col && va.yrel == vb.yrel) return;
-  if (ha) put_event<PASS>(b, g, ap, 1.0f);
-  if (hb) put_event<PASS>(b, g, bp, -1.0f);
+  if (ha) put_event<PASS>(b, g, band_base, ap, 1.0f);
+  if (hb) put_event<PASS>(b, g, band_base, bp, -1.0f);
+}
+
+// Fold one run's status block into a sticky one (first error, overflow bits,
+// largest counts), for a sequence of runs that is checked once at the end.
+__global__ void k_status_merge(const Status* src, Status* acc) {
+  if (threadIdx.x != 0) return;
+  if (!acc->err) acc->err = src->err;
+  acc->overflow |= src->overflow;
+  acc->n_records = max(acc->n_records, src->n_records);
+  acc->n_events = max(acc->n_events, src->n_events);
+  acc->n_lines = max(acc->n_lines, src->n_lines);
+}
+
+// Fill a layer with its background colour (Layer::new, src/scene.rs:483-501).
+__global__ void k_fill_rgba(rzp::C4* out, int w, int h, size_t row_stride, rzp::C4 c) {
+  const size_t n = (size_t)w * h;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t y = i / (size_t)w, x = i - y * (size_t)w;
+    out[y * row_stride + x] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -650,12 +801,33 @@ __device__ inline float alpha_fix(int64_t w) {
 
 constexpr int kFineWave = 64;  // one wave per tile
 
+// Per-pixel outputs of the fine stage.
+// Rasterizer::mask: the coverage itself, every pixel of the image.
+struct MaskOut {
+  float* out;
+  size_t row_stride;  // floats
+  __device__ void put(int x, int y, float alpha) const { out[(size_t)y * row_stride + x] = alpha; }
+};
+// Rasterizer::fill (src/rasterize.rs:69-115): pixels that mask_iter yields
+// (alpha >= 1e-6) get dst = dst.blend_over(paint.at(centre).with_alpha(alpha)).
+struct FillOut {
+  rzp::C4* img;
+  size_t row_stride;  // pixels
+  rzp::DPaint paint;
+  __device__ void put(int x, int y, float alpha) const {
+    if (!(alpha >= 1e-6f)) return;
+    const rzp::C4 c = rzp::paint_at(paint, x, y);
+    rzp::C4* d = img + (size_t)y * row_stride + x;
+    *d = rzp::blend_over(*d, rzp::mul(c, alpha));
+  }
+};
+
 // One 16x16 tile per wave (one workgroup per tile, so the hardware balances
 // tiles of very different record counts): records and events into LDS, then
 // four passes of 16-lane DPP row scans (one pixel row per 16 lanes) and
 // coalesced stores.
-template <int RULE>
-__global__ __launch_bounds__(kFineWave) void k_fine_mask(FineArgs a) {
+template <int RULE, class Out>
+__global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];
@@ -691,7 +863,7 @@ __global__ __launch_bounds__(kFineWave) void k_fine_mask(FineArgs a) {
         const int64_t r = a.redge[y];
         w += x == a.redge_col ? (int64_t)rint((double)r * a.redge_w) : r;
       }
-      a.out[(size_t)y * a.row_stride + x] = alpha_fix<RULE>(w);
+      o.put(x, y, alpha_fix<RULE>(w));
     }
   }
 }

@@ -1,0 +1,332 @@
+// HIP kernels of the batched scene render: Scene::group(fills).render(...)
+// (src/scene.rs:186-199, 268-357, 384-459) for thousands of paths in one
+// pass. Per render:
+//   k_scene_bbox    -- Path::bbox(tr) of every path (one wave per path)
+//   k_scene_setup   -- Pipeline::build_rec's view intersection and
+//                      render_rec's sub-view and alignment per path, its
+//                      binning grid on the canvas tile grid, its paint
+//   k_scene_offsets -- slices of the per-tile / per-band / per-row arrays
+//   (flatten and binning kernels of the mask pipeline, in their batched form)
+//   k_scene_bd      -- band backdrop and right-edge prefixes per path
+//   k_scene_rows    -- per canvas tile row, the paths crossing it, in order
+//   k_scene_fine    -- per canvas tile: every path over it in order, its
+//                      coverage, paint and source-over; one store per pixel
+#pragma once
+#include "rz_host.hpp"
+#include "rz_kernels.hpp"
+
+namespace rzk {
+
+// Static per-path description, uploaded once with the scene.
+struct ScenePath {
+  uint32_t seg_begin, seg_end;  // segments in the combined path
+  int rule;
+  int has_bbox_i;               // Path::bbox(identity) exists
+  rzh::BBox bbox_i;             // for BoundingBox paints
+  int units;
+  double paint_tr[6];
+  rzp::DPaint paint;            // stops already resolved (device pointers)
+};
+
+struct SceneView {
+  double vx0, vy0, vx1, vy1;  // view
+  int lx, ly, lw, lh;          // layer origin and size (Layer::new, src/scene.rs:483-501)
+  double tr[6];
+  int mode;                    // edge mode of the rasterizer's mask_iter
+};
+
+// Path::bbox(tr) per path: a wave folds its segments' boxes. The union of
+// segment boxes does not depend on the fold order (a quad or cubic whose
+// inner control points already lie in the running box has its curve inside
+// it), so lanes fold their own segments and the wave reduces min/max.
+__global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const ScenePath* sp, SceneView v,
+                                                    double4* boxes, int* has) {
+  const uint32_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const ScenePath& S = sp[i];
+  double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+  bool any = false;
+  for (uint32_t k = S.seg_begin + lane; k < S.seg_end; k += 64) {
+    const int kind = p.kinds[k];
+    const double* q = p.pts + 2 * (size_t)p.pt_off[k];
+    Pt c[4];
+    for (int j = 0; j < kind + 2; ++j) c[j] = rzg::tr_apply(v.tr, rzg::mkpt(q[2 * j], q[2 * j + 1]));
+    const rzh::BBox b = rzh::segment_bbox(kind, c, nullptr);
+    mnx = fmin(mnx, b.mn.x);
+    mny = fmin(mny, b.mn.y);
+    mxx = fmax(mxx, b.mx.x);
+    mxy = fmax(mxy, b.mx.y);
+    any = true;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
+    mny = fmin(mny, __shfl_xor(mny, o, 64));
+    mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+    mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+  }
+  const bool any_w = __ballot(any) != 0;
+  if (lane == 0) {
+    boxes[i] = make_double4(mnx, mny, mxx, mxy);
+    has[i] = any_w ? 1 : 0;
+  }
+}
+
+// Per path: bbox ∩ view (Pipeline::build_rec, src/scene.rs:276-293), the
+// sub-view rows/cols [floor(min) - layer, ceil(max) - layer + 1) clamped to
+// the layer (src/scene.rs:413-423, src/image.rs:588-605), the alignment
+// translate(-floor(min)) (:425-428) and the paint's pixel transform for the
+// aligned fill. Writes the counts that k_scene_offsets turns into slices.
+__global__ __launch_bounds__(kBlock) void k_scene_setup(const ScenePath* sp, uint32_t n,
+                                                         SceneView v, const double4* boxes,
+                                                         const int* has, PathInfo* pi,
+                                                         uint32_t* counts, int4* rects) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const ScenePath& S = sp[i];
+  PathInfo P;
+  P.live = 0;
+  P.rule = S.rule;
+  uint32_t nt = 0, nb = 0, nr = 0;
+  int4 rect = make_int4(0, 0, 0, 0);
+  const double4 b = boxes[i];
+  // range_intersect (src/geometry.rs:677-688) of view and bbox, both axes
+  const bool hit = has[i] && !(v.vx0 > b.z || b.x > v.vx1) && !(v.vy0 > b.w || b.y > v.vy1);
+  if (hit) {
+    const double bx0 = fmax(v.vx0, b.x), bx1 = fmin(v.vx1, b.z);
+    const double by0 = fmax(v.vy0, b.y), by1 = fmin(v.vy1, b.w);
+    const double fx = floor(bx0), fy = floor(by0);
+    int col_min = (int)fx - v.lx, col_max = (int)ceil(bx1) - v.lx + 1;
+    int row_min = (int)fy - v.ly, row_max = (int)ceil(by1) - v.ly + 1;
+    col_min = min(max(col_min, 0), v.lw);
+    col_max = min(max(col_max, col_min), v.lw);
+    row_min = min(max(row_min, 0), v.lh);
+    row_max = min(max(row_max, row_min), v.lh);
+    const int W = col_max - col_min, H = row_max - row_min;
+    rzh::Tr align;
+    const double av[6] = {1.0, 0.0, -fx, 0.0, 1.0, -fy};
+    for (int k = 0; k < 6; ++k) align.m[k] = av[k];
+    const rzh::Tr m = rzh::tr_mul(align, rzh::tr_of(v.tr));
+    for (int k = 0; k < 6; ++k) P.m[k] = m.m[k];
+    P.paint = S.paint;
+    bool draw = W > 0 && H > 0;
+    if (draw && S.paint.kind != rzp::PAINT_SOLID)
+      draw = rzh::paint_pixel_transform(S.units, S.paint_tr, m, S.has_bbox_i ? &S.bbox_i : nullptr,
+                                        P.paint.ptr);
+    if (draw) {
+      P.live = 1;
+      P.ox = col_min;
+      P.oy = row_min;
+      P.g.mode = v.mode;
+      P.g.W = W;
+      P.g.H = H;
+      P.g.wc = v.mode == rzg::EDGE_MASK ? (double)W - 1.0 : (double)W;
+      P.g.phx = col_min & 15;
+      P.g.phy = row_min & 15;
+      P.g.tiles_x = (W + P.g.phx + 15) >> 4;
+      P.g.tiles_y = (H + P.g.phy + 15) >> 4;
+      P.tx0 = col_min >> 4;
+      P.ty0 = row_min >> 4;
+      if (v.mode == rzg::EDGE_NONE) {
+        P.redge_col = INT32_MAX;
+        P.redge_w = 0.0;
+      } else {
+        const double X = rzg::redge_x(P.g), fxr = floor(X);
+        P.redge_col = (int)fxr;
+        P.redge_w = 1.0 - (X - fxr);
+      }
+      nt = (uint32_t)(P.g.tiles_x * P.g.tiles_y);
+      nb = (uint32_t)P.g.tiles_y;
+      nr = (uint32_t)H + 2u;
+      rect = make_int4(P.tx0, P.ty0, P.tx0 + P.g.tiles_x, P.ty0 + P.g.tiles_y);
+    }
+  }
+  pi[i] = P;
+  counts[i] = nt;
+  counts[n + i] = nb;
+  counts[2 * n + i] = nr;
+  rects[i] = rect;
+}
+
+// Exclusive prefix of the three per-path counts (one block), totals into the
+// status block; a total beyond its buffer flags overflow (the render is then
+// re-run with grown buffers).
+__global__ __launch_bounds__(kBlock) void k_scene_offsets(uint32_t n, const uint32_t* counts,
+                                                           PathInfo* pi, Status* st, uint32_t cap_t,
+                                                           uint32_t cap_b, uint32_t cap_r) {
+  uint32_t carry[3] = {0, 0, 0};
+  for (uint32_t start = 0; start < n; start += kBlock) {
+    const uint32_t i = start + threadIdx.x;
+    uint32_t pre[3];
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t v = i < n ? counts[c * n + i] : 0u;
+      uint32_t tot;
+      pre[c] = carry[c] + block_exclusive_scan(v, &tot);
+      carry[c] += tot;
+    }
+    if (i < n) {
+      pi[i].tile_base = pre[0];
+      pi[i].band_base = pre[1];
+      pi[i].redge_base = pre[2];
+    }
+  }
+  if (threadIdx.x == 0) {
+    st->pad[0] = carry[0];
+    st->pad[1] = carry[1];
+    st->pad[2] = carry[2];
+    if (carry[0] > cap_t || carry[1] > cap_b || carry[2] > cap_r) atomicOr(&st->overflow, 8u);
+  }
+}
+
+// Band-backdrop prefix along each tile row and right-edge prefix over the
+// rows, one block per path.
+__global__ __launch_bounds__(kBlock) void k_scene_bd(const PathInfo* pi, int32_t* bd,
+                                                      int32_t* redge, const Status* st) {
+  if (st->err || st->overflow) return;
+  const PathInfo& P = pi[blockIdx.x];
+  if (!P.live) return;
+  const int tiles_x = P.g.tiles_x;
+  for (int band = 0; band <= P.g.tiles_y; ++band) {
+    const bool edge = band == P.g.tiles_y;
+    uint32_t* row = edge ? (uint32_t*)(redge + P.redge_base)
+                         : (uint32_t*)(bd + P.tile_base + (size_t)band * tiles_x);
+    const int n = edge ? P.g.H + 2 : tiles_x;
+    uint32_t carry = 0;
+    for (int start = 0; start < n; start += kBlock) {
+      const int i = start + threadIdx.x;
+      const uint32_t v = i < n ? row[i] : 0u;
+      uint32_t tot;
+      const uint32_t pre = block_exclusive_scan(v, &tot);
+      if (i < n) row[i] = carry + pre + v;
+      carry += tot;
+    }
+  }
+}
+
+// For canvas tile row `ty`, the live paths whose tiles cross it, in path
+// order: rows[ty * n + k], count in row_count[ty]. One wave per row.
+__global__ __launch_bounds__(64) void k_scene_rows(const int4* rects, uint32_t n, uint32_t* rows,
+                                                    uint32_t* row_count, const Status* st) {
+  if (st->err || st->overflow) return;
+  const int ty = blockIdx.x;
+  const int lane = threadIdx.x;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t count = 0;
+  for (uint32_t start = 0; start < n; start += 64) {
+    const uint32_t i = start + lane;
+    bool hit = false;
+    if (i < n) {
+      const int4 r = rects[i];
+      hit = r.y <= ty && ty < r.w;
+    }
+    const uint64_t m = __ballot(hit);
+    if (hit) rows[(size_t)ty * n + count + (uint32_t)__popcll(m & below)] = i;
+    count += (uint32_t)__popcll(m);
+  }
+  if (lane == 0) row_count[ty] = count;
+}
+
+struct SceneFineArgs {
+  const PathInfo* pi;
+  const int4* rects;
+  const uint32_t* rows;
+  const uint32_t* row_count;
+  uint32_t n_paths;
+  const uint32_t* tile_off;
+  const float4* recs;
+  const int32_t* bd;
+  const uint32_t* band_ev_off;
+  const int4* events;
+  const int32_t* redge;
+  const Status* st;
+  rzp::C4 bg;
+  rzp::C4* out;       // layer image
+  size_t row_stride;  // pixels
+  int lw, lh;         // layer size
+  int tiles_x;        // canvas tiles per row
+};
+
+// One canvas tile per wave. Each lane owns 4 pixels (rows 4k + lane/16,
+// column lane % 16) and keeps their colours in registers from the background
+// to the final store. Paths crossing the tile are taken in path order; each
+// one's coverage is accumulated like k_fine's (records and events into LDS,
+// DPP row scans) in its own sub-view frame, then its paint is composited
+// source-over where mask_iter would yield a pixel (alpha >= 1e-6).
+__global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
+  if (a.st->err || a.st->overflow) return;
+  __shared__ int acc[256];
+  __shared__ int rowd[18];
+  const int lane = threadIdx.x;
+  const int TX = (int)(blockIdx.x % (uint32_t)a.tiles_x), TY = (int)(blockIdx.x / (uint32_t)a.tiles_x);
+  const int col = lane & 15;
+  rzp::C4 c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = a.bg;
+  const uint32_t nrow = a.row_count[TY];
+  const uint32_t* row = a.rows + (size_t)TY * a.n_paths;
+  for (uint32_t start = 0; start < nrow; start += 64) {
+    const uint32_t j = start + lane;
+    uint32_t pid = 0;
+    bool hit = false;
+    if (j < nrow) {
+      pid = row[j];
+      const int4 r = a.rects[pid];
+      hit = r.x <= TX && TX < r.z;
+    }
+    uint64_t m = __ballot(hit);
+    while (m) {
+      const int src = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, src);
+      const PathInfo& P = a.pi[q];
+      const int txl = TX - P.tx0, tyl = TY - P.ty0;
+      const uint32_t t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
+      const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
+      const uint32_t eb = a.band_ev_off[P.band_base + tyl], ee = a.band_ev_off[P.band_base + tyl + 1];
+      const bool local = rb != re || eb != ee;
+      if (local) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
+        if (lane < 18) rowd[lane] = 0;
+        __syncthreads();
+        for (uint32_t i = rb + lane; i < re; i += 64) fine_record(a.recs[i], acc, rowd);
+        for (uint32_t i = eb + lane; i < ee; i += 64) {
+          const int4 ev = a.events[i];
+          if (ev.y < txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+        }
+        __syncthreads();
+        if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
+        __syncthreads();
+      }
+      const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
+      const int xs = txl * 16 - P.g.phx + col;  // sub-view pixel column
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = 4 * k + (lane >> 4);
+        int64_t w = bd;
+        if (local) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
+        const int ys = tyl * 16 - P.g.phy + r;
+        if (xs >= 0 && xs < P.g.W && ys >= 0 && ys < P.g.H) {
+          if (xs >= P.redge_col) {
+            const int64_t rr = a.redge[P.redge_base + ys];
+            w += xs == P.redge_col ? (int64_t)rint((double)rr * P.redge_w) : rr;
+          }
+          const float alpha = P.rule == 0 ? alpha_fix<0>(w) : alpha_fix<1>(w);
+          if (alpha >= 1e-6f) {
+            const rzp::C4 pc = rzp::paint_at(P.paint, xs, ys);
+            c[k] = rzp::blend_over(c[k], rzp::mul(pc, alpha));
+          }
+        }
+      }
+      if (local) __syncthreads();  // LDS is reused by the next path
+    }
+  }
+  const int x = TX * 16 + col;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int y = TY * 16 + 4 * k + (lane >> 4);
+    if (x < a.lw && y < a.lh) a.out[(size_t)y * a.row_stride + x] = c[k];
+  }
+}
+
+}  // namespace rzk

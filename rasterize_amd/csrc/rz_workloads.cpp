@@ -1,5 +1,6 @@
 // Seeded synthetic inputs (include/rz_workloads.h) built with the reference's
 // LCG (benches/scene_bench.rs:51-112) and PathBuilder semantics.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -46,9 +47,35 @@ struct Rnd {
 
 }  // namespace
 
+namespace {
+
+// A paint with its own stop storage (rz_paint points into it).
+struct PaintStore {
+  rz_paint p{};
+  std::vector<double> pos;
+  std::vector<float> col;
+};
+
+// LinColor::from(RGBA) (src/color.rs:394-406, exact powf conversion)
+float srgb_to_linear(float v) {
+  if (v <= 0.04045f) return v / 12.92f;
+  return std::pow((v + 0.055f) / 1.055f, 2.4f);
+}
+void rgba_to_lin(uint8_t r, uint8_t g, uint8_t b, uint8_t a8, float* out) {
+  const float a = (float)a8 / 255.0f;
+  out[0] = srgb_to_linear((float)r / 255.0f) * a;
+  out[1] = srgb_to_linear((float)g / 255.0f) * a;
+  out[2] = srgb_to_linear((float)b / 255.0f) * a;
+  out[3] = a;
+}
+
+}  // namespace
+
 struct rzw_set {
   std::vector<rz::Path> paths;
   rz::Transform tr;
+  std::vector<std::unique_ptr<PaintStore>> paints;  // scenes: one per path
+  std::vector<int32_t> rules;
 };
 
 extern "C" {
@@ -121,6 +148,120 @@ rzw_set* rzw_random_walk(uint32_t seed, uint32_t n_segments, double canvas) {
               .pre_translate(-cx, -cy);
   return s;
 }
+
+rzw_set* rzw_scene(uint32_t seed, uint32_t n_paths, double canvas) {
+  Rnd rnd(seed);
+  rzw_set* s = new rzw_set();
+  const double lmin = std::log(8.0), lmax = std::log(1024.0);
+  auto rand_color = [&](float* out) {
+    const uint8_t r = (uint8_t)(rnd.uniform_u32() % 256), g = (uint8_t)(rnd.uniform_u32() % 256);
+    const uint8_t b = (uint8_t)(rnd.uniform_u32() % 256);
+    const uint8_t a = (uint8_t)(255.0 * rnd.uniform());
+    rgba_to_lin(r, g, b, a, out);
+  };
+  for (uint32_t i = 0; i < n_paths; ++i) {
+    // a star/blob: vertices at increasing angles, jagged radii, every edge a
+    // random line, quad or cubic
+    const double side = std::exp(lmin + (lmax - lmin) * rnd.uniform());
+    const rz::Point c = rnd.point(canvas);
+    const double g = rnd.gauss();
+    long nseg = std::lround(std::exp(std::log(60.0) + 1.4 * g));
+    nseg = nseg < 4 ? 4 : (nseg > 4000 ? 4000 : nseg);
+    const double R = 0.5 * side;
+    std::vector<double> th(nseg), rr(nseg);
+    for (long k = 0; k < nseg; ++k) {
+      th[k] = 2.0 * rz::PI * ((double)k + 0.3 * rnd.uniform()) / (double)nseg;
+      rr[k] = R * (0.55 + 0.45 * rnd.uniform());
+    }
+    auto at = [&](double a, double r) { return rz::Point(c.x + r * std::cos(a), c.y + r * std::sin(a)); };
+    rz::PathBuilder pb;
+    pb.move_to(at(th[0], rr[0]));
+    for (long k = 0; k < nseg; ++k) {
+      const long k1 = (k + 1) % nseg;
+      const double a0 = th[k], a1 = k1 ? th[k1] : th[0] + 2.0 * rz::PI;
+      const rz::Point next = at(th[k1], rr[k1]);
+      const double u = rnd.uniform();
+      if (u < 1.0 / 3.0) {
+        pb.line_to(next);
+      } else if (u < 2.0 / 3.0) {
+        pb.quad_to(at(0.5 * (a0 + a1), R * (0.3 + rnd.uniform())), next);
+      } else {
+        const rz::Point c1 = at(a0 + (a1 - a0) / 3.0, R * (0.3 + rnd.uniform()));
+        const rz::Point c2 = at(a0 + 2.0 * (a1 - a0) / 3.0, R * (0.3 + rnd.uniform()));
+        pb.cubic_to(c1, c2, next);
+      }
+    }
+    pb.close();
+    s->paths.push_back(pb.build());
+    // paint: 60% solid, 25% linear, 15% radial (UserSpaceOnUse, Pad, sRGB stops)
+    auto ps = std::make_unique<PaintStore>();
+    rz_paint& p = ps->p;
+    const double kind = rnd.uniform();
+    const double ident[6] = {1, 0, 0, 0, 1, 0};
+    for (int k = 0; k < 6; ++k) p.tr[k] = ident[k];
+    p.units = RZ_UNITS_USER_SPACE;
+    p.spread = RZ_SPREAD_PAD;
+    p.linear_colors = 0;
+    if (kind < 0.6) {
+      p.kind = RZ_PAINT_SOLID;
+      rand_color(p.color);
+    } else {
+      const int nst = 2 + (int)(rnd.uniform_u32() % 5);
+      std::vector<double> pos(nst);
+      for (int k = 0; k < nst; ++k) pos[k] = rnd.uniform();
+      std::sort(pos.begin(), pos.end());
+      for (int k = 0; k < nst; ++k) {
+        float col[4];
+        rand_color(col);
+        ps->pos.push_back(pos[k]);
+        ps->col.insert(ps->col.end(), col, col + 4);
+      }
+      p.n_stops = (uint32_t)nst;
+      if (kind < 0.85) {
+        p.kind = RZ_PAINT_LINEAR;
+        p.start[0] = c.x + R * (2.0 * rnd.uniform() - 1.0);
+        p.start[1] = c.y + R * (2.0 * rnd.uniform() - 1.0);
+        p.end[0] = c.x + R * (2.0 * rnd.uniform() - 1.0);
+        p.end[1] = c.y + R * (2.0 * rnd.uniform() - 1.0);
+      } else {
+        p.kind = RZ_PAINT_RADIAL;
+        p.center[0] = p.fcenter[0] = c.x + 0.3 * R * (2.0 * rnd.uniform() - 1.0);
+        p.center[1] = p.fcenter[1] = c.y + 0.3 * R * (2.0 * rnd.uniform() - 1.0);
+        p.radius = R * (0.2 + 0.8 * rnd.uniform());
+        p.fradius = 0.0;
+      }
+    }
+    p.stop_positions = ps->pos.empty() ? nullptr : ps->pos.data();
+    p.stop_colors = ps->col.empty() ? nullptr : ps->col.data();
+    s->paints.push_back(std::move(ps));
+    s->rules.push_back(rnd.uniform() < 0.8 ? RZ_NONZERO : RZ_EVENODD);
+  }
+  return s;
+}
+
+rzw_set* rzw_circles(uint32_t count, double size) {
+  Rnd rnd(0);
+  rzw_set* s = new rzw_set();
+  for (uint32_t i = 0; i < count; ++i) {
+    const double x = rnd.uniform() * size, y = rnd.uniform() * size;
+    const double r = rnd.uniform() * 10.0 + 30.0;
+    rz::PathBuilder pb;
+    pb.move_to(rz::Point(x, y)).circle(r);
+    s->paths.push_back(pb.build());
+    auto ps = std::make_unique<PaintStore>();
+    ps->p.kind = RZ_PAINT_SOLID;
+    const uint8_t cr = (uint8_t)(rnd.uniform_u32() % 256), cg = (uint8_t)(rnd.uniform_u32() % 256);
+    const uint8_t cb = (uint8_t)(rnd.uniform_u32() % 256);
+    rgba_to_lin(cr, cg, cb, 255, ps->p.color);
+    s->paints.push_back(std::move(ps));
+    s->rules.push_back(RZ_NONZERO);
+  }
+  return s;
+}
+
+void rzw_paint(const rzw_set* set, size_t i, rz_paint* out) { *out = set->paints[i]->p; }
+
+int32_t rzw_rule(const rzw_set* set, size_t i) { return set->rules[i]; }
 
 size_t rzw_count(const rzw_set* set) { return set ? set->paths.size() : 0; }
 

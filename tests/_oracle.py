@@ -70,9 +70,13 @@ def lib():
             "orc_path_bbox": (i, [P(orc_path), P(d), P(d)]),
             "orc_time_sd_mask": (d, [P(orc_path), P(d), d, i, sz, sz, i, i]),
             "orc_time_flatten": (d, [P(orc_path), P(d), d, i, P(sz)]),
+            "orc_fill": (i, [P(orc_path), P(d), d, i, P(orc_paint), P(C.c_float), sz, sz, sz, sz,
+                             sz]),
+            "orc_scene_render": (i, [sz, P(orc_path), P(orc_paint), P(C.c_int32), P(d), P(d),
+                                     P(C.c_float), d, P(C.c_float), sz, P(C.c_int64)]),
             "orc_bin_replay": (i, [P(orc_path), P(d), d, i, sz, sz, P(C.c_uint32), P(C.c_float),
                                    sz, P(C.c_int32), P(C.c_float), sz, P(C.c_int32), P(sz),
-                                   P(sz)]),
+                                   P(sz), i, i]),
         }
         for name, (res, args) in protos.items():
             f = getattr(_lib, name)
@@ -188,15 +192,17 @@ def arc_to(src, radii, rot, large, sweep, dst):
     return n, out
 
 
-def bin_replay(path, tr, W, H, mode, flatness=0.05):
+def bin_replay(path, tr, W, H, mode, flatness=0.05, phase=(0, 0)):
     """Sequential replay of the binning stage: (tile_counts, records[n,4] f32,
-    backdrop[tiles], events[n,4], right_edge[H] int32)."""
+    backdrop[tiles], events[n,4], right_edge[H] int32). `phase` = (phx, phy)
+    shifts the tile grid (tile column k spans x in [16k - phx, 16k + 16 - phx))."""
     p, t = opath(path), _tr(tr)
     nr, ne = C.c_size_t(), C.c_size_t()
     L = lib()
+    phx, phy = phase
     _check(L.orc_bin_replay(C.byref(p), _dp(t), flatness, mode, W, H, None, None, 0, None, None,
-                            0, None, C.byref(nr), C.byref(ne)))
-    tiles = ((W + 15) // 16) * ((H + 15) // 16)
+                            0, None, C.byref(nr), C.byref(ne), phx, phy))
+    tiles = ((W + phx + 15) // 16) * ((H + phy + 15) // 16)
     counts = np.zeros(tiles, np.uint32)
     recs = np.zeros((max(nr.value, 1), 4), np.float32)
     bd = np.zeros(tiles, np.int32)
@@ -207,5 +213,47 @@ def bin_replay(path, tr, W, H, mode, flatness=0.05):
                             recs.ctypes.data_as(C.POINTER(C.c_float)), recs.shape[0],
                             bd.ctypes.data_as(C.POINTER(C.c_int32)),
                             ev.ctypes.data_as(C.POINTER(C.c_float)), ev.shape[0],
-                            redge.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nr), C.byref(ne)))
+                            redge.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(nr), C.byref(ne),
+                            phx, phy))
     return counts, recs[: nr.value], bd, ev[: ne.value], redge[:H]
+
+
+def fill(path, tr, paint_abi, rule, img, flatness=0.05):
+    """SignedDifferenceRasterizer::fill into a copy of `img` ((H, W, 4) float32,
+    premultiplied linear RGBA). `paint_abi` is an rz_paint (same layout as orc_paint)."""
+    out = np.ascontiguousarray(img, dtype=np.float32).copy()
+    H, W = out.shape[:2]
+    p, t = opath(path), _tr(tr)
+    op = orc_paint.from_buffer_copy(paint_abi)
+    _check(lib().orc_fill(C.byref(p), _dp(t), flatness, rule, C.byref(op),
+                          out.ctypes.data_as(C.POINTER(C.c_float)), W * H, W, H, 0, W))
+    return out
+
+
+def paint_at(paint_abi, x, y):
+    op = orc_paint.from_buffer_copy(paint_abi)
+    out = (C.c_float * 4)()
+    lib().orc_paint_at(C.byref(op), x, y, out)
+    return np.array(out, dtype=np.float32)
+
+
+def scene_render(fills, tr, view=None, bg=None, flatness=0.05):
+    """Scene::group(fills).render(SignedDifferenceRasterizer, tr, view, bg):
+    (image (H, W, 4) float32, (x, y)). `fills` = [(path, paint_abi, rule)]."""
+    n = len(fills)
+    paths = (orc_path * max(n, 1))(*[opath(f[0]) for f in fills])
+    paints = (orc_paint * max(n, 1))(*[orc_paint.from_buffer_copy(f[1]) for f in fills])
+    rules = (C.c_int32 * max(n, 1))(*[int(f[2]) for f in fills])
+    t = _tr(tr)
+    v = (C.c_double * 4)(*view) if view is not None else None
+    b = (C.c_float * 4)(*bg) if bg is not None else None
+    xywh = (C.c_int64 * 4)()
+    rc = lib().orc_scene_render(n, paths, paints, rules, _dp(t), v, b, flatness, None, 0, xywh)
+    x, y, w, h = list(xywh)
+    img = np.zeros((h, w, 4), np.float32)
+    if w and h:
+        _check(lib().orc_scene_render(n, paths, paints, rules, _dp(t), v, b, flatness,
+                                      img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh))
+    elif rc not in (0, 2):
+        _check(rc)
+    return img, (x, y)

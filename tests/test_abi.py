@@ -95,3 +95,32 @@ def test_sin_cos_is_glibc_sincos():
         s, c = C.c_double(), C.c_double()
         libm.sincos(a, C.byref(s), C.byref(c))
         assert rz.sin_cos(a) == (s.value, c.value)
+
+
+def test_lincolor_from_rgba8_matches_oracle():
+    """LinColor::from(RGBA) (src/color.rs:394-406) on the host, bit-exact."""
+    import ctypes as C
+
+    import _oracle as O
+    from rasterize_amd import LinColor
+    rng = np.random.default_rng(0)
+    for rgba in [(0x89, 0x15, 0x51, 0x80), (255, 255, 255, 255), (0, 0, 0, 0), (10, 11, 12, 13)] + \
+            [tuple(int(v) for v in rng.integers(0, 256, 4)) for _ in range(50)]:
+        out = (C.c_float * 4)()
+        O.lib().orc_rgba_to_lin((C.c_uint8 * 4)(*rgba), out)
+        assert list(LinColor.from_rgba8(*rgba)) == list(out), rgba
+
+
+def test_paint_abi_layout_matches_oracle():
+    """rz_paint and the oracle's paint struct share one layout, so a paint
+    built by the package evaluates identically in the oracle."""
+    import ctypes as C
+
+    import _oracle as O
+    from rasterize_amd import GradRadial, LinColor, _abi
+    assert C.sizeof(_abi.rz_paint) == C.sizeof(O.orc_paint)
+    for (n1, t1), (n2, t2) in zip(_abi.rz_paint._fields_, O.orc_paint._fields_):
+        assert n1 == n2 and getattr(_abi.rz_paint, n1).offset == getattr(O.orc_paint, n2).offset
+    g = GradRadial([(0.0, LinColor(1, 0, 0, 1)), (1.0, LinColor(0, 0, 1, 1))], center=(0.5, 0.0),
+                   radius=0.5, fcenter=(0.25, 0.0), fradius=0.1, linear_colors=True)
+    assert np.allclose(O.paint_at(g.abi(), 0.675, 0.0), [0.5, 0.0, 0.5, 1.0], atol=1e-6)
