@@ -1,10 +1,13 @@
 """Benchmark of the MI355X coverage rasterizer.
 
-Workload (one GPU): Rasterizer::mask of one path of 200k segments (40% lines,
-30% quads, 25% cubics, 5% elliptic arcs; seeded random walk, random affine)
-into a 4096x4096 f32 mask with SignedDifferenceRasterizer semantics, nonzero
-fill. One step = the whole pipeline (flatten -> bin -> fine) with the path
-already resident in HBM. Prints one JSON line.
+Headline workload (one GPU): Rasterizer::mask of one path of 200k segments
+(40% lines, 30% quads, 25% cubics, 5% elliptic arcs; seeded random walk,
+random affine) into a 4096x4096 f32 mask with SignedDifferenceRasterizer
+semantics, nonzero fill. One step = the whole pipeline (flatten -> bin ->
+fine) with the path already resident in HBM. Also measured on the same GPU
+and reported in the same JSON line: the batched 10k-path scene render next to
+its plain one-pipeline-per-path form, and the reference's own many-circles
+scene benchmark. Prints one JSON line.
 """
 import argparse
 import ctypes as C
@@ -17,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_mask_c2_4096.json")
 
 
 def parse():
@@ -26,6 +30,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rule", choices=["nonzero", "evenodd"], default="nonzero")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-scene", action="store_true", help="skip the scene measurements")
+    ap.add_argument("--scene-only", action="store_true",
+                    help="only the scene measurements (for per-kernel profiles of them)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer 3 times (calibrates HBM counters)")
     ap.add_argument("--cpu-reps", type=int, default=4)
     return ap.parse_args()
 
@@ -49,43 +58,121 @@ def copy_bandwidth_gbps(torch):
     return 2.0 * n * 4 / (ms * 1e-3) / 1e9
 
 
+def kernel_stats(lib, _abi, ctx, run, reps):
+    """Per-kernel (launches, device ms) per run from events on the context's stream."""
+    lib.rz_profile(ctx.h, 1)
+    for _ in range(reps):
+        run()
+    stats = (_abi.rz_kernel_stat * 64)()
+    n = C.c_size_t()
+    ctx.check(lib.rz_profile_stats(ctx.h, stats, 64, C.byref(n)))
+    lib.rz_profile(ctx.h, 0)
+    return {stats[i].name.decode(): (stats[i].calls / reps, stats[i].total_ms / reps)
+            for i in range(n.value)}
+
+
+def settle(lib, _abi, ctx, run):
+    """Run until the context's buffers fit (a first run may grow them)."""
+    for _ in range(4):
+        run()
+        s = lib.rz_sync(ctx.h)
+        if s != _abi.RZ_E_CAPACITY:
+            break
+    ctx.check(s)
+
+
+def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
+    """Batched and plain Scene::render of a generated scene into a W x H layer."""
+    ds = rz.DeviceScene(ws.scene())
+    ctx = ds.ctx
+    out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    view = (0.0, 0.0, float(W), float(H))
+    stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
+    ident = rz.Transform.identity()
+    res = {}
+    for batched, frames in ((True, frames_b), (False, frames_p)):
+        def run():
+            ds.render_device(ident, view, out.data_ptr(), W, batched=batched)
+        settle(lib, _abi, ctx, run)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(frames):
+            run()
+        b.record(stream)
+        ctx.check(lib.rz_sync(ctx.h))
+        torch.cuda.synchronize()
+        res[batched] = a.elapsed_time(b) / frames
+    k = kernel_stats(lib, _abi, ctx, lambda: ds.render_device(ident, view, out.data_ptr(), W), 3)
+    ctx.check(lib.rz_sync(ctx.h))
+    ds.close()
+    segs = sum(p.segments_count() for p in ws.paths)
+    return {
+        "workload": name,
+        "paths": len(ws.paths),
+        "segments": segs,
+        "layer": [W, H],
+        "batched_ms_per_frame": round(res[True], 4),
+        "plain_ms_per_frame": round(res[False], 3),
+        "batched_over_plain": round(res[False] / res[True], 1),
+        "batched_mpix_s": round(W * H / res[True] / 1e3, 1),
+        "batched_msegs_s": round(segs / res[True] / 1e3, 2),
+        "batched_kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
                                    "only (bands of tile rows per GPU), --gpus must be 1"}))
         sys.exit(2)
-
     import numpy as np
     import torch
 
     import rasterize_amd as rz
     from rasterize_amd import _abi
 
+    if args.copy_calibration:
+        n = 64 << 20  # 256 MiB of f32
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n}))
+        return
+    lib = _abi.lib()
+    if args.scene_only:
+        scenes = [
+            scene_measure(torch, rz, lib, _abi, "scene-c3", rz.workload_scene(3, 10_000, 2048.0),
+                          2048, 2048, 5, 1),
+            scene_measure(torch, rz, lib, _abi, "circles", rz.workload_circles(1024, 1024.0),
+                          1024, 1024, 5, 1),
+        ]
+        print(json.dumps({"scenes": scenes}))
+        return
+
     W = H = 4096
     rule = rz.FillRule.NonZero if args.rule == "nonzero" else rz.FillRule.EvenOdd
     ws = rz.workload_random_walk(2, 200_000, 4096.0)
     path, tr = ws.paths[0], ws.transform
     n_seg = path.segments_count()
-    lib = _abi.lib()
+    n_pts = int(path.point_offsets[-1])
     ctx = rz.Context(0)
     dpath = C.c_void_p()
     ctx.check(lib.rz_path_upload(ctx.h, C.byref(path.abi()), C.byref(dpath)))
     out = torch.empty((H, W), dtype=torch.float32, device="cuda")
-    n_lines = len(rz.flatten(path, tr, 0.05, True, ctx=ctx))
 
     def step():
         ctx.check(lib.rz_mask_device(ctx.h, dpath, tr.abi(), 0.05, rule, _abi.RZ_EDGE_MASK, W, H,
                                      C.c_void_p(out.data_ptr()), W))
 
     # warmup (sizes the context's buffers; a capacity miss is re-run)
-    for i in range(max(1, args.warmup)):
-        for attempt in range(4):
-            step()
-            s = lib.rz_sync(ctx.h)
-            if s != _abi.RZ_E_CAPACITY:
-                break
-        ctx.check(s)
+    for _ in range(max(1, args.warmup)):
+        settle(lib, _abi, ctx, step)
+    nl, nr, ne = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    ctx.check(lib.rz_last_counts(ctx.h, C.byref(nl), C.byref(nr), C.byref(ne)))
+    n_lines, n_recs = nl.value, nr.value
 
     stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
     torch.cuda.synchronize()
@@ -103,35 +190,51 @@ def main():
     ms_step = ms_total / args.steps
 
     # per-kernel device time over a few profiled steps (events on the stream)
-    lib.rz_profile(ctx.h, 1)
-    prof_steps = min(10, args.steps)
-    for _ in range(prof_steps):
-        step()
-    stats = (_abi.rz_kernel_stat * 32)()
-    n = C.c_size_t()
-    ctx.check(lib.rz_profile_stats(ctx.h, stats, 32, C.byref(n)))
-    lib.rz_profile(ctx.h, 0)
-    kernels = {stats[i].name.decode(): (stats[i].calls, stats[i].total_ms) for i in range(n.value)}
-    per_step = {k: v[1] / prof_steps for k, v in kernels.items()}
+    kern = kernel_stats(lib, _abi, ctx, step, min(10, args.steps))
+    ctx.check(lib.rz_sync(ctx.h))
+    per_step = {k: v[1] for k, v in kern.items()}
     dom = max(per_step, key=per_step.get)
-    dom_ms = kernels[dom][1] / kernels[dom][0]
+    dom_ms = per_step[dom] / kern[dom][0]
 
-    # algorithmic bytes per launch
-    npts = int(path.point_offsets[-1])
-    flatten_bytes = 16 * npts + 4 * n_seg + 16 * n_lines
-    mask_bytes = 4 * W * H + 16 * n_lines
-    if dom.startswith("k_fine"):
-        alg = mask_bytes
-        unit_note = "4*W*H (f32 mask write) + 16*N_lines (line read)"
-    else:
-        alg = flatten_bytes
-        unit_note = "16*control points + 4*segments (read) + 16*N_lines (written)"
-    achieved = alg / (dom_ms * 1e-3) / 1e9
+    # algorithmic bytes per launch: inputs read once, outputs written once
+    seg_in = 16 * n_pts + 4 * n_seg
+    fine_bytes = (4 * W * H + 16 * n_recs, "4*W*H (f32 mask written) + 16*cell records (read)")
+    alg_def = {
+        "k_flat_count": (seg_in, "16*control points + 4*segments (read)"),
+        "k_flat_emit": (seg_in + 32 * n_lines,
+                        "16*control points + 4*segments (read) + 32*lines (f64, written)"),
+        "k_bin_lines<0>": (32 * n_lines, "32*lines (f64, read)"),
+        "k_bin_lines<1>": (32 * n_lines + 16 * n_recs, "32*lines (read) + 16*cell records (written)"),
+        "k_fine_mask<0>": fine_bytes,
+        "k_fine_mask<1>": fine_bytes,
+    }
+    alg, alg_note = alg_def.get(dom, (0, "n/a"))
+    achieved = alg / (dom_ms * 1e-3) / 1e9 if alg else 0.0
+    per_kernel_gbs = {k: round(alg_def[k][0] / (per_step[k] / kern[k][0] * 1e-3) / 1e9, 1)
+                      for k in per_step if k in alg_def}
+
+    traffic = None
+    traffic_src = None
+    if os.path.exists(TRAFFIC_FILE):
+        tf = json.load(open(TRAFFIC_FILE))
+        if dom in tf.get("kernels", {}):
+            traffic = tf["kernels"][dom]["hbm_bytes"]
+            traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT)
 
     copy_bw = copy_bandwidth_gbps(torch)
 
     mpix = W * H * args.steps / (ms_total * 1e-3) / 1e6
     msegs = n_seg * args.steps / (ms_total * 1e-3) / 1e6
+
+    scenes = None
+    if not args.no_scene:
+        scenes = [
+            scene_measure(torch, rz, lib, _abi, "batched scene: 10,000 star/blob paths, 60% solid / "
+                          "25% linear / 15% radial paint, 2048x2048 RGBA f32",
+                          rz.workload_scene(3, 10_000, 2048.0), 2048, 2048, 10, 2),
+            scene_measure(torch, rz, lib, _abi, "reference many-circles scene bench: 1024 circles, "
+                          "1024x1024 RGBA f32", rz.workload_circles(1024, 1024.0), 1024, 1024, 20, 2),
+        ]
 
     cpu = None
     if not args.no_cpu:
@@ -171,6 +274,7 @@ def main():
             "fill_rule": args.rule,
             "segments": n_seg,
             "lines": n_lines,
+            "cell_records": n_recs,
             "width": W,
             "height": H,
             "flatness": 0.05,
@@ -178,6 +282,7 @@ def main():
         "msegs_per_s": round(msegs, 3),
         "wall_s": round(wall, 3),
         "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
+        "kernels_gb_s": per_kernel_gbs,
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -185,12 +290,14 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes": alg,
-            "algorithmic_bytes_def": unit_note,
+            "algorithmic_bytes_def": alg_note,
             "kernel_ms": round(dom_ms, 4),
             "copy_bw_measured": round(copy_bw, 1),
         },
+        "scenes": scenes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -247,6 +247,9 @@ typedef struct rz_kernel_stat {
   double total_ms;
 } rz_kernel_stat;
 rz_status rz_profile(rz_ctx* ctx, int enable); /* also clears the statistics */
+/* Sizes of the last pipeline run that rz_sync (or a blocking call) collected:
+ * flattened lines, cell records and free-endpoint events. */
+rz_status rz_last_counts(rz_ctx* ctx, size_t* n_lines, size_t* n_records, size_t* n_events);
 rz_status rz_profile_stats(rz_ctx* ctx, rz_kernel_stat* out, size_t cap, size_t* n);
 
 #ifdef __cplusplus

@@ -113,6 +113,7 @@ PROTOTYPES = [
     ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
     ("rz_profile", C.c_int, [c_ctx, C.c_int]),
     ("rz_profile_stats", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
+    ("rz_last_counts", C.c_int, [c_ctx, P(C.c_size_t), P(C.c_size_t), P(C.c_size_t)]),
     # workloads
     ("rzw_cubics64", c_set, [C.c_uint32, C.c_double]),
     ("rzw_random_walk", c_set, [C.c_uint32, C.c_uint32, C.c_double]),

@@ -48,6 +48,7 @@ struct rz_ctx {
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   uint32_t rec_cap = 0, ev_cap = 0;
   rzk::Status* host_status = nullptr;  // pinned
+  rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
   int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
   uint32_t bin_blocks = 2048;          // grid of the per-line binning kernels
@@ -554,6 +555,7 @@ rz_status finish(rz_ctx* ctx) {
   RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
   rzk::Status s = *ctx->host_status;
   std::memset(ctx->host_status, 0, sizeof(s));
+  ctx->last = s;
   if (s.err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
   if (s.err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
   if (s.overflow) {
@@ -1024,6 +1026,14 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   if (param == RZ_PARAM_SPLIT_QUAD) *value = ctx->split_quad;
   else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
+  return RZ_OK;
+}
+
+rz_status rz_last_counts(rz_ctx* ctx, size_t* n_lines, size_t* n_records, size_t* n_events) {
+  if (!ctx) return RZ_E_ARG;
+  if (n_lines) *n_lines = ctx->last.n_lines;
+  if (n_records) *n_records = ctx->last.n_records;
+  if (n_events) *n_events = ctx->last.n_events;
   return RZ_OK;
 }
 
