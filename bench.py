@@ -120,6 +120,45 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
     }
 
 
+def atlas_measure(torch, rz, lib, _abi, frames):
+    """Glyph atlas: 65,536 glyph masks into one atlas, one pipeline per frame."""
+    gs = rz.workload_glyphs(4, 65_536, 16384)
+    W, H = gs.atlas_size
+    dg = rz.DeviceGlyphs(gs.paths, gs.slots)
+    ctx = dg.ctx
+    out = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+
+    def run():
+        dg.mask_device(out.data_ptr(), W)
+    settle(lib, _abi, ctx, run)
+    stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(frames):
+        run()
+    b.record(stream)
+    ctx.check(lib.rz_sync(ctx.h))
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / frames
+    k = kernel_stats(lib, _abi, ctx, run, 2)
+    ctx.check(lib.rz_sync(ctx.h))
+    dg.close()
+    px = sum(s.width * s.height for s in gs.slots)
+    segs = sum(p.segments_count() for p in gs.paths)
+    del out
+    return {
+        "workload": "glyph atlas: 65,536 glyphs of 8-200 quads (1-3 contours), masks 16-128 px, "
+                    f"one {W}x{H} f32 atlas, NonZero",
+        "glyphs": len(gs.paths),
+        "segments": segs,
+        "mask_pixels": px,
+        "ms_per_frame": round(ms, 4),
+        "mpix_s": round(px / ms / 1e3, 1),
+        "msegs_s": round(segs / ms / 1e3, 1),
+        "kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
+    }, gs
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -236,6 +275,11 @@ def main():
                           "1024x1024 RGBA f32", rz.workload_circles(1024, 1024.0), 1024, 1024, 20, 2),
         ]
 
+    atlas = None
+    glyph_set = None
+    if not args.no_scene:
+        atlas, glyph_set = atlas_measure(torch, rz, lib, _abi, 5)
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -254,6 +298,15 @@ def main():
                       "SignedDifferenceRasterizer::mask, one host thread",
             "msegs_per_s": round(n_seg * reps / secs / 1e6, 4),
         }
+        if glyph_set is not None:
+            # the glyph atlas on the host: the first 4096 glyphs, one at a time
+            t1 = time.perf_counter()
+            px = 0
+            for gp, sl in zip(glyph_set.paths[:4096], glyph_set.slots[:4096]):
+                O.sd_mask(gp, rz.Transform.identity(), sl.width, sl.height, 0)
+                px += sl.width * sl.height
+            atlas["cpu_mpix_s_1thread"] = round(px / (time.perf_counter() - t1) / 1e6, 2)
+            atlas["cpu_sample"] = "the oracle's SignedDifferenceRasterizer::mask of the first 4096 glyphs"
 
     rec = {
         "metric": "Mpix/s filled (Rasterizer::mask, 4096x4096, 200k-segment path)",
@@ -298,6 +351,7 @@ def main():
             "copy_bw_measured": round(copy_bw, 1),
         },
         "scenes": scenes,
+        "atlas": atlas,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

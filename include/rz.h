@@ -198,7 +198,8 @@ rz_status rz_fill_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, do
 /* ---- Scene::render ---------------------------------------------------------- */
 
 /* A scene: Scene::group of n Scene::fill(path, paint, fill_rule) nodes
- * (src/scene.rs:64-110), uploaded once and rendered many times. */
+ * (src/scene.rs:64-110), uploaded once and rendered many times. paints and
+ * fill_rules may be NULL (opaque black, NonZero), e.g. for glyph sets. */
 typedef struct rz_dscene rz_dscene;
 rz_status rz_scene_upload(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
                           const int32_t* fill_rules, size_t n, rz_dscene** out);
@@ -227,6 +228,33 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
                     const int32_t* fill_rules, size_t n, rz_transform tr, double flatness,
                     int edge_mode, const double* view, const float* bg, int batched, float* rgba,
                     size_t cap_px, int32_t* layer_xywh);
+
+/* ---- glyph atlas ------------------------------------------------------------ */
+
+/* Where glyph i goes: Rasterizer::mask of its path under `tr` into the atlas
+ * view of width x height pixels at (x, y) (src/image.rs:300-311). */
+typedef struct rz_atlas_slot {
+  rz_transform tr;
+  uint32_t x, y, width, height;
+} rz_atlas_slot;
+
+/* Glyph sets are uploaded with rz_scene_upload (paints and fill_rules may be
+ * NULL); rz_atlas_set places them (one slot per path, resident with the set). */
+rz_status rz_atlas_set(rz_ctx* ctx, rz_dscene* glyphs, const rz_atlas_slot* slots);
+
+/* Asynchronous Rasterizer::mask of every glyph into its view of the device
+ * atlas d_atlas (f32, row_stride floats per row), all glyphs in one pipeline.
+ * Every pixel of every view is written; other pixels are not touched. Views
+ * must not overlap. No reference counterpart (the reference masks glyphs one
+ * by one). */
+rz_status rz_mask_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flatness,
+                               int fill_rule, int edge_mode, float* d_atlas, size_t row_stride);
+
+/* Blocking form: n paths and slots, host atlas of atlas_width x atlas_height
+ * f32 (dense rows). */
+rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas_slot* slots,
+                        size_t n, double flatness, int fill_rule, int edge_mode, float* atlas,
+                        size_t atlas_width, size_t atlas_height);
 
 /* ---- launch parameters ----------------------------------------------------
  * No reference counterpart. RZ_PARAM_SPLIT_QUAD / RZ_PARAM_SPLIT_CUBIC: each

@@ -43,6 +43,16 @@ rzw_set* rzw_scene(uint32_t seed, uint32_t n_paths, double canvas);
  * [0, size)^2, opaque random colours, Rnd seed 0. */
 rzw_set* rzw_circles(uint32_t count, double size);
 
+/* A glyph set: n closed paths of U{8..200} quadratic Beziers split over 1-3
+ * contours (an outer blob and smaller inner blobs traced the other way round,
+ * i.e. holes), each fitted with a 1 px margin into its own mask of side
+ * U{16..128} px, shelf-packed (tallest first) into an atlas atlas_width px
+ * wide. Placement per glyph through rzw_slot, atlas size through
+ * rzw_atlas_size. */
+rzw_set* rzw_glyphs(uint32_t seed, uint32_t n, uint32_t atlas_width);
+void rzw_slot(const rzw_set* set, size_t i, rz_atlas_slot* out);
+void rzw_atlas_size(const rzw_set* set, uint32_t* width, uint32_t* height);
+
 /* Paint and fill rule of path i of a scene set. */
 void rzw_paint(const rzw_set* set, size_t i, rz_paint* out);
 int32_t rzw_rule(const rzw_set* set, size_t i);

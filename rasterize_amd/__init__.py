@@ -683,6 +683,85 @@ def workload_random_walk(seed=2, n_segments=200_000, canvas=4096.0):
     return WorkloadSet(lib().rzw_random_walk(seed, n_segments, canvas))
 
 
+class GlyphSet(WorkloadSet):
+    """Generated glyphs with their atlas placement (rz_atlas_slot per path)."""
+
+    def __init__(self, handle):
+        super().__init__(handle)
+        L = lib()
+        self.slots = []
+        for i in range(len(self.paths)):
+            sl = _abi.rz_atlas_slot()
+            L.rzw_slot(handle, i, C.byref(sl))
+            self.slots.append(sl)
+        w, h = C.c_uint32(), C.c_uint32()
+        L.rzw_atlas_size(handle, C.byref(w), C.byref(h))
+        self.atlas_size = (w.value, h.value)
+
+
+def atlas_slot(x, y, width, height, tr=None):
+    """Placement of one glyph in an atlas (rz_atlas_slot)."""
+    sl = _abi.rz_atlas_slot()
+    sl.tr = (tr or Transform.identity()).abi()
+    sl.x, sl.y, sl.width, sl.height = x, y, width, height
+    return sl
+
+
+def mask_atlas(paths, slots, atlas, fill_rule=FillRule.NonZero, flatness=DEFAULT_FLATNESS,
+               edge_mode=_abi.RZ_EDGE_MASK, ctx=None):
+    """Rasterizer::mask of each path into its view of a 2D float32 atlas, all in
+    one pipeline (rz_mask_atlas). Views must not overlap."""
+    ctx = ctx or default_context()
+    if atlas.dtype != np.float32 or atlas.ndim != 2 or not atlas.flags.c_contiguous:
+        raise TypeError("atlas must be a contiguous 2D float32 array")
+    n = len(paths)
+    keep = [p.abi() for p in paths]
+    parr = (C.POINTER(_abi.rz_path) * max(n, 1))(*[C.pointer(p) for p in keep])
+    sarr = (_abi.rz_atlas_slot * max(n, 1))(*slots)
+    ctx.check(lib().rz_mask_atlas(ctx.h, parr, sarr, n, flatness, int(fill_rule), edge_mode,
+                                  atlas.ctypes.data_as(C.POINTER(C.c_float)), atlas.shape[1],
+                                  atlas.shape[0]))
+    return atlas
+
+
+class DeviceGlyphs:
+    """A glyph set uploaded once with its placement, masked into a device atlas
+    by rz_mask_atlas_device."""
+
+    def __init__(self, paths, slots, ctx=None):
+        self.ctx = ctx or default_context()
+        n = len(paths)
+        keep = [p.abi() for p in paths]
+        parr = (C.POINTER(_abi.rz_path) * max(n, 1))(*[C.pointer(p) for p in keep])
+        h = C.c_void_p()
+        self.ctx.check(lib().rz_scene_upload(self.ctx.h, parr, None, None, n, C.byref(h)))
+        self.h = h
+        sarr = (_abi.rz_atlas_slot * max(n, 1))(*slots)
+        self.ctx.check(lib().rz_atlas_set(self.ctx.h, self.h, sarr))
+
+    def mask_device(self, d_atlas, row_stride, fill_rule=FillRule.NonZero,
+                    flatness=DEFAULT_FLATNESS, edge_mode=_abi.RZ_EDGE_MASK):
+        self.ctx.check(lib().rz_mask_atlas_device(self.ctx.h, self.h, flatness, int(fill_rule),
+                                                  edge_mode, C.c_void_p(d_atlas), row_stride))
+
+    def close(self):
+        if self.h:
+            lib().rz_scene_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def workload_glyphs(seed=4, n=65_536, atlas_width=16384):
+    """Glyph atlas: n star/blob glyphs of 8-200 quads, 1-3 contours, masks of
+    16-128 px shelf-packed into one atlas."""
+    return GlyphSet(lib().rzw_glyphs(seed, n, atlas_width))
+
+
 def workload_scene(seed=3, n_paths=10_000, canvas=2048.0):
     """Batched scene: star/blob paths with solid / linear / radial paints."""
     return WorkloadSet(lib().rzw_scene(seed, n_paths, canvas), with_paints=True)

@@ -67,6 +67,11 @@ class rz_paint(C.Structure):
     ]
 
 
+class rz_atlas_slot(C.Structure):
+    _fields_ = [("tr", rz_transform), ("x", C.c_uint32), ("y", C.c_uint32), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
 class rz_kernel_stat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("calls", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -109,6 +114,11 @@ PROTOTYPES = [
     ("rz_render", C.c_int, [c_ctx, P(P(rz_path)), P(rz_paint), P(C.c_int32), C.c_size_t,
                             rz_transform, C.c_double, C.c_int, P(C.c_double), P(C.c_float), C.c_int,
                             P(C.c_float), C.c_size_t, P(C.c_int32)]),
+    ("rz_atlas_set", C.c_int, [c_ctx, C.c_void_p, P(rz_atlas_slot)]),
+    ("rz_mask_atlas_device", C.c_int, [c_ctx, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_size_t]),
+    ("rz_mask_atlas", C.c_int, [c_ctx, P(P(rz_path)), P(rz_atlas_slot), C.c_size_t, C.c_double,
+                                C.c_int, C.c_int, P(C.c_float), C.c_size_t, C.c_size_t]),
     ("rz_set_param", C.c_int, [c_ctx, C.c_int, C.c_int]),
     ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
     ("rz_profile", C.c_int, [c_ctx, C.c_int]),
@@ -119,6 +129,9 @@ PROTOTYPES = [
     ("rzw_random_walk", c_set, [C.c_uint32, C.c_uint32, C.c_double]),
     ("rzw_scene", c_set, [C.c_uint32, C.c_uint32, C.c_double]),
     ("rzw_circles", c_set, [C.c_uint32, C.c_double]),
+    ("rzw_glyphs", c_set, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("rzw_slot", None, [c_set, C.c_size_t, P(rz_atlas_slot)]),
+    ("rzw_atlas_size", None, [c_set, P(C.c_uint32), P(C.c_uint32)]),
     ("rzw_paint", None, [c_set, C.c_size_t, P(rz_paint)]),
     ("rzw_rule", C.c_int32, [c_set, C.c_size_t]),
     ("rzw_count", C.c_size_t, [c_set]),
@@ -139,6 +152,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} not found: build it with `python -m rasterize_amd.build`")
+        # PyTorch ships its own libamdhip64.so.7. A process holds one library of
+        # that name, so it must be torch's when torch also runs on the GPU:
+        # load torch first (librz then binds to the same HIP runtime).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, res, args in PROTOTYPES:
             f = getattr(L, name)

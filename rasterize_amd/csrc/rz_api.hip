@@ -657,6 +657,10 @@ struct rz_dscene {
   const rzk::ScenePath* sp = nullptr;
   const uint32_t* slot_path = nullptr;
   std::vector<rzk::ScenePath> host_sp;  // same table on the host (plain form)
+  // glyph atlas placement (rz_atlas_set): device slots and slice totals
+  rzk::AtlasSlot* atlas_slots = nullptr;
+  uint32_t atlas_tiles = 0, atlas_bands = 0, atlas_rows = 0;
+  bool atlas_ready = false;
   // host copies of the paths, for Path::bbox(tr) in the plain form
   std::vector<std::vector<uint8_t>> kinds;
   std::vector<std::vector<uint32_t>> offs, subs;
@@ -687,6 +691,8 @@ void scene_release(rz_dscene* s) {
     delete d;
   }
   for (void* p : s->allocs) (void)hipFree(p);
+  if (s->atlas_slots) (void)hipFree(s->atlas_slots);
+  s->atlas_slots = nullptr;
   s->per.clear();
   s->allocs.clear();
   s->all = nullptr;
@@ -705,7 +711,7 @@ void layer_of(const double* v, int* lx, int* ly, int* lw, int* lh) {
 rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
                             const int32_t* rules, size_t n, rz_dscene** out) {
   if (!ctx || !out) return RZ_E_ARG;
-  if (n && (!paths || !paints || !rules)) return fail(ctx, RZ_E_ARG, "null scene arrays");
+  if (n && !paths) return fail(ctx, RZ_E_ARG, "null scene arrays");
   if (n > (1u << 24)) return fail(ctx, RZ_E_ARG, "too many paths");
   rz_dscene* s = new rz_dscene();
   s->n = (uint32_t)n;
@@ -731,7 +737,11 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
   for (size_t i = 0; i < n; ++i) {
     const rz_path* p = paths[i];
     RZ_TRY(validate_path(ctx, p));
-    if (rules[i] != RZ_NONZERO && rules[i] != RZ_EVENODD) return fail(ctx, RZ_E_ARG, "bad fill rule");
+    const int rule_i = rules ? rules[i] : RZ_NONZERO;
+    if (rule_i != RZ_NONZERO && rule_i != RZ_EVENODD) return fail(ctx, RZ_E_ARG, "bad fill rule");
+    rz_paint solid{};
+    solid.kind = RZ_PAINT_SOLID;
+    const rz_paint& paint_i = paints ? paints[i] : solid;
     const uint32_t seg0 = (uint32_t)kinds.size(), pt0 = (uint32_t)(pts.size() / 2);
     const uint32_t ns = p->n_segments, nsub = p->n_subpaths, npts = ns ? p->point_offsets[ns] : 0;
     s->kinds.emplace_back(p->kinds, p->kinds + ns);
@@ -750,16 +760,16 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
       for (uint32_t k = p->subpaths[j]; k <= p->subpaths[j + 1]; ++k) slot_path.push_back((uint32_t)i);
     }
     rzh::PaintPrep pp;
-    if (!rzh::prepare_paint_static(paints[i], &pp)) return fail(ctx, RZ_E_ARG, "invalid paint");
+    if (!rzh::prepare_paint_static(paint_i, &pp)) return fail(ctx, RZ_E_ARG, "invalid paint");
     rzk::ScenePath& S = s->host_sp[i];
     S = rzk::ScenePath{};
     S.seg_begin = seg0;
     S.seg_end = seg0 + ns;
-    S.rule = rules[i];
+    S.rule = rule_i;
     const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
     S.has_bbox_i = rzh::path_bbox(p, ident, &S.bbox_i) ? 1 : 0;
-    S.units = paints[i].units;
-    for (int k = 0; k < 6; ++k) S.paint_tr[k] = paints[i].tr[k];
+    S.units = paint_i.units;
+    for (int k = 0; k < 6; ++k) S.paint_tr[k] = paint_i.tr[k];
     S.paint = pp.p;
     stop_first[i] = stop_pos.size();
     stop_pos.insert(stop_pos.end(), pp.pos.begin(), pp.pos.end());
@@ -993,6 +1003,145 @@ rz_status enqueue_scene_plain(rz_ctx* ctx, const rz_dscene* s, const rz_transfor
   }
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->sticky_st, sizeof(rzk::Status),
                              hipMemcpyDeviceToHost, ctx->stream));
+  return RZ_OK;
+}
+
+// Glyph atlas: every path of the set masked into its own atlas view, in one
+// pipeline (flatten and binning in their batched form, one wave per glyph tile).
+rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, double flatness,
+                        float* d_atlas, size_t row_stride) {
+  const uint32_t n = s->n;
+  if (n == 0 || s->atlas_tiles == 0) return RZ_OK;
+  const rz_dpath* d = s->all;
+  if (ctx->line_cap == 0) {
+    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
+  }
+  if (ctx->rec_cap == 0) {
+    ctx->rec_cap = ctx->line_cap;
+    RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
+  }
+  if (ctx->ev_cap == 0) {
+    ctx->ev_cap = 1 << 14;
+    RZ_TRY(ensure(ctx, ctx->events, (size_t)ctx->ev_cap * 16));
+  }
+  const uint32_t ct = s->atlas_tiles, cb = s->atlas_bands, cr = s->atlas_rows;
+  const size_t sw = rzk::kStatusWords;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr;
+  RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
+  RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
+  RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
+  RZ_TRY(ensure(ctx, ctx->line_path, (size_t)ctx->line_cap * 4));
+  RZ_TRY(ensure(ctx, ctx->sc_pi, (size_t)n * sizeof(rzk::PathInfo)));
+  uint32_t* meta = (uint32_t*)ctx->meta.p;
+  rzk::Status* st = (rzk::Status*)meta;
+  uint32_t* tile_count = meta + sw;
+  int32_t* bd = (int32_t*)(meta + sw + ct);
+  uint32_t* band_ev = meta + sw + 2 * (size_t)ct;
+  int32_t* redge = (int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  uint32_t* tile_off = (uint32_t*)ctx->tile_off.p;
+  uint32_t* ev_off = (uint32_t*)ctx->ev_off.p;
+  rzk::PathInfo* pi = (rzk::PathInfo*)ctx->sc_pi.p;
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32,
+            dim3((uint32_t)std::min<size_t>(blocks_for(meta_u32), 4096)), dim3(rzk::kBlock), meta,
+            meta_u32, 0u);
+  RZ_LAUNCH(ctx, "k_atlas_setup", rzk::k_atlas_setup, dim3(blocks_for(n)), dim3(rzk::kBlock),
+            (const rzk::AtlasSlot*)s->atlas_slots, n, mode, pi);
+  rzk::Xform x{};
+  const double fl = 16.0 * flatness * flatness;
+  rzk::LineBufs lb;
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
+  rzk::BinBufs b;
+  b.tile_count = tile_count;
+  b.bd_diff = bd;
+  b.band_ev = band_ev;
+  b.recs = (float4*)ctx->recs.p;
+  b.rec_cap = ctx->rec_cap;
+  b.events = (int4*)ctx->events.p;
+  b.ev_cap = ctx->ev_cap;
+  b.redge = redge;
+  b.st = st;
+  rzg::BinGrid g0{};
+  const uint32_t gb = ctx->bin_blocks, gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
+  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, true>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
+            (const rzk::PathInfo*)pi, b);
+  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  RZ_TRY(check_launch(ctx, "atlas pass 0"));
+  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, tile_count, &st->n_records));
+  RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
+  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
+            bd, redge, (const rzk::Status*)st);
+  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, true>), dim3(gb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
+            (const rzk::PathInfo*)pi, b);
+  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
+            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  rzk::AtlasFineArgs a;
+  a.pi = pi;
+  a.slots = s->atlas_slots;
+  a.n = n;
+  a.tile_off = tile_off;
+  a.recs = (const float4*)ctx->recs.p;
+  a.bd = bd;
+  a.band_ev_off = ev_off;
+  a.events = (const int4*)ctx->events.p;
+  a.redge = redge;
+  a.st = st;
+  a.out = d_atlas;
+  a.row_stride = row_stride;
+  if (rule == RZ_NONZERO)
+    RZ_LAUNCH(ctx, "k_atlas_fine<0>", rzk::k_atlas_fine<0>, dim3(ct), dim3(64), a);
+  else
+    RZ_LAUNCH(ctx, "k_atlas_fine<1>", rzk::k_atlas_fine<1>, dim3(ct), dim3(64), a);
+  RZ_TRY(check_launch(ctx, "atlas pass 1"));
+  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  return RZ_OK;
+}
+
+rz_status atlas_set_impl(rz_ctx* ctx, rz_dscene* s, const rz_atlas_slot* slots) {
+  if (!ctx || !s) return RZ_E_ARG;
+  if (s->n && !slots) return fail(ctx, RZ_E_ARG, "slots is NULL");
+  std::vector<rzk::AtlasSlot> v(s->n);
+  uint64_t t = 0, b = 0, r = 0;
+  for (uint32_t i = 0; i < s->n; ++i) {
+    rzk::AtlasSlot& a = v[i];
+    const rz_atlas_slot& q = slots[i];
+    if (q.width > (1u << 16) || q.height > (1u << 16) || q.x > (1u << 30) || q.y > (1u << 30))
+      return fail(ctx, RZ_E_ARG, "atlas slot too large");
+    for (int k = 0; k < 6; ++k) a.m[k] = q.tr.m[k];
+    a.x = (int)q.x;
+    a.y = (int)q.y;
+    a.w = (int)q.width;
+    a.h = (int)q.height;
+    a.tile_base = (uint32_t)t;
+    a.band_base = (uint32_t)b;
+    a.redge_base = (uint32_t)r;
+    a.pad = 0;
+    const uint64_t tx = (q.width + 15) / 16, ty = (q.height + 15) / 16;
+    if (q.width && q.height) {
+      t += tx * ty;
+      b += ty;
+      r += q.height + 2;
+    }
+  }
+  if (t > 0xffffffffull || r > 0xffffffffull) return fail(ctx, RZ_E_ARG, "atlas too large");
+  if (s->atlas_slots) {
+    RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    RZ_HIP(ctx, hipFree(s->atlas_slots));
+    s->atlas_slots = nullptr;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(rzk::AtlasSlot));
+  if (e != hipSuccess) return fail(ctx, RZ_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  s->atlas_slots = (rzk::AtlasSlot*)p;
+  if (!v.empty()) RZ_HIP(ctx, hipMemcpy(p, v.data(), v.size() * sizeof(rzk::AtlasSlot), hipMemcpyHostToDevice));
+  s->atlas_tiles = (uint32_t)t;
+  s->atlas_bands = (uint32_t)b;
+  s->atlas_rows = (uint32_t)r;
+  s->atlas_ready = true;
   return RZ_OK;
 }
 
@@ -1413,6 +1562,51 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
   }
   if (st != RZ_OK) return st;
   RZ_HIP(ctx, hipMemcpy(rgba, ctx->out.p, px * 16, hipMemcpyDeviceToHost));
+  return RZ_OK;
+}
+
+rz_status rz_atlas_set(rz_ctx* ctx, rz_dscene* glyphs, const rz_atlas_slot* slots) {
+  return atlas_set_impl(ctx, glyphs, slots);
+}
+
+rz_status rz_mask_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flatness, int fill_rule,
+                               int edge_mode, float* d_atlas, size_t row_stride) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, 0, 0));
+  if (!glyphs || !glyphs->atlas_ready) return fail(ctx, RZ_E_ARG, "glyph placement not set");
+  if (glyphs->n && !d_atlas) return fail(ctx, RZ_E_ARG, "d_atlas is NULL");
+  return enqueue_atlas(ctx, glyphs, fill_rule, edge_mode, flatness, d_atlas, row_stride);
+}
+
+rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas_slot* slots,
+                        size_t n, double flatness, int fill_rule, int edge_mode, float* atlas,
+                        size_t atlas_width, size_t atlas_height) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, 0, 0));
+  if (n && (!slots || !atlas)) return fail(ctx, RZ_E_ARG, "null argument");
+  for (size_t i = 0; i < n; ++i)
+    if ((size_t)slots[i].x + slots[i].width > atlas_width ||
+        (size_t)slots[i].y + slots[i].height > atlas_height)
+      return fail(ctx, RZ_E_ARG, "glyph slot outside the atlas");
+  rz_dscene* s = nullptr;
+  RZ_TRY(scene_upload_impl(ctx, paths, nullptr, nullptr, n, &s));
+  struct Guard {
+    rz_ctx* c;
+    rz_dscene* s;
+    ~Guard() { rz_scene_free(c, s); }
+  } guard{ctx, s};
+  RZ_TRY(atlas_set_impl(ctx, s, slots));
+  const size_t bytes = atlas_width * atlas_height * sizeof(float);
+  RZ_TRY(ensure(ctx, ctx->out, std::max<size_t>(bytes, 4)));
+  // pixels outside every glyph view keep the caller's values
+  RZ_HIP(ctx, hipMemcpy(ctx->out.p, atlas, bytes, hipMemcpyHostToDevice));
+  rz_status st = RZ_OK;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    st = enqueue_atlas(ctx, s, fill_rule, edge_mode, flatness, (float*)ctx->out.p, atlas_width);
+    if (st != RZ_OK) return st;
+    st = finish(ctx);
+    if (st != RZ_E_CAPACITY) break;
+  }
+  if (st != RZ_OK) return st;
+  RZ_HIP(ctx, hipMemcpy(atlas, ctx->out.p, bytes, hipMemcpyDeviceToHost));
   return RZ_OK;
 }
 

@@ -330,3 +330,125 @@ __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
 }
 
 }  // namespace rzk
+
+namespace rzk {
+
+// ---------------------------------------------------------------------------
+// Glyph atlas: Rasterizer::mask of many independent paths, each into its own
+// sub-view of one atlas image (src/image.rs:300-311), in one pipeline.
+// ---------------------------------------------------------------------------
+
+// One glyph's placement, with its slices of the per-tile / per-band / per-row
+// arrays (prefix sums of its tile, band and row counts, made on the host).
+struct AtlasSlot {
+  double m[6];
+  int x, y, w, h;
+  uint32_t tile_base, band_base, redge_base, pad;
+};
+
+__global__ __launch_bounds__(kBlock) void k_atlas_setup(const AtlasSlot* slots, uint32_t n, int mode,
+                                                         PathInfo* pi) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const AtlasSlot& s = slots[i];
+  PathInfo P;
+  for (int k = 0; k < 6; ++k) P.m[k] = s.m[k];
+  P.live = s.w > 0 && s.h > 0;
+  P.rule = 0;
+  P.ox = s.x;
+  P.oy = s.y;
+  P.tx0 = P.ty0 = 0;
+  P.g.mode = mode;
+  P.g.W = s.w;
+  P.g.H = s.h;
+  P.g.wc = mode == rzg::EDGE_MASK ? (double)s.w - 1.0 : (double)s.w;
+  P.g.phx = P.g.phy = 0;
+  P.g.tiles_x = (s.w + 15) >> 4;
+  P.g.tiles_y = (s.h + 15) >> 4;
+  P.tile_base = s.tile_base;
+  P.band_base = s.band_base;
+  P.redge_base = s.redge_base;
+  if (mode == rzg::EDGE_NONE) {
+    P.redge_col = INT32_MAX;
+    P.redge_w = 0.0;
+  } else {
+    const double X = rzg::redge_x(P.g), fx = floor(X);
+    P.redge_col = (int)fx;
+    P.redge_w = 1.0 - (X - fx);
+  }
+  P.paint = rzp::DPaint{};
+  pi[i] = P;
+}
+
+struct AtlasFineArgs {
+  const PathInfo* pi;
+  const AtlasSlot* slots;
+  uint32_t n;
+  const uint32_t* tile_off;
+  const float4* recs;
+  const int32_t* bd;
+  const uint32_t* band_ev_off;
+  const int4* events;
+  const int32_t* redge;
+  const Status* st;
+  float* out;
+  size_t row_stride;  // floats
+};
+
+// One tile of one glyph per wave: the glyph is found by binary search over
+// the glyphs' first tiles; coverage as in k_fine, written into the glyph's
+// atlas view.
+template <int RULE>
+__global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
+  if (a.st->err || a.st->overflow) return;
+  __shared__ int acc[256];
+  __shared__ int rowd[18];
+  const int lane = threadIdx.x;
+  const uint32_t t = blockIdx.x;
+  uint32_t lo = 0, hi = a.n;  // last glyph with tile_base <= t
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.slots[mid].tile_base <= t) lo = mid;
+    else hi = mid;
+  }
+  const PathInfo& P = a.pi[lo];
+  if (!P.live) return;
+  const uint32_t local = t - P.tile_base;
+  const int txl = (int)(local % (uint32_t)P.g.tiles_x), tyl = (int)(local / (uint32_t)P.g.tiles_x);
+  const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
+  const uint32_t eb = a.band_ev_off[P.band_base + tyl], ee = a.band_ev_off[P.band_base + tyl + 1];
+  const bool local_work = rb != re || eb != ee;
+  if (local_work) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
+    if (lane < 18) rowd[lane] = 0;
+    __syncthreads();
+    for (uint32_t i = rb + lane; i < re; i += 64) fine_record(a.recs[i], acc, rowd);
+    for (uint32_t i = eb + lane; i < ee; i += 64) {
+      const int4 ev = a.events[i];
+      if (ev.y < txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+    }
+    __syncthreads();
+    if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
+    __syncthreads();
+  }
+  const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
+  const int col = lane & 15;
+  const int xs = txl * 16 + col;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = 4 * k + (lane >> 4);
+    int64_t w = bd;
+    if (local_work) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
+    const int ys = tyl * 16 + r;
+    if (xs < P.g.W && ys < P.g.H) {
+      if (xs >= P.redge_col) {
+        const int64_t rr = a.redge[P.redge_base + ys];
+        w += xs == P.redge_col ? (int64_t)rint((double)rr * P.redge_w) : rr;
+      }
+      a.out[(size_t)(P.oy + ys) * a.row_stride + (size_t)(P.ox + xs)] = alpha_fix<RULE>(w);
+    }
+  }
+}
+
+}  // namespace rzk

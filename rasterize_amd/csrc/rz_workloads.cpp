@@ -76,6 +76,8 @@ struct rzw_set {
   rz::Transform tr;
   std::vector<std::unique_ptr<PaintStore>> paints;  // scenes: one per path
   std::vector<int32_t> rules;
+  std::vector<rz_atlas_slot> slots;                 // glyph sets: atlas placement
+  uint32_t atlas_w = 0, atlas_h = 0;
 };
 
 extern "C" {
@@ -257,6 +259,85 @@ rzw_set* rzw_circles(uint32_t count, double size) {
     s->rules.push_back(RZ_NONZERO);
   }
   return s;
+}
+
+rzw_set* rzw_glyphs(uint32_t seed, uint32_t n, uint32_t atlas_width) {
+  Rnd rnd(seed);
+  rzw_set* s = new rzw_set();
+  std::vector<uint32_t> side(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t sz = 16u + rnd.uniform_u32() % 113u;  // U{16..128}
+    side[i] = sz;
+    const uint32_t q = 8u + rnd.uniform_u32() % 193u;    // U{8..200} quads
+    const uint32_t nc = 1u + rnd.uniform_u32() % 3u;     // 1..3 contours
+    // glyph space: unit square mapped to [1, sz - 1]^2 (1 px margin)
+    const double k = (double)sz - 2.0;
+    auto to_px = [&](double u, double v) { return rz::Point(1.0 + k * u, 1.0 + k * v); };
+    rz::PathBuilder pb;
+    uint32_t left = q;
+    for (uint32_t c = 0; c < nc; ++c) {
+      const uint32_t m = c + 1 == nc ? left : std::max<uint32_t>(4u, q / nc);
+      left = left > m ? left - m : 0u;
+      const uint32_t cnt = std::max<uint32_t>(m, 3u);
+      // outer contour: a blob round the centre; inner ones: small blobs
+      // inside it traced the other way round (holes)
+      double cx = 0.5, cy = 0.5, R = 0.5;
+      if (c > 0) {
+        R = 0.08 + 0.1 * rnd.uniform();
+        cx = 0.5 + 0.25 * (2.0 * rnd.uniform() - 1.0);
+        cy = 0.5 + 0.25 * (2.0 * rnd.uniform() - 1.0);
+      }
+      const double dir = c == 0 ? 1.0 : -1.0;
+      std::vector<double> th(cnt), rr(cnt);
+      for (uint32_t j = 0; j < cnt; ++j) {
+        th[j] = dir * 2.0 * rz::PI * ((double)j + 0.4 * rnd.uniform()) / (double)cnt;
+        rr[j] = R * (0.7 + 0.3 * rnd.uniform());
+      }
+      auto at = [&](double a, double r) { return to_px(cx + r * std::cos(a), cy + r * std::sin(a)); };
+      pb.move_to(at(th[0], rr[0]));
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t j1 = (j + 1) % cnt;
+        const double a0 = th[j], a1 = j1 ? th[j1] : th[0] + dir * 2.0 * rz::PI;
+        const double rm = R * (0.8 + 0.45 * rnd.uniform());
+        pb.quad_to(at(0.5 * (a0 + a1), rm), at(th[j1], rr[j1]));
+      }
+      pb.close();
+    }
+    s->paths.push_back(pb.build());
+    s->rules.push_back(RZ_NONZERO);
+  }
+  // shelf packing, tallest first, rows of atlas_width pixels
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return side[a] > side[b]; });
+  s->slots.resize(n);
+  uint32_t x = 0, y = 0, row_h = 0;
+  for (uint32_t i : order) {
+    const uint32_t sz = side[i];
+    if (x + sz > atlas_width) {
+      x = 0;
+      y += row_h;
+      row_h = 0;
+    }
+    rz_atlas_slot& sl = s->slots[i];
+    sl.tr = rz::Transform::identity().abi();
+    sl.x = x;
+    sl.y = y;
+    sl.width = sz;
+    sl.height = sz;
+    x += sz;
+    row_h = std::max(row_h, sz);
+  }
+  s->atlas_w = atlas_width;
+  s->atlas_h = y + row_h;
+  return s;
+}
+
+void rzw_slot(const rzw_set* set, size_t i, rz_atlas_slot* out) { *out = set->slots[i]; }
+
+void rzw_atlas_size(const rzw_set* set, uint32_t* width, uint32_t* height) {
+  *width = set->atlas_w;
+  *height = set->atlas_h;
 }
 
 void rzw_paint(const rzw_set* set, size_t i, rz_paint* out) { *out = set->paints[i]->p; }
