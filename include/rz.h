@@ -229,6 +229,31 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
                     int edge_mode, const double* view, const float* bg, int batched, float* rgba,
                     size_t cap_px, int32_t* layer_xywh);
 
+/* ---- differentiable render --------------------------------------------------- */
+
+/* Replace a resident scene's control points (device f64 x,y pairs of every
+ * path's points, in path order, rz_scene_points of them) and/or its solid
+ * paint colours (device f32 RGBA per path; other paints keep theirs). NULL
+ * leaves that part unchanged. Only the batched form sees the update (the
+ * plain form keeps the uploaded paths). No reference counterpart. */
+rz_status rz_scene_update_device(rz_ctx* ctx, rz_dscene* scene, const double* d_points,
+                                 const float* d_colors);
+rz_status rz_scene_points(rz_ctx* ctx, const rz_dscene* scene, size_t* n_points);
+
+/* Adjoint of rz_render_device (batched form) with the same arguments: given
+ * dL/d(layer) d_grad_rgba (device RGBA f32, grad_stride pixels per row) it
+ * adds dL/d(control points) to d_grad_points (f32 x,y per point of the
+ * scene), dL/d(solid paint colour) to d_grad_colors (f32 RGBA per path) and,
+ * if not NULL, dL/d(transform) to d_grad_tr (f64 [6]). The coverage gradient
+ * differentiates the exact box-filtered winding (edge terms of the signed
+ * area), the subdivision's fixed-t Bezier evaluation and source-over
+ * compositing; subdivision and tile lists are recomputed from the forward
+ * pass. No reference counterpart. */
+rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                    double flatness, int edge_mode, const double* view,
+                                    const float* bg, const float* d_grad_rgba, size_t grad_stride,
+                                    float* d_grad_points, float* d_grad_colors, double* d_grad_tr);
+
 /* ---- glyph atlas ------------------------------------------------------------ */
 
 /* Where glyph i goes: Rasterizer::mask of its path under `tr` into the atlas

@@ -628,6 +628,29 @@ class DeviceScene:
         self.ctx.check(lib().rz_render_device(self.ctx.h, self.h, tr.abi(), flatness, edge_mode, v, b,
                                               int(batched), C.c_void_p(d_rgba), row_stride))
 
+    def n_points(self):
+        n = C.c_size_t()
+        self.ctx.check(lib().rz_scene_points(self.ctx.h, self.h, C.byref(n)))
+        return n.value
+
+    def update_device(self, d_points=None, d_colors=None):
+        """Replace the control points (device f64 x,y per point) and/or solid
+        colours (device f32 RGBA per path) of the batched form."""
+        self.ctx.check(lib().rz_scene_update_device(
+            self.ctx.h, self.h, C.c_void_p(d_points) if d_points else None,
+            C.c_void_p(d_colors) if d_colors else None))
+
+    def backward_device(self, tr, view, d_grad_rgba, grad_stride, d_grad_points, d_grad_colors,
+                        d_grad_tr=None, bg=None, flatness=DEFAULT_FLATNESS,
+                        edge_mode=_abi.RZ_EDGE_ITER):
+        """Adjoint of render_device (batched): gradients are added to the buffers."""
+        v = (C.c_double * 4)(*view)
+        b = (C.c_float * 4)(*bg) if bg is not None else None
+        self.ctx.check(lib().rz_render_backward_device(
+            self.ctx.h, self.h, tr.abi(), flatness, edge_mode, v, b, C.c_void_p(d_grad_rgba),
+            grad_stride, C.c_void_p(d_grad_points), C.c_void_p(d_grad_colors),
+            C.c_void_p(d_grad_tr) if d_grad_tr else None))
+
     def close(self):
         if self.h:
             lib().rz_scene_free(self.ctx.h, self.h)
@@ -638,6 +661,50 @@ class DeviceScene:
             self.close()
         except Exception:
             pass
+
+
+def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLATNESS):
+    """Differentiable Scene::render (batched form) as a torch.autograd.Function:
+    `points` (N, 2) float tensor of every path's stored control points in scene
+    order, `colors` (n_paths, 4) solid premultiplied linear RGBA; returns the
+    (H, W, 4) float32 layer of `view`. Gradients flow to points and colors
+    through rz_render_backward_device."""
+    import torch
+
+    class _Render(torch.autograd.Function):
+        @staticmethod
+        def forward(fctx, pts, cols):
+            x0, y0 = int(np.floor(view[0])), int(np.floor(view[1]))
+            w, h = int(np.ceil(view[2])) - x0, int(np.ceil(view[3])) - y0
+            p64 = pts.detach().to(torch.float64).contiguous()
+            c32 = cols.detach().to(torch.float32).contiguous()
+            dscene.update_device(p64.data_ptr(), c32.data_ptr())
+            out = torch.empty((h, w, 4), dtype=torch.float32, device=pts.device)
+            L = lib()
+            for _ in range(4):
+                dscene.render_device(tr, view, out.data_ptr(), w, bg=bg, batched=True,
+                                     flatness=flatness)
+                s = L.rz_sync(dscene.ctx.h)
+                if s != _abi.RZ_E_CAPACITY:
+                    break
+            dscene.ctx.check(s)
+            fctx.keep = (p64, c32)
+            fctx.shape = (h, w)
+            fctx.dtypes = (pts.dtype, cols.dtype)
+            return out
+
+        @staticmethod
+        def backward(fctx, grad):
+            h, w = fctx.shape
+            g = grad.detach().to(torch.float32).contiguous()
+            gp = torch.zeros((dscene.n_points(), 2), dtype=torch.float32, device=g.device)
+            gc = torch.zeros((dscene.n, 4), dtype=torch.float32, device=g.device)
+            dscene.backward_device(tr, view, g.data_ptr(), w, gp.data_ptr(), gc.data_ptr(), bg=bg,
+                                   flatness=flatness)
+            dscene.ctx.check(lib().rz_sync(dscene.ctx.h))
+            return gp.to(fctx.dtypes[0]), gc.to(fctx.dtypes[1])
+
+    return _Render.apply(points, colors)
 
 
 class WorkloadSet:

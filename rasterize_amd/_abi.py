@@ -114,6 +114,11 @@ PROTOTYPES = [
     ("rz_render", C.c_int, [c_ctx, P(P(rz_path)), P(rz_paint), P(C.c_int32), C.c_size_t,
                             rz_transform, C.c_double, C.c_int, P(C.c_double), P(C.c_float), C.c_int,
                             P(C.c_float), C.c_size_t, P(C.c_int32)]),
+    ("rz_scene_update_device", C.c_int, [c_ctx, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rz_scene_points", C.c_int, [c_ctx, C.c_void_p, P(C.c_size_t)]),
+    ("rz_render_backward_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,
+                                            P(C.c_double), P(C.c_float), C.c_void_p, C.c_size_t,
+                                            C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rz_atlas_set", C.c_int, [c_ctx, C.c_void_p, P(rz_atlas_slot)]),
     ("rz_mask_atlas_device", C.c_int, [c_ctx, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p,
                                        C.c_size_t]),

@@ -13,6 +13,7 @@
 #include "rz_host.hpp"
 #include "rz_kernels.hpp"
 #include "rz_scene.hpp"
+#include "rz_backward.hpp"
 
 namespace {
 
@@ -44,6 +45,7 @@ struct rz_ctx {
   // batched scene render
   DBuf line_path;  // path of each line
   DBuf fl_counts;  // lines per flatten thread, then per-wave totals and offsets
+  DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
   DBuf sc_boxes, sc_has, sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   uint32_t rec_cap = 0, ev_cap = 0;
@@ -655,7 +657,10 @@ struct rz_dscene {
   std::vector<rz_dpath*> per;          // one resident path each, for the plain form
   std::vector<void*> allocs;           // ScenePath table, slot -> path map, stops
   const rzk::ScenePath* sp = nullptr;
+  rzk::ScenePath* sp_mut = nullptr;
   const uint32_t* slot_path = nullptr;
+  const uint32_t* subpaths = nullptr;  // combined path: subpath -> first segment
+  uint32_t n_points = 0;               // points of the combined path
   std::vector<rzk::ScenePath> host_sp;  // same table on the host (plain form)
   // glyph atlas placement (rz_atlas_set): device slots and slice totals
   rzk::AtlasSlot* atlas_slots = nullptr;
@@ -798,7 +803,7 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
     *dst = p;
     return RZ_OK;
   };
-  void *dpos = nullptr, *dcol = nullptr, *dsp = nullptr, *dslot = nullptr;
+  void *dpos = nullptr, *dcol = nullptr, *dsp = nullptr, *dslot = nullptr, *dsubs = nullptr;
   RZ_TRY(up(stop_pos.data(), stop_pos.size() * sizeof(double), &dpos));
   RZ_TRY(up(stop_col.data(), stop_col.size() * sizeof(rzp::C4), &dcol));
   for (size_t i = 0; i < n; ++i) {
@@ -808,8 +813,12 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
   }
   RZ_TRY(up(s->host_sp.data(), n * sizeof(rzk::ScenePath), &dsp));
   RZ_TRY(up(slot_path.data(), slot_path.size() * sizeof(uint32_t), &dslot));
+  RZ_TRY(up(subs.data(), subs.size() * sizeof(uint32_t), &dsubs));
   s->sp = (const rzk::ScenePath*)dsp;
+  s->sp_mut = (rzk::ScenePath*)dsp;
   s->slot_path = (const uint32_t*)dslot;
+  s->subpaths = (const uint32_t*)dsubs;
+  s->n_points = (uint32_t)(pts.size() / 2);
   guard.keep = true;
   *out = s;
   return RZ_OK;
@@ -1099,6 +1108,88 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
                              ctx->stream));
   return RZ_OK;
+}
+
+// Adjoint of the batched render (see rz_backward.hpp): the forward pipeline
+// again (its tile lists and records are recomputed, not kept), then the tile
+// pass (compositing adjoint) and the line pass (coverage and subdivision
+// adjoint). Gradients are accumulated into the caller's buffers.
+rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
+                                 const double* view, rzp::C4 bg, int mode, double flatness,
+                                 const float* grad_out, size_t grad_stride, float* grad_points,
+                                 float* grad_colors, double* grad_tr) {
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  if (lw == 0 || lh == 0 || s->n == 0) return RZ_OK;
+  RZ_TRY(ensure(ctx, ctx->out, (size_t)lw * lh * 16));
+  RZ_TRY(enqueue_scene_batched(ctx, s, tr, view, bg, mode, flatness, (rzp::C4*)ctx->out.p, (size_t)lw));
+  const size_t entries = ctx->sc_cap_t;
+  RZ_TRY(ensure(ctx, ctx->bw_w, entries * 256 * sizeof(float)));
+  RZ_TRY(ensure(ctx, ctx->bw_p, entries * 256 * sizeof(rzp::C4)));
+  const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b;
+  const size_t sw = rzk::kStatusWords;
+  uint32_t* meta = (uint32_t*)ctx->meta.p;
+  rzk::Status* st = (rzk::Status*)meta;
+  rzk::SceneFineArgs a;
+  a.pi = (const rzk::PathInfo*)ctx->sc_pi.p;
+  a.rects = (const int4*)ctx->sc_rects.p;
+  a.rows = (const uint32_t*)ctx->sc_rows.p;
+  a.row_count = (const uint32_t*)ctx->sc_rowcnt.p;
+  a.n_paths = s->n;
+  a.tile_off = (const uint32_t*)ctx->tile_off.p;
+  a.recs = (const float4*)ctx->recs.p;
+  a.bd = (const int32_t*)(meta + sw + ct);
+  a.band_ev_off = (const uint32_t*)ctx->ev_off.p;
+  a.events = (const int4*)ctx->events.p;
+  a.redge = (const int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  a.st = st;
+  a.bg = bg;
+  a.out = nullptr;
+  a.row_stride = 0;
+  a.lw = lw;
+  a.lh = lh;
+  a.tiles_x = (lw + 15) / 16;
+  rzk::BwdTileArgs bt;
+  bt.f = a;
+  bt.grad_out = (const rzp::C4*)grad_out;
+  bt.grad_stride = grad_stride;
+  bt.wbuf = (float*)ctx->bw_w.p;
+  bt.pbuf = (rzp::C4*)ctx->bw_p.p;
+  bt.grad_colors = grad_colors;
+  const uint32_t ntiles = (uint32_t)(((lw + 15) / 16) * ((lh + 15) / 16));
+  RZ_LAUNCH(ctx, "k_scene_bwd_tiles", rzk::k_scene_bwd_tiles, dim3(ntiles), dim3(64), bt);
+  // line pass over the same flatten items as the forward
+  const rz_dpath* d = s->all;
+  const int dq = ctx->split_quad, dc = ctx->split_cubic;
+  const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
+                 bc = d->view.blocks_cubic << dc;
+  const uint32_t nblk = bl + bq + bc;
+  if (nblk) {
+    const size_t nwaves = (size_t)nblk * rzk::kBlock / 64;
+    RZ_TRY(ensure(ctx, ctx->bw_tr, nwaves * 6 * sizeof(double)));
+    rzk::BwdLineArgs bl_args;
+    bl_args.pi = (const rzk::PathInfo*)ctx->sc_pi.p;
+    bl_args.slot_path = s->slot_path;
+    bl_args.subpaths = s->subpaths;
+    bl_args.wbuf = (const float*)ctx->bw_w.p;
+    bl_args.grad_points = grad_points;
+    bl_args.grad_tr_part = (double*)ctx->bw_tr.p;
+    rzk::Xform x{};
+    const double fl = 16.0 * flatness * flatness;
+    RZ_LAUNCH(ctx, "k_scene_bwd_lines", rzk::k_scene_bwd_lines, dim3(nblk), dim3(rzk::kBlock), d->view,
+              x, fl, bl, bq, dq, dc, (const rzk::PathInfo*)ctx->sc_pi.p, s->slot_path, bl_args,
+              (const rzk::Status*)st);
+    if (grad_tr)
+      RZ_LAUNCH(ctx, "k_sum_tr", rzk::k_sum_tr, dim3(1), dim3(rzk::kBlock),
+                (const double*)ctx->bw_tr.p, nwaves, grad_tr);
+  }
+  return check_launch(ctx, "scene backward");
+}
+
+__global__ void k_set_colors(rzk::ScenePath* sp, const float* colors, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && sp[i].paint.kind == rzp::PAINT_SOLID)
+    sp[i].paint.color = rzp::c4(colors[4 * i], colors[4 * i + 1], colors[4 * i + 2], colors[4 * i + 3]);
 }
 
 rz_status atlas_set_impl(rz_ctx* ctx, rz_dscene* s, const rz_atlas_slot* slots) {
@@ -1608,6 +1699,37 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
   if (st != RZ_OK) return st;
   RZ_HIP(ctx, hipMemcpy(atlas, ctx->out.p, bytes, hipMemcpyDeviceToHost));
   return RZ_OK;
+}
+
+rz_status rz_scene_update_device(rz_ctx* ctx, rz_dscene* scene, const double* d_points,
+                                 const float* d_colors) {
+  if (!ctx || !scene) return RZ_E_ARG;
+  if (d_points && scene->n_points)
+    RZ_HIP(ctx, hipMemcpyAsync((void*)scene->all->view.pts, d_points,
+                               (size_t)scene->n_points * 2 * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+  if (d_colors && scene->n)
+    RZ_LAUNCH(ctx, "k_set_colors", k_set_colors, dim3((scene->n + 255) / 256), dim3(256),
+              scene->sp_mut, d_colors, scene->n);
+  return check_launch(ctx, "scene update");
+}
+
+rz_status rz_scene_points(rz_ctx* ctx, const rz_dscene* scene, size_t* n_points) {
+  if (!ctx || !scene || !n_points) return RZ_E_ARG;
+  *n_points = scene->n_points;
+  return RZ_OK;
+}
+
+rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                    double flatness, int edge_mode, const double* view,
+                                    const float* bg, const float* d_grad_rgba, size_t grad_stride,
+                                    float* d_grad_points, float* d_grad_colors, double* d_grad_tr) {
+  RZ_TRY(check_scene_args(ctx, flatness, edge_mode, view));
+  if (!scene || !d_grad_rgba || !d_grad_points || !d_grad_colors)
+    return fail(ctx, RZ_E_ARG, "null argument");
+  const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
+  return enqueue_scene_backward(ctx, scene, tr, view, c, edge_mode, flatness, d_grad_rgba,
+                                grad_stride, d_grad_points, d_grad_colors, d_grad_tr);
 }
 
 }  // extern "C"

@@ -120,15 +120,18 @@ RZ_HD void child(const Node<2>& n, int right, Node<2>* c) { *c = n; }
 // the same operations the recursive split performs, so the recomputed node is
 // bit-identical). This keeps the traversal in registers with no scratch
 // memory; recomputation only happens above the deepest four levels.
+// leaf(start, end, depth, bits): the leaf is the root's subtree at `depth`
+// reached along `bits` (most significant first, 1 = right child), i.e. the
+// curve on t in [bits, bits + 1] / 2^depth of the root.
 template <int NP, typename Leaf>
-RZ_HD int traverse(const Node<NP>& root, double fl, Leaf&& leaf) {
+RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
   Node<NP> cur = root, s0, s1, s2, s3;
   int nst = 0, depth = 0;
   uint64_t bits = 0;
   while (true) {
     if (has_nans(cur)) return ST_NAN;
     if (flatness(cur) < fl) {
-      leaf(cur.p[0], cur.p[NP - 1]);
+      leaf(cur.p[0], cur.p[NP - 1], depth, bits);
       while (depth > 0 && (bits & 1)) {
         bits >>= 1;
         --depth;
@@ -165,6 +168,11 @@ RZ_HD int traverse(const Node<NP>& root, double fl, Leaf&& leaf) {
       bits <<= 1;
     }
   }
+}
+
+template <int NP, typename Leaf>
+RZ_HD int traverse(const Node<NP>& root, double fl, Leaf&& leaf) {
+  return traverse_db(root, fl, [&](Pt a, Pt b, int, uint64_t) { leaf(a, b); });
 }
 
 // ---------------------------------------------------------------------------
