@@ -35,6 +35,7 @@ def parse():
                     help="only the scene measurements (for per-kernel profiles of them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 3 times (calibrates HBM counters)")
+    ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the differentiable render")
     ap.add_argument("--cpu-reps", type=int, default=4)
     return ap.parse_args()
 
@@ -159,6 +160,56 @@ def atlas_measure(torch, rz, lib, _abi, frames):
     }, gs
 
 
+def diff_measure(torch, rz, steps):
+    """Differentiable render: 256 closed paths x 32 cubics, 1024x1024 RGBA,
+    L2 loss against a perturbed target, forward + adjoint backward + Adam."""
+    ds, tp, tc = rz.workload_diff(5, 256, 32, 1024.0)
+    dev = rz.DeviceScene(ds.scene())
+    view = (0.0, 0.0, 1024.0, 1024.0)
+    ident = rz.Transform.identity()
+    idx = torch.tensor(ds.index, device="cuda")
+    with torch.no_grad():
+        T = rz.render_torch(dev, torch.tensor(tp, device="cuda")[idx],
+                            torch.tensor(tc, dtype=torch.float32, device="cuda"), ident, view).clone()
+    U = torch.tensor(ds.points, dtype=torch.float32, device="cuda", requires_grad=True)
+    Cc = torch.tensor(ds.colors, dtype=torch.float32, device="cuda", requires_grad=True)
+    opt = torch.optim.Adam([{"params": [U], "lr": 0.25}, {"params": [Cc], "lr": 0.005}])
+
+    def step():
+        out = rz.render_torch(dev, U[idx], Cc, ident, view)
+        loss = ((out - T) ** 2).sum()
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        return loss
+
+    loss0 = float(step())
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(steps):
+        loss = step()
+    b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lossN = float(loss)
+    dev.close()
+    return {
+        "workload": "differentiable render: 256 closed paths x 32 cubics (control points U[0,1024)^2), "
+                    "solid RGBA, 1024x1024, L2 loss vs target (points + N(0,2px), colours + N(0,0.05))",
+        "parameters": int(U.numel() + Cc.numel()),
+        "adam_steps": steps,
+        "ms_per_step": round(wall * 1e3 / steps, 3),
+        "gpu_ms_per_step": round(a.elapsed_time(b) / steps, 3),
+        "loss_first": round(loss0, 2),
+        "loss_last": round(lossN, 2),
+        "gradient_check": "tests/test_gpu_backward.py: adjoint vs f64 central finite differences",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -188,7 +239,8 @@ def main():
             scene_measure(torch, rz, lib, _abi, "circles", rz.workload_circles(1024, 1024.0),
                           1024, 1024, 5, 1),
         ]
-        print(json.dumps({"scenes": scenes}))
+        diff = diff_measure(torch, rz, 5) if args.diff_steps > 0 else None
+        print(json.dumps({"scenes": scenes, "differentiable": diff}))
         return
 
     W = H = 4096
@@ -277,8 +329,11 @@ def main():
 
     atlas = None
     glyph_set = None
+    diff = None
     if not args.no_scene:
         atlas, glyph_set = atlas_measure(torch, rz, lib, _abi, 5)
+        if args.diff_steps > 0:
+            diff = diff_measure(torch, rz, args.diff_steps)
 
     cpu = None
     if not args.no_cpu:
@@ -352,6 +407,7 @@ def main():
         },
         "scenes": scenes,
         "atlas": atlas,
+        "differentiable": diff,
         "cpu_baseline": cpu,
     }
     print(json.dumps(rec), flush=True)

@@ -707,6 +707,55 @@ def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLA
     return _Render.apply(points, colors)
 
 
+class DiffScene:
+    """A scene of closed cubic paths parameterised for optimisation: `points`
+    (n_paths * 3 * n_cubics, 2) unique control points (consecutive cubics
+    share an endpoint), `colors` (n_paths, 4) solid premultiplied RGBA, and
+    `index` mapping each stored path point (4 per cubic, scene order) to its
+    unique point."""
+
+    def __init__(self, points, colors, n_cubics):
+        self.points = np.asarray(points, np.float64).reshape(-1, 2)
+        self.colors = np.asarray(colors, np.float32).reshape(-1, 4)
+        self.n_paths = self.colors.shape[0]
+        self.n_cubics = n_cubics
+        m = 3 * n_cubics
+        idx = []
+        for k in range(self.n_paths):
+            for i in range(n_cubics):
+                idx += [k * m + 3 * i, k * m + 3 * i + 1, k * m + 3 * i + 2, k * m + (3 * i + 3) % m]
+        self.index = np.array(idx, np.int64)
+
+    def paths(self, points=None):
+        u = self.points if points is None else np.asarray(points, np.float64).reshape(-1, 2)
+        stored = u[self.index].reshape(self.n_paths, -1)
+        nc = self.n_cubics
+        kinds = np.full(nc, 2, np.uint8)
+        offs = np.arange(nc + 1, dtype=np.uint32) * 4
+        subs = np.array([0, nc], np.uint32)
+        closed = np.array([1], np.uint8)
+        return [Path(kinds, offs, stored[k], subs, closed) for k in range(self.n_paths)]
+
+    def scene(self, points=None, colors=None, rule=FillRule.NonZero):
+        cols = self.colors if colors is None else np.asarray(colors, np.float32).reshape(-1, 4)
+        return Scene.group([Scene.fill(p, LinColor(*c), rule) for p, c in zip(self.paths(points), cols)])
+
+
+def workload_diff(seed=5, n_paths=256, n_cubics=32, size=1024.0):
+    """Differentiable render workload: closed paths of cubics with control
+    points uniform in [0, size)^2 and random translucent solid colours, plus a
+    target made by moving the points by N(0, 2 px) and the colours by
+    N(0, 0.05). Returns (scene, target_points, target_colors)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(0.0, size, (n_paths * 3 * n_cubics, 2))
+    a = rng.uniform(0.3, 0.9, (n_paths, 1))
+    cols = np.concatenate([rng.uniform(0.0, 1.0, (n_paths, 3)) * a, a], 1)
+    tp = pts + rng.normal(0.0, 2.0, pts.shape)
+    tc = np.clip(cols + rng.normal(0.0, 0.05, cols.shape), 0.0, 1.0)
+    tc[:, :3] = np.minimum(tc[:, :3], tc[:, 3:4])
+    return DiffScene(pts, cols, n_cubics), tp, tc
+
+
 class WorkloadSet:
     """Owned set of generated paths (include/rz_workloads.h); scene sets also
     carry a paint and a fill rule per path."""
