@@ -294,8 +294,9 @@ def main():
         "k_flat_count": (seg_in, "16*control points + 4*segments (read)"),
         "k_flat_emit": (seg_in + 32 * n_lines,
                         "16*control points + 4*segments (read) + 32*lines (f64, written)"),
-        "k_bin_lines<0>": (32 * n_lines, "32*lines (f64, read)"),
-        "k_bin_lines<1>": (32 * n_lines + 16 * n_recs, "32*lines (read) + 16*cell records (written)"),
+        "k_bin_lines": (32 * n_lines, "32*lines (f64, read)"),
+        "k_bin_scatter": (8 * n_lines + 32 * n_recs,
+                          "8*lines (tile ids, read) + 16*cell records (read) + 16*cell records (written)"),
         "k_fine_mask<0>": fine_bytes,
         "k_fine_mask<1>": fine_bytes,
     }

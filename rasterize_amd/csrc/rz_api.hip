@@ -48,7 +48,10 @@ struct rz_ctx {
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
   DBuf sc_boxes, sc_has, sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
+  DBuf bin_tmp;   // binning: kept records and their tiles, per line
+  DBuf long_buf;  // binning: long-line record list and event list
   uint32_t rec_cap = 0, ev_cap = 0;
+  uint32_t long_cap = 4096;  // long-line record list entries per part
   rzk::Status* host_status = nullptr;  // pinned
   rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
@@ -390,6 +393,39 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
   return check_launch(ctx, "flatten");
 }
 
+// Binning of the flattened lines around the scans: the count pass, then (after
+// the scans) the scatter of the kept records and the listed events. `fill`
+// (kLongParts + 1 words) must be zero when the count pass starts.
+template <bool MULTI>
+rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
+                            const rzk::PathInfo* pi, uint32_t* fill, rzk::BinBufs* b) {
+  ctx->long_cap = std::max<uint32_t>(ctx->long_cap, lb.cap / (16 * rzk::kLongParts));
+  const size_t nl = (size_t)rzk::kLongParts * ctx->long_cap;
+  RZ_TRY(ensure(ctx, ctx->bin_tmp, (size_t)lb.cap * 40));
+  RZ_TRY(ensure(ctx, ctx->long_buf, nl * 20 + (size_t)ctx->ev_cap * 20));
+  b->lrec = (float4*)ctx->bin_tmp.p;
+  b->lrec_stride = lb.cap;
+  b->ltile = (uint2*)((char*)ctx->bin_tmp.p + (size_t)lb.cap * 32);
+  char* lp = (char*)ctx->long_buf.p;
+  b->long_rec = (float4*)lp;
+  b->ev_list = (int4*)(lp + nl * 16);
+  b->long_tile = (uint32_t*)(lp + nl * 16 + (size_t)ctx->ev_cap * 16);
+  b->ev_band = b->long_tile + nl;
+  b->long_cap = ctx->long_cap;
+  b->fill = fill;
+  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(ctx->bin_blocks),
+            dim3(rzk::kBlock), (const double4*)lb.lines, (const uint32_t*)lb.line_path,
+            (const uint32_t*)lb.count, g0, pi, *b);
+  return RZ_OK;
+}
+
+rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::LineBufs& lb, const rzk::BinBufs& b) {
+  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(ctx->bin_blocks), dim3(rzk::kBlock),
+            (const uint32_t*)lb.count, b);
+  RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b);
+  return RZ_OK;
+}
+
 // Enqueue flatten, count pass, scans and scatter pass. Lines, records and
 // events land in ctx->lines / ctx->recs / ctx->events (capacity-checked on the
 // device; finish() grows them and asks for a re-run).
@@ -399,7 +435,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2);
+  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kLongParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, (tiles + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, (ty + 1) * 4));
@@ -443,11 +479,9 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.ev_cap = ctx->ev_cap;
   b.redge = P.redge;
   b.st = P.st;
-  const uint32_t gb = ctx->bin_blocks;
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, false>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)nullptr, (const uint32_t*)lb.count, P.grid.g,
-            (const rzk::PathInfo*)nullptr, b);
+  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr,
+                                 meta + meta_u32 - (rzk::kLongParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   RZ_TRY(check_launch(ctx, "bin pass 0"));
@@ -455,9 +489,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
   RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
             P.grid.g.tiles_x, P.redge, (int)(H + 2));
-  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, false>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)nullptr, (const uint32_t*)lb.count, P.grid.g,
-            (const rzk::PathInfo*)nullptr, b);
+  RZ_TRY(enqueue_bin_place(ctx, lb, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   return check_launch(ctx, "bin pass 1");
@@ -579,6 +611,8 @@ rz_status finish(rz_ctx* ctx) {
       ctx->ev_cap = s.n_events * 5 / 4 + 1024;
       RZ_TRY(ensure(ctx, ctx->events, (size_t)ctx->ev_cap * 16));
     }
+    if ((s.overflow & 16u) && s.n_long > ctx->long_cap)
+      ctx->long_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_long * 5 / 4 + 1024, 1u << 24);
     if (s.overflow & 8u) {
       // a batched scene's per-tile / per-band / per-row slices (totals in pad[])
       auto grow = [](uint32_t need, uint32_t cap) {
@@ -858,7 +892,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   }
   const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b, cr = ctx->sc_cap_r;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kLongParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -916,10 +950,8 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  const uint32_t gb = ctx->bin_blocks, gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, true>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
-            (const rzk::PathInfo*)pi, b);
+  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kLongParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "scene pass 0"));
@@ -927,9 +959,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, true>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
-            (const rzk::PathInfo*)pi, b);
+  RZ_TRY(enqueue_bin_place(ctx, lb, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
@@ -1036,7 +1066,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   }
   const uint32_t ct = s->atlas_tiles, cb = s->atlas_bands, cr = s->atlas_rows;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kLongParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -1071,10 +1101,8 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  const uint32_t gb = ctx->bin_blocks, gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_LAUNCH(ctx, "k_bin_lines<0>", (rzk::k_bin_lines<0, true>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
-            (const rzk::PathInfo*)pi, b);
+  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kLongParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
@@ -1082,9 +1110,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_LAUNCH(ctx, "k_bin_lines<1>", (rzk::k_bin_lines<1, true>), dim3(gb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
-            (const rzk::PathInfo*)pi, b);
+  RZ_TRY(enqueue_bin_place(ctx, lb, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;
@@ -1314,8 +1340,13 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   if (!ctx) return RZ_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DBuf* bufs[] = {&ctx->meta, &ctx->tile_off, &ctx->ev_off, &ctx->sums, &ctx->recs,
-                  &ctx->events, &ctx->out, &ctx->lines, &ctx->counts};
+  DBuf* bufs[] = {&ctx->meta,      &ctx->tile_off,  &ctx->ev_off,    &ctx->sums,
+                  &ctx->recs,      &ctx->events,    &ctx->out,       &ctx->lines,
+                  &ctx->counts,    &ctx->stops,     &ctx->sticky,    &ctx->line_path,
+                  &ctx->fl_counts, &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
+                  &ctx->sc_boxes,  &ctx->sc_has,    &ctx->sc_pi,     &ctx->sc_counts,
+                  &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->bin_tmp,
+                  &ctx->long_buf};
   for (DBuf* b : bufs) free_buf(*b);
   for (auto& r : ctx->pending) {
     (void)hipEventDestroy(r.a);

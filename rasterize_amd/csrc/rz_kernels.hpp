@@ -1,7 +1,8 @@
 // HIP kernels of the mask pipeline for gfx950:
 //   flatten  -- per-segment subdivision tree walk (f64, bit-exact)
-//   bin      -- flatten fused with cell binning: per-tile records, band
-//               backdrops, free-endpoint events (count pass, scatter pass)
+//   bin      -- cell binning of the flattened lines: per-tile records, band
+//               backdrops, free-endpoint events (count pass that keeps short
+//               lines' records, scans, scatter)
 //   fine     -- per-tile LDS accumulation of signed area/cover, DPP row scan,
 //               fill rule, coalesced f32 stores
 #pragma once
@@ -32,11 +33,13 @@ struct DPath {
 
 struct Status {
   uint32_t err;        // first error (rzg::Status), 0 if none
-  uint32_t overflow;   // bit0 records, bit1 events, bit2 lines outgrew their buffer
+  uint32_t overflow;   // bit0 records, bit1 events, bit2 lines, bit3 scene slices,
+                       // bit4 a wave's long-line list outgrew its buffer
   uint32_t n_records;  // total records (written by the scan)
   uint32_t n_events;
   uint32_t n_lines;    // flattened lines (atomic counter of the flatten kernel)
-  uint32_t pad[3];
+  uint32_t pad[3];     // a batched scene's slice totals
+  uint32_t n_long;     // longest long-line list of a wave, when one overflowed
 };
 constexpr int kStatusWords = sizeof(Status) / 4;
 
@@ -309,22 +312,42 @@ __global__ __launch_bounds__(kBlock) void k_flat_emit(DPath p, Xform x, double f
 }
 
 // ---------------------------------------------------------------------------
-// Binning of lines into cell records. PASS 0 counts records per tile,
-// accumulates band-bottom crossings and counts events per band; PASS 1 writes
-// records and events into the slots reserved by the scans.
+// Binning of lines into cell records, one geometry pass per line:
+//   k_bin_lines   counts records per tile, accumulates band-bottom crossings,
+//                 right-edge rows and events per band, and keeps what it
+//                 made: the records of a short line (at most two: 98% of
+//                 flattened lines) in per-line slots, those of a longer line
+//                 in a long-record list, events in an event list;
+//   (scans of the tile and band counts)
+//   k_bin_scatter moves the kept records into their tiles' slots;
+//   k_bin_events  moves the listed events into their bands' slots.
+// The lists are shared by all waves through kLongParts bump counters (a wave
+// takes its entries with one atomic), so no single counter is contended.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kLongParts = 16;
+
 struct BinBufs {
-  uint32_t* tile_count;   // PASS 0: counts; PASS 1: cursor (starts at tile offsets)
+  uint32_t* tile_count;   // counts; after the scan: cursors (start at the tile offsets)
   int32_t* bd_diff;       // [tiles] band backdrop differences
-  uint32_t* band_ev;      // PASS 0: counts per band; PASS 1: cursor per band
+  uint32_t* band_ev;      // counts per band; after the scan: cursors
   float4* recs;
   uint32_t rec_cap;
   int4* events;           // (band, col, yrel bits, sign bits)
   uint32_t ev_cap;
-  int32_t* redge;         // [H + 2] right-edge row differences (PASS 0)
+  int32_t* redge;         // [H + 2] right-edge row differences
+  uint2* ltile;           // [lines] tiles of a short line's records (~0u: none)
+  float4* lrec;           // [2][lrec_stride] its first and second record
+  uint32_t lrec_stride;
+  uint32_t* long_tile;    // [kLongParts][long_cap] long lines' records: tile
+  float4* long_rec;       //   ... and record
+  uint32_t long_cap;
+  uint32_t* fill;         // [kLongParts + 1] entries taken per long-record part, then events
+  int4* ev_list;          // [ev_cap] events of the count pass
+  uint32_t* ev_band;      // [ev_cap] their band slots (band base + band)
   Status* st;
 };
 
+// Count (PASS 0) or place (PASS 1) the free-endpoint event of vertex p.
 template <int PASS>
 __device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, uint32_t band_base, Pt p,
                                  float sign) {
@@ -340,28 +363,33 @@ __device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, uint32
   }
 }
 
-template <int PASS>
-__device__ inline void put_record(const BinBufs& b, int tile, const rzg::Rec& r) {
-  if (PASS == 0) {
-    atomicAdd(&b.tile_count[tile], 1u);
-  } else {
-    uint32_t slot = atomicAdd(&b.tile_count[tile], 1u);
-    if (slot < b.rec_cap) b.recs[slot] = make_float4(r.x0, r.y0, r.x1, r.y1);
-    else atomicOr(&b.st->overflow, 1u);
-  }
+__device__ inline int4 event_bits(const rzg::Event& ev) {
+  return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
 }
 
-// One thread per line (grid-stride, lanes of a wave on consecutive lines; the
-// next line is loaded while this one is binned). A line with at most two
-// records (the common case: flattened lines are a few pixels long and cross
-// at most one cell edge) keeps them in registers and takes part in a
-// wave-wide aggregation: the records of the wave that go to one tile get
-// consecutive slots from a single atomic, and all of a wave's atomics are in
-// flight together. Longer lines use one atomic per record. Lines beyond the
-// right clip only add to the right-edge column; two such lines on neighbouring
-// lanes that continue each other drop their shared endpoint (its two terms
-// cancel exactly).
-template <int PASS, bool MULTI>
+// Exclusive prefix sum of v over the wave (every lane must take part); the
+// wave total is returned in *total.
+__device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
+  uint32_t s = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = (uint32_t)__shfl_up((int)s, d, 64);
+    if (lane >= d) s += u;
+  }
+  *total = (uint32_t)__shfl((int)s, 63, 64);
+  return s - v;
+}
+
+// Count pass, one thread per line (grid-stride, lanes of a wave on consecutive
+// lines; the next line is loaded while this one is binned). A line with at
+// most two records keeps them in registers; the wave then counts them per
+// tile with one atomic per distinct tile (each round takes the tile of the
+// first pending record and groups every pending record of that tile) and
+// stores them in the line's slots. A longer line is binned a second time,
+// counting and listing each record. Lines beyond the right clip only add to
+// the right-edge column; two such lines on neighbouring lanes that continue
+// each other drop their shared endpoint (its two terms cancel exactly).
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
                                                        const uint32_t* line_path,
                                                        const uint32_t* n_lines, rzg::BinGrid g0,
@@ -371,16 +399,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   const uint32_t stride = gridDim.x * kBlock;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t part = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) % kLongParts;
   const double4 zero = make_double4(0, 0, 0, 0);
   uint32_t first = blockIdx.x * kBlock + threadIdx.x;
   double4 Lnext = first < n ? lines[first] : zero;
-  // the loop bound is uniform per wave so every lane reaches the ballots
+  // the loop bound is uniform per wave so every lane reaches the wave operations
   for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     const double4 L = Lnext;
     Lnext = i + stride < n ? lines[i + stride] : zero;
     int nrec = 0, t0 = -1, t1 = -1;
     rzg::Rec r0{}, r1{};  // separate registers: no dynamic indexing
+    int nev = 0;
+    int4 ev0{}, ev1{};
+    uint32_t evb0 = 0, evb1 = 0;
     bool have = i < n;
     // this line's path: grid and slices of the per-tile / per-band / per-row arrays
     uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
@@ -399,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
       have = false;
     }
     const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
-    if (PASS == 0) {
+    {
       const double nx = __shfl_down(L.x, 1, 64), ny = __shfl_down(L.y, 1, 64);
       const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
       const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
@@ -413,8 +445,9 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
         if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
       }
     }
+    rzg::EffLine e{};
     if (have && !proj) {
-      const rzg::EffLine e = rzg::effective_line(p0, p1, g);
+      e = rzg::effective_line(p0, p1, g);
       auto collect = [&](int t, const rzg::Rec& r) {
         if (nrec == 0) {
           t0 = (int)tb + t;
@@ -426,27 +459,139 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
         ++nrec;
       };
       auto bd = [&](int band, int col, int dir) {
-        if (PASS == 0) atomicAdd(&b.bd_diff[tb + band * g.tiles_x + col], dir);
+        atomicAdd(&b.bd_diff[tb + band * g.tiles_x + col], dir);
       };
       rzg::bin_piece(e.p[0], e.p[1], g, collect, bd);
       if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, collect, bd);
-      if (e.clip0) put_event<PASS>(b, g, bb, e.p[0], -1.0f);
-      if (e.clip1) put_event<PASS>(b, g, bb, e.n == 2 ? e.p[2] : e.p[1], 1.0f);
-      if (nrec > 2) {
-        // long line: regenerate its records with one atomic each
-        auto direct = [&](int t, const rzg::Rec& r) { put_record<PASS>(b, (int)tb + t, r); };
-        auto nobd = [&](int, int, int) {};
-        rzg::bin_piece(e.p[0], e.p[1], g, direct, nobd);
-        if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, direct, nobd);
-        nrec = 0;
+      rzg::Event ev;
+      if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
+        ev0 = event_bits(ev);
+        evb0 = bb + (uint32_t)ev.band;
+        atomicAdd(&b.band_ev[evb0], 1u);
+        nev = 1;
+      }
+      if (e.clip1 && rzg::make_event(e.n == 2 ? e.p[2] : e.p[1], 1.0f, g, &ev)) {
+        if (nev == 0) {
+          ev0 = event_bits(ev);
+          evb0 = bb + (uint32_t)ev.band;
+          atomicAdd(&b.band_ev[evb0], 1u);
+        } else {
+          ev1 = event_bits(ev);
+          evb1 = bb + (uint32_t)ev.band;
+          atomicAdd(&b.band_ev[evb1], 1u);
+        }
+        ++nev;
       }
     }
-    // Wave aggregation of the register records: each round takes the tile of
-    // the first pending record and groups every pending record of that tile;
-    // slots within a group are ordered by (lane, record).
+    const bool is_long = nrec > 2;
+    // short lines: count per tile, wave-aggregated, and keep the records
+    bool pend0 = !is_long && nrec >= 1, pend1 = !is_long && nrec == 2;
+    while (true) {
+      const uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
+      if ((m0 | m1) == 0) break;
+      const bool use0 = m0 != 0;
+      const int leader = __ffsll((unsigned long long)(use0 ? m0 : m1)) - 1;
+      const int T = __builtin_amdgcn_readlane(use0 ? t0 : t1, leader);
+      const bool mine0 = pend0 && t0 == T, mine1 = pend1 && t1 == T;
+      const uint32_t total = (uint32_t)(__popcll(__ballot(mine0)) + __popcll(__ballot(mine1)));
+      if (lane == leader) atomicAdd(&b.tile_count[T], total);
+      pend0 = pend0 && !mine0;
+      pend1 = pend1 && !mine1;
+    }
+    if (i < n) {
+      const bool keep = !is_long && nrec > 0;
+      b.ltile[i] = keep ? make_uint2((uint32_t)t0, nrec == 2 ? (uint32_t)t1 : ~0u)
+                        : make_uint2(~0u, ~0u);
+      if (keep) b.lrec[i] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
+      if (keep && nrec == 2) b.lrec[b.lrec_stride + i] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
+    }
+    // long lines: take list entries for the wave, bin again, count and list
+    uint32_t ltot;
+    const uint32_t lpre = wave_excl_scan(is_long ? (uint32_t)nrec : 0u, lane, &ltot);
+    if (ltot) {
+      uint32_t lbase = 0;
+      if (lane == 0) lbase = atomicAdd(&b.fill[part], ltot);
+      lbase = (uint32_t)__shfl((int)lbase, 0, 64);
+      // past the list's end the records are still counted (the record total
+      // then sizes the re-run exactly)
+      const bool fits = lbase + ltot <= b.long_cap;
+      if (!fits && lane == 0) {
+        atomicOr(&b.st->overflow, 16u);
+        atomicMax(&b.st->n_long, lbase + ltot);
+      }
+      if (is_long) {
+        size_t k = (size_t)part * b.long_cap + lbase + lpre;
+        auto list = [&](int t, const rzg::Rec& r) {
+          atomicAdd(&b.tile_count[tb + t], 1u);
+          if (fits) {
+            b.long_tile[k] = tb + (uint32_t)t;
+            b.long_rec[k] = make_float4(r.x0, r.y0, r.x1, r.y1);
+          }
+          ++k;
+        };
+        auto nobd = [&](int, int, int) {};
+        rzg::bin_piece(e.p[0], e.p[1], g, list, nobd);
+        if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, list, nobd);
+      }
+    }
+    // events: list them (at most two per line)
+    const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
+    if (e1) {
+      const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
+      uint32_t ebase = 0;
+      if (lane == 0) ebase = atomicAdd(&b.fill[kLongParts], etot);
+      ebase = (uint32_t)__shfl((int)ebase, 0, 64);
+      const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
+      if (ebase + etot > b.ev_cap) {
+        if (lane == 0) atomicOr(&b.st->overflow, 2u);
+      } else {
+        if (nev >= 1) {
+          b.ev_list[k] = ev0;
+          b.ev_band[k] = evb0;
+        }
+        if (nev == 2) {
+          b.ev_list[k + 1] = ev1;
+          b.ev_band[k + 1] = evb1;
+        }
+      }
+    }
+  }
+}
+
+// Place the kept records into the slots the scan reserved: first the two
+// slots of every line, then the long-record list, one entry per thread after
+// the lines. The records of a wave that go to one tile take consecutive slots
+// from a single atomic (ordered by lane, then record), and all of a wave's
+// atomics are in flight together.
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(const uint32_t* n_lines, BinBufs b) {
+  if (b.st->err || b.st->overflow) return;
+  const uint32_t n = *n_lines;
+  const uint32_t n_all = n + kLongParts * b.long_cap;
+  const uint32_t stride = gridDim.x * kBlock;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (uint32_t base = blockIdx.x * kBlock; base < n_all; base += stride) {
+    const uint32_t v = base + threadIdx.x;
+    int nrec = 0, t0 = -1, t1 = -1;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0;
+    if (v < n) {
+      const uint2 tt = b.ltile[v];
+      t0 = (int)tt.x;
+      t1 = (int)tt.y;
+      nrec = tt.x == ~0u ? 0 : (tt.y == ~0u ? 1 : 2);
+      if (nrec >= 1) r0 = b.lrec[v];
+      if (nrec == 2) r1 = b.lrec[b.lrec_stride + v];
+    } else if (v < n_all) {
+      const uint32_t p = (v - n) / b.long_cap, j = (v - n) - p * b.long_cap;
+      if (j < b.fill[p]) {
+        t0 = (int)b.long_tile[v - n];
+        r0 = b.long_rec[v - n];
+        nrec = 1;
+      }
+    }
     bool pend0 = nrec >= 1, pend1 = nrec == 2;
-    uint32_t tot0 = 0, tot1 = 0;    // group sizes this lane leads (for t0 / t1)
-    int ldr0 = 0, ldr1 = 0;         // lane leading the group of record 0 / 1
+    uint32_t tot0 = 0, tot1 = 0;      // group sizes this lane leads (for t0 / t1)
+    int ldr0 = 0, ldr1 = 0;           // lane leading the group of record 0 / 1
     bool sel0 = false, sel1 = false;  // ... through its t1 atomic
     uint32_t off0 = 0, off1 = 0;
     while (true) {
@@ -479,20 +624,29 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     uint32_t s0 = 0, s1 = 0;
     if (tot0) s0 = atomicAdd(&b.tile_count[t0], tot0);
     if (tot1) s1 = atomicAdd(&b.tile_count[t1], tot1);
-    if (PASS == 1) {
-      const uint32_t a0 = (uint32_t)__shfl((int)s0, ldr0, 64), b0 = (uint32_t)__shfl((int)s1, ldr0, 64);
-      const uint32_t a1 = (uint32_t)__shfl((int)s0, ldr1, 64), b1 = (uint32_t)__shfl((int)s1, ldr1, 64);
-      if (nrec >= 1) {
-        const uint32_t slot = (sel0 ? b0 : a0) + off0;
-        if (slot < b.rec_cap) b.recs[slot] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
-        else atomicOr(&b.st->overflow, 1u);
-      }
-      if (nrec == 2) {
-        const uint32_t slot = (sel1 ? b1 : a1) + off1;
-        if (slot < b.rec_cap) b.recs[slot] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
-        else atomicOr(&b.st->overflow, 1u);
-      }
+    const uint32_t a0 = (uint32_t)__shfl((int)s0, ldr0, 64), b0 = (uint32_t)__shfl((int)s1, ldr0, 64);
+    const uint32_t a1 = (uint32_t)__shfl((int)s0, ldr1, 64), b1 = (uint32_t)__shfl((int)s1, ldr1, 64);
+    if (nrec >= 1) {
+      const uint32_t slot = (sel0 ? b0 : a0) + off0;
+      if (slot < b.rec_cap) b.recs[slot] = r0;
+      else atomicOr(&b.st->overflow, 1u);
     }
+    if (nrec == 2) {
+      const uint32_t slot = (sel1 ? b1 : a1) + off1;
+      if (slot < b.rec_cap) b.recs[slot] = r1;
+      else atomicOr(&b.st->overflow, 1u);
+    }
+  }
+}
+
+// Place the events the count pass listed into their bands' slots.
+__global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b) {
+  if (b.st->err || b.st->overflow) return;
+  const uint32_t n = b.fill[kLongParts];
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const uint32_t slot = atomicAdd(&b.band_ev[b.ev_band[k]], 1u);
+    if (slot < b.ev_cap) b.events[slot] = b.ev_list[k];
+    else atomicOr(&b.st->overflow, 2u);
   }
 }
 
@@ -544,6 +698,7 @@ __global__ void k_status_merge(const Status* src, Status* acc) {
   acc->n_records = max(acc->n_records, src->n_records);
   acc->n_events = max(acc->n_events, src->n_events);
   acc->n_lines = max(acc->n_lines, src->n_lines);
+  acc->n_long = max(acc->n_long, src->n_long);
 }
 
 // Fill a layer with its background colour (Layer::new, src/scene.rs:483-501).

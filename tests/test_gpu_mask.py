@@ -121,6 +121,50 @@ def test_bin_bit_exact(case, mode, c1, c2):
     assert np.array_equal(_sort_events(ge), _sort_events(re)), "events differ"
 
 
+def _spokes(n, size):
+    """n thin triangles from the centre almost to the border, the last ones
+    crossing the right edge: nearly every line spans many cells."""
+    pb = PathBuilder()
+    c, r = size / 2.0, size * 0.55
+    for k in range(n):
+        a0 = 2.0 * math.pi * k / n
+        a1 = a0 + math.pi / n
+        pb.move_to((c, c))
+        pb.line_to((c + r * math.cos(a0), c + r * math.sin(a0)))
+        pb.line_to((c + r * math.cos(a1), c + r * math.sin(a1)))
+        pb.close()
+    return pb.build()
+
+
+@pytest.mark.parametrize("mode", [rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_ITER])
+def test_bin_long_lines_fresh_context(mode):
+    """Long lines only, on a fresh context: the first run outgrows the record
+    and long-line buffers and the re-run must still be bit-exact."""
+    path = _spokes(3000, 4096)
+    ctx = rz.Context(0)
+    try:
+        _say(f"bin spokes mode={mode}")
+        gc, gr, gb, ge, gx = rz.bin_stage(path, I, 4096, 4096, mode, ctx=ctx)
+    finally:
+        ctx.close()
+    rc, rr, rb, re, rx = O.bin_replay(path, I, 4096, 4096, mode)
+    _say(f"records {len(rr)} events {len(re)}")
+    assert len(rr) > 500_000
+    assert np.array_equal(gc, rc), "per-tile record counts differ"
+    assert np.array_equal(gb, rb), "band backdrops differ"
+    assert np.array_equal(gx, rx), "right-edge column differs"
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr)), "records differ"
+    assert np.array_equal(_sort_events(ge), _sort_events(re)), "events differ"
+
+
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_mask_long_lines(rule):
+    path = _spokes(3000, 4096)
+    g = SignedDifferenceRasterizer().mask_f32(path, I, 4096, 4096, rule)
+    r = O.sd_mask(path, I, 4096, 4096, rule)
+    check_close(g, r, f"spokes rule={rule}")
+
+
 # ---- masks: reference known answers on the GPU --------------------------------
 @pytest.mark.parametrize("R", [SignedDifferenceRasterizer, ActiveEdgeRasterizer])
 def test_rasterizer_octagon_evenodd(R):
