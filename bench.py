@@ -82,6 +82,13 @@ def settle(lib, _abi, ctx, run):
     ctx.check(s)
 
 
+def last_counts(lib, ctx):
+    """Flattened lines and cell records of the context's last run."""
+    nl, nr, ne = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    ctx.check(lib.rz_last_counts(ctx.h, C.byref(nl), C.byref(nr), C.byref(ne)))
+    return nl.value, nr.value
+
+
 def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
     """Batched and plain Scene::render of a generated scene into a W x H layer."""
     ds = rz.DeviceScene(ws.scene())
@@ -105,12 +112,15 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
         res[batched] = a.elapsed_time(b) / frames
     k = kernel_stats(lib, _abi, ctx, lambda: ds.render_device(ident, view, out.data_ptr(), W), 3)
     ctx.check(lib.rz_sync(ctx.h))
+    n_lines, n_recs = last_counts(lib, ctx)
     ds.close()
     segs = sum(p.segments_count() for p in ws.paths)
     return {
         "workload": name,
         "paths": len(ws.paths),
         "segments": segs,
+        "lines": n_lines,
+        "cell_records": n_recs,
         "layer": [W, H],
         "batched_ms_per_frame": round(res[True], 4),
         "plain_ms_per_frame": round(res[False], 3),
@@ -143,6 +153,7 @@ def atlas_measure(torch, rz, lib, _abi, frames):
     ms = a.elapsed_time(b) / frames
     k = kernel_stats(lib, _abi, ctx, run, 2)
     ctx.check(lib.rz_sync(ctx.h))
+    n_lines, n_recs = last_counts(lib, ctx)
     dg.close()
     px = sum(s.width * s.height for s in gs.slots)
     segs = sum(p.segments_count() for p in gs.paths)
@@ -152,6 +163,8 @@ def atlas_measure(torch, rz, lib, _abi, frames):
                     f"one {W}x{H} f32 atlas, NonZero",
         "glyphs": len(gs.paths),
         "segments": segs,
+        "lines": n_lines,
+        "cell_records": n_recs,
         "mask_pixels": px,
         "ms_per_frame": round(ms, 4),
         "mpix_s": round(px / ms / 1e3, 1),

@@ -48,15 +48,13 @@ struct rz_ctx {
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
   DBuf sc_boxes, sc_has, sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
-  DBuf bin_tmp;   // binning: kept records and their tiles, per line
-  DBuf long_buf;  // binning: long-line record list and event list
+  DBuf list_buf;  // binning: record list (records, tiles) and event list
   uint32_t rec_cap = 0, ev_cap = 0;
-  uint32_t long_cap = 4096;  // long-line record list entries per part
+  uint32_t part_cap = 0;  // overflow record-list entries per part
   rzk::Status* host_status = nullptr;  // pinned
   rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
   int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
-  uint32_t bin_blocks = 2048;          // grid of the per-line binning kernels
   uint32_t line_cap = 0;
   // kernel timing (rz_profile)
   bool prof = false;
@@ -394,33 +392,48 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
 }
 
 // Binning of the flattened lines around the scans: the count pass, then (after
-// the scans) the scatter of the kept records and the listed events. `fill`
-// (kLongParts + 1 words) must be zero when the count pass starts.
+// the scans) the scatter of the listed records and events. `fill`
+// (kParts + 1 words) must be zero when the count pass starts.
 template <bool MULTI>
 rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
                             const rzk::PathInfo* pi, uint32_t* fill, rzk::BinBufs* b) {
-  ctx->long_cap = std::max<uint32_t>(ctx->long_cap, lb.cap / (16 * rzk::kLongParts));
-  const size_t nl = (size_t)rzk::kLongParts * ctx->long_cap;
-  RZ_TRY(ensure(ctx, ctx->bin_tmp, (size_t)lb.cap * 40));
-  RZ_TRY(ensure(ctx, ctx->long_buf, nl * 20 + (size_t)ctx->ev_cap * 20));
-  b->lrec = (float4*)ctx->bin_tmp.p;
-  b->lrec_stride = lb.cap;
-  b->ltile = (uint2*)((char*)ctx->bin_tmp.p + (size_t)lb.cap * 32);
-  char* lp = (char*)ctx->long_buf.p;
-  b->long_rec = (float4*)lp;
+  // one count-pass wave per kWaveLines lines of capacity; a wave's region
+  // holds two entries per line (records are usually about half as many as
+  // lines), the overflow list takes the rest
+  const uint32_t wpb = rzk::kBlock / 64;
+  const uint32_t nw = (uint32_t)(((size_t)lb.cap + rzk::kWaveLines * wpb - 1) /
+                                 (rzk::kWaveLines * wpb) * wpb);
+  const uint32_t wave_cap = 2 * rzk::kWaveLines;
+  if (ctx->part_cap == 0) ctx->part_cap = 1u << 16;
+  const size_t no = (size_t)rzk::kParts * ctx->part_cap;
+  const size_t nl = (size_t)nw * wave_cap + no;
+  const size_t words = nl * 2 + no + (size_t)ctx->ev_cap + nw + (size_t)nw * rzk::kMap;
+  RZ_TRY(ensure(ctx, ctx->list_buf, (nl + ctx->ev_cap) * 16 + words * 4));
+  char* lp = (char*)ctx->list_buf.p;
+  b->list_rec = (float4*)lp;
   b->ev_list = (int4*)(lp + nl * 16);
-  b->long_tile = (uint32_t*)(lp + nl * 16 + (size_t)ctx->ev_cap * 16);
-  b->ev_band = b->long_tile + nl;
-  b->long_cap = ctx->long_cap;
+  uint32_t* w = (uint32_t*)(lp + (nl + ctx->ev_cap) * 16);
+  b->list_tile = w;
+  b->list_rank = w + nl;
+  b->ovf_wave = w + 2 * nl;
+  b->ev_band = b->ovf_wave + no;
+  b->wave_used = b->ev_band + ctx->ev_cap;
+  b->wave_map = b->wave_used + nw;
+  b->n_waves = nw;
+  b->wave_cap = wave_cap;
+  b->part_cap = ctx->part_cap;
   b->fill = fill;
-  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(ctx->bin_blocks),
-            dim3(rzk::kBlock), (const double4*)lb.lines, (const uint32_t*)lb.line_path,
-            (const uint32_t*)lb.count, g0, pi, *b);
+  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb), dim3(rzk::kBlock),
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
+            pi, *b);
   return RZ_OK;
 }
 
 rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::LineBufs& lb, const rzk::BinBufs& b) {
-  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(ctx->bin_blocks), dim3(rzk::kBlock),
+  // many waves, each over regions far apart (neighbouring regions hold
+  // records of the same tiles)
+  const uint32_t grid = std::min<uint32_t>(b.n_waves / (rzk::kBlock / 64), 8192);
+  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kBlock),
             (const uint32_t*)lb.count, b);
   RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b);
   return RZ_OK;
@@ -435,7 +448,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kLongParts + 1;
+  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, (tiles + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, (ty + 1) * 4));
@@ -471,6 +484,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, &lb));
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
+  b.tile_off = P.tile_off;
   b.bd_diff = P.bd;
   b.band_ev = P.band_ev;
   b.recs = (float4*)ctx->recs.p;
@@ -481,11 +495,11 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.st = P.st;
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
   RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr,
-                                 meta + meta_u32 - (rzk::kLongParts + 1), &b));
+                                 meta + meta_u32 - (rzk::kParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   RZ_TRY(check_launch(ctx, "bin pass 0"));
-  RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, P.tile_count, &P.st->n_records));
+  RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, nullptr, &P.st->n_records));
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
   RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
             P.grid.g.tiles_x, P.redge, (int)(H + 2));
@@ -611,8 +625,8 @@ rz_status finish(rz_ctx* ctx) {
       ctx->ev_cap = s.n_events * 5 / 4 + 1024;
       RZ_TRY(ensure(ctx, ctx->events, (size_t)ctx->ev_cap * 16));
     }
-    if ((s.overflow & 16u) && s.n_long > ctx->long_cap)
-      ctx->long_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_long * 5 / 4 + 1024, 1u << 24);
+    if ((s.overflow & 16u) && s.n_list > ctx->part_cap)
+      ctx->part_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_list * 5 / 4 + 1024, 1u << 28);
     if (s.overflow & 8u) {
       // a batched scene's per-tile / per-band / per-row slices (totals in pad[])
       auto grow = [](uint32_t need, uint32_t cap) {
@@ -892,7 +906,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   }
   const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b, cr = ctx->sc_cap_r;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kLongParts + 1;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -941,6 +955,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
+  b.tile_off = tile_off;
   b.bd_diff = bd;
   b.band_ev = band_ev;
   b.recs = (float4*)ctx->recs.p;
@@ -951,11 +966,11 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   b.st = st;
   rzg::BinGrid g0{};
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kLongParts + 1), &b));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "scene pass 0"));
-  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, tile_count, &st->n_records));
+  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, nullptr, &st->n_records));
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
@@ -1066,7 +1081,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   }
   const uint32_t ct = s->atlas_tiles, cb = s->atlas_bands, cr = s->atlas_rows;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kLongParts + 1;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -1092,6 +1107,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
+  b.tile_off = tile_off;
   b.bd_diff = bd;
   b.band_ev = band_ev;
   b.recs = (float4*)ctx->recs.p;
@@ -1102,11 +1118,11 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   b.st = st;
   rzg::BinGrid g0{};
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kLongParts + 1), &b));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kParts + 1), &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
-  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, tile_count, &st->n_records));
+  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, nullptr, &st->n_records));
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
@@ -1345,8 +1361,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                   &ctx->counts,    &ctx->stops,     &ctx->sticky,    &ctx->line_path,
                   &ctx->fl_counts, &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
                   &ctx->sc_boxes,  &ctx->sc_has,    &ctx->sc_pi,     &ctx->sc_counts,
-                  &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->bin_tmp,
-                  &ctx->long_buf};
+                  &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf};
   for (DBuf* b : bufs) free_buf(*b);
   for (auto& r : ctx->pending) {
     (void)hipEventDestroy(r.a);

@@ -301,85 +301,135 @@ struct Event {
   float sign;
 };
 
-// Enumerate the records, band-boundary crossings and free-endpoint events of
-// one effective piece a -> b.
+// The cell decomposition of one effective piece a -> b in three steps that
+// can run on different threads: the piece's band range, then per band its
+// part inside the band (and the band-bottom crossing), then per tile column
+// of that part its record. bin_piece composes them sequentially.
+
+// Bands whose closed y-range meets [ylo, yhi] of the piece: [*ty0, *ty1],
+// possibly empty. Returns false when the piece is entirely above or below the
+// image.
+RZ_HD bool piece_bands(Pt a, Pt b, const BinGrid& g, int* ty0, int* ty1) {
+  const double H = (double)g.H;
+  const double ylo = fmin(a.y, b.y), yhi = fmax(a.y, b.y);
+  if (!(yhi >= 0.0) || !(ylo <= H)) return false;
+  int t0 = ylo <= 0.0 ? 0 : (int)ceil((ylo + (double)g.phy) / (double)kTile) - 1;
+  if (t0 < 0) t0 = 0;
+  int t1 = (int)floor((fmin(yhi, H) + (double)g.phy) / (double)kTile);
+  if (t1 > g.tiles_y - 1) t1 = g.tiles_y - 1;
+  *ty0 = t0;
+  *ty1 = t1;
+  return true;
+}
+
+// The piece within band ty: its crossing of the band's bottom edge (column of
+// the first tile whose left region holds it, -1 if none or beyond the last
+// tile) and its part s -> e inside r0 <= y <= rb with that part's tile
+// columns [tx0, tx1] (any = false when the part is empty or a single point).
+struct BandPiece {
+  Pt s, e;
+  double r0;
+  int tx0, tx1;
+  int cross_col;
+  bool any;
+};
+
+RZ_HD void band_piece(Pt a, Pt b, const BinGrid& g, int ty, BandPiece* bp) {
+  const double H = (double)g.H;
+  const double ylo = fmin(a.y, b.y), yhi = fmax(a.y, b.y);
+  const double r0 = (double)(ty * kTile - g.phy);
+  const double rb = fmin(r0 + (double)kTile, H);
+  bp->r0 = r0;
+  bp->cross_col = -1;
+  bp->any = false;
+  // band-bottom crossing (a point on y = rb counts as inside the band)
+  if (ylo <= rb && rb < yhi) {
+    const Pt c = at_y(a, b, rb);
+    // first tile whose left region holds c (tile 0's is x <= 0, see tile_record)
+    const int col = c.x <= 0.0 ? 0 : tile_col(c.x, g) + 1;
+    if (col < g.tiles_x) bp->cross_col = col;
+  }
+  // piece of a -> b inside r0 <= y <= rb, direction preserved
+  Pt s = a, e = b;
+  if (a.y != b.y) {
+    if (a.y < b.y) {
+      if (s.y < r0) s = at_y(a, b, r0);
+      if (e.y > rb) e = at_y(a, b, rb);
+    } else {
+      if (s.y > rb) s = at_y(a, b, rb);
+      if (e.y < r0) e = at_y(a, b, r0);
+    }
+  } else if (a.y < r0 || a.y > rb) {
+    return;
+  }
+  if (s.x == e.x && s.y == e.y) return;  // touches the band in one point only
+  bp->s = s;
+  bp->e = e;
+  bp->tx0 = tile_col(fmin(s.x, e.x), g);
+  bp->tx1 = tile_col(fmax(s.x, e.x), g);
+  bp->any = true;
+}
+
+// The record of band part s -> e (band top r0) in tile column tx, if any.
+RZ_HD bool tile_record(Pt s, Pt e, double r0, int tx, const BinGrid& g, Rec* out) {
+  const double xlo = fmin(s.x, e.x), xhi = fmax(s.x, e.x);
+  const double c0 = (double)(tx * kTile - g.phx), c1 = c0 + (double)kTile;
+  if (xhi < c0 || xlo > c1) return false;
+  // Tile 0's left region is x <= 0 rather than x < 0: a line on x = 0
+  // deposits its whole cover into pixel 0, exactly like a line left of the
+  // image, so the vertical lines made by the left split (every part of the
+  // path left of the image) need no records at all.
+  const bool first = tx == 0;
+  if (first && xhi <= 0.0) return false;
+  // clip the band part to c0 <= x <= c1
+  Pt u = s, v = e;
+  if (s.x != e.x) {
+    if (s.x < e.x) {
+      if (u.x < c0) u = at_x(s, e, c0);
+      if (v.x > c1) v = at_x(s, e, c1);
+    } else {
+      if (u.x > c1) u = at_x(s, e, c1);
+      if (v.x < c0) v = at_x(s, e, c0);
+    }
+  }
+  // crossings of the cell's left edge x = c0 (inside means x < c0)
+  const bool in_s = first ? s.x <= 0.0 : s.x < c0;
+  const bool in_e = first ? e.x <= 0.0 : e.x < c0;
+  bool flag0 = false, flag1 = false;
+  if (in_s && !in_e) flag0 = u.y != r0;  // exits the left region: +V(u.y)
+  if (!in_s && in_e) flag1 = v.y != r0;  // enters the left region: -V(v.y)
+  // no vertical extent and no edge crossing: nothing to deposit
+  if (u.y == v.y && !flag0 && !flag1) return false;
+  Rec r;
+  r.x0 = (float)(u.x - c0);
+  r.y0 = (float)(u.y - r0);
+  r.x1 = (float)(v.x - c0);
+  r.y1 = (float)(v.y - r0);
+  // the flag is the sign bit; the magnitude stays the piece's x, which is 0
+  // except where tile 0's left region (x <= 0) ends inside the tile (phx > 0)
+  if (flag0) r.x0 = -r.x0;
+  if (flag1) r.x1 = -r.x1;
+  *out = r;
+  return true;
+}
+
+// Enumerate the records and band-boundary crossings of one effective piece
+// a -> b, band by band and left to right.
 //   rec(tile_index, Rec)        -- one cell record
 //   bd(band, col, dir)          -- crossing of the band's bottom edge at column col
 template <typename RecF, typename BdF>
 RZ_HD void bin_piece(Pt a, Pt b, const BinGrid& g, RecF&& rec, BdF&& bd) {
-  const double H = (double)g.H;
-  double ylo = fmin(a.y, b.y), yhi = fmax(a.y, b.y);
-  if (!(yhi >= 0.0) || !(ylo <= H)) return;  // entirely above or below the image
+  int ty0, ty1;
+  if (!piece_bands(a, b, g, &ty0, &ty1)) return;
   const int dir = b.y > a.y ? 1 : -1;
-  // bands whose closed y-range meets [ylo, yhi]
-  int ty0 = ylo <= 0.0 ? 0 : (int)ceil((ylo + (double)g.phy) / (double)kTile) - 1;
-  if (ty0 < 0) ty0 = 0;
-  int ty1 = (int)floor((fmin(yhi, H) + (double)g.phy) / (double)kTile);
-  if (ty1 > g.tiles_y - 1) ty1 = g.tiles_y - 1;
   for (int ty = ty0; ty <= ty1; ++ty) {
-    const double r0 = (double)(ty * kTile - g.phy);
-    const double rb = fmin(r0 + (double)kTile, H);
-    // band-bottom crossing (a point on y = rb counts as inside the band)
-    if (ylo <= rb && rb < yhi) {
-      Pt c = at_y(a, b, rb);
-      // first tile whose left region holds c (tile 0's is x <= 0, see below)
-      int col = c.x <= 0.0 ? 0 : tile_col(c.x, g) + 1;
-      if (col < g.tiles_x) bd(ty, col, dir);
-    }
-    // piece of a -> b inside r0 <= y <= rb, direction preserved
-    Pt s = a, e = b;
-    if (a.y != b.y) {
-      if (a.y < b.y) {
-        if (s.y < r0) s = at_y(a, b, r0);
-        if (e.y > rb) e = at_y(a, b, rb);
-      } else {
-        if (s.y > rb) s = at_y(a, b, rb);
-        if (e.y < r0) e = at_y(a, b, r0);
-      }
-    } else if (a.y < r0 || a.y > rb) {
-      continue;
-    }
-    if (s.x == e.x && s.y == e.y) continue;  // touches the band in one point only
-    const double xlo = fmin(s.x, e.x), xhi = fmax(s.x, e.x);
-    int tx0 = tile_col(xlo, g), tx1 = tile_col(xhi, g);
-    for (int tx = tx0; tx <= tx1; ++tx) {
-      const double c0 = (double)(tx * kTile - g.phx), c1 = c0 + (double)kTile;
-      if (xhi < c0 || xlo > c1) continue;
-      // Tile 0's left region is x <= 0 rather than x < 0: a line on x = 0
-      // deposits its whole cover into pixel 0, exactly like a line left of
-      // the image, so the vertical lines made by the left split (every part of
-      // the path left of the image) need no records at all.
-      const bool first = tx == 0;
-      if (first && xhi <= 0.0) continue;
-      // clip the band piece to c0 <= x <= c1
-      Pt u = s, v = e;
-      if (s.x != e.x) {
-        if (s.x < e.x) {
-          if (u.x < c0) u = at_x(s, e, c0);
-          if (v.x > c1) v = at_x(s, e, c1);
-        } else {
-          if (u.x > c1) u = at_x(s, e, c1);
-          if (v.x < c0) v = at_x(s, e, c0);
-        }
-      }
-      // crossings of the cell's left edge x = c0 (inside means x < c0)
-      const bool in_s = first ? s.x <= 0.0 : s.x < c0;
-      const bool in_e = first ? e.x <= 0.0 : e.x < c0;
-      bool flag0 = false, flag1 = false;
-      if (in_s && !in_e) flag0 = u.y != r0;  // exits the left region: +V(u.y)
-      if (!in_s && in_e) flag1 = v.y != r0;  // enters the left region: -V(v.y)
-      // no vertical extent and no edge crossing: nothing to deposit
-      if (u.y == v.y && !flag0 && !flag1) continue;
+    BandPiece bp;
+    band_piece(a, b, g, ty, &bp);
+    if (bp.cross_col >= 0) bd(ty, bp.cross_col, dir);
+    if (!bp.any) continue;
+    for (int tx = bp.tx0; tx <= bp.tx1; ++tx) {
       Rec r;
-      r.x0 = (float)(u.x - c0);
-      r.y0 = (float)(u.y - r0);
-      r.x1 = (float)(v.x - c0);
-      r.y1 = (float)(v.y - r0);
-      // the flag is the sign bit; the magnitude stays the piece's x, which is 0
-      // except where tile 0's left region (x <= 0) ends inside the tile (phx > 0)
-      if (flag0) r.x0 = -r.x0;
-      if (flag1) r.x1 = -r.x1;
-      rec(ty * g.tiles_x + tx, r);
+      if (tile_record(bp.s, bp.e, bp.r0, tx, g, &r)) rec(ty * g.tiles_x + tx, r);
     }
   }
 }

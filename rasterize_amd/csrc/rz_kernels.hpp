@@ -34,12 +34,12 @@ struct DPath {
 struct Status {
   uint32_t err;        // first error (rzg::Status), 0 if none
   uint32_t overflow;   // bit0 records, bit1 events, bit2 lines, bit3 scene slices,
-                       // bit4 a wave's long-line list outgrew its buffer
+                       // bit4 a part of the record list
   uint32_t n_records;  // total records (written by the scan)
   uint32_t n_events;
   uint32_t n_lines;    // flattened lines (atomic counter of the flatten kernel)
   uint32_t pad[3];     // a batched scene's slice totals
-  uint32_t n_long;     // longest long-line list of a wave, when one overflowed
+  uint32_t n_list;     // entries the fullest record-list part needed, when one overflowed
 };
 constexpr int kStatusWords = sizeof(Status) / 4;
 
@@ -312,22 +312,40 @@ __global__ __launch_bounds__(kBlock) void k_flat_emit(DPath p, Xform x, double f
 }
 
 // ---------------------------------------------------------------------------
-// Binning of lines into cell records, one geometry pass per line:
-//   k_bin_lines   counts records per tile, accumulates band-bottom crossings,
-//                 right-edge rows and events per band, and keeps what it
-//                 made: the records of a short line (at most two: 98% of
-//                 flattened lines) in per-line slots, those of a longer line
-//                 in a long-record list, events in an event list;
+// Binning of lines into cell records.
+//   k_bin_lines   the count pass: band-bottom crossings, right-edge rows,
+//                 events, and every cell record with its place among its
+//                 tile's records, which is only known up to one offset per
+//                 (wave, tile): a wave counts its records per tile in LDS and
+//                 adds each count to the tile's total with one atomic at its
+//                 end, which returns that offset;
 //   (scans of the tile and band counts)
-//   k_bin_scatter moves the kept records into their tiles' slots;
+//   k_bin_scatter writes each listed record to tile offset + wave offset +
+//                 rank (no atomics);
 //   k_bin_events  moves the listed events into their bands' slots.
-// The lists are shared by all waves through kLongParts bump counters (a wave
-// takes its entries with one atomic), so no single counter is contended.
+// Each wave lists its records in a region of its own; a wave that fills it
+// continues in an overflow list of kParts parts, each with its own bump
+// counter, taking kChunk entries at a time and padding what it leaves unused
+// with tile ~0u (the scatter skips padding).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kLongParts = 16;
+constexpr uint32_t kParts = 16;
+constexpr uint32_t kChunk = 128;
+// Lines per count-pass wave: each wave bins one range of 64 lines and ends, so
+// it never waits for its own atomics (a wave that went on to more lines would
+// wait for them at its next load: loads and atomics share one in-order
+// counter).
+constexpr uint32_t kWaveLines = 64;
+// Per-wave LDS map tile -> record count (open addressing). A tile holds
+// fewer than 2^24 records.
+constexpr int kMapBits = 7;
+constexpr uint32_t kMap = 1u << kMapBits;
+constexpr int kMapProbe = 16;
+constexpr uint32_t kMapNone = 0xffu;  // record ranked in its tile directly
+constexpr uint32_t kEmpty = ~0u;
 
 struct BinBufs {
-  uint32_t* tile_count;   // counts; after the scan: cursors (start at the tile offsets)
+  uint32_t* tile_count;   // [tiles] records per tile
+  const uint32_t* tile_off;  // [tiles] their exclusive prefix (after the scan)
   int32_t* bd_diff;       // [tiles] band backdrop differences
   uint32_t* band_ev;      // counts per band; after the scan: cursors
   float4* recs;
@@ -335,13 +353,15 @@ struct BinBufs {
   int4* events;           // (band, col, yrel bits, sign bits)
   uint32_t ev_cap;
   int32_t* redge;         // [H + 2] right-edge row differences
-  uint2* ltile;           // [lines] tiles of a short line's records (~0u: none)
-  float4* lrec;           // [2][lrec_stride] its first and second record
-  uint32_t lrec_stride;
-  uint32_t* long_tile;    // [kLongParts][long_cap] long lines' records: tile
-  float4* long_rec;       //   ... and record
-  uint32_t long_cap;
-  uint32_t* fill;         // [kLongParts + 1] entries taken per long-record part, then events
+  // record list: [n_waves][wave_cap] regions, then [kParts][part_cap] overflow parts
+  uint32_t* list_tile;    // tile (~0u: padding)
+  uint32_t* list_rank;    // map slot << 24 | rank within the wave's records of the tile
+  float4* list_rec;       // the record
+  uint32_t* ovf_wave;     // [kParts][part_cap] wave of each overflow entry
+  uint32_t n_waves, wave_cap, part_cap;
+  uint32_t* wave_used;    // [n_waves] entries of each wave's region
+  uint32_t* wave_map;     // [n_waves][kMap] the wave's offset in each mapped tile
+  uint32_t* fill;         // [kParts + 1] entries taken per overflow part, then listed events
   int4* ev_list;          // [ev_cap] events of the count pass
   uint32_t* ev_band;      // [ev_cap] their band slots (band base + band)
   Status* st;
@@ -367,8 +387,8 @@ __device__ inline int4 event_bits(const rzg::Event& ev) {
   return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
 }
 
-// Exclusive prefix sum of v over the wave (every lane must take part); the
-// wave total is returned in *total.
+// Exclusive prefix sum of v over the wave (every lane takes part); the wave
+// total goes to *total.
 __device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
   uint32_t s = v;
 #pragma unroll
@@ -380,261 +400,380 @@ __device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total)
   return s - v;
 }
 
-// Count pass, one thread per line (grid-stride, lanes of a wave on consecutive
-// lines; the next line is loaded while this one is binned). A line with at
-// most two records keeps them in registers; the wave then counts them per
-// tile with one atomic per distinct tile (each round takes the tile of the
-// first pending record and groups every pending record of that tile) and
-// stores them in the line's slots. A longer line is binned a second time,
-// counting and listing each record. Lines beyond the right clip only add to
-// the right-edge column; two such lines on neighbouring lanes that continue
-// each other drop their shared endpoint (its two terms cancel exactly).
+// Lane that owns task j of a wave whose lanes own consecutive task ranges
+// starting at `pre` (an exclusive prefix sum): the last lane with pre <= j.
+// Every lane takes part.
+__device__ inline int task_owner(uint32_t pre, uint32_t j) {
+  int o = 0;
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) {
+    const uint32_t p = (uint32_t)__shfl((int)pre, o + s, 64);
+    if (p <= j) o += s;
+  }
+  return o;
+}
+
+// Order this wave's LDS writes before its following LDS reads (LDS executes a
+// wave's operations in issue order; this keeps the compiler from moving them).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Slot of `tile` in a wave's LDS map, inserting it if absent; -1 when the
+// probe sequence is full.
+__device__ inline int map_slot(uint32_t* keys, uint32_t tile) {
+  uint32_t h = (tile * 2654435761u) >> (32 - kMapBits);
+  for (int step = 0; step < kMapProbe; ++step) {
+    const uint32_t k = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
+    if (k == tile) return (int)h;
+    if (k == kEmpty) {
+      const uint32_t prev = atomicCAS(&keys[h], kEmpty, tile);
+      if (prev == kEmpty || prev == tile) return (int)h;
+    }
+    h = (h + 1) & (kMap - 1);
+  }
+  return -1;
+}
+
+// A line's effective polyline and its path's grid, staged in LDS for the band
+// tasks of its wave.
+struct LineStage {
+  double px[3], py[3];
+  int ty0[2];  // first band of each piece
+  int nb0;     // bands of piece 0
+  int H, tiles_x, phx, phy;
+  uint32_t tb;  // the path's first tile
+};
+
+// A band part, staged for the tile tasks.
+struct BandStage {
+  double sx, sy, ex, ey, r0;
+  int ty, tx0;
+  int line;  // lane of its line
+  int pad;
+};
+
+// The part of a line's grid the band and tile steps use.
+__device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
+  rzg::BinGrid g{};
+  g.H = S.H;
+  g.tiles_x = S.tiles_x;
+  g.phx = S.phx;
+  g.phy = S.phy;
+  return g;
+}
+
+// Count pass. A wave takes 64 consecutive lines, one per lane: right clip and
+// left split, right-edge column, events, and the band range of each piece.
+// The (line, band) pairs of the 64 lines are then dealt out to the lanes 64
+// at a time, and the (band part, tile column) pairs of each such round
+// likewise, so a line crossing many cells keeps many lanes busy rather than
+// one. Each round's records are grouped by tile across the lanes; a group
+// takes consecutive ranks from its tile's count in the wave's LDS map, and
+// the records go to the wave's region of the record list. Lines beyond the
+// right clip only add to the right-edge column; two such lines on
+// neighbouring lanes that continue each other drop their shared endpoint (its
+// two terms cancel exactly).
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
                                                        const uint32_t* line_path,
                                                        const uint32_t* n_lines, rzg::BinGrid g0,
                                                        const PathInfo* pi, BinBufs b) {
+  __shared__ LineStage lstage[kBlock];
+  __shared__ BandStage bstage[kBlock];
+  __shared__ uint32_t mkey[kBlock / 64][kMap];
+  __shared__ uint32_t mcnt[kBlock / 64][kMap];
   if (b.st->err || (b.st->overflow & 4u)) return;
   const uint32_t n = *n_lines;
-  const uint32_t stride = gridDim.x * kBlock;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint32_t part = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) % kLongParts;
-  const double4 zero = make_double4(0, 0, 0, 0);
-  uint32_t first = blockIdx.x * kBlock + threadIdx.x;
-  double4 Lnext = first < n ? lines[first] : zero;
-  // the loop bound is uniform per wave so every lane reaches the wave operations
-  for (uint32_t base = blockIdx.x * kBlock; base < n; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    const double4 L = Lnext;
-    Lnext = i + stride < n ? lines[i + stride] : zero;
-    int nrec = 0, t0 = -1, t1 = -1;
-    rzg::Rec r0{}, r1{};  // separate registers: no dynamic indexing
-    int nev = 0;
-    int4 ev0{}, ev1{};
-    uint32_t evb0 = 0, evb1 = 0;
-    bool have = i < n;
-    // this line's path: grid and slices of the per-tile / per-band / per-row arrays
-    uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
-    rzg::BinGrid g = g0;
-    if (MULTI && have) {
-      pid = line_path[i];
-      const PathInfo& P = pi[pid];
-      g = P.g;
-      tb = P.tile_base;
-      bb = P.band_base;
-      rb0 = P.redge_base;
-    }
-    const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
-    if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
-      set_err(b.st, rzg::ST_NAN);
-      have = false;
-    }
-    const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
-    {
-      const double nx = __shfl_down(L.x, 1, 64), ny = __shfl_down(L.y, 1, 64);
-      const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
-      const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
-      const bool pproj = __shfl_up((int)proj, 1, 64) != 0;
-      const uint32_t npid = __shfl_down(pid, 1, 64), ppid = __shfl_up(pid, 1, 64);
-      if (proj) {
-        const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
-        const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
-        auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
-        if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
-        if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
-      }
-    }
-    rzg::EffLine e{};
-    if (have && !proj) {
-      e = rzg::effective_line(p0, p1, g);
-      auto collect = [&](int t, const rzg::Rec& r) {
-        if (nrec == 0) {
-          t0 = (int)tb + t;
-          r0 = r;
-        } else {
-          t1 = (int)tb + t;  // only used when there are exactly two
-          r1 = r;
-        }
-        ++nrec;
-      };
-      auto bd = [&](int band, int col, int dir) {
-        atomicAdd(&b.bd_diff[tb + band * g.tiles_x + col], dir);
-      };
-      rzg::bin_piece(e.p[0], e.p[1], g, collect, bd);
-      if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, collect, bd);
-      rzg::Event ev;
-      if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
-        ev0 = event_bits(ev);
-        evb0 = bb + (uint32_t)ev.band;
-        atomicAdd(&b.band_ev[evb0], 1u);
-        nev = 1;
-      }
-      if (e.clip1 && rzg::make_event(e.n == 2 ? e.p[2] : e.p[1], 1.0f, g, &ev)) {
-        if (nev == 0) {
-          ev0 = event_bits(ev);
-          evb0 = bb + (uint32_t)ev.band;
-          atomicAdd(&b.band_ev[evb0], 1u);
-        } else {
-          ev1 = event_bits(ev);
-          evb1 = bb + (uint32_t)ev.band;
-          atomicAdd(&b.band_ev[evb1], 1u);
-        }
-        ++nev;
-      }
-    }
-    const bool is_long = nrec > 2;
-    // short lines: count per tile, wave-aggregated, and keep the records
-    bool pend0 = !is_long && nrec >= 1, pend1 = !is_long && nrec == 2;
+  const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t i = wave * kWaveLines + (uint32_t)lane;
+  if (wave * kWaveLines >= n) return;  // the scatter reads the regions of live waves only
+  LineStage* wl = lstage + (threadIdx.x & ~63u);
+  BandStage* wb = bstage + (threadIdx.x & ~63u);
+  uint32_t* keys = mkey[threadIdx.x >> 6];
+  uint32_t* cnts = mcnt[threadIdx.x >> 6];
+  for (uint32_t q = (uint32_t)lane; q < kMap; q += 64) {
+    keys[q] = kEmpty;
+    cnts[q] = 0;
+  }
+  const uint32_t part = wave % kParts;
+  const size_t region = (size_t)wave * b.wave_cap;
+  const size_t obase = (size_t)b.n_waves * b.wave_cap + (size_t)part * b.part_cap;
+  uint32_t used = 0;                   // entries of the wave's region
+  uint32_t cbase = 0, cused = kChunk;  // the wave's current overflow chunk (none yet)
+  bool lover = false;                  // the overflow part is full (the run is re-done)
+
+  // Rank and list one record per lane (has = false: none); every lane takes part.
+  auto emit = [&](bool has, int tile, const rzg::Rec& r) {
+    // group equal tiles: each group's leader takes the group's ranks
+    bool pend = has;
+    uint32_t tot = 0;  // size of the group this lane leads
+    int ldr = 0;       // lane leading this lane's group
+    uint32_t off = 0;  // place within the group
     while (true) {
-      const uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
-      if ((m0 | m1) == 0) break;
-      const bool use0 = m0 != 0;
-      const int leader = __ffsll((unsigned long long)(use0 ? m0 : m1)) - 1;
-      const int T = __builtin_amdgcn_readlane(use0 ? t0 : t1, leader);
-      const bool mine0 = pend0 && t0 == T, mine1 = pend1 && t1 == T;
-      const uint32_t total = (uint32_t)(__popcll(__ballot(mine0)) + __popcll(__ballot(mine1)));
-      if (lane == leader) atomicAdd(&b.tile_count[T], total);
-      pend0 = pend0 && !mine0;
-      pend1 = pend1 && !mine1;
-    }
-    if (i < n) {
-      const bool keep = !is_long && nrec > 0;
-      b.ltile[i] = keep ? make_uint2((uint32_t)t0, nrec == 2 ? (uint32_t)t1 : ~0u)
-                        : make_uint2(~0u, ~0u);
-      if (keep) b.lrec[i] = make_float4(r0.x0, r0.y0, r0.x1, r0.y1);
-      if (keep && nrec == 2) b.lrec[b.lrec_stride + i] = make_float4(r1.x0, r1.y0, r1.x1, r1.y1);
-    }
-    // long lines: take list entries for the wave, bin again, count and list
-    uint32_t ltot;
-    const uint32_t lpre = wave_excl_scan(is_long ? (uint32_t)nrec : 0u, lane, &ltot);
-    if (ltot) {
-      uint32_t lbase = 0;
-      if (lane == 0) lbase = atomicAdd(&b.fill[part], ltot);
-      lbase = (uint32_t)__shfl((int)lbase, 0, 64);
-      // past the list's end the records are still counted (the record total
-      // then sizes the re-run exactly)
-      const bool fits = lbase + ltot <= b.long_cap;
-      if (!fits && lane == 0) {
-        atomicOr(&b.st->overflow, 16u);
-        atomicMax(&b.st->n_long, lbase + ltot);
-      }
-      if (is_long) {
-        size_t k = (size_t)part * b.long_cap + lbase + lpre;
-        auto list = [&](int t, const rzg::Rec& r) {
-          atomicAdd(&b.tile_count[tb + t], 1u);
-          if (fits) {
-            b.long_tile[k] = tb + (uint32_t)t;
-            b.long_rec[k] = make_float4(r.x0, r.y0, r.x1, r.y1);
-          }
-          ++k;
-        };
-        auto nobd = [&](int, int, int) {};
-        rzg::bin_piece(e.p[0], e.p[1], g, list, nobd);
-        if (e.n == 2) rzg::bin_piece(e.p[1], e.p[2], g, list, nobd);
+      const uint64_t m = __ballot(pend);
+      if (m == 0) break;
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      const int T = __builtin_amdgcn_readlane(tile, leader);
+      const bool mine = pend && tile == T;
+      const uint64_t gm = __ballot(mine);
+      if (lane == leader) tot = (uint32_t)__popcll(gm);
+      if (mine) {
+        ldr = leader;
+        off = (uint32_t)__popcll(gm & below);
+        pend = false;
       }
     }
-    // events: list them (at most two per line)
-    const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
-    if (e1) {
-      const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
-      uint32_t ebase = 0;
-      if (lane == 0) ebase = atomicAdd(&b.fill[kLongParts], etot);
-      ebase = (uint32_t)__shfl((int)ebase, 0, 64);
-      const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
-      if (ebase + etot > b.ev_cap) {
-        if (lane == 0) atomicOr(&b.st->overflow, 2u);
+    int slot = (int)kMapNone;
+    uint32_t base = 0;
+    if (tot) {
+      const int h = map_slot(keys, (uint32_t)tile);
+      if (h >= 0) {
+        slot = h;
+        base = atomicAdd(&cnts[h], tot);
       } else {
-        if (nev >= 1) {
-          b.ev_list[k] = ev0;
-          b.ev_band[k] = evb0;
+        // the map is full: rank directly among all of the tile's records
+        base = atomicAdd(&b.tile_count[tile], tot);
+      }
+    }
+    const uint32_t rank = ((uint32_t)__shfl(slot, ldr, 64) << 24) | (__shfl((int)base, ldr, 64) + off);
+    // append to the wave's region, or past its end to the overflow list
+    const uint64_t hm = __ballot(has);
+    const uint32_t cnt = (uint32_t)__popcll(hm);
+    const uint32_t lrank = (uint32_t)__popcll(hm & below);
+    if (cnt && used + cnt <= b.wave_cap) {
+      if (has) {
+        const size_t q = region + used + lrank;
+        b.list_tile[q] = (uint32_t)tile;
+        b.list_rank[q] = rank;
+        b.list_rec[q] = make_float4(r.x0, r.y0, r.x1, r.y1);
+      }
+      used += cnt;
+    } else if (cnt) {
+      if (cused + cnt > kChunk) {
+        // pad the rest of the chunk and take a new one
+        if (!lover)
+          for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_tile[obase + cbase + q] = ~0u;
+        uint32_t nb = 0;
+        if (lane == 0) nb = atomicAdd(&b.fill[part], kChunk);
+        nb = (uint32_t)__shfl((int)nb, 0, 64);
+        if (nb + kChunk > b.part_cap) {
+          if (lane == 0) {
+            atomicOr(&b.st->overflow, 16u);
+            atomicMax(&b.st->n_list, nb + kChunk);
+          }
+          lover = true;
         }
-        if (nev == 2) {
-          b.ev_list[k + 1] = ev1;
-          b.ev_band[k + 1] = evb1;
-        }
+        cbase = nb;
+        cused = 0;
+      }
+      if (has && !lover) {
+        const size_t q = obase + cbase + cused + lrank;
+        b.list_tile[q] = (uint32_t)tile;
+        b.list_rank[q] = rank;
+        b.list_rec[q] = make_float4(r.x0, r.y0, r.x1, r.y1);
+        b.ovf_wave[q - (size_t)b.n_waves * b.wave_cap] = wave;
+      }
+      cused += cnt;
+    }
+  };
+
+  bool have = i < n;
+  uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
+  rzg::BinGrid g = g0;
+  if (MULTI && have) {
+    pid = line_path[i];
+    const PathInfo& P = pi[pid];
+    g = P.g;
+    tb = P.tile_base;
+    bb = P.band_base;
+    rb0 = P.redge_base;
+  }
+  const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
+  const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
+  if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
+    set_err(b.st, rzg::ST_NAN);
+    have = false;
+  }
+  const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
+  {
+    const double nx = __shfl_down(L.x, 1, 64), ny = __shfl_down(L.y, 1, 64);
+    const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
+    const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
+    const bool pproj = __shfl_up((int)proj, 1, 64) != 0;
+    const uint32_t npid = __shfl_down(pid, 1, 64), ppid = __shfl_up(pid, 1, 64);
+    if (proj) {
+      const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
+      const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
+      auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
+      if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
+      if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
+    }
+  }
+  int nev = 0;
+  int4 ev0{}, ev1{};
+  uint32_t evb0 = 0, evb1 = 0;
+  int nb0 = 0, nb1 = 0;
+  if (have && !proj) {
+    const rzg::EffLine e = rzg::effective_line(p0, p1, g);
+    rzg::Event ev;
+    if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
+      ev0 = event_bits(ev);
+      evb0 = bb + (uint32_t)ev.band;
+      nev = 1;
+    }
+    if (e.clip1 && rzg::make_event(e.n == 2 ? e.p[2] : e.p[1], 1.0f, g, &ev)) {
+      (nev ? ev1 : ev0) = event_bits(ev);
+      (nev ? evb1 : evb0) = bb + (uint32_t)ev.band;
+      ++nev;
+    }
+    int ty0 = 0, ty1 = -1, uy0 = 0, uy1 = -1;
+    if (rzg::piece_bands(e.p[0], e.p[1], g, &ty0, &ty1) && ty1 >= ty0) nb0 = ty1 - ty0 + 1;
+    if (e.n == 2 && rzg::piece_bands(e.p[1], e.p[2], g, &uy0, &uy1) && uy1 >= uy0)
+      nb1 = uy1 - uy0 + 1;
+    LineStage& S = wl[lane];
+    S.px[0] = e.p[0].x;
+    S.py[0] = e.p[0].y;
+    S.px[1] = e.p[1].x;
+    S.py[1] = e.p[1].y;
+    S.px[2] = e.p[2].x;
+    S.py[2] = e.p[2].y;
+    S.ty0[0] = ty0;
+    S.ty0[1] = uy0;
+    S.nb0 = nb0;
+    S.H = g.H;
+    S.tiles_x = g.tiles_x;
+    S.phx = g.phx;
+    S.phy = g.phy;
+    S.tb = tb;
+  }
+  // events: count, then list them (at most two per line)
+  if (nev >= 1) atomicAdd(&b.band_ev[evb0], 1u);
+  if (nev == 2) atomicAdd(&b.band_ev[evb1], 1u);
+  const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
+  if (e1) {
+    const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
+    uint32_t ebase = 0;
+    if (lane == 0) ebase = atomicAdd(&b.fill[kParts], etot);
+    ebase = (uint32_t)__shfl((int)ebase, 0, 64);
+    const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
+    if (ebase + etot > b.ev_cap) {
+      if (lane == 0) atomicOr(&b.st->overflow, 2u);
+    } else {
+      if (nev >= 1) {
+        b.ev_list[k] = ev0;
+        b.ev_band[k] = evb0;
+      }
+      if (nev == 2) {
+        b.ev_list[k + 1] = ev1;
+        b.ev_band[k + 1] = evb1;
       }
     }
   }
+  // band tasks, 64 per round
+  uint32_t n_band;
+  const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
+  wave_lds_sync();
+  for (uint32_t j0 = 0; j0 < n_band; j0 += 64) {
+    const uint32_t j = j0 + lane;
+    const int o = task_owner(pre1, j < n_band ? j : n_band - 1);
+    const uint32_t opre = (uint32_t)__shfl((int)pre1, o, 64);
+    int nt = 0;
+    if (j < n_band) {
+      const LineStage& S = wl[o];
+      const int k = (int)(j - opre);
+      const int pc = k < S.nb0 ? 0 : 1;
+      const int ty = pc == 0 ? S.ty0[0] + k : S.ty0[1] + (k - S.nb0);
+      const Pt a = rzg::mkpt(S.px[pc], S.py[pc]), c = rzg::mkpt(S.px[pc + 1], S.py[pc + 1]);
+      const rzg::BinGrid gt = stage_grid(S);
+      rzg::BandPiece bp;
+      rzg::band_piece(a, c, gt, ty, &bp);
+      if (bp.cross_col >= 0)
+        atomicAdd(&b.bd_diff[S.tb + ty * gt.tiles_x + bp.cross_col], c.y > a.y ? 1 : -1);
+      if (bp.any) {
+        nt = bp.tx1 - bp.tx0 + 1;
+        BandStage& B = wb[lane];
+        B.sx = bp.s.x;
+        B.sy = bp.s.y;
+        B.ex = bp.e.x;
+        B.ey = bp.e.y;
+        B.r0 = bp.r0;
+        B.ty = ty;
+        B.tx0 = bp.tx0;
+        B.line = o;
+      }
+    }
+    // tile tasks of this round's band parts, 64 per round
+    uint32_t n_tile;
+    const uint32_t pre2 = wave_excl_scan((uint32_t)nt, lane, &n_tile);
+    wave_lds_sync();
+    for (uint32_t k0 = 0; k0 < n_tile; k0 += 64) {
+      const uint32_t k = k0 + lane;
+      const int o2 = task_owner(pre2, k < n_tile ? k : n_tile - 1);
+      const uint32_t opre2 = (uint32_t)__shfl((int)pre2, o2, 64);
+      bool has = false;
+      int tile = -1;
+      rzg::Rec r{};
+      if (k < n_tile) {
+        const BandStage& B = wb[o2];
+        const LineStage& S = wl[B.line];
+        const rzg::BinGrid gt = stage_grid(S);
+        const int tx = B.tx0 + (int)(k - opre2);
+        has = rzg::tile_record(rzg::mkpt(B.sx, B.sy), rzg::mkpt(B.ex, B.ey), B.r0, tx, gt, &r);
+        tile = (int)S.tb + B.ty * S.tiles_x + tx;
+      }
+      emit(has, tile, r);
+    }
+    wave_lds_sync();  // the next band round re-stages the band parts
+  }
+  if (!lover && cused < kChunk)
+    for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_tile[obase + cbase + q] = ~0u;
+  // add the wave's counts to the tiles' totals; each atomic returns the
+  // wave's offset among the tile's records
+  wave_lds_sync();
+  for (uint32_t q = (uint32_t)lane; q < kMap; q += 64) {
+    const uint32_t t = keys[q];
+    if (t != kEmpty) b.wave_map[(size_t)wave * kMap + q] = atomicAdd(&b.tile_count[t], cnts[q]);
+  }
+  if (lane == 0) b.wave_used[wave] = used;
 }
 
-// Place the kept records into the slots the scan reserved: first the two
-// slots of every line, then the long-record list, one entry per thread after
-// the lines. The records of a wave that go to one tile take consecutive slots
-// from a single atomic (ordered by lane, then record), and all of a wave's
-// atomics are in flight together.
+// Write one listed record to its slot: tile offset + the wave's offset in the
+// tile (through its map slot, kMapNone: already included) + rank.
+__device__ inline void place_record(const BinBufs& b, uint32_t tile, uint32_t rank, uint32_t wave,
+                                    const float4& r) {
+  const uint32_t h = rank >> 24;
+  uint32_t slot = b.tile_off[tile] + (rank & 0xffffffu);
+  if (h != kMapNone) slot += b.wave_map[(size_t)wave * kMap + h];
+  if (slot < b.rec_cap) b.recs[slot] = r;
+  else atomicOr(&b.st->overflow, 1u);
+}
+
+// Place the listed records: one wave per count-pass region (grid-stride over
+// the regions of the waves that had lines), then the overflow parts one entry
+// per thread.
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(const uint32_t* n_lines, BinBufs b) {
   if (b.st->err || b.st->overflow) return;
-  const uint32_t n = *n_lines;
-  const uint32_t n_all = n + kLongParts * b.long_cap;
+  const uint32_t live = (*n_lines + kWaveLines - 1) / kWaveLines;
+  const uint32_t wpb = kBlock / 64;
+  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < live; w += gridDim.x * wpb) {
+    const uint32_t used = b.wave_used[w];
+    const size_t region = (size_t)w * b.wave_cap;
+    for (uint32_t j = threadIdx.x & 63; j < used; j += 64)
+      place_record(b, b.list_tile[region + j], b.list_rank[region + j], w, b.list_rec[region + j]);
+  }
   const uint32_t stride = gridDim.x * kBlock;
-  const int lane = threadIdx.x & 63;
-  const uint64_t below = (1ull << lane) - 1ull;
-  for (uint32_t base = blockIdx.x * kBlock; base < n_all; base += stride) {
-    const uint32_t v = base + threadIdx.x;
-    int nrec = 0, t0 = -1, t1 = -1;
-    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0;
-    if (v < n) {
-      const uint2 tt = b.ltile[v];
-      t0 = (int)tt.x;
-      t1 = (int)tt.y;
-      nrec = tt.x == ~0u ? 0 : (tt.y == ~0u ? 1 : 2);
-      if (nrec >= 1) r0 = b.lrec[v];
-      if (nrec == 2) r1 = b.lrec[b.lrec_stride + v];
-    } else if (v < n_all) {
-      const uint32_t p = (v - n) / b.long_cap, j = (v - n) - p * b.long_cap;
-      if (j < b.fill[p]) {
-        t0 = (int)b.long_tile[v - n];
-        r0 = b.long_rec[v - n];
-        nrec = 1;
-      }
-    }
-    bool pend0 = nrec >= 1, pend1 = nrec == 2;
-    uint32_t tot0 = 0, tot1 = 0;      // group sizes this lane leads (for t0 / t1)
-    int ldr0 = 0, ldr1 = 0;           // lane leading the group of record 0 / 1
-    bool sel0 = false, sel1 = false;  // ... through its t1 atomic
-    uint32_t off0 = 0, off1 = 0;
-    while (true) {
-      const uint64_t m0 = __ballot(pend0), m1 = __ballot(pend1);
-      if ((m0 | m1) == 0) break;
-      const bool use0 = m0 != 0;
-      const int leader = __ffsll((unsigned long long)(use0 ? m0 : m1)) - 1;
-      const int T = __builtin_amdgcn_readlane(use0 ? t0 : t1, leader);
-      const bool mine0 = pend0 && t0 == T, mine1 = pend1 && t1 == T;
-      const uint64_t g0 = __ballot(mine0), g1 = __ballot(mine1);
-      const uint32_t total = (uint32_t)(__popcll(g0) + __popcll(g1));
-      if (lane == leader) {
-        if (use0) tot0 = total;
-        else tot1 = total;
-      }
-      const uint32_t off = (uint32_t)(__popcll(g0 & below) + __popcll(g1 & below));
-      if (mine0) {
-        ldr0 = leader;
-        sel0 = !use0;
-        off0 = off;
-        pend0 = false;
-      }
-      if (mine1) {
-        ldr1 = leader;
-        sel1 = !use0;
-        off1 = off + (mine0 ? 1u : 0u);
-        pend1 = false;
-      }
-    }
-    uint32_t s0 = 0, s1 = 0;
-    if (tot0) s0 = atomicAdd(&b.tile_count[t0], tot0);
-    if (tot1) s1 = atomicAdd(&b.tile_count[t1], tot1);
-    const uint32_t a0 = (uint32_t)__shfl((int)s0, ldr0, 64), b0 = (uint32_t)__shfl((int)s1, ldr0, 64);
-    const uint32_t a1 = (uint32_t)__shfl((int)s0, ldr1, 64), b1 = (uint32_t)__shfl((int)s1, ldr1, 64);
-    if (nrec >= 1) {
-      const uint32_t slot = (sel0 ? b0 : a0) + off0;
-      if (slot < b.rec_cap) b.recs[slot] = r0;
-      else atomicOr(&b.st->overflow, 1u);
-    }
-    if (nrec == 2) {
-      const uint32_t slot = (sel1 ? b1 : a1) + off1;
-      if (slot < b.rec_cap) b.recs[slot] = r1;
-      else atomicOr(&b.st->overflow, 1u);
+  const size_t obase = (size_t)b.n_waves * b.wave_cap;
+  for (uint32_t p = 0; p < kParts; ++p) {
+    const uint32_t np = min(b.fill[p], b.part_cap);
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < np; k += stride) {
+      const size_t o = (size_t)p * b.part_cap + k;
+      const uint32_t t = b.list_tile[obase + o];
+      if (t != ~0u) place_record(b, t, b.list_rank[obase + o], b.ovf_wave[o], b.list_rec[obase + o]);
     }
   }
 }
@@ -642,7 +781,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(const uint32_t* n_lines,
 // Place the events the count pass listed into their bands' slots.
 __global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b) {
   if (b.st->err || b.st->overflow) return;
-  const uint32_t n = b.fill[kLongParts];
+  const uint32_t n = b.fill[kParts];
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
     const uint32_t slot = atomicAdd(&b.band_ev[b.ev_band[k]], 1u);
     if (slot < b.ev_cap) b.events[slot] = b.ev_list[k];
@@ -698,7 +837,7 @@ __global__ void k_status_merge(const Status* src, Status* acc) {
   acc->n_records = max(acc->n_records, src->n_records);
   acc->n_events = max(acc->n_events, src->n_events);
   acc->n_lines = max(acc->n_lines, src->n_lines);
-  acc->n_long = max(acc->n_long, src->n_long);
+  acc->n_list = max(acc->n_list, src->n_list);
 }
 
 // Fill a layer with its background colour (Layer::new, src/scene.rs:483-501).
