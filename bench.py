@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 3 times (calibrates HBM counters)")
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the differentiable render")
-    ap.add_argument("--cpu-reps", type=int, default=4)
+    ap.add_argument("--cpu-reps", type=int, default=16)
     return ap.parse_args()
 
 
@@ -307,7 +307,8 @@ def main():
         "k_flat_count": (seg_in, "16*control points + 4*segments (read)"),
         "k_flat_emit": (seg_in + 32 * n_lines,
                         "16*control points + 4*segments (read) + 32*lines (f64, written)"),
-        "k_bin_lines": (32 * n_lines, "32*lines (f64, read)"),
+        "k_bin_lines": (32 * n_lines + 16 * n_recs,
+                        "32*lines (f64, read) + 16*cell records (written, to the record list)"),
         "k_bin_scatter": (8 * n_lines + 32 * n_recs,
                           "8*lines (tile ids, read) + 16*cell records (read) + 16*cell records (written)"),
         "k_fine_mask<0>": fine_bytes,
