@@ -252,6 +252,13 @@ struct SceneFineArgs {
 // one's coverage is accumulated like k_fine's (records and events into LDS,
 // DPP row scans) in its own sub-view frame, then its paint is composited
 // source-over where mask_iter would yield a pixel (alpha >= 1e-6).
+//
+// The paths of the tile row are looked at 64 at a time, one per lane, which
+// also loads the path's tile slot, record and event ranges and backdrop, so
+// the path-by-path loop starts from values already in registers. A path with
+// no records or events in the tile, a zero backdrop and no pixel at or right
+// of its right-edge column has zero winding on the whole tile (alpha 0
+// everywhere) and is skipped.
 __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
@@ -266,46 +273,62 @@ __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
   const uint32_t* row = a.rows + (size_t)TY * a.n_paths;
   for (uint32_t start = 0; start < nrow; start += 64) {
     const uint32_t j = start + lane;
-    uint32_t pid = 0;
-    bool hit = false;
+    uint32_t pid = 0, t = 0, rb = 0, re = 0, eb = 0, ee = 0;
+    int bdv = 0, txl = 0, tyl = 0;
+    bool draw = false;
     if (j < nrow) {
       pid = row[j];
       const int4 r = a.rects[pid];
-      hit = r.x <= TX && TX < r.z;
+      if (r.x <= TX && TX < r.z) {
+        const PathInfo& P = a.pi[pid];
+        txl = TX - P.tx0;
+        tyl = TY - P.ty0;
+        t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
+        rb = a.tile_off[t];
+        re = a.tile_off[t + 1];
+        eb = a.band_ev_off[P.band_base + tyl];
+        ee = a.band_ev_off[P.band_base + tyl + 1];
+        bdv = a.bd[t];
+        const bool right = txl * 16 - P.g.phx + 15 >= P.redge_col;
+        draw = rb != re || eb != ee || bdv != 0 || right;
+      }
     }
-    uint64_t m = __ballot(hit);
+    uint64_t m = __ballot(draw);
     while (m) {
       const int src = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
       const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, src);
+      const uint32_t qrb = (uint32_t)__builtin_amdgcn_readlane((int)rb, src);
+      const uint32_t qre = (uint32_t)__builtin_amdgcn_readlane((int)re, src);
+      const uint32_t qeb = (uint32_t)__builtin_amdgcn_readlane((int)eb, src);
+      const uint32_t qee = (uint32_t)__builtin_amdgcn_readlane((int)ee, src);
+      const int qbd = __builtin_amdgcn_readlane(bdv, src);
+      const int qtxl = __builtin_amdgcn_readlane(txl, src);
+      const int qtyl = __builtin_amdgcn_readlane(tyl, src);
       const PathInfo& P = a.pi[q];
-      const int txl = TX - P.tx0, tyl = TY - P.ty0;
-      const uint32_t t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
-      const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
-      const uint32_t eb = a.band_ev_off[P.band_base + tyl], ee = a.band_ev_off[P.band_base + tyl + 1];
-      const bool local = rb != re || eb != ee;
+      const bool local = qrb != qre || qeb != qee;
       if (local) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
         if (lane < 18) rowd[lane] = 0;
         __syncthreads();
-        for (uint32_t i = rb + lane; i < re; i += 64) fine_record(a.recs[i], acc, rowd);
-        for (uint32_t i = eb + lane; i < ee; i += 64) {
+        for (uint32_t i = qrb + lane; i < qre; i += 64) fine_record(a.recs[i], acc, rowd);
+        for (uint32_t i = qeb + lane; i < qee; i += 64) {
           const int4 ev = a.events[i];
-          if (ev.y < txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+          if (ev.y < qtxl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
         }
         __syncthreads();
         if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
         __syncthreads();
       }
-      const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
-      const int xs = txl * 16 - P.g.phx + col;  // sub-view pixel column
+      const int64_t bd = (int64_t)qbd << rzg::kFixShift;
+      const int xs = qtxl * 16 - P.g.phx + col;  // sub-view pixel column
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * k + (lane >> 4);
         int64_t w = bd;
         if (local) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
-        const int ys = tyl * 16 - P.g.phy + r;
+        const int ys = qtyl * 16 - P.g.phy + r;
         if (xs >= 0 && xs < P.g.W && ys >= 0 && ys < P.g.H) {
           if (xs >= P.redge_col) {
             const int64_t rr = a.redge[P.redge_base + ys];
