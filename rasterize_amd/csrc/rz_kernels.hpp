@@ -387,30 +387,44 @@ __device__ inline int4 event_bits(const rzg::Event& ev) {
   return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
 }
 
+// Cross-lane steps on DPP (no LDS round trip). update_dpp(old, src, ctrl,
+// row_mask, bank_mask, bound_ctrl) gives `old` to lanes whose source lies
+// outside their 16-lane row or whose row is masked off. Controls: row_shr:n
+// 0x110 + n, wave_shl:1 0x130, wave_shr:1 0x138, row_bcast:15 0x142,
+// row_bcast:31 0x143.
+__device__ inline int dpp_add_scan(int x) {  // inclusive sum over the wave
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // rows 1, 3 += rows 0, 2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // rows 2, 3 += rows 0..1
+  return x;
+}
+__device__ inline int dpp_max_scan(int x) {  // inclusive max over the wave, x >= -1
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xa, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xc, 0xf, false));
+  return x;
+}
+__device__ inline int lane_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ inline int lane_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ inline double lane_next(double v) {
+  return __hiloint2double(lane_next(__double2hiint(v)), lane_next(__double2loint(v)));
+}
+__device__ inline double lane_prev(double v) {
+  return __hiloint2double(lane_prev(__double2hiint(v)), lane_prev(__double2loint(v)));
+}
+
 // Exclusive prefix sum of v over the wave (every lane takes part); the wave
 // total goes to *total.
 __device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
-  uint32_t s = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t u = (uint32_t)__shfl_up((int)s, d, 64);
-    if (lane >= d) s += u;
-  }
-  *total = (uint32_t)__shfl((int)s, 63, 64);
-  return s - v;
-}
-
-// Lane that owns task j of a wave whose lanes own consecutive task ranges
-// starting at `pre` (an exclusive prefix sum): the last lane with pre <= j.
-// Every lane takes part.
-__device__ inline int task_owner(uint32_t pre, uint32_t j) {
-  int o = 0;
-#pragma unroll
-  for (int s = 32; s >= 1; s >>= 1) {
-    const uint32_t p = (uint32_t)__shfl((int)pre, o + s, 64);
-    if (p <= j) o += s;
-  }
-  return o;
+  const uint32_t inc = (uint32_t)dpp_add_scan((int)v);
+  *total = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+  return inc - v;
 }
 
 // Order this wave's LDS writes before its following LDS reads (LDS executes a
@@ -419,6 +433,22 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Owner lane of task j0 + lane, lane o owning tasks [pre, pre + n). Each lane
+// whose range starts in [j0, j0 + 64) marks its start in the wave's 64-entry
+// LDS `marks`; the owner is the running maximum of the marks, or `carry`, the
+// owner of task j0 - 1 (-1 before the first round), which is updated. Every
+// lane takes part.
+__device__ inline int task_owner(int* marks, uint32_t pre, uint32_t n, uint32_t j0, int lane,
+                                 int* carry) {
+  marks[lane] = -1;
+  wave_lds_sync();
+  if (n > 0 && pre >= j0 && pre - j0 < 64u) marks[pre - j0] = lane;
+  wave_lds_sync();
+  const int o = max(dpp_max_scan(marks[lane]), *carry);
+  *carry = __builtin_amdgcn_readlane(o, 63);
+  return o;
 }
 
 // Slot of `tile` in a wave's LDS map, inserting it if absent; -1 when the
@@ -485,6 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   __shared__ BandStage bstage[kBlock];
   __shared__ uint32_t mkey[kBlock / 64][kMap];
   __shared__ uint32_t mcnt[kBlock / 64][kMap];
+  __shared__ int mark_all[kBlock / 64][64];
   if (b.st->err || (b.st->overflow & 4u)) return;
   const uint32_t n = *n_lines;
   const int lane = threadIdx.x & 63;
@@ -496,6 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   BandStage* wb = bstage + (threadIdx.x & ~63u);
   uint32_t* keys = mkey[threadIdx.x >> 6];
   uint32_t* cnts = mcnt[threadIdx.x >> 6];
+  int* marks = mark_all[threadIdx.x >> 6];
   for (uint32_t q = (uint32_t)lane; q < kMap; q += 64) {
     keys[q] = kEmpty;
     cnts[q] = 0;
@@ -601,11 +633,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   }
   const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
   {
-    const double nx = __shfl_down(L.x, 1, 64), ny = __shfl_down(L.y, 1, 64);
-    const double px = __shfl_up(L.z, 1, 64), py = __shfl_up(L.w, 1, 64);
-    const bool nproj = __shfl_down((int)proj, 1, 64) != 0;
-    const bool pproj = __shfl_up((int)proj, 1, 64) != 0;
-    const uint32_t npid = __shfl_down(pid, 1, 64), ppid = __shfl_up(pid, 1, 64);
+    const double nx = lane_next(L.x), ny = lane_next(L.y);
+    const double px = lane_prev(L.z), py = lane_prev(L.w);
+    const bool nproj = lane_next((int)proj) != 0;
+    const bool pproj = lane_prev((int)proj) != 0;
+    const uint32_t npid = (uint32_t)lane_next((int)pid), ppid = (uint32_t)lane_prev((int)pid);
     if (proj) {
       const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
       const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
@@ -678,9 +710,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   uint32_t n_band;
   const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
   wave_lds_sync();
+  int carry1 = -1;
   for (uint32_t j0 = 0; j0 < n_band; j0 += 64) {
     const uint32_t j = j0 + lane;
-    const int o = task_owner(pre1, j < n_band ? j : n_band - 1);
+    const int o = task_owner(marks, pre1, (uint32_t)(nb0 + nb1), j0, lane, &carry1);
     const uint32_t opre = (uint32_t)__shfl((int)pre1, o, 64);
     int nt = 0;
     if (j < n_band) {
@@ -711,9 +744,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     uint32_t n_tile;
     const uint32_t pre2 = wave_excl_scan((uint32_t)nt, lane, &n_tile);
     wave_lds_sync();
+    int carry2 = -1;
     for (uint32_t k0 = 0; k0 < n_tile; k0 += 64) {
       const uint32_t k = k0 + lane;
-      const int o2 = task_owner(pre2, k < n_tile ? k : n_tile - 1);
+      const int o2 = task_owner(marks, pre2, (uint32_t)nt, k0, lane, &carry2);
       const uint32_t opre2 = (uint32_t)__shfl((int)pre2, o2, 64);
       bool has = false;
       int tile = -1;
