@@ -712,6 +712,7 @@ struct rz_dscene {
   std::vector<rzk::ScenePath> host_sp;  // same table on the host (plain form)
   // glyph atlas placement (rz_atlas_set): device slots and slice totals
   rzk::AtlasSlot* atlas_slots = nullptr;
+  uint32_t* atlas_tile_glyph = nullptr;  // glyph of each atlas tile
   uint32_t atlas_tiles = 0, atlas_bands = 0, atlas_rows = 0;
   bool atlas_ready = false;
   // host copies of the paths, for Path::bbox(tr) in the plain form
@@ -746,6 +747,8 @@ void scene_release(rz_dscene* s) {
   for (void* p : s->allocs) (void)hipFree(p);
   if (s->atlas_slots) (void)hipFree(s->atlas_slots);
   s->atlas_slots = nullptr;
+  if (s->atlas_tile_glyph) (void)hipFree(s->atlas_tile_glyph);
+  s->atlas_tile_glyph = nullptr;
   s->per.clear();
   s->allocs.clear();
   s->all = nullptr;
@@ -1131,7 +1134,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;
   a.pi = pi;
-  a.slots = s->atlas_slots;
+  a.tile_glyph = s->atlas_tile_glyph;
   a.n = n;
   a.tile_off = tile_off;
   a.recs = (const float4*)ctx->recs.p;
@@ -1261,16 +1264,29 @@ rz_status atlas_set_impl(rz_ctx* ctx, rz_dscene* s, const rz_atlas_slot* slots) 
     }
   }
   if (t > 0xffffffffull || r > 0xffffffffull) return fail(ctx, RZ_E_ARG, "atlas too large");
-  if (s->atlas_slots) {
+  std::vector<uint32_t> tile_glyph(t);
+  for (uint32_t i = 0; i < s->n; ++i) {
+    const uint64_t end = i + 1 < s->n ? v[i + 1].tile_base : t;
+    for (uint64_t k = v[i].tile_base; k < end; ++k) tile_glyph[k] = i;
+  }
+  if (s->atlas_slots || s->atlas_tile_glyph) {
     RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    RZ_HIP(ctx, hipFree(s->atlas_slots));
+    if (s->atlas_slots) RZ_HIP(ctx, hipFree(s->atlas_slots));
+    if (s->atlas_tile_glyph) RZ_HIP(ctx, hipFree(s->atlas_tile_glyph));
     s->atlas_slots = nullptr;
+    s->atlas_tile_glyph = nullptr;
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, std::max<size_t>(v.size(), 1) * sizeof(rzk::AtlasSlot));
   if (e != hipSuccess) return fail(ctx, RZ_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   s->atlas_slots = (rzk::AtlasSlot*)p;
   if (!v.empty()) RZ_HIP(ctx, hipMemcpy(p, v.data(), v.size() * sizeof(rzk::AtlasSlot), hipMemcpyHostToDevice));
+  void* q = nullptr;
+  e = hipMalloc(&q, std::max<size_t>(tile_glyph.size(), 1) * sizeof(uint32_t));
+  if (e != hipSuccess) return fail(ctx, RZ_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  s->atlas_tile_glyph = (uint32_t*)q;
+  if (!tile_glyph.empty())
+    RZ_HIP(ctx, hipMemcpy(q, tile_glyph.data(), tile_glyph.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   s->atlas_tiles = (uint32_t)t;
   s->atlas_bands = (uint32_t)b;
   s->atlas_rows = (uint32_t)r;

@@ -1162,29 +1162,34 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   const int lane = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
-  if (lane < 18) rowd[lane] = 0;
-  __syncthreads();
   const uint32_t rb = a.tile_off[tile], re = a.tile_off[tile + 1];
-  for (uint32_t i = rb + lane; i < re; i += kFineWave) fine_record(a.recs[i], acc, rowd);
   const uint32_t eb = a.band_ev_off[ty], ee = a.band_ev_off[ty + 1];
-  for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
-    const int4 ev = a.events[i];
-    if (ev.y < tx) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+  // a tile with no records and no events of its band has the winding of its
+  // backdrop (plus the right-edge column) everywhere: no LDS work
+  const bool local = rb != re || eb != ee;
+  if (local) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
+    if (lane < 18) rowd[lane] = 0;
+    __syncthreads();
+    for (uint32_t i = rb + lane; i < re; i += kFineWave) fine_record(a.recs[i], acc, rowd);
+    for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
+      const int4 ev = a.events[i];
+      if (ev.y < tx) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+    }
+    __syncthreads();
+    // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
+    if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
+    __syncthreads();
   }
-  __syncthreads();
-  // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
-  if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
-  __syncthreads();
   const int64_t bd = (int64_t)a.bd[tile] << rzg::kFixShift;
   const int col = lane & 15;
   const int x = tx * 16 + col;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int row = 4 * k + (lane >> 4);
-    const int v = row_scan16(acc[row * 16 + col]);
-    int64_t w = bd + (int64_t)(int32_t)(rowd[row] + v);
+    int64_t w = bd;
+    if (local) w += (int64_t)(int32_t)(rowd[row] + row_scan16(acc[row * 16 + col]));
     const int y = ty * 16 + row;
     if (x < a.W && y < a.H) {
       if (x >= a.redge_col) {

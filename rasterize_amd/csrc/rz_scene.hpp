@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_atlas_setup(const AtlasSlot* slots, 
 
 struct AtlasFineArgs {
   const PathInfo* pi;
-  const AtlasSlot* slots;
+  const uint32_t* tile_glyph;  // glyph of each atlas tile
   uint32_t n;
   const uint32_t* tile_off;
   const float4* recs;
@@ -418,9 +418,9 @@ struct AtlasFineArgs {
   size_t row_stride;  // floats
 };
 
-// One tile of one glyph per wave: the glyph is found by binary search over
-// the glyphs' first tiles; coverage as in k_fine, written into the glyph's
-// atlas view.
+// One tile of one glyph per wave (the glyph from a per-tile table made when
+// the slots are set); coverage as in k_fine, written into the glyph's atlas
+// view.
 template <int RULE>
 __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   if (a.st->err || a.st->overflow) return;
@@ -428,13 +428,7 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   __shared__ int rowd[18];
   const int lane = threadIdx.x;
   const uint32_t t = blockIdx.x;
-  uint32_t lo = 0, hi = a.n;  // last glyph with tile_base <= t
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a.slots[mid].tile_base <= t) lo = mid;
-    else hi = mid;
-  }
-  const PathInfo& P = a.pi[lo];
+  const PathInfo& P = a.pi[a.tile_glyph[t]];
   if (!P.live) return;
   const uint32_t local = t - P.tile_base;
   const int txl = (int)(local % (uint32_t)P.g.tiles_x), tyl = (int)(local / (uint32_t)P.g.tiles_x);
