@@ -62,6 +62,8 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];
+  __shared__ RowStage rs[64];
+  __shared__ int marks[64];
   const int lane = threadIdx.x;
   const int TX = (int)(blockIdx.x % (uint32_t)a.tiles_x), TY = (int)(blockIdx.x / (uint32_t)a.tiles_x);
   const int col = lane & 15;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
         for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
         if (lane < 18) rowd[lane] = 0;
         __syncthreads();
-        for (uint32_t i = rb + lane; i < re; i += 64) fine_record(a.recs[i], acc, rowd);
+        tile_records(a.recs, rb, re, acc, rowd, rs, marks, lane);
         for (uint32_t i = eb + lane; i < ee; i += 64) {
           const int4 ev = a.events[i];
           if (ev.y < txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));

@@ -1033,65 +1033,115 @@ __device__ inline void dep(int* acc, int row, int col, float v) {
   if ((unsigned)col < 16u) atomicAdd(&acc[row * 16 + col], fix(v));
 }
 
-// signed_difference_line (src/rasterize.rs:395-469) for one cell record, in
-// cell coordinates; deposits beyond column 15 belong to tiles to the right
-// and are accounted for by their backdrops.
-__device__ inline void fine_record(float4 r, int* acc, int* rowd) {
-  float x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
-  // records are clipped to the cell; anything else is skipped rather than
-  // allowed to index outside the tile
-  if (!(fabsf(x0) <= 16.0f && fabsf(x1) <= 16.0f && y0 >= 0.0f && y0 <= 16.0f && y1 >= 0.0f &&
-        y1 <= 16.0f))
-    return;
-  if (signbit(x0)) add_v(rowd, y0, 1.0f);
-  if (signbit(x1)) add_v(rowd, y1, -1.0f);
-  x0 = fabsf(x0);
-  x1 = fabsf(x1);
-  if (y0 == y1) return;
-  float dir = 1.0f;
-  if (y0 > y1) {
-    dir = -1.0f;
-    float t = x0; x0 = x1; x1 = t;
-    t = y0; y0 = y1; y1 = t;
-  }
-  const float dxdy = (x1 - x0) / (y1 - y0);
-  int ys = (int)y0;
-  int ye = (int)ceilf(y1);
-  if (ye > 16) ye = 16;
-  float x_next = x0;
-  for (int y = ys; y < ye; ++y) {
-    const float x = x_next;
-    const float dy = fminf((float)(y + 1), y1) - fmaxf((float)y, y0);
-    const float d = dir * dy;
-    x_next = x + dxdy * dy;
-    const float xa = fminf(x, x_next), xb = fmaxf(x, x_next);
-    const float xa_floor = fmaxf(floorf(xa), 0.0f);
-    const int xai = (int)xa_floor;
-    const float xb_ceil = ceilf(xb);
-    const int xbi = (int)xb_ceil;
-    if (xbi <= xai + 1) {
-      const float xmf = 0.5f * (x + x_next) - xa_floor;
-      dep(acc, y, xai, d * (1.0f - xmf));
-      dep(acc, y, xai + 1, d * xmf);
+// signed_difference_line (src/rasterize.rs:395-469) for the cell records of
+// one tile, in cell coordinates; deposits beyond column 15 belong to tiles to
+// the right and are accounted for by their backdrops.
+//
+// A record crosses one to 16 pixel rows. The records are taken 64 at a time,
+// one per lane (left-edge crossings, direction, slope and row range), and
+// their (record, row) pairs are then dealt out to the lanes 64 at a time, so
+// a record crossing many rows keeps many lanes busy rather than one. A row's
+// x range is computed from the record's start and slope (x0 + dx/dy (y - y0)).
+
+// A record prepared for its row steps, staged in LDS for the wave.
+struct RowStage {
+  float x0, y0, y1, dxdy, dir;
+  int ys;   // first row
+  uint32_t pre;  // first row step of the record among its round's steps
+  int pad;
+};
+
+// Deposit the part of a prepared record within pixel row y.
+__device__ inline void fine_row(int* acc, const RowStage& R, int y) {
+  const float ylo = fmaxf((float)y, R.y0);
+  const float x = R.x0 + R.dxdy * (ylo - R.y0);
+  const float dy = fminf((float)(y + 1), R.y1) - ylo;
+  const float d = R.dir * dy;
+  const float x_next = x + R.dxdy * dy;
+  const float xa = fminf(x, x_next), xb = fmaxf(x, x_next);
+  const float xa_floor = fmaxf(floorf(xa), 0.0f);
+  const int xai = (int)xa_floor;
+  const float xb_ceil = ceilf(xb);
+  const int xbi = (int)xb_ceil;
+  if (xbi <= xai + 1) {
+    const float xmf = 0.5f * (x + x_next) - xa_floor;
+    dep(acc, y, xai, d * (1.0f - xmf));
+    dep(acc, y, xai + 1, d * xmf);
+  } else {
+    const float s = 1.0f / (xb - xa);
+    const float xaf = xa - xa_floor;
+    const float xbf = xb - xb_ceil + 1.0f;
+    const float a0 = 0.5f * s * (1.0f - xaf) * (1.0f - xaf);
+    const float am = 0.5f * s * xbf * xbf;
+    dep(acc, y, xai, d * a0);
+    if (xbi == xai + 2) {
+      dep(acc, y, xai + 1, d * (1.0f - a0 - am));
     } else {
-      const float s = 1.0f / (xb - xa);
-      const float xaf = xa - xa_floor;
-      const float xbf = xb - xb_ceil + 1.0f;
-      const float a0 = 0.5f * s * (1.0f - xaf) * (1.0f - xaf);
-      const float am = 0.5f * s * xbf * xbf;
-      dep(acc, y, xai, d * a0);
-      if (xbi == xai + 2) {
-        dep(acc, y, xai + 1, d * (1.0f - a0 - am));
-      } else {
-        const float a1 = s * (1.5f - xaf);
-        dep(acc, y, xai + 1, d * (a1 - a0));
-        const int hi = xbi - 1 < 16 ? xbi - 1 : 16;
-        for (int xi = xai + 2; xi < hi; ++xi) dep(acc, y, xi, d * s);
-        const float a2 = a1 + (float)(xbi - xai - 3) * s;
-        dep(acc, y, xbi - 1, d * (1.0f - a2 - am));
-      }
-      dep(acc, y, xbi, d * am);
+      const float a1 = s * (1.5f - xaf);
+      dep(acc, y, xai + 1, d * (a1 - a0));
+      const int hi = xbi - 1 < 16 ? xbi - 1 : 16;
+      for (int xi = xai + 2; xi < hi; ++xi) dep(acc, y, xi, d * s);
+      const float a2 = a1 + (float)(xbi - xai - 3) * s;
+      dep(acc, y, xbi - 1, d * (1.0f - a2 - am));
     }
+    dep(acc, y, xbi, d * am);
+  }
+}
+
+// Records [rb, re) of one tile into the wave's LDS accumulators. `rs` and
+// `marks` are the wave's 64-entry LDS scratch. Every lane of the wave takes
+// part.
+__device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re, int* acc,
+                                    int* rowd, RowStage* rs, int* marks, int lane) {
+  for (uint32_t r0 = rb; r0 < re; r0 += 64) {
+    const uint32_t i = r0 + (uint32_t)lane;
+    int nrows = 0;
+    RowStage R{};
+    if (i < re) {
+      const float4 r = recs[i];
+      float x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
+      // records are clipped to the cell; anything else is skipped rather than
+      // allowed to index outside the tile
+      if (fabsf(x0) <= 16.0f && fabsf(x1) <= 16.0f && y0 >= 0.0f && y0 <= 16.0f && y1 >= 0.0f &&
+          y1 <= 16.0f) {
+        if (signbit(x0)) add_v(rowd, y0, 1.0f);
+        if (signbit(x1)) add_v(rowd, y1, -1.0f);
+        x0 = fabsf(x0);
+        x1 = fabsf(x1);
+        if (y0 != y1) {
+          float dir = 1.0f;
+          if (y0 > y1) {
+            dir = -1.0f;
+            float t = x0; x0 = x1; x1 = t;
+            t = y0; y0 = y1; y1 = t;
+          }
+          const int ys = (int)y0;
+          int ye = (int)ceilf(y1);
+          if (ye > 16) ye = 16;
+          nrows = ye - ys;
+          R.x0 = x0;
+          R.y0 = y0;
+          R.y1 = y1;
+          R.dxdy = (x1 - x0) / (y1 - y0);
+          R.dir = dir;
+          R.ys = ys;
+        }
+      }
+    }
+    uint32_t n_rows;
+    R.pre = wave_excl_scan((uint32_t)nrows, lane, &n_rows);
+    if (nrows > 0) rs[lane] = R;
+    wave_lds_sync();
+    int carry = -1;
+    for (uint32_t k0 = 0; k0 < n_rows; k0 += 64) {
+      const int o = task_owner(marks, R.pre, (uint32_t)nrows, k0, lane, &carry);
+      const uint32_t k = k0 + (uint32_t)lane;
+      if (k < n_rows) {
+        const RowStage& Q = rs[o];
+        fine_row(acc, Q, Q.ys + (int)(k - Q.pre));
+      }
+    }
+    wave_lds_sync();  // the next round re-stages the records
   }
 }
 
@@ -1159,6 +1209,8 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];
+  __shared__ RowStage rs[64];
+  __shared__ int marks[64];
   const int lane = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
@@ -1172,7 +1224,7 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
     for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
     if (lane < 18) rowd[lane] = 0;
     __syncthreads();
-    for (uint32_t i = rb + lane; i < re; i += kFineWave) fine_record(a.recs[i], acc, rowd);
+    tile_records(a.recs, rb, re, acc, rowd, rs, marks, lane);
     for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
       const int4 ev = a.events[i];
       if (ev.y < tx) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
