@@ -304,9 +304,8 @@ def main():
     seg_in = 16 * n_pts + 4 * n_seg
     fine_bytes = (4 * W * H + 16 * n_recs, "4*W*H (f32 mask written) + 16*cell records (read)")
     alg_def = {
-        "k_flat_count": (seg_in, "16*control points + 4*segments (read)"),
-        "k_flat_emit": (seg_in + 32 * n_lines,
-                        "16*control points + 4*segments (read) + 32*lines (f64, written)"),
+        "k_flat_lines": (seg_in + 32 * n_lines,
+                         "16*control points + 4*segments (read) + 32*lines (f64, written)"),
         "k_bin_lines": (32 * n_lines + 16 * n_recs,
                         "32*lines (f64, read) + 16*cell records (written, to the record list)"),
         "k_bin_scatter": (8 * n_lines + 32 * n_recs,

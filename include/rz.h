@@ -284,8 +284,10 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
 /* ---- launch parameters ----------------------------------------------------
  * No reference counterpart. RZ_PARAM_SPLIT_QUAD / RZ_PARAM_SPLIT_CUBIC: each
  * quad / cubic segment's subdivision tree is processed by 2^value threads
- * (one per subtree at that depth), 0..8. Results do not depend on them. */
-enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1 };
+ * (one per subtree at that depth), 0..8; setting either fixes both for the
+ * context. RZ_PARAM_SPLIT_AUTO = 1 (the default) chooses them per call from
+ * the segments' expected line counts. Results do not depend on them. */
+enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 

@@ -44,17 +44,18 @@ struct rz_ctx {
   rzk::Status* sticky_st = nullptr;  // set while such a render is enqueued
   // batched scene render
   DBuf line_path;  // path of each line
-  DBuf fl_counts;  // lines per flatten thread, then per-wave totals and offsets
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
   DBuf sc_boxes, sc_has, sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   DBuf list_buf;  // binning: record list (records, tiles) and event list
+  DBuf wave_lines;  // per flatten wave: its lines' first index and count
   uint32_t rec_cap = 0, ev_cap = 0;
   uint32_t part_cap = 0;  // overflow record-list entries per part
   rzk::Status* host_status = nullptr;  // pinned
   rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
   int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
+  bool split_auto = true;               // choose them per call (split_depths)
   uint32_t line_cap = 0;
   // kernel timing (rz_profile)
   bool prof = false;
@@ -67,12 +68,20 @@ struct rz_ctx {
   std::vector<rz_kernel_stat> stats;
 };
 
+// Mean expected line count of a path's quads and cubics: (flatness /
+// (16 tol^2))^(1/4) per segment (a segment flattens into about 1.4 times that
+// many lines), at tol = 0.05 under the transform the estimate was made with.
+struct SplitEst {
+  double quad = 1.0, cubic = 1.0;
+};
+
 struct rz_dpath {
   rzk::DPath view{};
   std::vector<void*> allocs;
   uint32_t n_order = 0;  // padded length of `order`
   uint32_t n_slots = 0;
   bool has_bbox = false;  // Path::bbox(identity), for bounding-box paints
+  SplitEst est;           // expected line counts of its quads / cubics
   rzh::BBox bbox{};
 };
 
@@ -136,6 +145,65 @@ void push_prof(rz_ctx* ctx, const char* name, hipEvent_t a, hipEvent_t b);
   } while (0)
 
 uint32_t blocks_for(size_t n) { return (uint32_t)((n + rzk::kBlock - 1) / rzk::kBlock); }
+
+// Sums behind a SplitEst.
+struct SplitSum {
+  double q = 0, c = 0;
+  size_t nq = 0, nc = 0;
+  SplitEst est() const {
+    SplitEst e;
+    e.quad = nq ? q / (double)nq : 1.0;
+    e.cubic = nc ? c / (double)nc : 1.0;
+    return e;
+  }
+};
+
+void split_sum(const rz_path* p, const double* m, SplitSum* s) {
+  const double fl = 16.0 * 0.05 * 0.05;
+  for (uint32_t k = 0; k < p->n_segments; ++k) {
+    const double* q = p->points + 2 * (size_t)p->point_offsets[k];
+    double f;
+    if (p->kinds[k] == RZ_QUAD) {
+      rzg::Node<3> n;
+      for (int i = 0; i < 3; ++i) n.p[i] = rzg::tr_apply(m, rzg::mkpt(q[2 * i], q[2 * i + 1]));
+      f = rzg::flatness(n);
+    } else if (p->kinds[k] == RZ_CUBIC) {
+      rzg::Node<4> n;
+      for (int i = 0; i < 4; ++i) n.p[i] = rzg::tr_apply(m, rzg::mkpt(q[2 * i], q[2 * i + 1]));
+      f = rzg::flatness(n);
+    } else {
+      continue;
+    }
+    const double e = f > fl && std::isfinite(f) ? std::pow(f / fl, 0.25) : 1.0;
+    if (p->kinds[k] == RZ_QUAD) {
+      s->q += e;
+      ++s->nq;
+    } else {
+      s->c += e;
+      ++s->nc;
+    }
+  }
+}
+
+// Virtual-subtree depths for a call: about four lines per thread for quads,
+// two for cubics (measured best on the 200k-segment path and the glyph atlas).
+// `scale` is the transform's area scale (|det|), applied on top of the
+// estimate's own transform.
+void split_depths(rz_ctx* ctx, const SplitEst& e, double scale, double flatness, int* dq,
+                  int* dc) {
+  if (!ctx->split_auto) {
+    *dq = ctx->split_quad;
+    *dc = ctx->split_cubic;
+    return;
+  }
+  const double f = 1.41 * std::sqrt(std::sqrt(scale > 0.0 ? scale : 1.0) * 0.05 / flatness);
+  auto depth = [](double lines, double per_thread) {
+    const double d = std::round(std::log2(std::max(lines, 1.0) / per_thread));
+    return (int)std::min(6.0, std::max(0.0, d));
+  };
+  *dq = depth(e.quad * f, 4.0);
+  *dc = depth(e.cubic * f, 2.0);
+}
 
 rz_status check_launch(rz_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
@@ -303,6 +371,9 @@ static rz_status path_upload_impl(rz_ctx* ctx, const rz_path* p, rz_dpath** out)
   }
   const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
   d->has_bbox = rzh::path_bbox(p, ident, &d->bbox);
+  SplitSum sum;
+  split_sum(p, ident.m, &sum);
+  d->est = sum.est();
   *out = d;
   return RZ_OK;
 }
@@ -350,45 +421,40 @@ struct Pipe {
   int32_t* redge;  // [H + 2] right-edge row differences, then (scanned) per-row sums
 };
 
-// Flatten every slot of `d` into ctx->lines (and, for a batched scene, the
-// path of each line into ctx->line_path): count pass, scan of the per-wave
-// totals, emit pass. The total goes to st->n_lines.
+// Flatten every slot of `d` into the line parts of ctx->lines (and, for a
+// batched scene, the path of each line into ctx->line_path). `line_fill`
+// (kLineParts words) must be zero; the number of lines goes to st->n_lines.
 rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, double fl,
                           const rzk::PathInfo* pi, const uint32_t* slot_path, rzk::Status* st,
-                          rzk::LineBufs* lb) {
-  const int dq = ctx->split_quad, dc = ctx->split_cubic;
+                          uint32_t* line_fill, int dq, int dc, rzk::LineBufs* lb) {
   const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
                  bc = d->view.blocks_cubic << dc;
   const uint32_t nblk = bl + bq + bc;
   lb->lines = (double4*)ctx->lines.p;
   lb->line_path = pi ? (uint32_t*)ctx->line_path.p : nullptr;
-  lb->cap = ctx->line_cap;
-  lb->count = &st->n_lines;
+  lb->part_cap = ctx->line_cap / rzk::kLineParts;
+  lb->fill = line_fill;
   lb->st = st;
-  if (!nblk) return RZ_OK;
-  const size_t nthreads = (size_t)nblk * rzk::kBlock, nwaves = nthreads / 64;
-  RZ_TRY(ensure(ctx, ctx->fl_counts, (nthreads + 2 * (nwaves + 1)) * 4));
-  uint32_t* counts = (uint32_t*)ctx->fl_counts.p;
-  uint32_t* wave_tot = counts + nthreads;
-  uint32_t* wave_off = wave_tot + nwaves + 1;
-  if (pi) {
-    RZ_LAUNCH(ctx, "k_flat_count", rzk::k_flat_count<true>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
-              fl, bl, bq, dq, dc, pi, slot_path, counts, wave_tot, st);
-  } else {
-    RZ_LAUNCH(ctx, "k_flat_count", rzk::k_flat_count<false>, dim3(nblk), dim3(rzk::kBlock), d->view,
-              x, fl, bl, bq, dq, dc, pi, slot_path, counts, wave_tot, st);
+  lb->n_waves = nblk * (rzk::kBlock / 64);
+  RZ_TRY(ensure(ctx, ctx->wave_lines, (size_t)std::max<uint32_t>(lb->n_waves, 1) * sizeof(uint2)));
+  lb->wave_lines = (uint2*)ctx->wave_lines.p;
+  if (nblk) {
+    if (pi) {
+      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<true>, dim3(nblk), dim3(rzk::kBlock), d->view,
+                x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    } else {
+      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<false>, dim3(nblk), dim3(rzk::kBlock),
+                d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    }
   }
-  RZ_TRY(scan_u32(ctx, wave_tot, wave_off, nwaves, nullptr, &st->n_lines));
-  if (pi) {
-    RZ_LAUNCH(ctx, "k_flat_emit", rzk::k_flat_emit<true>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
-              fl, bl, bq, dq, dc, pi, slot_path, (const uint32_t*)counts, (const uint32_t*)wave_off,
-              *lb);
-  } else {
-    RZ_LAUNCH(ctx, "k_flat_emit", rzk::k_flat_emit<false>, dim3(nblk), dim3(rzk::kBlock), d->view, x,
-              fl, bl, bq, dq, dc, pi, slot_path, (const uint32_t*)counts, (const uint32_t*)wave_off,
-              *lb);
-  }
+  RZ_LAUNCH(ctx, "k_line_total", rzk::k_line_total, dim3(1), dim3(64), (const uint32_t*)line_fill, st);
   return check_launch(ctx, "flatten");
+}
+
+// Line capacity: kLineParts parts of whole 64-line chunks.
+uint32_t line_cap_for(size_t lines) {
+  const size_t unit = (size_t)rzk::kLineParts * 64;
+  return (uint32_t)std::min<size_t>((lines + unit - 1) / unit * unit, (size_t)1 << 30);
 }
 
 // Binning of the flattened lines around the scans: the count pass, then (after
@@ -397,16 +463,15 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
 template <bool MULTI>
 rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
                             const rzk::PathInfo* pi, uint32_t* fill, rzk::BinBufs* b) {
-  // one count-pass wave per kWaveLines lines of capacity; a wave's region
-  // holds two entries per line (records are usually about half as many as
-  // lines), the overflow list takes the rest
+  // one count-pass wave per flatten wave; the record list holds two entries
+  // per line slot (records are usually as many as lines or fewer), the
+  // overflow list takes the rest
   const uint32_t wpb = rzk::kBlock / 64;
-  const uint32_t nw = (uint32_t)(((size_t)lb.cap + rzk::kWaveLines * wpb - 1) /
-                                 (rzk::kWaveLines * wpb) * wpb);
-  const uint32_t wave_cap = 2 * rzk::kWaveLines;
+  const uint32_t nw = lb.n_waves;
   if (ctx->part_cap == 0) ctx->part_cap = 1u << 16;
   const size_t no = (size_t)rzk::kParts * ctx->part_cap;
-  const size_t nl = (size_t)nw * wave_cap + no;
+  const size_t nreg = 2 * (size_t)ctx->line_cap;
+  const size_t nl = nreg + no;
   const size_t words = nl * 2 + no + (size_t)ctx->ev_cap + nw + (size_t)nw * rzk::kMap;
   RZ_TRY(ensure(ctx, ctx->list_buf, (nl + ctx->ev_cap) * 16 + words * 4));
   char* lp = (char*)ctx->list_buf.p;
@@ -420,21 +485,22 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   b->wave_used = b->ev_band + ctx->ev_cap;
   b->wave_map = b->wave_used + nw;
   b->n_waves = nw;
-  b->wave_cap = wave_cap;
+  b->wave_lines = lb.wave_lines;
+  b->list_regions = nreg;
   b->part_cap = ctx->part_cap;
   b->fill = fill;
+  if (nw == 0) return RZ_OK;
   RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb), dim3(rzk::kBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, (const uint32_t*)lb.count, g0,
-            pi, *b);
+            (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi, *b);
   return RZ_OK;
 }
 
-rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::LineBufs& lb, const rzk::BinBufs& b) {
+rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b) {
+  if (b.n_waves == 0) return RZ_OK;
   // many waves, each over regions far apart (neighbouring regions hold
   // records of the same tiles)
   const uint32_t grid = std::min<uint32_t>(b.n_waves / (rzk::kBlock / 64), 8192);
-  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kBlock),
-            (const uint32_t*)lb.count, b);
+  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kBlock), b);
   RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b);
   return RZ_OK;
 }
@@ -448,12 +514,12 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kParts + 1;
+  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kParts + 1 + rzk::kLineParts;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, (tiles + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, (ty + 1) * 4));
   if (ctx->line_cap == 0) {
-    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    ctx->line_cap = line_cap_for(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16));
     RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
   }
   if (ctx->rec_cap == 0) {
@@ -481,7 +547,11 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   const double fl = 16.0 * flatness * flatness;
   // flatten: quad and cubic slots are split into 2^dq / 2^dc virtual subtrees
   rzk::LineBufs lb;
-  RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, &lb));
+  uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
+  int dq, dc;
+  split_depths(ctx, d->est, std::fabs(x.m[0] * x.m[4] - x.m[1] * x.m[3]), flatness, &dq, &dc);
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, bin_fill + rzk::kParts + 1, dq, dc,
+                         &lb));
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
   b.tile_off = P.tile_off;
@@ -494,8 +564,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.redge = P.redge;
   b.st = P.st;
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr,
-                                 meta + meta_u32 - (rzk::kParts + 1), &b));
+  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr, bin_fill, &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   RZ_TRY(check_launch(ctx, "bin pass 0"));
@@ -503,7 +572,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
   RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
             P.grid.g.tiles_x, P.redge, (int)(H + 2));
-  RZ_TRY(enqueue_bin_place(ctx, lb, b));
+  RZ_TRY(enqueue_bin_place(ctx, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   return check_launch(ctx, "bin pass 1");
@@ -611,8 +680,8 @@ rz_status finish(rz_ctx* ctx) {
       ctx->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_records * 5 / 4 + 1024, 0xffffffffu);
       RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
     }
-    if (s.n_lines > ctx->line_cap) {
-      ctx->line_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_lines * 5 / 4 + 1024, 0xffffffffu);
+    if ((s.overflow & 4u) && s.n_line_part > ctx->line_cap / rzk::kLineParts) {
+      ctx->line_cap = line_cap_for(((size_t)s.n_line_part * 5 / 4 + 1024) * rzk::kLineParts);
       RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
       // binning was skipped, so its record count is unknown: most lines give
       // one record, so size for that rather than re-running twice
@@ -713,6 +782,7 @@ struct rz_dscene {
   // glyph atlas placement (rz_atlas_set): device slots and slice totals
   rzk::AtlasSlot* atlas_slots = nullptr;
   uint32_t* atlas_tile_glyph = nullptr;  // glyph of each atlas tile
+  SplitEst atlas_est;                    // line estimate under the slots' transforms
   uint32_t atlas_tiles = 0, atlas_bands = 0, atlas_rows = 0;
   bool atlas_ready = false;
   // host copies of the paths, for Path::bbox(tr) in the plain form
@@ -896,7 +966,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
     ctx->sc_cap_r = (uint32_t)std::max<size_t>((size_t)lh * 4 + n * 66, 1 << 14);
   }
   if (ctx->line_cap == 0) {
-    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    ctx->line_cap = line_cap_for(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16));
     RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
   }
   if (ctx->rec_cap == 0) {
@@ -909,7 +979,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   }
   const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b, cr = ctx->sc_cap_r;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1 + rzk::kLineParts;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -955,7 +1025,11 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   rzk::Xform x{};
   const double fl = 16.0 * flatness * flatness;
   rzk::LineBufs lb;
-  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
+  uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
+  int dq, dc;
+  split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
+                         &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -969,7 +1043,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   b.st = st;
   rzg::BinGrid g0{};
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kParts + 1), &b));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, bin_fill, &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "scene pass 0"));
@@ -977,7 +1051,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_TRY(enqueue_bin_place(ctx, lb, b));
+  RZ_TRY(enqueue_bin_place(ctx, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
@@ -1071,7 +1145,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   if (n == 0 || s->atlas_tiles == 0) return RZ_OK;
   const rz_dpath* d = s->all;
   if (ctx->line_cap == 0) {
-    ctx->line_cap = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16), 1u << 28);
+    ctx->line_cap = line_cap_for(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16));
     RZ_TRY(ensure(ctx, ctx->lines, (size_t)ctx->line_cap * 32));
   }
   if (ctx->rec_cap == 0) {
@@ -1084,7 +1158,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   }
   const uint32_t ct = s->atlas_tiles, cb = s->atlas_bands, cr = s->atlas_rows;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1;
+  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1 + rzk::kLineParts;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -1107,7 +1181,11 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   rzk::Xform x{};
   const double fl = 16.0 * flatness * flatness;
   rzk::LineBufs lb;
-  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, &lb));
+  uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
+  int dq, dc;
+  split_depths(ctx, s->atlas_est, 1.0, flatness, &dq, &dc);
+  RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
+                         &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -1121,7 +1199,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   b.st = st;
   rzg::BinGrid g0{};
   const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, meta + meta_u32 - (rzk::kParts + 1), &b));
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, bin_fill, &b));
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
@@ -1129,7 +1207,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_TRY(enqueue_bin_place(ctx, lb, b));
+  RZ_TRY(enqueue_bin_place(ctx, b));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;
@@ -1205,7 +1283,8 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
   RZ_LAUNCH(ctx, "k_scene_bwd_tiles", rzk::k_scene_bwd_tiles, dim3(ntiles), dim3(64), bt);
   // line pass over the same flatten items as the forward
   const rz_dpath* d = s->all;
-  const int dq = ctx->split_quad, dc = ctx->split_cubic;
+  int dq, dc;
+  split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
   const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
                  bc = d->view.blocks_cubic << dc;
   const uint32_t nblk = bl + bq + bc;
@@ -1264,6 +1343,12 @@ rz_status atlas_set_impl(rz_ctx* ctx, rz_dscene* s, const rz_atlas_slot* slots) 
     }
   }
   if (t > 0xffffffffull || r > 0xffffffffull) return fail(ctx, RZ_E_ARG, "atlas too large");
+  SplitSum sum;
+  for (uint32_t i = 0; i < s->n; ++i) {
+    const rz_path hp = s->host_path(i);
+    split_sum(&hp, slots[i].tr.m, &sum);
+  }
+  s->atlas_est = sum.est();
   std::vector<uint32_t> tile_glyph(t);
   for (uint32_t i = 0; i < s->n; ++i) {
     const uint64_t end = i + 1 < s->n ? v[i + 1].tile_base : t;
@@ -1313,9 +1398,17 @@ extern "C" {
 rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   if (!ctx) return RZ_E_ARG;
   if (value < 0 || value > 8) return fail(ctx, RZ_E_ARG, "parameter out of range 0..8");
-  if (param == RZ_PARAM_SPLIT_QUAD) ctx->split_quad = value;
-  else if (param == RZ_PARAM_SPLIT_CUBIC) ctx->split_cubic = value;
-  else return fail(ctx, RZ_E_ARG, "unknown parameter");
+  if (param == RZ_PARAM_SPLIT_QUAD) {
+    ctx->split_quad = value;
+    ctx->split_auto = false;
+  } else if (param == RZ_PARAM_SPLIT_CUBIC) {
+    ctx->split_cubic = value;
+    ctx->split_auto = false;
+  } else if (param == RZ_PARAM_SPLIT_AUTO) {
+    ctx->split_auto = value != 0;
+  } else {
+    return fail(ctx, RZ_E_ARG, "unknown parameter");
+  }
   return RZ_OK;
 }
 
@@ -1323,6 +1416,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   if (!ctx || !value) return RZ_E_ARG;
   if (param == RZ_PARAM_SPLIT_QUAD) *value = ctx->split_quad;
   else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
+  else if (param == RZ_PARAM_SPLIT_AUTO) *value = ctx->split_auto ? 1 : 0;
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
   return RZ_OK;
 }
@@ -1375,9 +1469,10 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   DBuf* bufs[] = {&ctx->meta,      &ctx->tile_off,  &ctx->ev_off,    &ctx->sums,
                   &ctx->recs,      &ctx->events,    &ctx->out,       &ctx->lines,
                   &ctx->counts,    &ctx->stops,     &ctx->sticky,    &ctx->line_path,
-                  &ctx->fl_counts, &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
+                  &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
                   &ctx->sc_boxes,  &ctx->sc_has,    &ctx->sc_pi,     &ctx->sc_counts,
-                  &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf};
+                  &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
+                  &ctx->wave_lines};
   for (DBuf* b : bufs) free_buf(*b);
   for (auto& r : ctx->pending) {
     (void)hipEventDestroy(r.a);

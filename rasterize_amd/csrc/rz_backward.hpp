@@ -279,7 +279,7 @@ __device__ inline void bernstein(double t, double* w) {
   }
 }
 
-// One flatten item of the backward pass: the same subtree as k_flat_emit, each
+// One flatten item of the backward pass: the same subtree as k_flat_lines, each
 // leaf line's endpoint gradients pushed to the slot's control points.
 template <int NP>
 __device__ void bwd_line_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,

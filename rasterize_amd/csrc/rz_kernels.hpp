@@ -40,6 +40,7 @@ struct Status {
   uint32_t n_lines;    // flattened lines (atomic counter of the flatten kernel)
   uint32_t pad[3];     // a batched scene's slice totals
   uint32_t n_list;     // entries the fullest record-list part needed, when one overflowed
+  uint32_t n_line_part;  // lines the fullest line part needed, when one overflowed
 };
 constexpr int kStatusWords = sizeof(Status) / 4;
 
@@ -142,251 +143,6 @@ __global__ __launch_bounds__(kBlock) void k_flatten_emit(DPath p, Xform x, doubl
   }
 }
 
-// ---------------------------------------------------------------------------
-// Flatten to lines for binning. Line slots take one thread each; every quad /
-// cubic slot is split into 2^D0 virtual subtrees (the subtree at depth D0
-// along path bits j), one thread each: the thread walks from the root along
-// j; a leaf met above depth D0 is processed only by the leftmost j under it.
-// Every leaf is produced exactly once, with the same arithmetic as the
-// slot-wide traversal, and a curve with hundreds of lines is spread over many
-// threads. The order of lines in the buffer does not matter for binning, so
-// each wave reserves its output with one atomic.
-// ---------------------------------------------------------------------------
-struct LineBufs {
-  double4* lines;
-  uint32_t* line_path;  // batched scenes: path of each line
-  uint32_t cap;
-  uint32_t* count;  // total lines (device)
-  Status* st;
-};
-
-// Walk from the slot's root along j for D0 levels. Returns the subtree root
-// in *n and whether a traversal from it is needed; *leaf is set when a leaf
-// above D0 is owned by this j.
-template <int NP>
-__device__ inline bool walk_to_subtree(rzg::Node<NP>* n, uint32_t j, int D0, double fl, bool* leaf,
-                                       int* err) {
-  *leaf = false;
-  for (int k = 0; k < D0; ++k) {
-    if (rzg::has_nans(*n)) {
-      *err = rzg::ST_NAN;
-      return false;
-    }
-    if (rzg::flatness(*n) < fl) {
-      *leaf = (j & ((1u << (D0 - k)) - 1u)) == 0;
-      return false;
-    }
-    rzg::Node<NP> c;
-    rzg::child(*n, (int)((j >> (D0 - 1 - k)) & 1u), &c);
-    *n = c;
-  }
-  return true;
-}
-
-__device__ inline uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
-  return v;
-}
-
-// The subtree a thread of the flatten kernels handles. Returns the node to
-// traverse (walk = true) or a single leaf owned by this thread (leaf = true).
-template <int NP, bool MULTI>
-__device__ inline void flat_subtree(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
-                                    const Xform& x, double fl, const PathInfo* pi,
-                                    const uint32_t* slot_path, rzg::Node<NP>* n, bool* walk,
-                                    bool* leaf, int* err, uint32_t* pid) {
-  *walk = *leaf = false;
-  *err = rzg::ST_OK;
-  *pid = 0;
-  if (MULTI && active) {
-    *pid = slot_path[slot];
-    active = pi[*pid].live != 0;
-  }
-  if (active) {
-    *n = load_node<NP>(p, slot, MULTI ? pi[*pid].m : x.m);
-    *walk = walk_to_subtree(n, j, D0, fl, leaf, err);
-  }
-}
-
-// Count pass: lines per thread, and per wave their total (no atomics: the
-// wave totals are scanned into the wave's output offsets).
-template <int NP, bool MULTI>
-__device__ void flat_count_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
-                                const Xform& x, double fl, const PathInfo* pi,
-                                const uint32_t* slot_path, uint32_t* counts, uint32_t* wave_tot,
-                                Status* st) {
-  rzg::Node<NP> n;
-  bool walk, leaf;
-  int err;
-  uint32_t pid;
-  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
-  uint32_t cnt = leaf ? 1u : 0u;
-  if (walk) {
-    const int r = rzg::traverse(n, fl, [&](Pt, Pt) { ++cnt; });
-    if (r != rzg::ST_OK) err = r;
-  }
-  if (err != rzg::ST_OK) {
-    set_err(st, (uint32_t)err);
-    cnt = 0;
-  }
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  counts[gid] = cnt;
-  const uint32_t incl = wave_incl_scan(cnt);
-  if ((threadIdx.x & 63) == 63) wave_tot[gid >> 6] = incl;
-}
-
-// Emit pass: the thread's lines at its wave's offset plus its prefix in the wave.
-template <int NP, bool MULTI>
-__device__ void flat_emit_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
-                               const Xform& x, double fl, const PathInfo* pi,
-                               const uint32_t* slot_path, const uint32_t* counts,
-                               const uint32_t* wave_off, const LineBufs& lb) {
-  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t cnt = counts[gid];
-  const uint32_t incl = wave_incl_scan(cnt);
-  if (!cnt) return;
-  uint32_t o = wave_off[gid >> 6] + incl - cnt;
-  if (o + cnt > lb.cap) {
-    atomicOr(&lb.st->overflow, 4u);
-    return;
-  }
-  rzg::Node<NP> n;
-  bool walk, leaf;
-  int err;
-  uint32_t pid;
-  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
-  if (leaf) {
-    lb.lines[o] = make_double4(n.p[0].x, n.p[0].y, n.p[NP - 1].x, n.p[NP - 1].y);
-    if (MULTI) lb.line_path[o] = pid;
-    return;
-  }
-  rzg::traverse(n, fl, [&](Pt a, Pt c) {
-    if (MULTI) lb.line_path[o] = pid;
-    lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y);
-  });
-}
-
-// Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
-// cubic virtual items. Every lane of a wave reaches the wave scans.
-#define RZ_FLAT_DISPATCH(ITEM, ...)                                                        \
-  const uint32_t blk = blockIdx.x, t = threadIdx.x;                                        \
-  if (blk < bl) {                                                                          \
-    const uint32_t slot = p.order[blk * kBlock + t];                                       \
-    ITEM<2, MULTI>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);         \
-  } else if (blk < bl + bq) {                                                              \
-    const uint32_t v = (blk - bl) * kBlock + t;                                            \
-    const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];                     \
-    ITEM<3, MULTI>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi, slot_path,  \
-                   __VA_ARGS__);                                                           \
-  } else {                                                                                 \
-    const uint32_t v = (blk - bl - bq) * kBlock + t;                                       \
-    const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];   \
-    ITEM<4, MULTI>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi, slot_path,  \
-                   __VA_ARGS__);                                                           \
-  }
-
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_flat_count(DPath p, Xform x, double fl, uint32_t bl,
-                                                        uint32_t bq, int dq, int dc,
-                                                        const PathInfo* pi,
-                                                        const uint32_t* slot_path,
-                                                        uint32_t* counts, uint32_t* wave_tot,
-                                                        Status* st) {
-  if (st->overflow) return;  // a batched scene whose slices did not fit
-  RZ_FLAT_DISPATCH(flat_count_item, counts, wave_tot, st)
-}
-
-template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_flat_emit(DPath p, Xform x, double fl, uint32_t bl,
-                                                       uint32_t bq, int dq, int dc,
-                                                       const PathInfo* pi, const uint32_t* slot_path,
-                                                       const uint32_t* counts,
-                                                       const uint32_t* wave_off, LineBufs lb) {
-  if (lb.st->err || lb.st->overflow) return;
-  RZ_FLAT_DISPATCH(flat_emit_item, counts, wave_off, lb)
-}
-
-// ---------------------------------------------------------------------------
-// Binning of lines into cell records.
-//   k_bin_lines   the count pass: band-bottom crossings, right-edge rows,
-//                 events, and every cell record with its place among its
-//                 tile's records, which is only known up to one offset per
-//                 (wave, tile): a wave counts its records per tile in LDS and
-//                 adds each count to the tile's total with one atomic at its
-//                 end, which returns that offset;
-//   (scans of the tile and band counts)
-//   k_bin_scatter writes each listed record to tile offset + wave offset +
-//                 rank (no atomics);
-//   k_bin_events  moves the listed events into their bands' slots.
-// Each wave lists its records in a region of its own; a wave that fills it
-// continues in an overflow list of kParts parts, each with its own bump
-// counter, taking kChunk entries at a time and padding what it leaves unused
-// with tile ~0u (the scatter skips padding).
-// ---------------------------------------------------------------------------
-constexpr uint32_t kParts = 16;
-constexpr uint32_t kChunk = 128;
-// Lines per count-pass wave: each wave bins one range of 64 lines and ends, so
-// it never waits for its own atomics (a wave that went on to more lines would
-// wait for them at its next load: loads and atomics share one in-order
-// counter).
-constexpr uint32_t kWaveLines = 64;
-// Per-wave LDS map tile -> record count (open addressing). A tile holds
-// fewer than 2^24 records.
-constexpr int kMapBits = 7;
-constexpr uint32_t kMap = 1u << kMapBits;
-constexpr int kMapProbe = 16;
-constexpr uint32_t kMapNone = 0xffu;  // record ranked in its tile directly
-constexpr uint32_t kEmpty = ~0u;
-
-struct BinBufs {
-  uint32_t* tile_count;   // [tiles] records per tile
-  const uint32_t* tile_off;  // [tiles] their exclusive prefix (after the scan)
-  int32_t* bd_diff;       // [tiles] band backdrop differences
-  uint32_t* band_ev;      // counts per band; after the scan: cursors
-  float4* recs;
-  uint32_t rec_cap;
-  int4* events;           // (band, col, yrel bits, sign bits)
-  uint32_t ev_cap;
-  int32_t* redge;         // [H + 2] right-edge row differences
-  // record list: [n_waves][wave_cap] regions, then [kParts][part_cap] overflow parts
-  uint32_t* list_tile;    // tile (~0u: padding)
-  uint32_t* list_rank;    // map slot << 24 | rank within the wave's records of the tile
-  float4* list_rec;       // the record
-  uint32_t* ovf_wave;     // [kParts][part_cap] wave of each overflow entry
-  uint32_t n_waves, wave_cap, part_cap;
-  uint32_t* wave_used;    // [n_waves] entries of each wave's region
-  uint32_t* wave_map;     // [n_waves][kMap] the wave's offset in each mapped tile
-  uint32_t* fill;         // [kParts + 1] entries taken per overflow part, then listed events
-  int4* ev_list;          // [ev_cap] events of the count pass
-  uint32_t* ev_band;      // [ev_cap] their band slots (band base + band)
-  Status* st;
-};
-
-// Count (PASS 0) or place (PASS 1) the free-endpoint event of vertex p.
-template <int PASS>
-__device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, uint32_t band_base, Pt p,
-                                 float sign) {
-  rzg::Event ev;
-  if (!rzg::make_event(p, sign, g, &ev)) return;
-  if (PASS == 0) {
-    atomicAdd(&b.band_ev[band_base + ev.band], 1u);
-  } else {
-    uint32_t slot = atomicAdd(&b.band_ev[band_base + ev.band], 1u);
-    if (slot < b.ev_cap)
-      b.events[slot] = make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
-    else atomicOr(&b.st->overflow, 2u);
-  }
-}
-
-__device__ inline int4 event_bits(const rzg::Event& ev) {
-  return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
-}
-
 // Cross-lane steps on DPP (no LDS round trip). update_dpp(old, src, ctrl,
 // row_mask, bank_mask, bound_ctrl) gives `old` to lanes whose source lies
 // outside their 16-lane row or whose row is masked off. Controls: row_shr:n
@@ -417,6 +173,263 @@ __device__ inline double lane_next(double v) {
 }
 __device__ inline double lane_prev(double v) {
   return __hiloint2double(lane_prev(__double2hiint(v)), lane_prev(__double2loint(v)));
+}
+
+// ---------------------------------------------------------------------------
+// Flatten to lines for binning, one kernel. Line slots take one thread each;
+// every quad / cubic slot is split into 2^D0 virtual subtrees (the subtree at
+// depth D0 along path bits j), one thread each: the thread walks from the
+// root along j; a leaf met above depth D0 is processed only by the leftmost j
+// under it. Every leaf is produced exactly once, with the same arithmetic as
+// the slot-wide traversal, and a curve with hundreds of lines is spread over
+// many threads.
+//
+// A thread keeps the first kLineKeep lines of its subtree in LDS while it
+// counts them (consecutive lines share their endpoints, so kLineKeep + 1
+// points); a wave then takes the range for its lines with one atomic on one
+// of kLineParts counters and each thread writes its lines there in order, so
+// the lines of a wave stay consecutive (binning wants neighbouring lines
+// together). Only a thread with more lines walks its subtree a second time.
+// For the 200k-segment path, 99.9% of the lines come from threads with at
+// most 8 lines.
+// ---------------------------------------------------------------------------
+constexpr int kLineKeep = 8;
+constexpr uint32_t kLineParts = 64;
+
+struct LineBufs {
+  double4* lines;       // [kLineParts][part_cap]
+  uint32_t* line_path;  // batched scenes: path of each line
+  uint32_t part_cap;    // lines per part
+  uint32_t* fill;       // [kLineParts] lines per part
+  uint2* wave_lines;    // per flatten wave: first line, line count
+  uint32_t n_waves;     // flatten waves
+  Status* st;
+};
+
+// Walk from the slot's root along j for D0 levels. Returns the subtree root
+// in *n and whether a traversal from it is needed; *leaf is set when a leaf
+// above D0 is owned by this j.
+template <int NP>
+__device__ inline bool walk_to_subtree(rzg::Node<NP>* n, uint32_t j, int D0, double fl, bool* leaf,
+                                       int* err) {
+  *leaf = false;
+  for (int k = 0; k < D0; ++k) {
+    if (rzg::has_nans(*n)) {
+      *err = rzg::ST_NAN;
+      return false;
+    }
+    if (rzg::flatness(*n) < fl) {
+      *leaf = (j & ((1u << (D0 - k)) - 1u)) == 0;
+      return false;
+    }
+    rzg::Node<NP> c;
+    rzg::child(*n, (int)((j >> (D0 - 1 - k)) & 1u), &c);
+    *n = c;
+  }
+  return true;
+}
+
+// The subtree a thread of the flatten kernel handles. Returns the node to
+// traverse (walk = true) or a single leaf owned by this thread (leaf = true).
+template <int NP, bool MULTI>
+__device__ inline void flat_subtree(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                                    const Xform& x, double fl, const PathInfo* pi,
+                                    const uint32_t* slot_path, rzg::Node<NP>* n, bool* walk,
+                                    bool* leaf, int* err, uint32_t* pid) {
+  *walk = *leaf = false;
+  *err = rzg::ST_OK;
+  *pid = 0;
+  if (MULTI && active) {
+    *pid = slot_path[slot];
+    active = pi[*pid].live != 0;
+  }
+  if (active) {
+    *n = load_node<NP>(p, slot, MULTI ? pi[*pid].m : x.m);
+    *walk = walk_to_subtree(n, j, D0, fl, leaf, err);
+  }
+}
+
+// One flatten thread: count (keeping the first lines), take the wave's range,
+// write. Every lane of the wave takes part.
+template <int NP, bool MULTI>
+__device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                          const Xform& x, double fl, const PathInfo* pi, const uint32_t* slot_path,
+                          const LineBufs& lb, double2* keep, uint32_t part, uint32_t wave) {
+  rzg::Node<NP> n;
+  bool walk, leaf;
+  int err;
+  uint32_t pid;
+  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
+  uint32_t cnt = 0;
+  if (leaf) {
+    keep[0] = make_double2(n.p[0].x, n.p[0].y);
+    keep[1] = make_double2(n.p[NP - 1].x, n.p[NP - 1].y);
+    cnt = 1;
+  }
+  if (walk) {
+    const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) {
+      if (cnt < (uint32_t)kLineKeep) {
+        if (cnt == 0) keep[0] = make_double2(a.x, a.y);
+        keep[cnt + 1] = make_double2(c.x, c.y);
+      }
+      ++cnt;
+    });
+    if (r != rzg::ST_OK) err = r;
+  }
+  if (err != rzg::ST_OK) {
+    set_err(lb.st, (uint32_t)err);
+    cnt = 0;
+  }
+  const int lane = threadIdx.x & 63;
+  const uint32_t incl = (uint32_t)dpp_add_scan((int)cnt);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(&lb.fill[part], total);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+  const bool fits = base + total <= lb.part_cap;
+  // the wave's lines, in order, for the binning wave of the same index
+  if (lane == 0) lb.wave_lines[wave] = make_uint2(part * lb.part_cap + base, fits ? total : 0u);
+  if (total == 0) return;
+  if (!fits) {
+    if (lane == 0) {
+      atomicOr(&lb.st->overflow, 4u);
+      atomicMax(&lb.st->n_line_part, base + total);
+    }
+    return;
+  }
+  if (!cnt) return;
+  size_t o = (size_t)part * lb.part_cap + base + incl - cnt;
+  if (cnt <= (uint32_t)kLineKeep) {
+    for (uint32_t k = 0; k < cnt; ++k, ++o) {
+      const double2 a = keep[k], c = keep[k + 1];
+      lb.lines[o] = make_double4(a.x, a.y, c.x, c.y);
+      if (MULTI) lb.line_path[o] = pid;
+    }
+    return;
+  }
+  // more lines than were kept: walk the subtree again
+  if (leaf) {
+    lb.lines[o] = make_double4(n.p[0].x, n.p[0].y, n.p[NP - 1].x, n.p[NP - 1].y);
+    if (MULTI) lb.line_path[o] = pid;
+    return;
+  }
+  rzg::traverse(n, fl, [&](Pt a, Pt c) {
+    if (MULTI) lb.line_path[o] = pid;
+    lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y);
+  });
+}
+
+// Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
+// cubic virtual items. Every lane of a wave reaches the wave operations.
+#define RZ_FLAT_DISPATCH(ITEM, ...)                                                        \
+  const uint32_t blk = blockIdx.x, t = threadIdx.x;                                        \
+  if (blk < bl) {                                                                          \
+    const uint32_t slot = p.order[blk * kBlock + t];                                       \
+    ITEM<2, MULTI>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);         \
+  } else if (blk < bl + bq) {                                                              \
+    const uint32_t v = (blk - bl) * kBlock + t;                                            \
+    const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];                     \
+    ITEM<3, MULTI>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi, slot_path,  \
+                   __VA_ARGS__);                                                           \
+  } else {                                                                                 \
+    const uint32_t v = (blk - bl - bq) * kBlock + t;                                       \
+    const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];   \
+    ITEM<4, MULTI>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi, slot_path,  \
+                   __VA_ARGS__);                                                           \
+  }
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_flat_lines(DPath p, Xform x, double fl, uint32_t bl,
+                                                        uint32_t bq, int dq, int dc,
+                                                        const PathInfo* pi,
+                                                        const uint32_t* slot_path, LineBufs lb) {
+  __shared__ double2 keep_all[kBlock * (kLineKeep + 1)];
+  if (lb.st->err || lb.st->overflow) return;  // e.g. a batched scene whose slices did not fit
+  double2* keep = keep_all + threadIdx.x * (kLineKeep + 1);
+  const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t part = wave % kLineParts;
+  RZ_FLAT_DISPATCH(flat_item, lb, keep, part, wave)
+}
+
+// Total of the lines (reporting).
+__global__ void k_line_total(const uint32_t* fill, Status* st) {
+  uint32_t v = fill[threadIdx.x];
+  for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+  if (threadIdx.x == 0) st->n_lines = v;
+}
+
+// ---------------------------------------------------------------------------
+// Binning of lines into cell records.
+//   k_bin_lines   the count pass: band-bottom crossings, right-edge rows,
+//                 events, and every cell record with its place among its
+//                 tile's records, which is only known up to one offset per
+//                 (wave, tile): a wave counts its records per tile in LDS and
+//                 adds each count to the tile's total with one atomic at its
+//                 end, which returns that offset;
+//   (scans of the tile and band counts)
+//   k_bin_scatter writes each listed record to tile offset + wave offset +
+//                 rank (no atomics);
+//   k_bin_events  moves the listed events into their bands' slots.
+// Each wave lists its records in a region of its own; a wave that fills it
+// continues in an overflow list of kParts parts, each with its own bump
+// counter, taking kChunk entries at a time and padding what it leaves unused
+// with tile ~0u (the scatter skips padding).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kParts = 16;
+constexpr uint32_t kChunk = 128;
+// Per-wave LDS map tile -> record count (open addressing). A tile holds
+// fewer than 2^24 records.
+constexpr int kMapBits = 7;
+constexpr uint32_t kMap = 1u << kMapBits;
+constexpr int kMapProbe = 16;
+constexpr uint32_t kMapNone = 0xffu;  // record ranked in its tile directly
+constexpr uint32_t kEmpty = ~0u;
+
+struct BinBufs {
+  uint32_t* tile_count;   // [tiles] records per tile
+  const uint32_t* tile_off;  // [tiles] their exclusive prefix (after the scan)
+  int32_t* bd_diff;       // [tiles] band backdrop differences
+  uint32_t* band_ev;      // counts per band; after the scan: cursors
+  float4* recs;
+  uint32_t rec_cap;
+  int4* events;           // (band, col, yrel bits, sign bits)
+  uint32_t ev_cap;
+  int32_t* redge;         // [H + 2] right-edge row differences
+  // record list: two entries per line slot (each binning wave's region is
+  // those of its lines), then [kParts][part_cap] overflow parts
+  uint32_t* list_tile;    // tile (~0u: padding)
+  uint32_t* list_rank;    // map slot << 24 | rank within the wave's records of the tile
+  float4* list_rec;       // the record
+  uint32_t* ovf_wave;     // [kParts][part_cap] wave of each overflow entry
+  uint32_t n_waves, part_cap;
+  size_t list_regions;    // entries before the overflow parts
+  const uint2* wave_lines;  // per flatten wave: first line, line count (one binning wave each)
+  uint32_t* wave_used;    // [n_waves] entries of each wave's region
+  uint32_t* wave_map;     // [n_waves][kMap] the wave's offset in each mapped tile
+  uint32_t* fill;         // [kParts + 1] entries taken per overflow part, then listed events
+  int4* ev_list;          // [ev_cap] events of the count pass
+  uint32_t* ev_band;      // [ev_cap] their band slots (band base + band)
+  Status* st;
+};
+
+// Count (PASS 0) or place (PASS 1) the free-endpoint event of vertex p.
+template <int PASS>
+__device__ inline void put_event(const BinBufs& b, const rzg::BinGrid& g, uint32_t band_base, Pt p,
+                                 float sign) {
+  rzg::Event ev;
+  if (!rzg::make_event(p, sign, g, &ev)) return;
+  if (PASS == 0) {
+    atomicAdd(&b.band_ev[band_base + ev.band], 1u);
+  } else {
+    uint32_t slot = atomicAdd(&b.band_ev[band_base + ev.band], 1u);
+    if (slot < b.ev_cap)
+      b.events[slot] = make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
+    else atomicOr(&b.st->overflow, 2u);
+  }
+}
+
+__device__ inline int4 event_bits(const rzg::Event& ev) {
+  return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
 }
 
 // Exclusive prefix sum of v over the wave (every lane takes part); the wave
@@ -495,7 +508,8 @@ __device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
   return g;
 }
 
-// Count pass. A wave takes 64 consecutive lines, one per lane: right clip and
+// Count pass. A wave takes the lines of the flatten wave of the same index
+// (consecutive along the path), 64 at a time, one per lane: right clip and
 // left split, right-edge column, events, and the band range of each piece.
 // The (line, band) pairs of the 64 lines are then dealt out to the lanes 64
 // at a time, and the (band part, tile column) pairs of each such round
@@ -508,21 +522,20 @@ __device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
 // two terms cancel exactly).
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
-                                                       const uint32_t* line_path,
-                                                       const uint32_t* n_lines, rzg::BinGrid g0,
+                                                       const uint32_t* line_path, rzg::BinGrid g0,
                                                        const PathInfo* pi, BinBufs b) {
   __shared__ LineStage lstage[kBlock];
   __shared__ BandStage bstage[kBlock];
   __shared__ uint32_t mkey[kBlock / 64][kMap];
   __shared__ uint32_t mcnt[kBlock / 64][kMap];
   __shared__ int mark_all[kBlock / 64][64];
-  if (b.st->err || (b.st->overflow & 4u)) return;
-  const uint32_t n = *n_lines;
+  if (b.st->err || b.st->overflow) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const uint32_t i = wave * kWaveLines + (uint32_t)lane;
-  if (wave * kWaveLines >= n) return;  // the scatter reads the regions of live waves only
+  if (wave >= b.n_waves) return;
+  const uint2 wl_range = b.wave_lines[wave];
+  if (wl_range.y == 0) return;  // the scatter skips the same regions
   LineStage* wl = lstage + (threadIdx.x & ~63u);
   BandStage* wb = bstage + (threadIdx.x & ~63u);
   uint32_t* keys = mkey[threadIdx.x >> 6];
@@ -533,8 +546,9 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     cnts[q] = 0;
   }
   const uint32_t part = wave % kParts;
-  const size_t region = (size_t)wave * b.wave_cap;
-  const size_t obase = (size_t)b.n_waves * b.wave_cap + (size_t)part * b.part_cap;
+  const size_t region = 2 * (size_t)wl_range.x;
+  const uint32_t region_cap = 2 * wl_range.y;
+  const size_t obase = b.list_regions + (size_t)part * b.part_cap;
   uint32_t used = 0;                   // entries of the wave's region
   uint32_t cbase = 0, cused = kChunk;  // the wave's current overflow chunk (none yet)
   bool lover = false;                  // the overflow part is full (the run is re-done)
@@ -577,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     const uint64_t hm = __ballot(has);
     const uint32_t cnt = (uint32_t)__popcll(hm);
     const uint32_t lrank = (uint32_t)__popcll(hm & below);
-    if (cnt && used + cnt <= b.wave_cap) {
+    if (cnt && used + cnt <= region_cap) {
       if (has) {
         const size_t q = region + used + lrank;
         b.list_tile[q] = (uint32_t)tile;
@@ -608,161 +622,165 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
         b.list_tile[q] = (uint32_t)tile;
         b.list_rank[q] = rank;
         b.list_rec[q] = make_float4(r.x0, r.y0, r.x1, r.y1);
-        b.ovf_wave[q - (size_t)b.n_waves * b.wave_cap] = wave;
+        b.ovf_wave[q - b.list_regions] = wave;
       }
       cused += cnt;
     }
   };
 
-  bool have = i < n;
-  uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
-  rzg::BinGrid g = g0;
-  if (MULTI && have) {
-    pid = line_path[i];
-    const PathInfo& P = pi[pid];
-    g = P.g;
-    tb = P.tile_base;
-    bb = P.band_base;
-    rb0 = P.redge_base;
-  }
-  const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
-  const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
-  if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
-    set_err(b.st, rzg::ST_NAN);
-    have = false;
-  }
-  const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
-  {
-    const double nx = lane_next(L.x), ny = lane_next(L.y);
-    const double px = lane_prev(L.z), py = lane_prev(L.w);
-    const bool nproj = lane_next((int)proj) != 0;
-    const bool pproj = lane_prev((int)proj) != 0;
-    const uint32_t npid = (uint32_t)lane_next((int)pid), ppid = (uint32_t)lane_prev((int)pid);
-    if (proj) {
-      const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
-      const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
-      auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
-      if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
-      if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
+  for (uint32_t c0 = 0; c0 < wl_range.y; c0 += 64) {
+    const size_t i = (size_t)wl_range.x + c0 + (uint32_t)lane;
+    bool have = c0 + (uint32_t)lane < wl_range.y;
+    uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
+    rzg::BinGrid g = g0;
+    if (MULTI && have) {
+      pid = line_path[i];
+      const PathInfo& P = pi[pid];
+      g = P.g;
+      tb = P.tile_base;
+      bb = P.band_base;
+      rb0 = P.redge_base;
     }
-  }
-  int nev = 0;
-  int4 ev0{}, ev1{};
-  uint32_t evb0 = 0, evb1 = 0;
-  int nb0 = 0, nb1 = 0;
-  if (have && !proj) {
-    const rzg::EffLine e = rzg::effective_line(p0, p1, g);
-    rzg::Event ev;
-    if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
-      ev0 = event_bits(ev);
-      evb0 = bb + (uint32_t)ev.band;
-      nev = 1;
+    const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
+    const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
+    if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
+      set_err(b.st, rzg::ST_NAN);
+      have = false;
     }
-    if (e.clip1 && rzg::make_event(e.n == 2 ? e.p[2] : e.p[1], 1.0f, g, &ev)) {
-      (nev ? ev1 : ev0) = event_bits(ev);
-      (nev ? evb1 : evb0) = bb + (uint32_t)ev.band;
-      ++nev;
-    }
-    int ty0 = 0, ty1 = -1, uy0 = 0, uy1 = -1;
-    if (rzg::piece_bands(e.p[0], e.p[1], g, &ty0, &ty1) && ty1 >= ty0) nb0 = ty1 - ty0 + 1;
-    if (e.n == 2 && rzg::piece_bands(e.p[1], e.p[2], g, &uy0, &uy1) && uy1 >= uy0)
-      nb1 = uy1 - uy0 + 1;
-    LineStage& S = wl[lane];
-    S.px[0] = e.p[0].x;
-    S.py[0] = e.p[0].y;
-    S.px[1] = e.p[1].x;
-    S.py[1] = e.p[1].y;
-    S.px[2] = e.p[2].x;
-    S.py[2] = e.p[2].y;
-    S.ty0[0] = ty0;
-    S.ty0[1] = uy0;
-    S.nb0 = nb0;
-    S.H = g.H;
-    S.tiles_x = g.tiles_x;
-    S.phx = g.phx;
-    S.phy = g.phy;
-    S.tb = tb;
-  }
-  // events: count, then list them (at most two per line)
-  if (nev >= 1) atomicAdd(&b.band_ev[evb0], 1u);
-  if (nev == 2) atomicAdd(&b.band_ev[evb1], 1u);
-  const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
-  if (e1) {
-    const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
-    uint32_t ebase = 0;
-    if (lane == 0) ebase = atomicAdd(&b.fill[kParts], etot);
-    ebase = (uint32_t)__shfl((int)ebase, 0, 64);
-    const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
-    if (ebase + etot > b.ev_cap) {
-      if (lane == 0) atomicOr(&b.st->overflow, 2u);
-    } else {
-      if (nev >= 1) {
-        b.ev_list[k] = ev0;
-        b.ev_band[k] = evb0;
-      }
-      if (nev == 2) {
-        b.ev_list[k + 1] = ev1;
-        b.ev_band[k + 1] = evb1;
+    const bool proj = have && g.mode != rzg::EDGE_NONE && p0.x > g.wc && p1.x > g.wc;
+    {
+      const double nx = lane_next(L.x), ny = lane_next(L.y);
+      const double px = lane_prev(L.z), py = lane_prev(L.w);
+      const bool nproj = lane_next((int)proj) != 0;
+      const bool pproj = lane_prev((int)proj) != 0;
+      const uint32_t npid = (uint32_t)lane_next((int)pid), ppid = (uint32_t)lane_prev((int)pid);
+      if (proj) {
+        const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
+        const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
+        auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
+        if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
+        if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
       }
     }
-  }
-  // band tasks, 64 per round
-  uint32_t n_band;
-  const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
-  wave_lds_sync();
-  int carry1 = -1;
-  for (uint32_t j0 = 0; j0 < n_band; j0 += 64) {
-    const uint32_t j = j0 + lane;
-    const int o = task_owner(marks, pre1, (uint32_t)(nb0 + nb1), j0, lane, &carry1);
-    const uint32_t opre = (uint32_t)__shfl((int)pre1, o, 64);
-    int nt = 0;
-    if (j < n_band) {
-      const LineStage& S = wl[o];
-      const int k = (int)(j - opre);
-      const int pc = k < S.nb0 ? 0 : 1;
-      const int ty = pc == 0 ? S.ty0[0] + k : S.ty0[1] + (k - S.nb0);
-      const Pt a = rzg::mkpt(S.px[pc], S.py[pc]), c = rzg::mkpt(S.px[pc + 1], S.py[pc + 1]);
-      const rzg::BinGrid gt = stage_grid(S);
-      rzg::BandPiece bp;
-      rzg::band_piece(a, c, gt, ty, &bp);
-      if (bp.cross_col >= 0)
-        atomicAdd(&b.bd_diff[S.tb + ty * gt.tiles_x + bp.cross_col], c.y > a.y ? 1 : -1);
-      if (bp.any) {
-        nt = bp.tx1 - bp.tx0 + 1;
-        BandStage& B = wb[lane];
-        B.sx = bp.s.x;
-        B.sy = bp.s.y;
-        B.ex = bp.e.x;
-        B.ey = bp.e.y;
-        B.r0 = bp.r0;
-        B.ty = ty;
-        B.tx0 = bp.tx0;
-        B.line = o;
+    int nev = 0;
+    int4 ev0{}, ev1{};
+    uint32_t evb0 = 0, evb1 = 0;
+    int nb0 = 0, nb1 = 0;
+    if (have && !proj) {
+      const rzg::EffLine e = rzg::effective_line(p0, p1, g);
+      rzg::Event ev;
+      if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
+        ev0 = event_bits(ev);
+        evb0 = bb + (uint32_t)ev.band;
+        nev = 1;
+      }
+      if (e.clip1 && rzg::make_event(e.n == 2 ? e.p[2] : e.p[1], 1.0f, g, &ev)) {
+        (nev ? ev1 : ev0) = event_bits(ev);
+        (nev ? evb1 : evb0) = bb + (uint32_t)ev.band;
+        ++nev;
+      }
+      int ty0 = 0, ty1 = -1, uy0 = 0, uy1 = -1;
+      if (rzg::piece_bands(e.p[0], e.p[1], g, &ty0, &ty1) && ty1 >= ty0) nb0 = ty1 - ty0 + 1;
+      if (e.n == 2 && rzg::piece_bands(e.p[1], e.p[2], g, &uy0, &uy1) && uy1 >= uy0)
+        nb1 = uy1 - uy0 + 1;
+      LineStage& S = wl[lane];
+      S.px[0] = e.p[0].x;
+      S.py[0] = e.p[0].y;
+      S.px[1] = e.p[1].x;
+      S.py[1] = e.p[1].y;
+      S.px[2] = e.p[2].x;
+      S.py[2] = e.p[2].y;
+      S.ty0[0] = ty0;
+      S.ty0[1] = uy0;
+      S.nb0 = nb0;
+      S.H = g.H;
+      S.tiles_x = g.tiles_x;
+      S.phx = g.phx;
+      S.phy = g.phy;
+      S.tb = tb;
+    }
+    // events: count, then list them (at most two per line)
+    if (nev >= 1) atomicAdd(&b.band_ev[evb0], 1u);
+    if (nev == 2) atomicAdd(&b.band_ev[evb1], 1u);
+    const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
+    if (e1) {
+      const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
+      uint32_t ebase = 0;
+      if (lane == 0) ebase = atomicAdd(&b.fill[kParts], etot);
+      ebase = (uint32_t)__shfl((int)ebase, 0, 64);
+      const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
+      if (ebase + etot > b.ev_cap) {
+        if (lane == 0) atomicOr(&b.st->overflow, 2u);
+      } else {
+        if (nev >= 1) {
+          b.ev_list[k] = ev0;
+          b.ev_band[k] = evb0;
+        }
+        if (nev == 2) {
+          b.ev_list[k + 1] = ev1;
+          b.ev_band[k + 1] = evb1;
+        }
       }
     }
-    // tile tasks of this round's band parts, 64 per round
-    uint32_t n_tile;
-    const uint32_t pre2 = wave_excl_scan((uint32_t)nt, lane, &n_tile);
+    // band tasks, 64 per round
+    uint32_t n_band;
+    const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
     wave_lds_sync();
-    int carry2 = -1;
-    for (uint32_t k0 = 0; k0 < n_tile; k0 += 64) {
-      const uint32_t k = k0 + lane;
-      const int o2 = task_owner(marks, pre2, (uint32_t)nt, k0, lane, &carry2);
-      const uint32_t opre2 = (uint32_t)__shfl((int)pre2, o2, 64);
-      bool has = false;
-      int tile = -1;
-      rzg::Rec r{};
-      if (k < n_tile) {
-        const BandStage& B = wb[o2];
-        const LineStage& S = wl[B.line];
+    int carry1 = -1;
+    for (uint32_t j0 = 0; j0 < n_band; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      const int o = task_owner(marks, pre1, (uint32_t)(nb0 + nb1), j0, lane, &carry1);
+      const uint32_t opre = (uint32_t)__shfl((int)pre1, o, 64);
+      int nt = 0;
+      if (j < n_band) {
+        const LineStage& S = wl[o];
+        const int k = (int)(j - opre);
+        const int pc = k < S.nb0 ? 0 : 1;
+        const int ty = pc == 0 ? S.ty0[0] + k : S.ty0[1] + (k - S.nb0);
+        const Pt a = rzg::mkpt(S.px[pc], S.py[pc]), c = rzg::mkpt(S.px[pc + 1], S.py[pc + 1]);
         const rzg::BinGrid gt = stage_grid(S);
-        const int tx = B.tx0 + (int)(k - opre2);
-        has = rzg::tile_record(rzg::mkpt(B.sx, B.sy), rzg::mkpt(B.ex, B.ey), B.r0, tx, gt, &r);
-        tile = (int)S.tb + B.ty * S.tiles_x + tx;
+        rzg::BandPiece bp;
+        rzg::band_piece(a, c, gt, ty, &bp);
+        if (bp.cross_col >= 0)
+          atomicAdd(&b.bd_diff[S.tb + ty * gt.tiles_x + bp.cross_col], c.y > a.y ? 1 : -1);
+        if (bp.any) {
+          nt = bp.tx1 - bp.tx0 + 1;
+          BandStage& B = wb[lane];
+          B.sx = bp.s.x;
+          B.sy = bp.s.y;
+          B.ex = bp.e.x;
+          B.ey = bp.e.y;
+          B.r0 = bp.r0;
+          B.ty = ty;
+          B.tx0 = bp.tx0;
+          B.line = o;
+        }
       }
-      emit(has, tile, r);
+      // tile tasks of this round's band parts, 64 per round
+      uint32_t n_tile;
+      const uint32_t pre2 = wave_excl_scan((uint32_t)nt, lane, &n_tile);
+      wave_lds_sync();
+      int carry2 = -1;
+      for (uint32_t k0 = 0; k0 < n_tile; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const int o2 = task_owner(marks, pre2, (uint32_t)nt, k0, lane, &carry2);
+        const uint32_t opre2 = (uint32_t)__shfl((int)pre2, o2, 64);
+        bool has = false;
+        int tile = -1;
+        rzg::Rec r{};
+        if (k < n_tile) {
+          const BandStage& B = wb[o2];
+          const LineStage& S = wl[B.line];
+          const rzg::BinGrid gt = stage_grid(S);
+          const int tx = B.tx0 + (int)(k - opre2);
+          has = rzg::tile_record(rzg::mkpt(B.sx, B.sy), rzg::mkpt(B.ex, B.ey), B.r0, tx, gt, &r);
+          tile = (int)S.tb + B.ty * S.tiles_x + tx;
+        }
+        emit(has, tile, r);
+      }
+      wave_lds_sync();  // the next band round re-stages the band parts
     }
-    wave_lds_sync();  // the next band round re-stages the band parts
+    wave_lds_sync();  // the next lines re-stage the polylines
   }
   if (!lover && cused < kChunk)
     for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_tile[obase + cbase + q] = ~0u;
@@ -790,18 +808,19 @@ __device__ inline void place_record(const BinBufs& b, uint32_t tile, uint32_t ra
 // Place the listed records: one wave per count-pass region (grid-stride over
 // the regions of the waves that had lines), then the overflow parts one entry
 // per thread.
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(const uint32_t* n_lines, BinBufs b) {
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
   if (b.st->err || b.st->overflow) return;
-  const uint32_t live = (*n_lines + kWaveLines - 1) / kWaveLines;
   const uint32_t wpb = kBlock / 64;
-  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < live; w += gridDim.x * wpb) {
+  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < b.n_waves; w += gridDim.x * wpb) {
+    const uint2 r = b.wave_lines[w];
+    if (r.y == 0) continue;
     const uint32_t used = b.wave_used[w];
-    const size_t region = (size_t)w * b.wave_cap;
+    const size_t region = 2 * (size_t)r.x;
     for (uint32_t j = threadIdx.x & 63; j < used; j += 64)
       place_record(b, b.list_tile[region + j], b.list_rank[region + j], w, b.list_rec[region + j]);
   }
   const uint32_t stride = gridDim.x * kBlock;
-  const size_t obase = (size_t)b.n_waves * b.wave_cap;
+  const size_t obase = b.list_regions;
   for (uint32_t p = 0; p < kParts; ++p) {
     const uint32_t np = min(b.fill[p], b.part_cap);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < np; k += stride) {
@@ -872,6 +891,7 @@ __global__ void k_status_merge(const Status* src, Status* acc) {
   acc->n_events = max(acc->n_events, src->n_events);
   acc->n_lines = max(acc->n_lines, src->n_lines);
   acc->n_list = max(acc->n_list, src->n_list);
+  acc->n_line_part = max(acc->n_line_part, src->n_line_part);
 }
 
 // Fill a layer with its background colour (Layer::new, src/scene.rs:483-501).
