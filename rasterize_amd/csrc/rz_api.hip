@@ -472,15 +472,14 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   const size_t no = (size_t)rzk::kParts * ctx->part_cap;
   const size_t nreg = 2 * (size_t)ctx->line_cap;
   const size_t nl = nreg + no;
-  const size_t words = nl * 2 + no + (size_t)ctx->ev_cap + nw + (size_t)nw * rzk::kMap;
-  RZ_TRY(ensure(ctx, ctx->list_buf, (nl + ctx->ev_cap) * 16 + words * 4));
+  const size_t words = no + (size_t)ctx->ev_cap + nw + (size_t)nw * rzk::kMap;
+  RZ_TRY(ensure(ctx, ctx->list_buf, (nl + ctx->ev_cap) * 16 + nl * 8 + words * 4));
   char* lp = (char*)ctx->list_buf.p;
   b->list_rec = (float4*)lp;
   b->ev_list = (int4*)(lp + nl * 16);
-  uint32_t* w = (uint32_t*)(lp + (nl + ctx->ev_cap) * 16);
-  b->list_tile = w;
-  b->list_rank = w + nl;
-  b->ovf_wave = w + 2 * nl;
+  b->list_meta = (uint2*)(lp + (nl + ctx->ev_cap) * 16);
+  uint32_t* w = (uint32_t*)(lp + (nl + ctx->ev_cap) * 16 + nl * 8);
+  b->ovf_wave = w;
   b->ev_band = b->ovf_wave + no;
   b->wave_used = b->ev_band + ctx->ev_cap;
   b->wave_map = b->wave_used + nw;

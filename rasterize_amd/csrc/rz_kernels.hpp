@@ -397,8 +397,8 @@ struct BinBufs {
   int32_t* redge;         // [H + 2] right-edge row differences
   // record list: two entries per line slot (each binning wave's region is
   // those of its lines), then [kParts][part_cap] overflow parts
-  uint32_t* list_tile;    // tile (~0u: padding)
-  uint32_t* list_rank;    // map slot << 24 | rank within the wave's records of the tile
+  uint2* list_meta;       // tile (~0u: padding), map slot << 24 | rank within the wave's
+                          // records of the tile
   float4* list_rec;       // the record
   uint32_t* ovf_wave;     // [kParts][part_cap] wave of each overflow entry
   uint32_t n_waves, part_cap;
@@ -553,40 +553,20 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
   uint32_t cbase = 0, cused = kChunk;  // the wave's current overflow chunk (none yet)
   bool lover = false;                  // the overflow part is full (the run is re-done)
 
-  // Rank and list one record per lane (has = false: none); every lane takes part.
+  // Rank and list one record per lane (has = false: none); every lane takes
+  // part. Each record takes its rank from its tile's count in the wave's LDS
+  // map (lanes with equal tiles are serialized by the LDS atomic).
   auto emit = [&](bool has, int tile, const rzg::Rec& r) {
-    // group equal tiles: each group's leader takes the group's ranks
-    bool pend = has;
-    uint32_t tot = 0;  // size of the group this lane leads
-    int ldr = 0;       // lane leading this lane's group
-    uint32_t off = 0;  // place within the group
-    while (true) {
-      const uint64_t m = __ballot(pend);
-      if (m == 0) break;
-      const int leader = __ffsll((unsigned long long)m) - 1;
-      const int T = __builtin_amdgcn_readlane(tile, leader);
-      const bool mine = pend && tile == T;
-      const uint64_t gm = __ballot(mine);
-      if (lane == leader) tot = (uint32_t)__popcll(gm);
-      if (mine) {
-        ldr = leader;
-        off = (uint32_t)__popcll(gm & below);
-        pend = false;
-      }
-    }
-    int slot = (int)kMapNone;
-    uint32_t base = 0;
-    if (tot) {
+    uint32_t rank = 0;
+    if (has) {
       const int h = map_slot(keys, (uint32_t)tile);
       if (h >= 0) {
-        slot = h;
-        base = atomicAdd(&cnts[h], tot);
+        rank = ((uint32_t)h << 24) | atomicAdd(&cnts[h], 1u);
       } else {
         // the map is full: rank directly among all of the tile's records
-        base = atomicAdd(&b.tile_count[tile], tot);
+        rank = (kMapNone << 24) | atomicAdd(&b.tile_count[tile], 1u);
       }
     }
-    const uint32_t rank = ((uint32_t)__shfl(slot, ldr, 64) << 24) | (__shfl((int)base, ldr, 64) + off);
     // append to the wave's region, or past its end to the overflow list
     const uint64_t hm = __ballot(has);
     const uint32_t cnt = (uint32_t)__popcll(hm);
@@ -594,8 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     if (cnt && used + cnt <= region_cap) {
       if (has) {
         const size_t q = region + used + lrank;
-        b.list_tile[q] = (uint32_t)tile;
-        b.list_rank[q] = rank;
+        b.list_meta[q] = make_uint2((uint32_t)tile, rank);
         b.list_rec[q] = make_float4(r.x0, r.y0, r.x1, r.y1);
       }
       used += cnt;
@@ -603,7 +582,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
       if (cused + cnt > kChunk) {
         // pad the rest of the chunk and take a new one
         if (!lover)
-          for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_tile[obase + cbase + q] = ~0u;
+          for (uint32_t q = cused + lane; q < kChunk; q += 64)
+            b.list_meta[obase + cbase + q] = make_uint2(~0u, 0u);
         uint32_t nb = 0;
         if (lane == 0) nb = atomicAdd(&b.fill[part], kChunk);
         nb = (uint32_t)__shfl((int)nb, 0, 64);
@@ -619,8 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
       }
       if (has && !lover) {
         const size_t q = obase + cbase + cused + lrank;
-        b.list_tile[q] = (uint32_t)tile;
-        b.list_rank[q] = rank;
+        b.list_meta[q] = make_uint2((uint32_t)tile, rank);
         b.list_rec[q] = make_float4(r.x0, r.y0, r.x1, r.y1);
         b.ovf_wave[q - b.list_regions] = wave;
       }
@@ -783,7 +762,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
     wave_lds_sync();  // the next lines re-stage the polylines
   }
   if (!lover && cused < kChunk)
-    for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_tile[obase + cbase + q] = ~0u;
+    for (uint32_t q = cused + lane; q < kChunk; q += 64) b.list_meta[obase + cbase + q] = make_uint2(~0u, 0u);
   // add the wave's counts to the tiles' totals; each atomic returns the
   // wave's offset among the tile's records
   wave_lds_sync();
@@ -817,7 +796,10 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
     const uint32_t used = b.wave_used[w];
     const size_t region = 2 * (size_t)r.x;
     for (uint32_t j = threadIdx.x & 63; j < used; j += 64)
-      place_record(b, b.list_tile[region + j], b.list_rank[region + j], w, b.list_rec[region + j]);
+    {
+      const uint2 mt = b.list_meta[region + j];
+      place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
+    }
   }
   const uint32_t stride = gridDim.x * kBlock;
   const size_t obase = b.list_regions;
@@ -825,8 +807,8 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
     const uint32_t np = min(b.fill[p], b.part_cap);
     for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < np; k += stride) {
       const size_t o = (size_t)p * b.part_cap + k;
-      const uint32_t t = b.list_tile[obase + o];
-      if (t != ~0u) place_record(b, t, b.list_rank[obase + o], b.ovf_wave[o], b.list_rec[obase + o]);
+      const uint2 mt = b.list_meta[obase + o];
+      if (mt.x != ~0u) place_record(b, mt.x, mt.y, b.ovf_wave[o], b.list_rec[obase + o]);
     }
   }
 }
