@@ -220,17 +220,30 @@ rz_status check_launch(rz_ctx* ctx, const char* what) {
 // Exclusive scan of n u32 values: out[0..n] (out[n] = total). Optionally
 // copies the offsets to `cursor` and the total to `total`.
 rz_status scan_u32(rz_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* cursor,
-                   uint32_t* total) {
+                   uint32_t* total, const rzk::SmallScan& x = rzk::SmallScan{}) {
   size_t nb = std::max<size_t>(1, (n + rzk::kScanTile - 1) / rzk::kScanTile);
   RZ_TRY(ensure(ctx, ctx->sums, nb * sizeof(uint32_t)));
   uint32_t* sums = (uint32_t*)ctx->sums.p;
   RZ_LAUNCH(ctx, "k_scan_tiles", rzk::k_scan_tiles, dim3((uint32_t)nb), dim3(rzk::kBlock), in,
                      out, n, sums);
   RZ_LAUNCH(ctx, "k_scan_sums", rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), sums, nb, out, n,
-                     total);
+                     total, x);
   RZ_LAUNCH(ctx, "k_scan_add", rzk::k_scan_add, dim3((uint32_t)nb), dim3(rzk::kBlock), out, n,
                      (const uint32_t*)sums, cursor);
   return check_launch(ctx, "scan");
+}
+
+// The tile scan and the band scan: a short band array rides along with the
+// tile scan's second pass.
+rz_status scan_tiles_bands(rz_ctx* ctx, const uint32_t* tiles_in, uint32_t* tiles_out, size_t nt,
+                           uint32_t* n_records, uint32_t* bands, uint32_t* bands_out, size_t nb,
+                           uint32_t* n_events) {
+  if (nb <= 4096) {
+    const rzk::SmallScan x{bands, bands_out, bands, n_events, (uint32_t)nb};
+    return scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records, x);
+  }
+  RZ_TRY(scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records));
+  return scan_u32(ctx, bands, bands_out, nb, bands, n_events);
 }
 
 rz_status validate_path(rz_ctx* ctx, const rz_path* p) {
@@ -447,7 +460,6 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
                 d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     }
   }
-  RZ_LAUNCH(ctx, "k_line_total", rzk::k_line_total, dim3(1), dim3(64), (const uint32_t*)line_fill, st);
   return check_launch(ctx, "flatten");
 }
 
@@ -494,13 +506,13 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   return RZ_OK;
 }
 
-rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b) {
+rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* line_fill) {
   if (b.n_waves == 0) return RZ_OK;
   // many waves, each over regions far apart (neighbouring regions hold
   // records of the same tiles)
   const uint32_t grid = std::min<uint32_t>(b.n_waves / (rzk::kBlock / 64), 8192);
   RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kBlock), b);
-  RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b);
+  RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b, line_fill);
   return RZ_OK;
 }
 
@@ -567,11 +579,11 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   RZ_TRY(check_launch(ctx, "bin pass 0"));
-  RZ_TRY(scan_u32(ctx, P.tile_count, P.tile_off, tiles, nullptr, &P.st->n_records));
-  RZ_TRY(scan_u32(ctx, P.band_ev, P.ev_off, ty, P.band_ev, &P.st->n_events));
+  RZ_TRY(scan_tiles_bands(ctx, P.tile_count, P.tile_off, tiles, &P.st->n_records, P.band_ev,
+                          P.ev_off, ty, &P.st->n_events));
   RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
             P.grid.g.tiles_x, P.redge, (int)(H + 2));
-  RZ_TRY(enqueue_bin_place(ctx, b));
+  RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, false>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   return check_launch(ctx, "bin pass 1");
@@ -1046,11 +1058,11 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "scene pass 0"));
-  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, nullptr, &st->n_records));
-  RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
+  RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
+                          &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_TRY(enqueue_bin_place(ctx, b));
+  RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
@@ -1202,11 +1214,11 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
-  RZ_TRY(scan_u32(ctx, tile_count, tile_off, ct, nullptr, &st->n_records));
-  RZ_TRY(scan_u32(ctx, band_ev, ev_off, cb, band_ev, &st->n_events));
+  RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
+                          &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
-  RZ_TRY(enqueue_bin_place(ctx, b));
+  RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;

@@ -351,12 +351,6 @@ __global__ __launch_bounds__(kBlock) void k_flat_lines(DPath p, Xform x, double 
   RZ_FLAT_DISPATCH(flat_item, lb, keep, part, wave)
 }
 
-// Total of the lines (reporting).
-__global__ void k_line_total(const uint32_t* fill, Status* st) {
-  uint32_t v = fill[threadIdx.x];
-  for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
-  if (threadIdx.x == 0) st->n_lines = v;
-}
 
 // ---------------------------------------------------------------------------
 // Binning of lines into cell records.
@@ -813,8 +807,14 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
   }
 }
 
-// Place the events the count pass listed into their bands' slots.
-__global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b) {
+// Place the events the count pass listed into their bands' slots; the first
+// wave also totals the lines (reporting).
+__global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b, const uint32_t* line_fill) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    uint32_t v = line_fill[threadIdx.x];
+    for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    if (threadIdx.x == 0) b.st->n_lines = v;
+  }
   if (b.st->err || b.st->overflow) return;
   const uint32_t n = b.fill[kParts];
   for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
@@ -941,10 +941,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint3
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// A whole exclusive scan done by one block (a short array riding along with
+// another scan's second pass).
+struct SmallScan {
+  const uint32_t* in;
+  uint32_t* out;     // [n + 1]
+  uint32_t* cursor;  // copy of the offsets, or null
+  uint32_t* total;   // or null
+  uint32_t n;        // 0: none
+};
+
 // Pass 2: one block scans the block totals (any count) and writes the grand
-// total to out[n] and *total.
+// total to out[n] and *total; then, if given, the short array `x`.
 __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* sums, size_t nb, uint32_t* out,
-                                                       size_t n, uint32_t* total) {
+                                                       size_t n, uint32_t* total, SmallScan x) {
   uint32_t carry = 0;
   for (size_t start = 0; start < nb; start += kBlock) {
     size_t i = start + threadIdx.x;
@@ -958,6 +968,23 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* sums, size_t nb,
   if (threadIdx.x == 0) {
     out[n] = carry;
     if (total) *total = carry;
+  }
+  carry = 0;
+  for (uint32_t start = 0; start < x.n; start += kBlock) {
+    const uint32_t i = start + threadIdx.x;
+    const uint32_t v = i < x.n ? x.in[i] : 0u;
+    uint32_t tot;
+    const uint32_t pre = block_exclusive_scan(v, &tot);
+    if (i < x.n) {
+      x.out[i] = carry + pre;
+      if (x.cursor) x.cursor[i] = carry + pre;
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && x.n) {
+    x.out[x.n] = carry;
+    if (x.total) *x.total = carry;
   }
 }
 
