@@ -157,6 +157,30 @@ def test_bin_long_lines_fresh_context(mode):
     assert np.array_equal(_sort_events(ge), _sort_events(re)), "events differ"
 
 
+@pytest.mark.parametrize("dq,dc", [(0, 0), (1, 2), (3, 5), (6, 6)])
+def test_bin_split_depths_agree(dq, dc, c2):
+    """The virtual-subtree depths only spread the flatten work over threads:
+    the binning must come out bit-identical for any of them (and for the
+    depths chosen automatically by the default context)."""
+    path, tr = c2
+    rc, rr, rb, re, rx = rz.bin_stage(path, tr, 4096, 4096)
+    ctx = rz.Context(0)
+    try:
+        L = rz._abi.lib()
+        ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_SPLIT_QUAD, dq))
+        ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_SPLIT_CUBIC, dc))
+        v = C.c_int()
+        ctx.check(L.rz_get_param(ctx.h, rz._abi.RZ_PARAM_SPLIT_AUTO, C.byref(v)))
+        assert v.value == 0
+        _say(f"bin c2 split dq={dq} dc={dc}")
+        gc, gr, gb, ge, gx = rz.bin_stage(path, tr, 4096, 4096, ctx=ctx)
+    finally:
+        ctx.close()
+    assert np.array_equal(gc, rc) and np.array_equal(gb, rb) and np.array_equal(gx, rx)
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr))
+    assert np.array_equal(_sort_events(ge), _sort_events(re))
+
+
 @pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
 def test_mask_long_lines(rule):
     path = _spokes(3000, 4096)
