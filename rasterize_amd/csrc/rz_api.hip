@@ -49,6 +49,7 @@ struct rz_ctx {
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   DBuf list_buf;  // binning: record list (records, tiles) and event list
   DBuf wave_lines;  // per flatten wave: its lines' first index and count
+  DBuf atlas_tiles;  // per atlas tile: what the fine stage needs (AtlasTile)
   uint32_t rec_cap = 0, ev_cap = 0;
   uint32_t part_cap = 0;  // overflow record-list entries per part
   rzk::Status* host_status = nullptr;  // pinned
@@ -1222,8 +1223,11 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
             d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;
-  a.pi = pi;
-  a.tile_glyph = s->atlas_tile_glyph;
+  RZ_TRY(ensure(ctx, ctx->atlas_tiles, (size_t)ct * sizeof(rzk::AtlasTile)));
+  RZ_LAUNCH(ctx, "k_atlas_tiles", rzk::k_atlas_tiles, dim3(blocks_for(ct)), dim3(rzk::kBlock),
+            (const rzk::PathInfo*)pi, (const uint32_t*)s->atlas_tile_glyph, ct,
+            (rzk::AtlasTile*)ctx->atlas_tiles.p);
+  a.tiles = (const rzk::AtlasTile*)ctx->atlas_tiles.p;
   a.n = n;
   a.tile_off = tile_off;
   a.recs = (const float4*)ctx->recs.p;
@@ -1483,7 +1487,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                   &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
                   &ctx->sc_boxes,  &ctx->sc_has,    &ctx->sc_pi,     &ctx->sc_counts,
                   &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
-                  &ctx->wave_lines};
+                  &ctx->wave_lines, &ctx->atlas_tiles};
   for (DBuf* b : bufs) free_buf(*b);
   for (auto& r : ctx->pending) {
     (void)hipEventDestroy(r.a);

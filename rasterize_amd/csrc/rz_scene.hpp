@@ -405,9 +405,43 @@ __global__ __launch_bounds__(kBlock) void k_atlas_setup(const AtlasSlot* slots, 
   pi[i] = P;
 }
 
+// What the fine stage needs of one atlas tile, gathered once per run so the
+// per-tile work starts from a single load.
+struct AtlasTile {
+  double redge_w;  // the glyph's right-edge weight
+  int x, y;        // atlas pixel of the tile's top-left corner
+  int w, h;        // glyph pixels inside the tile (0: nothing to draw)
+  int txl;         // tile column within the glyph
+  int rcol;        // right-edge column relative to the tile's first column
+  uint32_t band;   // band slot (the glyph's band base + band)
+  uint32_t redge;  // right-edge slot of the tile's first row
+};
+
+__global__ __launch_bounds__(kBlock) void k_atlas_tiles(const PathInfo* pi,
+                                                         const uint32_t* tile_glyph, uint32_t n_tiles,
+                                                         AtlasTile* tiles) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n_tiles) return;
+  const PathInfo& P = pi[tile_glyph[t]];
+  AtlasTile T{};
+  if (P.live) {
+    const uint32_t local = t - P.tile_base;
+    const int txl = (int)(local % (uint32_t)P.g.tiles_x), tyl = (int)(local / (uint32_t)P.g.tiles_x);
+    T.redge_w = P.redge_w;
+    T.x = P.ox + txl * 16;
+    T.y = P.oy + tyl * 16;
+    T.w = min(16, P.g.W - txl * 16);
+    T.h = min(16, P.g.H - tyl * 16);
+    T.txl = txl;
+    T.rcol = P.redge_col == INT32_MAX ? INT32_MAX : P.redge_col - txl * 16;
+    T.band = P.band_base + (uint32_t)tyl;
+    T.redge = P.redge_base + (uint32_t)tyl * 16u;
+  }
+  tiles[t] = T;
+}
+
 struct AtlasFineArgs {
-  const PathInfo* pi;
-  const uint32_t* tile_glyph;  // glyph of each atlas tile
+  const AtlasTile* tiles;
   uint32_t n;
   const uint32_t* tile_off;
   const float4* recs;
@@ -420,9 +454,8 @@ struct AtlasFineArgs {
   size_t row_stride;  // floats
 };
 
-// One tile of one glyph per wave (the glyph from a per-tile table made when
-// the slots are set); coverage as in k_fine, written into the glyph's atlas
-// view.
+// One tile of one glyph per wave (from its AtlasTile); coverage as in k_fine,
+// written into the glyph's atlas view.
 template <int RULE>
 __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   if (a.st->err || a.st->overflow) return;
@@ -432,12 +465,11 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   __shared__ int marks[64];
   const int lane = threadIdx.x;
   const uint32_t t = blockIdx.x;
-  const PathInfo& P = a.pi[a.tile_glyph[t]];
-  if (!P.live) return;
-  const uint32_t local = t - P.tile_base;
-  const int txl = (int)(local % (uint32_t)P.g.tiles_x), tyl = (int)(local / (uint32_t)P.g.tiles_x);
+  const AtlasTile T = a.tiles[t];
   const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
-  const uint32_t eb = a.band_ev_off[P.band_base + tyl], ee = a.band_ev_off[P.band_base + tyl + 1];
+  const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
+  if (T.w <= 0 || T.h <= 0) return;
+  const uint32_t eb = a.band_ev_off[T.band], ee = a.band_ev_off[T.band + 1];
   const bool local_work = rb != re || eb != ee;
   if (local_work) {
 #pragma unroll
@@ -447,27 +479,24 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
     tile_records(a.recs, rb, re, acc, rowd, rs, marks, lane);
     for (uint32_t i = eb + lane; i < ee; i += 64) {
       const int4 ev = a.events[i];
-      if (ev.y < txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+      if (ev.y < T.txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
     }
     __syncthreads();
     if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
     __syncthreads();
   }
-  const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
   const int col = lane & 15;
-  const int xs = txl * 16 + col;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int r = 4 * k + (lane >> 4);
     int64_t w = bd;
     if (local_work) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
-    const int ys = tyl * 16 + r;
-    if (xs < P.g.W && ys < P.g.H) {
-      if (xs >= P.redge_col) {
-        const int64_t rr = a.redge[P.redge_base + ys];
-        w += xs == P.redge_col ? (int64_t)rint((double)rr * P.redge_w) : rr;
+    if (col < T.w && r < T.h) {
+      if (col >= T.rcol) {
+        const int64_t rr = a.redge[T.redge + (uint32_t)r];
+        w += col == T.rcol ? (int64_t)rint((double)rr * T.redge_w) : rr;
       }
-      a.out[(size_t)(P.oy + ys) * a.row_stride + (size_t)(P.ox + xs)] = alpha_fix<RULE>(w);
+      a.out[(size_t)(T.y + r) * a.row_stride + (size_t)(T.x + col)] = alpha_fix<RULE>(w);
     }
   }
 }
