@@ -452,13 +452,14 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
   lb->n_waves = nblk * (rzk::kBlock / 64);
   RZ_TRY(ensure(ctx, ctx->wave_lines, (size_t)std::max<uint32_t>(lb->n_waves, 1) * sizeof(uint2)));
   lb->wave_lines = (uint2*)ctx->wave_lines.p;
+  const uint32_t fsub = rzk::kBlock / rzk::kFlatBlock;
   if (nblk) {
     if (pi) {
-      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<true>, dim3(nblk), dim3(rzk::kBlock), d->view,
-                x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<true>, dim3(nblk * fsub),
+                dim3(rzk::kFlatBlock), d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     } else {
-      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<false>, dim3(nblk), dim3(rzk::kBlock),
-                d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<false>, dim3(nblk * fsub),
+                dim3(rzk::kFlatBlock), d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     }
   }
   return check_launch(ctx, "flatten");
@@ -479,8 +480,8 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   // one count-pass wave per flatten wave; the record list holds two entries
   // per line slot (records are usually as many as lines or fewer), the
   // overflow list takes the rest
-  const uint32_t wpb = rzk::kBlock / 64;
-  const uint32_t nw = lb.n_waves;
+  const uint32_t wpb = rzk::kBinBlock / 64;
+  const uint32_t nw = lb.n_waves;  // a multiple of kBlock / 64
   if (ctx->part_cap == 0) ctx->part_cap = 1u << 16;
   const size_t no = (size_t)rzk::kParts * ctx->part_cap;
   const size_t nreg = 2 * (size_t)ctx->line_cap;
@@ -502,7 +503,7 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   b->part_cap = ctx->part_cap;
   b->fill = fill;
   if (nw == 0) return RZ_OK;
-  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb), dim3(rzk::kBlock),
+  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb), dim3(rzk::kBinBlock),
             (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi, *b);
   return RZ_OK;
 }

@@ -23,7 +23,13 @@ namespace rzg {
 static constexpr double kEpsilon = 2.220446049250313e-16;  // f64::EPSILON
 static constexpr int kTile = 16;                            // tile edge in pixels
 static constexpr int kFixShift = 20;                        // LDS fixed point: 2^-20
-static constexpr int kMaxDepth = 62;                        // subdivision depth limit
+static constexpr int kMaxDepth = 62;
+#ifndef RZ_STACK_Q
+#define RZ_STACK_Q 4  // register stack entries of the quad traversal
+#endif
+#ifndef RZ_STACK_C
+#define RZ_STACK_C 4  // register stack entries of the cubic traversal
+#endif                        // subdivision depth limit
 
 enum SegKind { KIND_LINE = 0, KIND_QUAD = 1, KIND_CUBIC = 2 };
 enum EdgeMode { EDGE_MASK = 0, EDGE_ITER = 1, EDGE_NONE = 2 };
@@ -115,17 +121,22 @@ RZ_HD void child(const Node<2>& n, int right, Node<2>* c) { *c = n; }
 // left child is visited first.
 //
 // The reference keeps an explicit stack. Here the pending right siblings live
-// in a 4-entry register shift stack; when that underflows the next node is
-// recomputed from the root along its path bits (depth splits from the root are
-// the same operations the recursive split performs, so the recomputed node is
-// bit-identical). This keeps the traversal in registers with no scratch
-// memory; recomputation only happens above the deepest four levels.
+// in a register shift stack of kStack(NP) entries; when that underflows the
+// next node is recomputed from the root along its path bits (depth splits
+// from the root are the same operations the recursive split performs, so the
+// recomputed node is bit-identical). This keeps the traversal in registers
+// with no scratch memory; recomputation only happens above the deepest
+// kStack(NP) levels.
 // leaf(start, end, depth, bits): the leaf is the root's subtree at `depth`
 // reached along `bits` (most significant first, 1 = right child), i.e. the
 // curve on t in [bits, bits + 1] / 2^depth of the root.
+template <int NP>
+constexpr int kStack() { return RZ_STACK_Q * (NP == 3) + RZ_STACK_C * (NP == 4) + 4 * (NP == 2); }
+
 template <int NP, typename Leaf>
 RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
-  Node<NP> cur = root, s0, s1, s2, s3;
+  constexpr int S = kStack<NP>();
+  Node<NP> cur = root, st[S];
   int nst = 0, depth = 0;
   uint64_t bits = 0;
   while (true) {
@@ -139,10 +150,9 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       if (depth == 0) return ST_OK;
       bits |= 1;
       if (nst > 0) {
-        cur = s0;
-        s0 = s1;
-        s1 = s2;
-        s2 = s3;
+        cur = st[0];
+#pragma unroll
+        for (int k = 0; k + 1 < S; ++k) st[k] = st[k + 1];
         --nst;
       } else {
         Node<NP> n = root;
@@ -158,11 +168,10 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       Node<NP> l, r;
       child(cur, 0, &l);
       child(cur, 1, &r);
-      s3 = s2;
-      s2 = s1;
-      s1 = s0;
-      s0 = r;
-      nst = nst < 4 ? nst + 1 : 4;
+#pragma unroll
+      for (int k = S - 1; k > 0; --k) st[k] = st[k - 1];
+      st[0] = r;
+      nst = nst < S ? nst + 1 : S;
       cur = l;
       ++depth;
       bits <<= 1;

@@ -322,7 +322,8 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
 // Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
 // cubic virtual items. Every lane of a wave reaches the wave operations.
 #define RZ_FLAT_DISPATCH(ITEM, ...)                                                        \
-  const uint32_t blk = blockIdx.x, t = threadIdx.x;                                        \
+  const uint32_t blk = blockIdx.x / (kBlock / kFlatBlock),                                 \
+                 t = (blockIdx.x % (kBlock / kFlatBlock)) * kFlatBlock + threadIdx.x;      \
   if (blk < bl) {                                                                          \
     const uint32_t slot = p.order[blk * kBlock + t];                                       \
     ITEM<2, MULTI>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);         \
@@ -338,15 +339,20 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
                    __VA_ARGS__);                                                           \
   }
 
+// Workgroup size of the flatten: the slot blocks above are kBlock threads,
+// each run as kBlock / kFlatBlock workgroups. One wave per workgroup: a wave
+// with long subtrees then holds only its own LDS and wave slot.
+constexpr uint32_t kFlatBlock = 64;
+
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_flat_lines(DPath p, Xform x, double fl, uint32_t bl,
-                                                        uint32_t bq, int dq, int dc,
-                                                        const PathInfo* pi,
-                                                        const uint32_t* slot_path, LineBufs lb) {
-  __shared__ double2 keep_all[kBlock * (kLineKeep + 1)];
+__global__ __launch_bounds__(kFlatBlock) void k_flat_lines(DPath p, Xform x, double fl, uint32_t bl,
+                                                            uint32_t bq, int dq, int dc,
+                                                            const PathInfo* pi,
+                                                            const uint32_t* slot_path, LineBufs lb) {
+  __shared__ double2 keep_all[kFlatBlock * (kLineKeep + 1)];
   if (lb.st->err || lb.st->overflow) return;  // e.g. a batched scene whose slices did not fit
   double2* keep = keep_all + threadIdx.x * (kLineKeep + 1);
-  const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t wave = blockIdx.x * (kFlatBlock / 64) + (threadIdx.x >> 6);
   const uint32_t part = wave % kLineParts;
   RZ_FLAT_DISPATCH(flat_item, lb, keep, part, wave)
 }
@@ -514,19 +520,23 @@ __device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
 // right clip only add to the right-edge column; two such lines on
 // neighbouring lanes that continue each other drop their shared endpoint (its
 // two terms cancel exactly).
+// Workgroup size of the count pass. Its waves are independent and their
+// work varies with their lines, so one wave per workgroup.
+constexpr uint32_t kBinBlock = 64;
+
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_bin_lines(const double4* lines,
-                                                       const uint32_t* line_path, rzg::BinGrid g0,
-                                                       const PathInfo* pi, BinBufs b) {
-  __shared__ LineStage lstage[kBlock];
-  __shared__ BandStage bstage[kBlock];
-  __shared__ uint32_t mkey[kBlock / 64][kMap];
-  __shared__ uint32_t mcnt[kBlock / 64][kMap];
-  __shared__ int mark_all[kBlock / 64][64];
+__global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
+                                                          const uint32_t* line_path, rzg::BinGrid g0,
+                                                          const PathInfo* pi, BinBufs b) {
+  __shared__ LineStage lstage[kBinBlock];
+  __shared__ BandStage bstage[kBinBlock];
+  __shared__ uint32_t mkey[kBinBlock / 64][kMap];
+  __shared__ uint32_t mcnt[kBinBlock / 64][kMap];
+  __shared__ int mark_all[kBinBlock / 64][64];
   if (b.st->err || b.st->overflow) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint32_t wave = blockIdx.x * (kBinBlock / 64) + (threadIdx.x >> 6);
   if (wave >= b.n_waves) return;
   const uint2 wl_range = b.wave_lines[wave];
   if (wl_range.y == 0) return;  // the scatter skips the same regions
