@@ -219,31 +219,36 @@ rz_status check_launch(rz_ctx* ctx, const char* what) {
   } while (0)
 
 // Exclusive scan of n u32 values: out[0..n] (out[n] = total). Optionally
-// copies the offsets to `cursor` and the total to `total`.
+// copies the offsets to `cursor` and the total to `total`; the short array
+// `x` and the rows of `r` are done by extra blocks of the second kernel.
 rz_status scan_u32(rz_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uint32_t* cursor,
-                   uint32_t* total, const rzk::SmallScan& x = rzk::SmallScan{}) {
+                   uint32_t* total, const rzk::SmallScan& x = rzk::SmallScan{},
+                   const rzk::RowPrefix& r = rzk::RowPrefix{}) {
   size_t nb = std::max<size_t>(1, (n + rzk::kScanTile - 1) / rzk::kScanTile);
   RZ_TRY(ensure(ctx, ctx->sums, nb * sizeof(uint32_t)));
   uint32_t* sums = (uint32_t*)ctx->sums.p;
   RZ_LAUNCH(ctx, "k_scan_tiles", rzk::k_scan_tiles, dim3((uint32_t)nb), dim3(rzk::kBlock), in,
                      out, n, sums);
-  RZ_LAUNCH(ctx, "k_scan_sums", rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), sums, nb, out, n,
-                     total, x);
-  RZ_LAUNCH(ctx, "k_scan_add", rzk::k_scan_add, dim3((uint32_t)nb), dim3(rzk::kBlock), out, n,
-                     (const uint32_t*)sums, cursor);
+  const bool scanned = nb > rzk::kScanDirect;
+  if (scanned)
+    RZ_LAUNCH(ctx, "k_scan_sums", rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), sums, nb, out, n,
+                       total);
+  const uint32_t extra = (x.n ? 1u : 0u) + r.rows + (r.n_tail > 0 ? 1u : 0u);
+  RZ_LAUNCH(ctx, "k_scan_add", rzk::k_scan_add, dim3((uint32_t)nb + extra), dim3(rzk::kBlock), out,
+                     n, (const uint32_t*)sums, (uint32_t)nb, scanned, cursor, total, x, r);
   return check_launch(ctx, "scan");
 }
 
 // The tile scan and the band scan: a short band array rides along with the
-// tile scan's second pass.
+// tile scan's second kernel, as do the rows of `r`.
 rz_status scan_tiles_bands(rz_ctx* ctx, const uint32_t* tiles_in, uint32_t* tiles_out, size_t nt,
                            uint32_t* n_records, uint32_t* bands, uint32_t* bands_out, size_t nb,
-                           uint32_t* n_events) {
+                           uint32_t* n_events, const rzk::RowPrefix& r = rzk::RowPrefix{}) {
   if (nb <= 4096) {
     const rzk::SmallScan x{bands, bands_out, bands, n_events, (uint32_t)nb};
-    return scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records, x);
+    return scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records, x, r);
   }
-  RZ_TRY(scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records));
+  RZ_TRY(scan_u32(ctx, tiles_in, tiles_out, nt, nullptr, n_records, rzk::SmallScan{}, r));
   return scan_u32(ctx, bands, bands_out, nb, bands, n_events);
 }
 
@@ -347,6 +352,9 @@ static rz_status path_upload_impl(rz_ctx* ctx, const rz_path* p, rz_dpath** out)
     slot_kind[s] = RZ_LINE;
     ++s;
   }
+  std::vector<uint32_t> gap_slots;
+  for (uint32_t t = 0; t < nslots; ++t)
+    if (slot_flags[t] & 1) gap_slots.push_back(t);
   // processing order: kind-grouped, each group padded to whole blocks
   std::vector<uint32_t> order;
   uint32_t nb[3] = {0, 0, 0};
@@ -366,12 +374,14 @@ static rz_status path_upload_impl(rz_ctx* ctx, const rz_path* p, rz_dpath** out)
       (st = upload_array(ctx, d, slot_item, &v.slot_item)) != RZ_OK ||
       (st = upload_array(ctx, d, slot_flags, &v.slot_flags)) != RZ_OK ||
       (st = upload_array(ctx, d, close_pts, &v.close_pts)) != RZ_OK ||
-      (st = upload_array(ctx, d, order, &v.order)) != RZ_OK) {
+      (st = upload_array(ctx, d, order, &v.order)) != RZ_OK ||
+      (st = upload_array(ctx, d, gap_slots, &v.gap_slots)) != RZ_OK) {
     dpath_release(d);
     delete d;
     return st;
   }
   v.n_slots = nslots;
+  v.n_gaps = (uint32_t)gap_slots.size();
   v.blocks_line = nb[0];
   v.blocks_quad = nb[1];
   v.blocks_cubic = nb[2];
@@ -476,7 +486,8 @@ uint32_t line_cap_for(size_t lines) {
 // (kParts + 1 words) must be zero when the count pass starts.
 template <bool MULTI>
 rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
-                            const rzk::PathInfo* pi, uint32_t* fill, rzk::BinBufs* b) {
+                            const rzk::PathInfo* pi, const rzk::GapArgs& ga, uint32_t* fill,
+                            rzk::BinBufs* b) {
   // one count-pass wave per flatten wave; the record list holds two entries
   // per line slot (records are usually as many as lines or fewer), the
   // overflow list takes the rest
@@ -503,8 +514,11 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   b->part_cap = ctx->part_cap;
   b->fill = fill;
   if (nw == 0) return RZ_OK;
-  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb), dim3(rzk::kBinBlock),
-            (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi, *b);
+  // then the gap slots, 64 per wave
+  const uint32_t gap_blocks = (ga.p.n_gaps + rzk::kBinBlock - 1) / rzk::kBinBlock;
+  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb + gap_blocks),
+            dim3(rzk::kBinBlock), (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi,
+            *b, ga);
   return RZ_OK;
 }
 
@@ -512,8 +526,9 @@ rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* 
   if (b.n_waves == 0) return RZ_OK;
   // many waves, each over regions far apart (neighbouring regions hold
   // records of the same tiles)
-  const uint32_t grid = std::min<uint32_t>(b.n_waves / (rzk::kBlock / 64), 8192);
-  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kBlock), b);
+  const uint32_t wpb = rzk::kScatterBlock / 64;
+  const uint32_t grid = (std::min(b.n_waves, rzk::kScatterWaves) + wpb - 1) / wpb;
+  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kScatterBlock), b);
   RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b, line_fill);
   return RZ_OK;
 }
@@ -576,18 +591,15 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.ev_cap = ctx->ev_cap;
   b.redge = P.redge;
   b.st = P.st;
-  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr, bin_fill, &b));
-  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, false>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
+  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr, rzk::GapArgs{d->view, x, nullptr},
+                                   bin_fill, &b));
   RZ_TRY(check_launch(ctx, "bin pass 0"));
+  // with the backdrop differences along each tile row and the right-edge
+  // row differences
+  const rzk::RowPrefix rp{P.bd, P.grid.g.tiles_x, (uint32_t)ty, P.redge, (int)(H + 2)};
   RZ_TRY(scan_tiles_bands(ctx, P.tile_count, P.tile_off, tiles, &P.st->n_records, P.band_ev,
-                          P.ev_off, ty, &P.st->n_events));
-  RZ_LAUNCH(ctx, "k_bd_prefix", rzk::k_bd_prefix, dim3((uint32_t)ty + 1), dim3(rzk::kBlock), P.bd,
-            P.grid.g.tiles_x, P.redge, (int)(H + 2));
+                          P.ev_off, ty, &P.st->n_events, rp));
   RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
-  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, false>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, P.grid.g, b, (const rzk::PathInfo*)nullptr, (const uint32_t*)nullptr);
   return check_launch(ctx, "bin pass 1");
 }
 
@@ -1055,18 +1067,14 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, bin_fill, &b));
-  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, rzk::GapArgs{d->view, x, s->slot_path},
+                                  bin_fill, &b));
   RZ_TRY(check_launch(ctx, "scene pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
   RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
-  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
             (const int4*)ctx->sc_rects.p, n, (uint32_t*)ctx->sc_rows.p, (uint32_t*)ctx->sc_rowcnt.p,
             (const rzk::Status*)st);
@@ -1211,18 +1219,14 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  const uint32_t gs = std::max<uint32_t>(1, blocks_for(d->n_slots));
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, bin_fill, &b));
-  RZ_LAUNCH(ctx, "k_gap_events<0>", (rzk::k_gap_events<0, true>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, rzk::GapArgs{d->view, x, s->slot_path},
+                                  bin_fill, &b));
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
   RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
             bd, redge, (const rzk::Status*)st);
   RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
-  RZ_LAUNCH(ctx, "k_gap_events<1>", (rzk::k_gap_events<1, true>), dim3(gs), dim3(rzk::kBlock),
-            d->view, x, g0, b, (const rzk::PathInfo*)pi, s->slot_path);
   rzk::AtlasFineArgs a;
   RZ_TRY(ensure(ctx, ctx->atlas_tiles, (size_t)ct * sizeof(rzk::AtlasTile)));
   RZ_LAUNCH(ctx, "k_atlas_tiles", rzk::k_atlas_tiles, dim3(blocks_for(ct)), dim3(rzk::kBlock),

@@ -27,6 +27,8 @@ struct DPath {
   const uint8_t* slot_flags;  // bit0: start is a gap, bit1: end is a gap
   const double* close_pts;    // per subpath: closing line (x0,y0,x1,y1) untransformed
   const uint32_t* order;      // processing order: slot indices grouped by kind, padded with ~0u
+  const uint32_t* gap_slots;  // slots whose start is a gap (slot_flags bit0)
+  uint32_t n_gaps;
   uint32_t n_slots;
   uint32_t blocks_line, blocks_quad, blocks_cubic;  // block ranges per kind in `order`
 };
@@ -432,6 +434,34 @@ __device__ inline int4 event_bits(const rzg::Event& ev) {
   return make_int4(ev.band, ev.col, __float_as_int(ev.yrel), __float_as_int(ev.sign));
 }
 
+// Count (per band) and list up to two events per lane: one atomic per wave
+// takes the list entries. Every lane of the wave takes part.
+__device__ inline void list_events(const BinBufs& b, int nev, const int4& ev0, uint32_t evb0,
+                                   const int4& ev1, uint32_t evb1, int lane) {
+  const uint64_t below = (1ull << lane) - 1ull;
+  if (nev >= 1) atomicAdd(&b.band_ev[evb0], 1u);
+  if (nev == 2) atomicAdd(&b.band_ev[evb1], 1u);
+  const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
+  if (!e1) return;
+  const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
+  uint32_t ebase = 0;
+  if (lane == 0) ebase = atomicAdd(&b.fill[kParts], etot);
+  ebase = (uint32_t)__shfl((int)ebase, 0, 64);
+  const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
+  if (ebase + etot > b.ev_cap) {
+    if (lane == 0) atomicOr(&b.st->overflow, 2u);
+    return;
+  }
+  if (nev >= 1) {
+    b.ev_list[k] = ev0;
+    b.ev_band[k] = evb0;
+  }
+  if (nev == 2) {
+    b.ev_list[k + 1] = ev1;
+    b.ev_band[k + 1] = evb1;
+  }
+}
+
 // Exclusive prefix sum of v over the wave (every lane takes part); the wave
 // total goes to *total.
 __device__ inline uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
@@ -508,6 +538,68 @@ __device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
   return g;
 }
 
+// A gap between the previous segment's end `a` and a segment's start `b`
+// (arc endpoints are recomputed by sin/cos and miss their neighbours by
+// ~1e-13): the chain has a free end at a (+V) and a free start at b (-V).
+// Both events are made here; a pair that lands on the same band, column and
+// f32 row offset contributes exactly zero and is dropped. Ends created by the
+// right clip are reported by the clip itself. The events are counted and
+// listed like the lines' (k_bin_events places them). One lane per gap slot
+// (i = ~0u: none); every lane of the wave takes part.
+struct GapArgs {
+  DPath p;
+  Xform x;
+  const uint32_t* slot_path;  // batched scenes and atlases: path of each slot
+};
+
+template <bool MULTI>
+__device__ void gap_events(const GapArgs& ga, const rzg::BinGrid& g0, const BinBufs& b,
+                           const PathInfo* pi, uint32_t i, int lane) {
+  int nev = 0;
+  int4 ev0{}, ev1{};
+  uint32_t evb0 = 0, evb1 = 0;
+  if (i != ~0u) {
+    const uint32_t slot = ga.p.gap_slots[i];
+    rzg::BinGrid g = g0;
+    const double* m = ga.x.m;
+    uint32_t band_base = 0;
+    bool live = true;
+    if (MULTI) {
+      const PathInfo& P = pi[ga.slot_path[slot]];
+      live = P.live != 0;
+      g = P.g;
+      m = P.m;
+      band_base = P.band_base;
+    }
+    if (live) {
+      const uint32_t k = ga.p.slot_item[slot];  // a segment (closing lines never start a gap)
+      const double* qa = ga.p.pts + 2 * (size_t)(ga.p.pt_off[k] - 1);  // previous segment's last point
+      const double* qb = ga.p.pts + 2 * (size_t)ga.p.pt_off[k];
+      const Pt a = rzg::tr_apply(m, rzg::mkpt(qa[0], qa[1]));
+      const Pt bb = rzg::tr_apply(m, rzg::mkpt(qb[0], qb[1]));
+      const bool clip = g.mode != rzg::EDGE_NONE;
+      const bool ea = !(clip && a.x > g.wc), eb = !(clip && bb.x > g.wc);
+      rzg::Event va, vb;
+      const Pt ap = rzg::mkpt(fmax(a.x, 0.0), a.y), bp = rzg::mkpt(fmax(bb.x, 0.0), bb.y);
+      const bool ha = ea && rzg::make_event(ap, 1.0f, g, &va);
+      const bool hb = eb && rzg::make_event(bp, -1.0f, g, &vb);
+      if (!(ha && hb && va.band == vb.band && va.col == vb.col && va.yrel == vb.yrel)) {
+        if (ha) {
+          ev0 = event_bits(va);
+          evb0 = band_base + (uint32_t)va.band;
+          nev = 1;
+        }
+        if (hb) {
+          (nev ? ev1 : ev0) = event_bits(vb);
+          (nev ? evb1 : evb0) = band_base + (uint32_t)vb.band;
+          ++nev;
+        }
+      }
+    }
+  }
+  list_events(b, nev, ev0, evb0, ev1, evb1, lane);
+}
+
 // Count pass. A wave takes the lines of the flatten wave of the same index
 // (consecutive along the path), 64 at a time, one per lane: right clip and
 // left split, right-edge column, events, and the band range of each piece.
@@ -527,7 +619,7 @@ constexpr uint32_t kBinBlock = 64;
 template <bool MULTI>
 __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
                                                           const uint32_t* line_path, rzg::BinGrid g0,
-                                                          const PathInfo* pi, BinBufs b) {
+                                                          const PathInfo* pi, BinBufs b, GapArgs ga) {
   __shared__ LineStage lstage[kBinBlock];
   __shared__ BandStage bstage[kBinBlock];
   __shared__ uint32_t mkey[kBinBlock / 64][kMap];
@@ -537,7 +629,12 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
   const uint32_t wave = blockIdx.x * (kBinBlock / 64) + (threadIdx.x >> 6);
-  if (wave >= b.n_waves) return;
+  if (wave >= b.n_waves) {
+    // the waves past the lines' take the gap slots, 64 each
+    const uint32_t i = (wave - b.n_waves) * 64 + (uint32_t)lane;
+    gap_events<MULTI>(ga, g0, b, pi, i < ga.p.n_gaps ? i : ~0u, lane);
+    return;
+  }
   const uint2 wl_range = b.wave_lines[wave];
   if (wl_range.y == 0) return;  // the scatter skips the same regions
   LineStage* wl = lstage + (threadIdx.x & ~63u);
@@ -682,29 +779,7 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
       S.phy = g.phy;
       S.tb = tb;
     }
-    // events: count, then list them (at most two per line)
-    if (nev >= 1) atomicAdd(&b.band_ev[evb0], 1u);
-    if (nev == 2) atomicAdd(&b.band_ev[evb1], 1u);
-    const uint64_t e1 = __ballot(nev >= 1), e2 = __ballot(nev == 2);
-    if (e1) {
-      const uint32_t etot = (uint32_t)(__popcll(e1) + __popcll(e2));
-      uint32_t ebase = 0;
-      if (lane == 0) ebase = atomicAdd(&b.fill[kParts], etot);
-      ebase = (uint32_t)__shfl((int)ebase, 0, 64);
-      const uint32_t k = ebase + (uint32_t)(__popcll(e1 & below) + __popcll(e2 & below));
-      if (ebase + etot > b.ev_cap) {
-        if (lane == 0) atomicOr(&b.st->overflow, 2u);
-      } else {
-        if (nev >= 1) {
-          b.ev_list[k] = ev0;
-          b.ev_band[k] = evb0;
-        }
-        if (nev == 2) {
-          b.ev_list[k + 1] = ev1;
-          b.ev_band[k + 1] = evb1;
-        }
-      }
-    }
+    list_events(b, nev, ev0, evb0, ev1, evb1, lane);
     // band tasks, 64 per round
     uint32_t n_band;
     const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
@@ -791,9 +866,12 @@ __device__ inline void place_record(const BinBufs& b, uint32_t tile, uint32_t ra
 // Place the listed records: one wave per count-pass region (grid-stride over
 // the regions of the waves that had lines), then the overflow parts one entry
 // per thread.
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
+constexpr uint32_t kScatterBlock = 256;
+constexpr uint32_t kScatterWaves = 32768;  // waves of the scatter grid at most
+
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b) {
   if (b.st->err || b.st->overflow) return;
-  const uint32_t wpb = kBlock / 64;
+  const uint32_t wpb = kScatterBlock / 64;
   for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < b.n_waves; w += gridDim.x * wpb) {
     const uint2 r = b.wave_lines[w];
     if (r.y == 0) continue;
@@ -805,11 +883,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BinBufs b) {
       place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
     }
   }
-  const uint32_t stride = gridDim.x * kBlock;
+  const uint32_t stride = gridDim.x * kScatterBlock;
   const size_t obase = b.list_regions;
   for (uint32_t p = 0; p < kParts; ++p) {
     const uint32_t np = min(b.fill[p], b.part_cap);
-    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < np; k += stride) {
+    for (uint32_t k = blockIdx.x * kScatterBlock + threadIdx.x; k < np; k += stride) {
       const size_t o = (size_t)p * b.part_cap + k;
       const uint2 mt = b.list_meta[obase + o];
       if (mt.x != ~0u) place_record(b, mt.x, mt.y, b.ovf_wave[o], b.list_rec[obase + o]);
@@ -832,45 +910,6 @@ __global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b, const uint32_t
     if (slot < b.ev_cap) b.events[slot] = b.ev_list[k];
     else atomicOr(&b.st->overflow, 2u);
   }
-}
-
-// A gap between the previous segment's end `a` and this segment's start `b`
-// (arc endpoints are recomputed by sin/cos and miss their neighbours by
-// ~1e-13): the chain has a free end at a (+V) and a free start at b (-V).
-// Both events are made here; a pair that lands on the same band, column and
-// f32 row offset contributes exactly zero and is dropped. Ends created by the
-// right clip are reported by the clip itself. One thread per slot.
-template <int PASS, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_gap_events(DPath p, Xform x, rzg::BinGrid g0, BinBufs b,
-                                                        const PathInfo* pi,
-                                                        const uint32_t* slot_path) {
-  if (b.st->err || b.st->overflow) return;
-  const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
-  if (slot >= p.n_slots || !(p.slot_flags[slot] & 1)) return;
-  rzg::BinGrid g = g0;
-  const double* m = x.m;
-  uint32_t band_base = 0;
-  if (MULTI) {
-    const PathInfo& P = pi[slot_path[slot]];
-    if (!P.live) return;
-    g = P.g;
-    m = P.m;
-    band_base = P.band_base;
-  }
-  const uint32_t k = p.slot_item[slot];  // a segment (closing lines never start a gap)
-  const double* qa = p.pts + 2 * (size_t)(p.pt_off[k] - 1);  // previous segment's last point
-  const double* qb = p.pts + 2 * (size_t)p.pt_off[k];
-  const Pt a = rzg::tr_apply(m, rzg::mkpt(qa[0], qa[1]));
-  const Pt bb = rzg::tr_apply(m, rzg::mkpt(qb[0], qb[1]));
-  const bool clip = g.mode != rzg::EDGE_NONE;
-  const bool ea = !(clip && a.x > g.wc), eb = !(clip && bb.x > g.wc);
-  rzg::Event va, vb;
-  const Pt ap = rzg::mkpt(fmax(a.x, 0.0), a.y), bp = rzg::mkpt(fmax(bb.x, 0.0), bb.y);
-  const bool ha = ea && rzg::make_event(ap, 1.0f, g, &va);
-  const bool hb = eb && rzg::make_event(bp, -1.0f, g, &vb);
-  if (ha && hb && va.band == vb.band && va.col == vb.col && va.yrel == vb.yrel) return;
-  if (ha) put_event<PASS>(b, g, band_base, ap, 1.0f);
-  if (hb) put_event<PASS>(b, g, band_base, bp, -1.0f);
 }
 
 // Fold one run's status block into a sticky one (first error, overflow bits,
@@ -931,6 +970,16 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
   return base + x - v;
 }
 
+// A whole exclusive scan done by one block (a short array riding along with
+// another scan's second pass).
+struct SmallScan {
+  const uint32_t* in;
+  uint32_t* out;     // [n + 1]
+  uint32_t* cursor;  // copy of the offsets, or null
+  uint32_t* total;   // or null
+  uint32_t n;        // 0: none
+};
+
 // Pass 1: per-tile exclusive scan of `in` into `out`, block totals into sums.
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint32_t* out, size_t n,
                                                         uint32_t* sums) {
@@ -951,20 +1000,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* in, uint3
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// A whole exclusive scan done by one block (a short array riding along with
-// another scan's second pass).
-struct SmallScan {
-  const uint32_t* in;
-  uint32_t* out;     // [n + 1]
-  uint32_t* cursor;  // copy of the offsets, or null
-  uint32_t* total;   // or null
-  uint32_t n;        // 0: none
-};
+// Many block totals (more than kScanDirect): one block scans them in place
+// and writes the grand total to out[n] and *total.
+constexpr uint32_t kScanDirect = 4096;
 
-// Pass 2: one block scans the block totals (any count) and writes the grand
-// total to out[n] and *total; then, if given, the short array `x`.
 __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* sums, size_t nb, uint32_t* out,
-                                                       size_t n, uint32_t* total, SmallScan x) {
+                                                       size_t n, uint32_t* total) {
   uint32_t carry = 0;
   for (size_t start = 0; start < nb; start += kBlock) {
     size_t i = start + threadIdx.x;
@@ -979,30 +1020,89 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(uint32_t* sums, size_t nb,
     out[n] = carry;
     if (total) *total = carry;
   }
-  carry = 0;
-  for (uint32_t start = 0; start < x.n; start += kBlock) {
-    const uint32_t i = start + threadIdx.x;
-    const uint32_t v = i < x.n ? x.in[i] : 0u;
+}
+
+// Sum of v over the block (every thread takes part).
+__device__ inline uint32_t block_sum(uint32_t v) {
+  uint32_t tot;
+  (void)block_exclusive_scan(v, &tot);
+  return tot;
+}
+
+// Inclusive prefix along rows of a signed array (two's-complement sums, so
+// the unsigned scan is exact): `rows` rows of `len` values from `p`, then, if
+// n_tail > 0, one more row of n_tail values from `tail`.
+struct RowPrefix {
+  int32_t* p;
+  int len;
+  uint32_t rows;
+  int32_t* tail;
+  int n_tail;
+};
+
+__device__ void row_prefix_block(const RowPrefix& r, uint32_t row) {
+  const bool tail = row == r.rows;
+  uint32_t* v = (uint32_t*)(tail ? r.tail : r.p + (size_t)row * r.len);
+  const int n = tail ? r.n_tail : r.len;
+  uint32_t carry = 0;
+  for (int start = 0; start < n; start += kBlock) {
+    const int i = start + threadIdx.x;
+    const uint32_t x = i < n ? v[i] : 0u;
     uint32_t tot;
-    const uint32_t pre = block_exclusive_scan(v, &tot);
-    if (i < x.n) {
-      x.out[i] = carry + pre;
-      if (x.cursor) x.cursor[i] = carry + pre;
-    }
+    const uint32_t pre = block_exclusive_scan(x, &tot);
+    if (i < n) v[i] = carry + pre + x;
     carry += tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && x.n) {
-    x.out[x.n] = carry;
-    if (x.total) *x.total = carry;
   }
 }
 
-// Pass 3: add block offsets; optionally copy to a cursor array.
+// Pass 2: add block offsets; optionally copy to a cursor array. Unless the
+// block totals were scanned in place (`scanned`), each block sums those
+// before its own. The last block writes the grand total to out[n] and
+// *total. The block after the scan's scans the short array `x` (if any), the
+// blocks after that take the rows of `r`.
 __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, size_t n, const uint32_t* sums,
-                                                      uint32_t* cursor) {
+                                                      uint32_t nb, bool scanned, uint32_t* cursor,
+                                                      uint32_t* total, SmallScan x, RowPrefix r) {
+  if (blockIdx.x >= nb) {
+    uint32_t k = blockIdx.x - nb;
+    if (x.n && k == 0) {
+      uint32_t carry = 0;
+      for (uint32_t start = 0; start < x.n; start += kBlock) {
+        const uint32_t i = start + threadIdx.x;
+        const uint32_t v = i < x.n ? x.in[i] : 0u;
+        uint32_t tot;
+        const uint32_t pre = block_exclusive_scan(v, &tot);
+        if (i < x.n) {
+          x.out[i] = carry + pre;
+          if (x.cursor) x.cursor[i] = carry + pre;
+        }
+        carry += tot;
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        x.out[x.n] = carry;
+        if (x.total) *x.total = carry;
+      }
+      return;
+    }
+    if (x.n) --k;
+    row_prefix_block(r, k);
+    return;
+  }
+  uint32_t off;
+  if (scanned) {
+    off = sums[blockIdx.x];
+  } else {
+    uint32_t part = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += kBlock) part += sums[j];
+    off = block_sum(part);
+  }
+  if (!scanned && blockIdx.x == nb - 1 && threadIdx.x == 0) {
+    const uint32_t t = off + sums[blockIdx.x];
+    out[n] = t;
+    if (total) *total = t;
+  }
   size_t base = (size_t)blockIdx.x * kScanTile;
-  uint32_t off = sums[blockIdx.x];
   for (int k = threadIdx.x; k < kScanTile; k += kBlock) {
     size_t i = base + k;
     if (i < n) {
@@ -1010,25 +1110,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* out, size_t n, co
       out[i] = v;
       if (cursor) cursor[i] = v;
     }
-  }
-}
-
-// Inclusive prefix of the band backdrop differences along each tile row, one
-// block per band, and of the right-edge row differences in the last block
-// (two's-complement sums, so the unsigned scan is exact).
-__global__ __launch_bounds__(kBlock) void k_bd_prefix(int32_t* bd, int tiles_x, int32_t* redge,
-                                                       int n_redge) {
-  const bool edge = blockIdx.x == gridDim.x - 1;
-  uint32_t* row = (uint32_t*)(edge ? redge : bd + (size_t)blockIdx.x * tiles_x);
-  const int n = edge ? n_redge : tiles_x;
-  uint32_t carry = 0;
-  for (int start = 0; start < n; start += kBlock) {
-    const int i = start + threadIdx.x;
-    const uint32_t v = i < n ? row[i] : 0u;
-    uint32_t tot;
-    const uint32_t pre = block_exclusive_scan(v, &tot);
-    if (i < n) row[i] = carry + pre + v;
-    carry += tot;
   }
 }
 
