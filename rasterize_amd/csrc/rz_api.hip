@@ -528,8 +528,9 @@ rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* 
   // records of the same tiles)
   const uint32_t wpb = rzk::kScatterBlock / 64;
   const uint32_t grid = (std::min(b.n_waves, rzk::kScatterWaves) + wpb - 1) / wpb;
-  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid), dim3(rzk::kScatterBlock), b);
-  RZ_LAUNCH(ctx, "k_bin_events", rzk::k_bin_events, dim3(64), dim3(rzk::kBlock), b, line_fill);
+  // then 64 blocks for the events
+  RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid + 64), dim3(rzk::kScatterBlock), b,
+            line_fill, grid);
   return RZ_OK;
 }
 

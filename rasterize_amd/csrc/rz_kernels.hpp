@@ -370,8 +370,8 @@ __global__ __launch_bounds__(kFlatBlock) void k_flat_lines(DPath p, Xform x, dou
 //                 end, which returns that offset;
 //   (scans of the tile and band counts)
 //   k_bin_scatter writes each listed record to tile offset + wave offset +
-//                 rank (no atomics);
-//   k_bin_events  moves the listed events into their bands' slots.
+//                 rank (no atomics), and moves the listed events into their
+//                 bands' slots.
 // Each wave lists its records in a region of its own; a wave that fills it
 // continues in an overflow list of kParts parts, each with its own bump
 // counter, taking kChunk entries at a time and padding what it leaves unused
@@ -544,7 +544,7 @@ __device__ inline rzg::BinGrid stage_grid(const LineStage& S) {
 // Both events are made here; a pair that lands on the same band, column and
 // f32 row offset contributes exactly zero and is dropped. Ends created by the
 // right clip are reported by the clip itself. The events are counted and
-// listed like the lines' (k_bin_events places them). One lane per gap slot
+// listed like the lines' (k_bin_scatter places them). One lane per gap slot
 // (i = ~0u: none); every lane of the wave takes part.
 struct GapArgs {
   DPath p;
@@ -869,10 +869,34 @@ __device__ inline void place_record(const BinBufs& b, uint32_t tile, uint32_t ra
 constexpr uint32_t kScatterBlock = 256;
 constexpr uint32_t kScatterWaves = 32768;  // waves of the scatter grid at most
 
-__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b) {
+// Place the events the count pass listed into their bands' slots (blocks
+// from `first` on; the first of them also totals the lines, for reporting).
+__device__ void place_events(const BinBufs& b, const uint32_t* line_fill, uint32_t first) {
+  const uint32_t blk = blockIdx.x - first, nblk = gridDim.x - first;
+  if (blk == 0 && threadIdx.x < 64) {
+    uint32_t v = line_fill[threadIdx.x];
+    for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    if (threadIdx.x == 0) b.st->n_lines = v;
+  }
+  if (b.st->err || b.st->overflow) return;
+  const uint32_t n = b.fill[kParts];
+  for (uint32_t k = blk * kScatterBlock + threadIdx.x; k < n; k += nblk * kScatterBlock) {
+    const uint32_t slot = atomicAdd(&b.band_ev[b.ev_band[k]], 1u);
+    if (slot < b.ev_cap) b.events[slot] = b.ev_list[k];
+    else atomicOr(&b.st->overflow, 2u);
+  }
+}
+
+// Blocks [0, n_rec_blocks) place the listed records, the rest the events.
+__global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b, const uint32_t* line_fill,
+                                                               uint32_t n_rec_blocks) {
+  if (blockIdx.x >= n_rec_blocks) {
+    place_events(b, line_fill, n_rec_blocks);
+    return;
+  }
   if (b.st->err || b.st->overflow) return;
   const uint32_t wpb = kScatterBlock / 64;
-  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < b.n_waves; w += gridDim.x * wpb) {
+  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < b.n_waves; w += n_rec_blocks * wpb) {
     const uint2 r = b.wave_lines[w];
     if (r.y == 0) continue;
     const uint32_t used = b.wave_used[w];
@@ -883,7 +907,7 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b) {
       place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
     }
   }
-  const uint32_t stride = gridDim.x * kScatterBlock;
+  const uint32_t stride = n_rec_blocks * kScatterBlock;
   const size_t obase = b.list_regions;
   for (uint32_t p = 0; p < kParts; ++p) {
     const uint32_t np = min(b.fill[p], b.part_cap);
@@ -892,23 +916,6 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b) {
       const uint2 mt = b.list_meta[obase + o];
       if (mt.x != ~0u) place_record(b, mt.x, mt.y, b.ovf_wave[o], b.list_rec[obase + o]);
     }
-  }
-}
-
-// Place the events the count pass listed into their bands' slots; the first
-// wave also totals the lines (reporting).
-__global__ __launch_bounds__(kBlock) void k_bin_events(BinBufs b, const uint32_t* line_fill) {
-  if (blockIdx.x == 0 && threadIdx.x < 64) {
-    uint32_t v = line_fill[threadIdx.x];
-    for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
-    if (threadIdx.x == 0) b.st->n_lines = v;
-  }
-  if (b.st->err || b.st->overflow) return;
-  const uint32_t n = b.fill[kParts];
-  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
-    const uint32_t slot = atomicAdd(&b.band_ev[b.ev_band[k]], 1u);
-    if (slot < b.ev_cap) b.events[slot] = b.ev_list[k];
-    else atomicOr(&b.st->overflow, 2u);
   }
 }
 
@@ -1044,14 +1051,16 @@ __device__ void row_prefix_block(const RowPrefix& r, uint32_t row) {
   const bool tail = row == r.rows;
   uint32_t* v = (uint32_t*)(tail ? r.tail : r.p + (size_t)row * r.len);
   const int n = tail ? r.n_tail : r.len;
-  uint32_t carry = 0;
-  for (int start = 0; start < n; start += kBlock) {
-    const int i = start + threadIdx.x;
-    const uint32_t x = i < n ? v[i] : 0u;
-    uint32_t tot;
-    const uint32_t pre = block_exclusive_scan(x, &tot);
-    if (i < n) v[i] = carry + pre + x;
-    carry += tot;
+  // each thread takes `per` consecutive values: one block scan of their sums
+  const int per = (n + kBlock - 1) / kBlock;
+  const int i0 = (int)threadIdx.x * per, i1 = min(i0 + per, n);
+  uint32_t sum = 0;
+  for (int i = i0; i < i1; ++i) sum += v[i];
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(sum, &tot);
+  for (int i = i0; i < i1; ++i) {
+    run += v[i];
+    v[i] = run;
   }
 }
 
