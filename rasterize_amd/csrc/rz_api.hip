@@ -1073,8 +1073,8 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   RZ_TRY(check_launch(ctx, "scene pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
-  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
-            bd, redge, (const rzk::Status*)st);
+  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(64), (const rzk::PathInfo*)pi, bd,
+            redge, (const rzk::Status*)st);
   RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
   RZ_LAUNCH(ctx, "k_scene_rows", rzk::k_scene_rows, dim3((uint32_t)cty), dim3(64),
             (const int4*)ctx->sc_rects.p, n, (uint32_t*)ctx->sc_rows.p, (uint32_t*)ctx->sc_rowcnt.p,
@@ -1225,8 +1225,8 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
-  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(rzk::kBlock), (const rzk::PathInfo*)pi,
-            bd, redge, (const rzk::Status*)st);
+  RZ_LAUNCH(ctx, "k_scene_bd", rzk::k_scene_bd, dim3(n), dim3(64), (const rzk::PathInfo*)pi, bd,
+            redge, (const rzk::Status*)st);
   RZ_TRY(enqueue_bin_place(ctx, b, lb.fill));
   rzk::AtlasFineArgs a;
   RZ_TRY(ensure(ctx, ctx->atlas_tiles, (size_t)ct * sizeof(rzk::AtlasTile)));

@@ -179,27 +179,35 @@ __global__ __launch_bounds__(kBlock) void k_scene_offsets(uint32_t n, const uint
 }
 
 // Band-backdrop prefix along each tile row and right-edge prefix over the
-// rows, one block per path.
-__global__ __launch_bounds__(kBlock) void k_scene_bd(const PathInfo* pi, int32_t* bd,
-                                                      int32_t* redge, const Status* st) {
+// rows (inclusive, two's-complement sums), one wave per path: lane r walks
+// tile row r; for the right-edge column each lane takes a run of consecutive
+// rows, and one wave scan of the run totals gives each run's carry.
+__global__ __launch_bounds__(64) void k_scene_bd(const PathInfo* pi, int32_t* bd, int32_t* redge,
+                                                  const Status* st) {
   if (st->err || st->overflow) return;
   const PathInfo& P = pi[blockIdx.x];
   if (!P.live) return;
+  const int lane = threadIdx.x;
   const int tiles_x = P.g.tiles_x;
-  for (int band = 0; band <= P.g.tiles_y; ++band) {
-    const bool edge = band == P.g.tiles_y;
-    uint32_t* row = edge ? (uint32_t*)(redge + P.redge_base)
-                         : (uint32_t*)(bd + P.tile_base + (size_t)band * tiles_x);
-    const int n = edge ? P.g.H + 2 : tiles_x;
+  for (int band = lane; band < P.g.tiles_y; band += 64) {
+    uint32_t* row = (uint32_t*)(bd + P.tile_base + (size_t)band * tiles_x);
     uint32_t carry = 0;
-    for (int start = 0; start < n; start += kBlock) {
-      const int i = start + threadIdx.x;
-      const uint32_t v = i < n ? row[i] : 0u;
-      uint32_t tot;
-      const uint32_t pre = block_exclusive_scan(v, &tot);
-      if (i < n) row[i] = carry + pre + v;
-      carry += tot;
+#pragma unroll 8
+    for (int i = 0; i < tiles_x; ++i) {
+      carry += row[i];
+      row[i] = carry;
     }
+  }
+  uint32_t* e = (uint32_t*)(redge + P.redge_base);
+  const int n = P.g.H + 2;
+  const int per = (n + 63) / 64;
+  const int i0 = lane * per, i1 = min(i0 + per, n);
+  uint32_t sum = 0;
+  for (int i = i0; i < i1; ++i) sum += e[i];
+  uint32_t run = (uint32_t)dpp_add_scan((int)sum) - sum;  // exclusive
+  for (int i = i0; i < i1; ++i) {
+    run += e[i];
+    e[i] = run;
   }
 }
 
