@@ -19,7 +19,7 @@ HIPCC_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-parameter", "-Wno-unused-function",
+    "-Wall", "-Wshadow", "-Wno-unused-parameter", "-Wno-unused-function",
 ]
 
 

@@ -1139,23 +1139,33 @@ __device__ inline int row_scan16(int v) {
 }
 
 // Add sign * V(yc) to the per-row base, V(yc)(row) = clamp(yc - row, 0, 1),
-// stored as differences between consecutive rows (prefix-summed later).
-__device__ inline void add_v(int* rowd, float yc, float sign) {
+// stored as differences between consecutive rows (prefix-summed later). The
+// difference of row 0 goes to the lane's own sum r0 (added to rowd[0] once
+// per tile by flush_row0: an atomic on one address from many lanes would be
+// serialized lane by lane).
+__device__ inline void add_v(int* rowd, uint32_t& r0, float yc, float sign) {
   if (!(yc > 0.0f)) return;
   if (yc >= 16.0f) {
-    atomicAdd(&rowd[0], (int)(sign * kFix));
+    r0 += (uint32_t)(int)(sign * kFix);
     return;
   }
   int k = (int)yc;
   float f = yc - (float)k;
   int one = (int)(sign * kFix), ff = fix(sign * f);
   if (k == 0) {
-    atomicAdd(&rowd[0], ff);
+    r0 += (uint32_t)ff;
   } else {
-    atomicAdd(&rowd[0], one);
+    r0 += (uint32_t)one;
     atomicAdd(&rowd[k], ff - one);
   }
   atomicAdd(&rowd[k + 1], -ff);
+}
+
+// Add the lanes' row-0 sums to rowd[0] (two's-complement sums, exact in any
+// order). Every lane of the wave takes part.
+__device__ inline void flush_row0(int* rowd, uint32_t r0, int lane) {
+  const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(dpp_add_scan((int)r0), 63);
+  if (lane == 0 && t) atomicAdd(&rowd[0], (int)t);
 }
 
 __device__ inline void dep(int* acc, int row, int col, float v) {
@@ -1217,11 +1227,11 @@ __device__ inline void fine_row(int* acc, const RowStage& R, int y) {
   }
 }
 
-// Records [rb, re) of one tile into the wave's LDS accumulators. `rs` and
-// `marks` are the wave's 64-entry LDS scratch. Every lane of the wave takes
-// part.
+// Records [rb, re) of one tile into the wave's LDS accumulators (the row-0
+// differences into the lane's row0, see add_v). `rs` and `marks` are the
+// wave's 64-entry LDS scratch. Every lane of the wave takes part.
 __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re, int* acc,
-                                    int* rowd, RowStage* rs, int* marks, int lane) {
+                                    int* rowd, uint32_t& row0, RowStage* rs, int* marks, int lane) {
   for (uint32_t r0 = rb; r0 < re; r0 += 64) {
     const uint32_t i = r0 + (uint32_t)lane;
     int nrows = 0;
@@ -1233,8 +1243,8 @@ __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re
       // allowed to index outside the tile
       if (fabsf(x0) <= 16.0f && fabsf(x1) <= 16.0f && y0 >= 0.0f && y0 <= 16.0f && y1 >= 0.0f &&
           y1 <= 16.0f) {
-        if (signbit(x0)) add_v(rowd, y0, 1.0f);
-        if (signbit(x1)) add_v(rowd, y1, -1.0f);
+        if (signbit(x0)) add_v(rowd, row0, y0, 1.0f);
+        if (signbit(x1)) add_v(rowd, row0, y1, -1.0f);
         x0 = fabsf(x0);
         x1 = fabsf(x1);
         if (y0 != y1) {
@@ -1353,11 +1363,13 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
     for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
     if (lane < 18) rowd[lane] = 0;
     __syncthreads();
-    tile_records(a.recs, rb, re, acc, rowd, rs, marks, lane);
+    uint32_t r0 = 0;
+    tile_records(a.recs, rb, re, acc, rowd, r0, rs, marks, lane);
     for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
       const int4 ev = a.events[i];
-      if (ev.y < tx) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+      if (ev.y < tx) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
     }
+    flush_row0(rowd, r0, lane);
     __syncthreads();
     // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
     if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);

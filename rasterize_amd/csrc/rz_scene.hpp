@@ -322,11 +322,13 @@ __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
         for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
         if (lane < 18) rowd[lane] = 0;
         __syncthreads();
-        tile_records(a.recs, qrb, qre, acc, rowd, rs, marks, lane);
+        uint32_t r0 = 0;
+        tile_records(a.recs, qrb, qre, acc, rowd, r0, rs, marks, lane);
         for (uint32_t i = qeb + lane; i < qee; i += 64) {
           const int4 ev = a.events[i];
-          if (ev.y < qtxl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+          if (ev.y < qtxl) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
         }
+        flush_row0(rowd, r0, lane);
         __syncthreads();
         if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
         __syncthreads();
@@ -484,11 +486,13 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
     for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
     if (lane < 18) rowd[lane] = 0;
     __syncthreads();
-    tile_records(a.recs, rb, re, acc, rowd, rs, marks, lane);
+    uint32_t r0 = 0;
+    tile_records(a.recs, rb, re, acc, rowd, r0, rs, marks, lane);
     for (uint32_t i = eb + lane; i < ee; i += 64) {
       const int4 ev = a.events[i];
-      if (ev.y < T.txl) add_v(rowd, __int_as_float(ev.z), __int_as_float(ev.w));
+      if (ev.y < T.txl) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
     }
+    flush_row0(rowd, r0, lane);
     __syncthreads();
     if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
     __syncthreads();
