@@ -126,17 +126,27 @@ RZ_HD void child(const Node<2>& n, int right, Node<2>* c) { *c = n; }
 // from the root are the same operations the recursive split performs, so the
 // recomputed node is bit-identical). This keeps the traversal in registers
 // with no scratch memory; recomputation only happens above the deepest
-// kStack(NP) levels.
+// kStack(NP) levels. A stack entry leaves out the right sibling's first
+// point: when it is popped, the leaf just emitted is the rightmost one under
+// the left sibling, whose last point is that sibling's last point (a right
+// child copies its parent's last point), i.e. the parent's midpoint, computed
+// by the same expression as the right sibling's first point.
 // leaf(start, end, depth, bits): the leaf is the root's subtree at `depth`
 // reached along `bits` (most significant first, 1 = right child), i.e. the
 // curve on t in [bits, bits + 1] / 2^depth of the root.
 template <int NP>
 constexpr int kStack() { return RZ_STACK_Q * (NP == 3) + RZ_STACK_C * (NP == 4) + 4 * (NP == 2); }
 
+template <int NP>
+struct Tail {  // a node without its first point
+  Pt p[NP - 1];
+};
+
 template <int NP, typename Leaf>
 RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
   constexpr int S = kStack<NP>();
-  Node<NP> cur = root, st[S];
+  Node<NP> cur = root;
+  Tail<NP> st[S];
   int nst = 0, depth = 0;
   uint64_t bits = 0;
   while (true) {
@@ -150,7 +160,9 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       if (depth == 0) return ST_OK;
       bits |= 1;
       if (nst > 0) {
-        cur = st[0];
+        cur.p[0] = cur.p[NP - 1];
+#pragma unroll
+        for (int i = 1; i < NP; ++i) cur.p[i] = st[0].p[i - 1];
 #pragma unroll
         for (int k = 0; k + 1 < S; ++k) st[k] = st[k + 1];
         --nst;
@@ -170,7 +182,8 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       child(cur, 1, &r);
 #pragma unroll
       for (int k = S - 1; k > 0; --k) st[k] = st[k - 1];
-      st[0] = r;
+#pragma unroll
+      for (int i = 1; i < NP; ++i) st[0].p[i - 1] = r.p[i];
       nst = nst < S ? nst + 1 : S;
       cur = l;
       ++depth;

@@ -1,7 +1,7 @@
 // HIP kernels of the batched scene render: Scene::group(fills).render(...)
 // (src/scene.rs:186-199, 268-357, 384-459) for thousands of paths in one
 // pass. Per render:
-//   k_scene_bbox    -- Path::bbox(tr) of every path (one wave per path)
+//   k_scene_bbox    -- Path::bbox(tr) of every path (a lane per segment)
 //   k_scene_setup   -- Pipeline::build_rec's view intersection and
 //                      render_rec's sub-view and alignment per path, its
 //                      binning grid on the canvas tile grid, its paint
@@ -35,40 +35,66 @@ struct SceneView {
   int mode;                    // edge mode of the rasterizer's mask_iter
 };
 
-// Path::bbox(tr) per path: a wave folds its segments' boxes. The union of
-// segment boxes does not depend on the fold order (a quad or cubic whose
-// inner control points already lie in the running box has its curve inside
-// it), so lanes fold their own segments and the wave reduces min/max.
-__global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const ScenePath* sp, SceneView v,
-                                                    double4* boxes, int* has) {
-  const uint32_t i = blockIdx.x;
+// Path::bbox(tr) of every path as ordered keys, so that the union is an
+// integer max: key[4i + 0, 1] = ~okey(min x, min y), key[4i + 2, 3] =
+// okey(max x, max y), all starting at zero. The union of segment boxes does
+// not depend on the fold order (a quad or cubic whose inner control points
+// already lie in the running box has its curve inside it), so each lane takes
+// one segment (closing lines lie inside the segments' box) and each path of
+// the wave gets one atomic per key. NaN coordinates drop out as in fmin/fmax.
+__device__ inline unsigned long long okey(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double okey_value(unsigned long long k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
+__global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const uint32_t* slot_path, SceneView v,
+                                                    unsigned long long* keys) {
+  const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
   const int lane = threadIdx.x;
-  const ScenePath& S = sp[i];
+  bool have = false;
+  uint32_t pid = 0;
   double mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
-  bool any = false;
-  for (uint32_t k = S.seg_begin + lane; k < S.seg_end; k += 64) {
-    const int kind = p.kinds[k];
-    const double* q = p.pts + 2 * (size_t)p.pt_off[k];
-    Pt c[4];
-    for (int j = 0; j < kind + 2; ++j) c[j] = rzg::tr_apply(v.tr, rzg::mkpt(q[2 * j], q[2 * j + 1]));
-    const rzh::BBox b = rzh::segment_bbox(kind, c, nullptr);
-    mnx = fmin(mnx, b.mn.x);
-    mny = fmin(mny, b.mn.y);
-    mxx = fmax(mxx, b.mx.x);
-    mxy = fmax(mxy, b.mx.y);
-    any = true;
+  if (slot < p.n_slots) {
+    const uint32_t k = p.slot_item[slot];
+    if (!(k & 0x80000000u)) {
+      have = true;
+      pid = slot_path[slot];
+      const int kind = p.kinds[k];
+      const double* q = p.pts + 2 * (size_t)p.pt_off[k];
+      Pt c[4];
+      for (int j = 0; j < kind + 2; ++j) c[j] = rzg::tr_apply(v.tr, rzg::mkpt(q[2 * j], q[2 * j + 1]));
+      const rzh::BBox b = rzh::segment_bbox(kind, c, nullptr);
+      mnx = fmin(mnx, b.mn.x);
+      mny = fmin(mny, b.mn.y);
+      mxx = fmax(mxx, b.mx.x);
+      mxy = fmax(mxy, b.mx.y);
+    }
   }
+  unsigned long long k0 = ~okey(mnx), k1 = ~okey(mny), k2 = okey(mxx), k3 = okey(mxy);
+  uint64_t todo = __ballot(have);
+  while (todo) {
+    const int first = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, first);
+    const bool mine = have && pid == q;
+    todo &= ~__ballot(mine);
+    unsigned long long a0 = mine ? k0 : 0ull, a1 = mine ? k1 : 0ull;
+    unsigned long long a2 = mine ? k2 : 0ull, a3 = mine ? k3 : 0ull;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    mnx = fmin(mnx, __shfl_xor(mnx, o, 64));
-    mny = fmin(mny, __shfl_xor(mny, o, 64));
-    mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
-    mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
-  }
-  const bool any_w = __ballot(any) != 0;
-  if (lane == 0) {
-    boxes[i] = make_double4(mnx, mny, mxx, mxy);
-    has[i] = any_w ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+      a0 = max(a0, (unsigned long long)__shfl_xor((long long)a0, o, 64));
+      a1 = max(a1, (unsigned long long)__shfl_xor((long long)a1, o, 64));
+      a2 = max(a2, (unsigned long long)__shfl_xor((long long)a2, o, 64));
+      a3 = max(a3, (unsigned long long)__shfl_xor((long long)a3, o, 64));
+    }
+    if (lane == first) {
+      atomicMax(&keys[4 * (size_t)q + 0], a0);
+      atomicMax(&keys[4 * (size_t)q + 1], a1);
+      atomicMax(&keys[4 * (size_t)q + 2], a2);
+      atomicMax(&keys[4 * (size_t)q + 3], a3);
+    }
   }
 }
 
@@ -78,9 +104,9 @@ __global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const ScenePath* sp,
 // translate(-floor(min)) (:425-428) and the paint's pixel transform for the
 // aligned fill. Writes the counts that k_scene_offsets turns into slices.
 __global__ __launch_bounds__(kBlock) void k_scene_setup(const ScenePath* sp, uint32_t n,
-                                                         SceneView v, const double4* boxes,
-                                                         const int* has, PathInfo* pi,
-                                                         uint32_t* counts, int4* rects) {
+                                                         SceneView v, const unsigned long long* keys,
+                                                         PathInfo* pi, uint32_t* counts,
+                                                         int4* rects) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const ScenePath& S = sp[i];
@@ -89,9 +115,16 @@ __global__ __launch_bounds__(kBlock) void k_scene_setup(const ScenePath* sp, uin
   P.rule = S.rule;
   uint32_t nt = 0, nb = 0, nr = 0;
   int4 rect = make_int4(0, 0, 0, 0);
-  const double4 b = boxes[i];
+  // the path's box (k_scene_bbox); a key still zero saw only NaN coordinates
+  const unsigned long long* kk = keys + 4 * (size_t)i;
+  double4 b;
+  b.x = kk[0] ? okey_value(~kk[0]) : INFINITY;
+  b.y = kk[1] ? okey_value(~kk[1]) : INFINITY;
+  b.z = kk[2] ? okey_value(kk[2]) : -INFINITY;
+  b.w = kk[3] ? okey_value(kk[3]) : -INFINITY;
+  const bool has = S.seg_end > S.seg_begin;
   // range_intersect (src/geometry.rs:677-688) of view and bbox, both axes
-  const bool hit = has[i] && !(v.vx0 > b.z || b.x > v.vx1) && !(v.vy0 > b.w || b.y > v.vy1);
+  const bool hit = has && !(v.vx0 > b.z || b.x > v.vx1) && !(v.vy0 > b.w || b.y > v.vy1);
   if (hit) {
     const double bx0 = fmax(v.vx0, b.x), bx1 = fmin(v.vx1, b.z);
     const double by0 = fmax(v.vy0, b.y), by1 = fmin(v.vy1, b.w);
@@ -154,21 +187,22 @@ __global__ __launch_bounds__(kBlock) void k_scene_setup(const ScenePath* sp, uin
 __global__ __launch_bounds__(kBlock) void k_scene_offsets(uint32_t n, const uint32_t* counts,
                                                            PathInfo* pi, Status* st, uint32_t cap_t,
                                                            uint32_t cap_b, uint32_t cap_r) {
-  uint32_t carry[3] = {0, 0, 0};
-  for (uint32_t start = 0; start < n; start += kBlock) {
-    const uint32_t i = start + threadIdx.x;
-    uint32_t pre[3];
-    for (int c = 0; c < 3; ++c) {
-      const uint32_t v = i < n ? counts[c * n + i] : 0u;
-      uint32_t tot;
-      pre[c] = carry[c] + block_exclusive_scan(v, &tot);
-      carry[c] += tot;
-    }
-    if (i < n) {
-      pi[i].tile_base = pre[0];
-      pi[i].band_base = pre[1];
-      pi[i].redge_base = pre[2];
-    }
+  // each thread takes a run of consecutive paths: one block scan of the run
+  // totals gives each run's first offsets
+  const uint32_t per = (n + kBlock - 1) / kBlock;
+  const uint32_t i0 = min(threadIdx.x * per, n), i1 = min(i0 + per, n);
+  uint32_t run[3] = {0, 0, 0}, carry[3];
+  for (uint32_t i = i0; i < i1; ++i)
+    for (int c = 0; c < 3; ++c) run[c] += counts[c * n + i];
+  for (int c = 0; c < 3; ++c) {
+    const uint32_t pre = block_exclusive_scan(run[c], &carry[c]);
+    run[c] = pre;
+  }
+  for (uint32_t i = i0; i < i1; ++i) {
+    pi[i].tile_base = run[0];
+    pi[i].band_base = run[1];
+    pi[i].redge_base = run[2];
+    for (int c = 0; c < 3; ++c) run[c] += counts[c * n + i];
   }
   if (threadIdx.x == 0) {
     st->pad[0] = carry[0];
@@ -219,17 +253,23 @@ __global__ __launch_bounds__(64) void k_scene_rows(const int4* rects, uint32_t n
   const int ty = blockIdx.x;
   const int lane = threadIdx.x;
   const uint64_t below = (1ull << lane) - 1ull;
+  constexpr int kBatch = 8;  // rows of 64 paths whose rects are loaded together
   uint32_t count = 0;
-  for (uint32_t start = 0; start < n; start += 64) {
-    const uint32_t i = start + lane;
-    bool hit = false;
-    if (i < n) {
-      const int4 r = rects[i];
-      hit = r.y <= ty && ty < r.w;
+  for (uint32_t start = 0; start < n; start += 64 * kBatch) {
+    int4 r[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const uint32_t i = start + 64 * b + lane;
+      r[b] = i < n ? rects[i] : make_int4(0, 0, 0, 0);
     }
-    const uint64_t m = __ballot(hit);
-    if (hit) rows[(size_t)ty * n + count + (uint32_t)__popcll(m & below)] = i;
-    count += (uint32_t)__popcll(m);
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const uint32_t i = start + 64 * b + lane;
+      const bool hit = i < n && r[b].y <= ty && ty < r[b].w;
+      const uint64_t m = __ballot(hit);
+      if (hit) rows[(size_t)ty * n + count + (uint32_t)__popcll(m & below)] = i;
+      count += (uint32_t)__popcll(m);
+    }
   }
   if (lane == 0) row_count[ty] = count;
 }
