@@ -126,9 +126,12 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
           }
           W = (float)w * (1.0f / kFix);
           const float alpha = alpha_of(P.rule, W);
-          b.pbuf[e] = c[k];
-          if (alpha >= 1e-6f)
+          if (alpha >= 1e-6f) {
+            // the colour below is needed only where the path is drawn (the
+            // reverse sweep tests the same alpha)
+            b.pbuf[e] = c[k];
             c[k] = rzp::blend_over(c[k], rzp::mul(rzp::paint_at(P.paint, xs, ys), alpha));
+          }
         }
         b.wbuf[e] = W;
       }

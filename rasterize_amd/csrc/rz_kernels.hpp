@@ -1300,18 +1300,20 @@ struct FineArgs {
 };
 
 // FillRule::alpha_from_winding (src/path.rs:32-46) on a 2^-20 fixed-point winding
+// (the magnitudes converted to f32 are below 2^21, so the 32-bit conversion
+// is exact and equals the 64-bit one)
 template <int RULE>
 __device__ inline float alpha_fix(int64_t w) {
   if (RULE == 0) {
     int64_t a = w < 0 ? -w : w;
     if (a >= ((int64_t)1 << rzg::kFixShift)) return 1.0f;
-    float v = (float)a * (1.0f / kFix);
+    float v = (float)(int32_t)a * (1.0f / kFix);
     return v < 1e-6f ? 0.0f : v;
   } else {
     // ((w + 1) rem_euclid 2 - 1).abs(): exact on the fixed-point value
     const int64_t one = (int64_t)1 << rzg::kFixShift;
     int64_t u = (w + one) & ((one << 1) - 1);
-    int64_t d = u - one;
+    int32_t d = (int32_t)(u - one);
     return (float)(d < 0 ? -d : d) * (1.0f / kFix);
   }
 }
