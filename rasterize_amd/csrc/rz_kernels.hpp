@@ -895,17 +895,34 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b, const 
     return;
   }
   if (b.st->err || b.st->overflow) return;
-  const uint32_t wpb = kScatterBlock / 64;
-  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < b.n_waves; w += n_rec_blocks * wpb) {
-    const uint2 r = b.wave_lines[w];
-    if (r.y == 0) continue;
-    const uint32_t used = b.wave_used[w];
-    const size_t region = 2 * (size_t)r.x;
-    for (uint32_t j = threadIdx.x & 63; j < used; j += 64)
-    {
-      const uint2 mt = b.list_meta[region + j];
-      place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
+  const uint32_t wpb = kScatterBlock / 64, wstride = n_rec_blocks * wpb;
+  // the next region's line range and entry count are loaded while the
+  // current region's entries are placed
+  uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6);
+  uint2 r = make_uint2(0u, 0u);
+  uint32_t used = 0;
+  if (w < b.n_waves) {
+    r = b.wave_lines[w];
+    used = b.wave_used[w];
+  }
+  while (w < b.n_waves) {
+    const uint32_t wn = w + wstride;
+    uint2 rn = make_uint2(0u, 0u);
+    uint32_t usedn = 0;
+    if (wn < b.n_waves) {
+      rn = b.wave_lines[wn];
+      usedn = b.wave_used[wn];
     }
+    if (r.y != 0) {
+      const size_t region = 2 * (size_t)r.x;
+      for (uint32_t j = threadIdx.x & 63; j < used; j += 64) {
+        const uint2 mt = b.list_meta[region + j];
+        place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
+      }
+    }
+    w = wn;
+    r = rn;
+    used = usedn;
   }
   const uint32_t stride = n_rec_blocks * kScatterBlock;
   const size_t obase = b.list_regions;
