@@ -708,20 +708,31 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
     }
   };
 
+  // each group's line (and path) is loaded while the previous group is binned
+  double4 fL = make_double4(0, 0, 0, 0);
+  uint32_t fpid = 0;
+  auto fetch = [&](uint32_t c) {
+    const size_t i = (size_t)wl_range.x + c + (uint32_t)lane;
+    if (c + (uint32_t)lane < wl_range.y) {
+      fL = lines[i];
+      if (MULTI) fpid = line_path[i];
+    }
+  };
+  fetch(0);
   for (uint32_t c0 = 0; c0 < wl_range.y; c0 += 64) {
-    const size_t i = (size_t)wl_range.x + c0 + (uint32_t)lane;
     bool have = c0 + (uint32_t)lane < wl_range.y;
-    uint32_t pid = 0, tb = 0, bb = 0, rb0 = 0;
+    const double4 L = have ? fL : make_double4(0, 0, 0, 0);
+    uint32_t pid = have ? fpid : 0u;
+    if (c0 + 64 < wl_range.y) fetch(c0 + 64);
+    uint32_t tb = 0, bb = 0, rb0 = 0;
     rzg::BinGrid g = g0;
     if (MULTI && have) {
-      pid = line_path[i];
       const PathInfo& P = pi[pid];
       g = P.g;
       tb = P.tile_base;
       bb = P.band_base;
       rb0 = P.redge_base;
     }
-    const double4 L = have ? lines[i] : make_double4(0, 0, 0, 0);
     const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
     if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
       set_err(b.st, rzg::ST_NAN);
