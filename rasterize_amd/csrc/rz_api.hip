@@ -1242,13 +1242,11 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   RZ_TRY(ensure(ctx, ctx->atlas_tiles, (size_t)ct * sizeof(rzk::AtlasTile)));
   RZ_LAUNCH(ctx, "k_atlas_tiles", rzk::k_atlas_tiles, dim3(blocks_for(ct)), dim3(rzk::kBlock),
             (const rzk::PathInfo*)pi, (const uint32_t*)s->atlas_tile_glyph, ct,
+            (const uint32_t*)tile_off, (const uint32_t*)ev_off, (const int32_t*)bd,
             (rzk::AtlasTile*)ctx->atlas_tiles.p);
   a.tiles = (const rzk::AtlasTile*)ctx->atlas_tiles.p;
   a.n = n;
-  a.tile_off = tile_off;
   a.recs = (const float4*)ctx->recs.p;
-  a.bd = bd;
-  a.band_ev_off = ev_off;
   a.events = (const int4*)ctx->events.p;
   a.redge = redge;
   a.st = st;

@@ -455,21 +455,26 @@ __global__ __launch_bounds__(kBlock) void k_atlas_setup(const AtlasSlot* slots, 
   pi[i] = P;
 }
 
-// What the fine stage needs of one atlas tile, gathered once per run so the
-// per-tile work starts from a single load.
+// What the fine stage needs of one atlas tile, gathered once per run (after
+// the scans) so that the per-tile work starts from a single load.
 struct AtlasTile {
   double redge_w;  // the glyph's right-edge weight
   int x, y;        // atlas pixel of the tile's top-left corner
   int w, h;        // glyph pixels inside the tile (0: nothing to draw)
   int txl;         // tile column within the glyph
   int rcol;        // right-edge column relative to the tile's first column
-  uint32_t band;   // band slot (the glyph's band base + band)
   uint32_t redge;  // right-edge slot of the tile's first row
+  uint32_t rb, re;  // the tile's records
+  uint32_t eb, ee;  // the events of its band
+  int bd;           // its band backdrop
 };
 
+// After the scans: one thread per atlas tile.
 __global__ __launch_bounds__(kBlock) void k_atlas_tiles(const PathInfo* pi,
                                                          const uint32_t* tile_glyph, uint32_t n_tiles,
-                                                         AtlasTile* tiles) {
+                                                         const uint32_t* tile_off,
+                                                         const uint32_t* band_ev_off,
+                                                         const int32_t* bd, AtlasTile* tiles) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= n_tiles) return;
   const PathInfo& P = pi[tile_glyph[t]];
@@ -484,8 +489,13 @@ __global__ __launch_bounds__(kBlock) void k_atlas_tiles(const PathInfo* pi,
     T.h = min(16, P.g.H - tyl * 16);
     T.txl = txl;
     T.rcol = P.redge_col == INT32_MAX ? INT32_MAX : P.redge_col - txl * 16;
-    T.band = P.band_base + (uint32_t)tyl;
     T.redge = P.redge_base + (uint32_t)tyl * 16u;
+    const uint32_t band = P.band_base + (uint32_t)tyl;
+    T.rb = tile_off[t];
+    T.re = tile_off[t + 1];
+    T.eb = band_ev_off[band];
+    T.ee = band_ev_off[band + 1];
+    T.bd = bd[t];
   }
   tiles[t] = T;
 }
@@ -493,10 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_atlas_tiles(const PathInfo* pi,
 struct AtlasFineArgs {
   const AtlasTile* tiles;
   uint32_t n;
-  const uint32_t* tile_off;
   const float4* recs;
-  const int32_t* bd;
-  const uint32_t* band_ev_off;
   const int4* events;
   const int32_t* redge;
   const Status* st;
@@ -516,10 +523,9 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   const int lane = threadIdx.x;
   const uint32_t t = blockIdx.x;
   const AtlasTile T = a.tiles[t];
-  const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
-  const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
   if (T.w <= 0 || T.h <= 0) return;
-  const uint32_t eb = a.band_ev_off[T.band], ee = a.band_ev_off[T.band + 1];
+  const uint32_t rb = T.rb, re = T.re, eb = T.eb, ee = T.ee;
+  const int64_t bd = (int64_t)T.bd << rzg::kFixShift;
   const bool local_work = rb != re || eb != ee;
   if (local_work) {
 #pragma unroll
