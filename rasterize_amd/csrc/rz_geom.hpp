@@ -11,8 +11,10 @@
 
 #if defined(__HIPCC__)
 #define RZ_HD __host__ __device__ inline
+#define RZ_UNROLL _Pragma("unroll")
 #else
 #define RZ_HD inline
+#define RZ_UNROLL
 #endif
 
 namespace rzg {
@@ -161,9 +163,9 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       bits |= 1;
       if (nst > 0) {
         cur.p[0] = cur.p[NP - 1];
-#pragma unroll
+RZ_UNROLL
         for (int i = 1; i < NP; ++i) cur.p[i] = st[0].p[i - 1];
-#pragma unroll
+RZ_UNROLL
         for (int k = 0; k + 1 < S; ++k) st[k] = st[k + 1];
         --nst;
       } else {
@@ -180,9 +182,9 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
       Node<NP> l, r;
       child(cur, 0, &l);
       child(cur, 1, &r);
-#pragma unroll
+RZ_UNROLL
       for (int k = S - 1; k > 0; --k) st[k] = st[k - 1];
-#pragma unroll
+RZ_UNROLL
       for (int i = 1; i < NP; ++i) st[0].p[i - 1] = r.p[i];
       nst = nst < S ? nst + 1 : S;
       cur = l;
