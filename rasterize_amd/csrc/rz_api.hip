@@ -979,7 +979,9 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
   return RZ_OK;
 }
 
-// The batched form: one pipeline for every path of the scene.
+// The batched form: one pipeline for every path of the scene. With d_out
+// null it stops before the fine stage (the adjoint needs only the binned
+// state).
 rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
                                 const double* view, rzp::C4 bg, int mode, double flatness,
                                 rzp::C4* d_out, size_t row_stride) {
@@ -1108,7 +1110,9 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.lw = lw;
   a.lh = lh;
   a.tiles_x = ctx_x;
-  RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
+  // no layer (the adjoint's own tile pass recomposites): the binned state only
+  if (d_out)
+    RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
   RZ_TRY(check_launch(ctx, "scene pass 1"));
   RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
                              ctx->stream));
@@ -1273,8 +1277,9 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
   int lx, ly, lw, lh;
   layer_of(view, &lx, &ly, &lw, &lh);
   if (lw == 0 || lh == 0 || s->n == 0) return RZ_OK;
-  RZ_TRY(ensure(ctx, ctx->out, (size_t)lw * lh * 16));
-  RZ_TRY(enqueue_scene_batched(ctx, s, tr, view, bg, mode, flatness, (rzp::C4*)ctx->out.p, (size_t)lw));
+  // the forward pipeline up to the fine stage (k_scene_bwd_tiles composites
+  // the paths again itself)
+  RZ_TRY(enqueue_scene_batched(ctx, s, tr, view, bg, mode, flatness, nullptr, 0));
   const size_t entries = ctx->sc_cap_t;
   RZ_TRY(ensure(ctx, ctx->bw_w, entries * 256 * sizeof(float)));
   RZ_TRY(ensure(ctx, ctx->bw_p, entries * 256 * sizeof(rzp::C4)));
