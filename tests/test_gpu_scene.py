@@ -124,3 +124,25 @@ def test_scene_batched_is_deterministic(small_scene):
     a = small_scene.scene().render(SignedDifferenceRasterizer(), S, view).image
     b = small_scene.scene().render(SignedDifferenceRasterizer(), S, view).image
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_scene_empty_outside_and_clipped_paths(small_scene, batched):
+    """Fills without segments, fills entirely outside the view and fills cut by
+    the view's edges, between ordinary ones: the per-segment box reduction
+    must leave an empty path undrawn and give clipped paths their sub-views."""
+    empty = PathBuilder().build()
+    far = PathBuilder().move_to((5000.0, 5000.0)).line_to((5100.0, 5000.0)).line_to((5050.0, 5080.0)).close().build()
+    edge = (PathBuilder().move_to((-40.0, 30.0)).cubic_to((600.0, -50.0), (1200.0, 1300.0), (1100.0, 90.0))
+            .line_to((120.0, 1070.0)).close().build())
+    ws = small_scene
+    paint = ws.paints[1]
+    order = list(range(10)) + ["e", "f"] + list(range(10, 25)) + ["c", "e"] + list(range(25, 40))
+    pick = {"e": (empty, paint, 0), "f": (far, paint, 0), "c": (edge, paint, 1)}
+    items = [(ws.paths[k], ws.paints[k], ws.rules[k]) if isinstance(k, int) else pick[k] for k in order]
+    scene = Scene.group([Scene.fill(p, q, r) for p, q, r in items])
+    view = (0.0, 0.0, 256.0, 256.0)
+    _say(f"scene empty/outside/clipped batched={batched}")
+    got = scene.render(SignedDifferenceRasterizer(), S, view, batched=batched)
+    ref = O.scene_render([(p, q.abi(), r) for p, q, r in items], S, view)
+    check_layer(got, ref, f"scene empty/outside/clipped batched={batched}")
