@@ -69,9 +69,11 @@ struct rz_ctx {
   std::vector<rz_kernel_stat> stats;
 };
 
-// Mean expected line count of a path's quads and cubics: (flatness /
-// (16 tol^2))^(1/4) per segment (a segment flattens into about 1.4 times that
-// many lines), at tol = 0.05 under the transform the estimate was made with.
+// Typical expected line count of a path's quads and cubics: the geometric
+// mean over segments of (flatness / (16 tol^2))^(1/4) (a segment flattens into
+// about 1.4 times that many lines), at tol = 0.05 under the transform the
+// estimate was made with. The geometric mean keeps a few large curves of a
+// scene from splitting its many small ones.
 struct SplitEst {
   double quad = 1.0, cubic = 1.0;
 };
@@ -153,12 +155,12 @@ uint32_t blocks_for(size_t n) { return (uint32_t)((n + rzk::kBlock - 1) / rzk::k
 
 // Sums behind a SplitEst.
 struct SplitSum {
-  double q = 0, c = 0;
+  double q = 0, c = 0;  // sums of log counts
   size_t nq = 0, nc = 0;
   SplitEst est() const {
     SplitEst e;
-    e.quad = nq ? q / (double)nq : 1.0;
-    e.cubic = nc ? c / (double)nc : 1.0;
+    e.quad = nq ? std::exp(q / (double)nq) : 1.0;
+    e.cubic = nc ? std::exp(c / (double)nc) : 1.0;
     return e;
   }
 };
@@ -179,7 +181,7 @@ void split_sum(const rz_path* p, const double* m, SplitSum* s) {
     } else {
       continue;
     }
-    const double e = f > fl && std::isfinite(f) ? std::pow(f / fl, 0.25) : 1.0;
+    const double e = f > fl && std::isfinite(f) ? 0.25 * std::log(f / fl) : 0.0;  // log of the count
     if (p->kinds[k] == RZ_QUAD) {
       s->q += e;
       ++s->nq;
