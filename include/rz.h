@@ -286,8 +286,12 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
  * quad / cubic segment's subdivision tree is processed by 2^value threads
  * (one per subtree at that depth), 0..8; setting either fixes both for the
  * context. RZ_PARAM_SPLIT_AUTO = 1 (the default) chooses them per call from
- * the segments' expected line counts. Results do not depend on them. */
-enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2 };
+ * the segments' expected line counts. RZ_PARAM_SCAN_DIRECT (0..16, default
+ * 12): a scan of up to 2^value blocks of 2048 counts has each block sum the
+ * totals before its own; a longer one scans the totals in a kernel of their
+ * own. Results do not depend on any of them. */
+enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2,
+       RZ_PARAM_SCAN_DIRECT = 3 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 

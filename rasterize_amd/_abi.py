@@ -16,7 +16,7 @@ STATUS_NAMES = {
     RZ_E_DEPTH: "RZ_E_DEPTH", RZ_E_NOMEM: "RZ_E_NOMEM", RZ_E_CAPACITY: "RZ_E_CAPACITY",
 }
 RZ_NONZERO, RZ_EVENODD = 0, 1
-RZ_PARAM_SPLIT_QUAD, RZ_PARAM_SPLIT_CUBIC, RZ_PARAM_SPLIT_AUTO = 0, 1, 2
+RZ_PARAM_SPLIT_QUAD, RZ_PARAM_SPLIT_CUBIC, RZ_PARAM_SPLIT_AUTO, RZ_PARAM_SCAN_DIRECT = 0, 1, 2, 3
 RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
 RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
 RZ_PAINT_SOLID, RZ_PAINT_LINEAR, RZ_PAINT_RADIAL = 0, 1, 2

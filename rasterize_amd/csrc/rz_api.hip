@@ -57,6 +57,7 @@ struct rz_ctx {
   // launch parameters (rz_set_param)
   int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
   bool split_auto = true;               // choose them per call (split_depths)
+  uint32_t scan_direct = rzk::kScanDirect;  // block totals summed by the add pass itself
   uint32_t line_cap = 0;
   // kernel timing (rz_profile)
   bool prof = false;
@@ -235,7 +236,7 @@ rz_status scan_u32(rz_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n, uin
   uint32_t* sums = (uint32_t*)ctx->sums.p;
   RZ_LAUNCH(ctx, "k_scan_tiles", rzk::k_scan_tiles, dim3((uint32_t)nb), dim3(rzk::kBlock), in,
                      out, n, sums);
-  const bool scanned = nb > rzk::kScanDirect;
+  const bool scanned = nb > ctx->scan_direct;
   if (scanned)
     RZ_LAUNCH(ctx, "k_scan_sums", rzk::k_scan_sums, dim3(1), dim3(rzk::kBlock), sums, nb, out, n,
                        total);
@@ -1433,6 +1434,11 @@ extern "C" {
 
 rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   if (!ctx) return RZ_E_ARG;
+  if (param == RZ_PARAM_SCAN_DIRECT) {
+    if (value < 0 || value > 16) return fail(ctx, RZ_E_ARG, "parameter out of range 0..16");
+    ctx->scan_direct = 1u << value;
+    return RZ_OK;
+  }
   if (value < 0 || value > 8) return fail(ctx, RZ_E_ARG, "parameter out of range 0..8");
   if (param == RZ_PARAM_SPLIT_QUAD) {
     ctx->split_quad = value;
@@ -1453,6 +1459,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   if (param == RZ_PARAM_SPLIT_QUAD) *value = ctx->split_quad;
   else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
   else if (param == RZ_PARAM_SPLIT_AUTO) *value = ctx->split_auto ? 1 : 0;
+  else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
   return RZ_OK;
 }

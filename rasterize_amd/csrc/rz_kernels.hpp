@@ -606,12 +606,13 @@ __device__ void gap_events(const GapArgs& ga, const rzg::BinGrid& g0, const BinB
 // The (line, band) pairs of the 64 lines are then dealt out to the lanes 64
 // at a time, and the (band part, tile column) pairs of each such round
 // likewise, so a line crossing many cells keeps many lanes busy rather than
-// one. Each round's records are grouped by tile across the lanes; a group
-// takes consecutive ranks from its tile's count in the wave's LDS map, and
-// the records go to the wave's region of the record list. Lines beyond the
-// right clip only add to the right-edge column; two such lines on
-// neighbouring lanes that continue each other drop their shared endpoint (its
-// two terms cancel exactly).
+// one. Each record takes its rank from its tile's count in the wave's LDS
+// map (one LDS atomic) and goes to the wave's region of the record list.
+// Lines beyond the right clip only add to the right-edge column; two such
+// lines on neighbouring lanes that continue each other drop their shared
+// endpoint (its two terms cancel exactly). Waves past the flatten waves take
+// the gap slots.
+
 // Workgroup size of the count pass. Its waves are independent and their
 // work varies with their lines, so one wave per workgroup.
 constexpr uint32_t kBinBlock = 64;

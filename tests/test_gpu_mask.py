@@ -181,6 +181,30 @@ def test_bin_split_depths_agree(dq, dc, c2):
     assert np.array_equal(_sort_events(ge), _sort_events(re))
 
 
+@pytest.mark.parametrize("direct_log2", [0, 3])
+def test_bin_scan_forms_agree(direct_log2, c2):
+    """The tile scan's second pass sums the block totals itself up to
+    2^RZ_PARAM_SCAN_DIRECT blocks and has them scanned by a kernel of their
+    own beyond: both forms (C2 has 32 blocks), with the band scan and the
+    backdrop / right-edge row prefixes riding along, must bin identically."""
+    path, tr = c2
+    rc, rr, rb, re, rx = rz.bin_stage(path, tr, 4096, 4096)
+    ctx = rz.Context(0)
+    try:
+        L = rz._abi.lib()
+        ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_SCAN_DIRECT, direct_log2))
+        v = C.c_int()
+        ctx.check(L.rz_get_param(ctx.h, rz._abi.RZ_PARAM_SCAN_DIRECT, C.byref(v)))
+        assert v.value == direct_log2
+        _say(f"bin c2 scan direct up to 2^{direct_log2} blocks")
+        gc, gr, gb, ge, gx = rz.bin_stage(path, tr, 4096, 4096, ctx=ctx)
+    finally:
+        ctx.close()
+    assert np.array_equal(gc, rc) and np.array_equal(gb, rb) and np.array_equal(gx, rx)
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr))
+    assert np.array_equal(_sort_events(ge), _sort_events(re))
+
+
 @pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
 def test_mask_long_lines(rule):
     path = _spokes(3000, 4096)
