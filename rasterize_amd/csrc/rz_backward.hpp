@@ -53,10 +53,28 @@ __device__ inline float dalpha_of(int rule, float W) {
   return f < 0.0f ? -1.0f : 1.0f;
 }
 
+// Path q over canvas tile (TX, TY): its tile slot, record and event ranges
+// and backdrop. Returns whether its winding can be nonzero anywhere on the
+// tile (records, events, a backdrop, or pixels at or right of its right-edge
+// column); where not, the path contributes nothing to the tile.
+__device__ inline bool path_tile(const SceneFineArgs& a, uint32_t q, int TX, int TY, uint32_t* t,
+                                 uint32_t* rb, uint32_t* re, uint32_t* eb, uint32_t* ee) {
+  const PathInfo& P = a.pi[q];
+  const int txl = TX - P.tx0, tyl = TY - P.ty0;
+  *t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
+  *rb = a.tile_off[*t];
+  *re = a.tile_off[*t + 1];
+  *eb = a.band_ev_off[P.band_base + tyl];
+  *ee = a.band_ev_off[P.band_base + tyl + 1];
+  const bool right = txl * 16 - P.g.phx + 15 >= P.redge_col;
+  return *rb != *re || *eb != *ee || a.bd[*t] != 0 || right;
+}
+
 // Per canvas tile: the forward composite again, keeping each path's winding
 // and the colour below it; then the paths in reverse order with the
 // transmittance of the paths above, giving dL/dW per pixel and path and
-// dL/dc per path.
+// dL/dc per path. Paths with zero winding over the tile (alpha 0 on all of
+// it) change neither sweep; only their dL/dW slot is cleared.
 __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
   const SceneFineArgs& a = b.f;
   if (a.st->err || a.st->overflow) return;
@@ -82,16 +100,22 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
       const int4 r = a.rects[pid];
       hit = r.x <= TX && TX < r.z;
     }
-    uint64_t m = __ballot(hit);
+    // a path with zero winding over the tile is skipped (the reverse sweep
+    // writes its zero dL/dW)
+    uint32_t lt = 0, lrb = 0, lre = 0, leb = 0, lee = 0;
+    const bool draw = hit && path_tile(a, pid, TX, TY, &lt, &lrb, &lre, &leb, &lee);
+    uint64_t m = __ballot(draw);
     while (m) {
       const int src = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
       const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, src);
       const PathInfo& P = a.pi[q];
       const int txl = TX - P.tx0, tyl = TY - P.ty0;
-      const uint32_t t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
-      const uint32_t rb = a.tile_off[t], re = a.tile_off[t + 1];
-      const uint32_t eb = a.band_ev_off[P.band_base + tyl], ee = a.band_ev_off[P.band_base + tyl + 1];
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)lt, src);
+      const uint32_t rb = (uint32_t)__builtin_amdgcn_readlane((int)lrb, src);
+      const uint32_t re = (uint32_t)__builtin_amdgcn_readlane((int)lre, src);
+      const uint32_t eb = (uint32_t)__builtin_amdgcn_readlane((int)leb, src);
+      const uint32_t ee = (uint32_t)__builtin_amdgcn_readlane((int)lee, src);
       const bool local = rb != re || eb != ee;
       if (local) {
 #pragma unroll
@@ -158,14 +182,24 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
       const int4 r = a.rects[pid];
       hit = r.x <= TX && TX < r.z;
     }
-    uint64_t m = __ballot(hit);
+    uint32_t lt = 0, lrb, lre, leb, lee;
+    const bool draw = hit && path_tile(a, pid, TX, TY, &lt, &lrb, &lre, &leb, &lee);
+    // paths skipped by the forward sweep: dL/dW = 0 on their tile slot
+    uint64_t z = __ballot(hit && !draw);
+    while (z) {
+      const int src = __ffsll((unsigned long long)z) - 1;
+      z &= z - 1;
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)lt, src);
+      reinterpret_cast<float4*>(b.wbuf + (size_t)t * 256)[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    uint64_t m = __ballot(draw);
     while (m) {
       const int src = 63 - __clzll((long long)m);
       m &= ~(1ull << src);
       const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, src);
       const PathInfo& P = a.pi[q];
       const int txl = TX - P.tx0, tyl = TY - P.ty0;
-      const uint32_t t = P.tile_base + (uint32_t)(tyl * P.g.tiles_x + txl);
+      const uint32_t t = (uint32_t)__builtin_amdgcn_readlane((int)lt, src);
       const int xs = txl * 16 - P.g.phx + col;
       rzp::C4 dc = rzp::splat(0.0f);
 #pragma unroll
