@@ -58,7 +58,16 @@ RZ_HD C4 s2l(C4 x) { return c4(s2l1(x.r), s2l1(x.g), s2l1(x.b), s2l1(x.a)); }
 RZ_HD C4 unmultiply(C4 c) { return c.a <= 1e-6f ? splat(0.0f) : div(c, splat(c.a)); }
 // LinColor::into_srgb / into_linear (src/color.rs:322-332)
 RZ_HD C4 into_srgb(C4 c) { return mul(l2s(unmultiply(c)), splat(c.a)); }
-RZ_HD C4 into_linear(C4 c) { return mul(s2l(unmultiply(c)), splat(c.a)); }
+// The alpha lane of unmultiply is c.a / c.a: exactly 1 for finite c.a above
+// the threshold and NaN (the same NaN as c.a - c.a) for an infinite or NaN
+// one, so it needs no division.
+RZ_HD C4 into_linear(C4 c) {
+  const bool zero = c.a <= 1e-6f;
+  const float inv = zero ? 0.0f : c.a;
+  const float la = zero ? s2l1(0.0f) : (c.a < INFINITY ? s2l1(1.0f) : s2l1(c.a - c.a));
+  const C4 u = zero ? splat(0.0f) : c4(c.r / inv, c.g / inv, c.b / inv, 0.0f);
+  return mul(c4(s2l1(u.r), s2l1(u.g), s2l1(u.b), la), splat(c.a));
+}
 // Color::lerp for LinColor: other * t + self * (1 - t) (src/color.rs:352-354)
 RZ_HD C4 lerp(C4 self, C4 other, float t) { return add(mul(other, t), mul(self, 1.0f - t)); }
 // Color::blend_over for LinColor: other + self * (1 - other.alpha)
