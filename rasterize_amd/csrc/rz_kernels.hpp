@@ -719,20 +719,33 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
       if (MULTI) fpid = line_path[i];
     }
   };
+  // batched forms: the group's path data, loaded during the previous group
+  rzg::BinGrid fg = g0;
+  uint32_t ftb = 0, fbb = 0, frb0 = 0;
+  auto fetch_path = [&](bool h) {
+    if (MULTI && h) {
+      const PathInfo& P = pi[fpid];
+      fg = P.g;
+      ftb = P.tile_base;
+      fbb = P.band_base;
+      frb0 = P.redge_base;
+    }
+  };
   fetch(0);
+  fetch_path((uint32_t)lane < wl_range.y);
   for (uint32_t c0 = 0; c0 < wl_range.y; c0 += 64) {
     bool have = c0 + (uint32_t)lane < wl_range.y;
     const double4 L = have ? fL : make_double4(0, 0, 0, 0);
     uint32_t pid = have ? fpid : 0u;
-    if (c0 + 64 < wl_range.y) fetch(c0 + 64);
+    const bool next = c0 + 64 < wl_range.y;
+    if (next) fetch(c0 + 64);
     uint32_t tb = 0, bb = 0, rb0 = 0;
     rzg::BinGrid g = g0;
     if (MULTI && have) {
-      const PathInfo& P = pi[pid];
-      g = P.g;
-      tb = P.tile_base;
-      bb = P.band_base;
-      rb0 = P.redge_base;
+      g = fg;
+      tb = ftb;
+      bb = fbb;
+      rb0 = frb0;
     }
     const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
     if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
@@ -792,6 +805,7 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
       S.tb = tb;
     }
     list_events(b, nev, ev0, evb0, ev1, evb1, lane);
+    if (MULTI && next) fetch_path(c0 + 64 + (uint32_t)lane < wl_range.y);
     // band tasks, 64 per round
     uint32_t n_band;
     const uint32_t pre1 = wave_excl_scan((uint32_t)(nb0 + nb1), lane, &n_band);
