@@ -193,9 +193,32 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
       reinterpret_cast<float4*>(b.wbuf + (size_t)t * 256)[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     uint64_t m = __ballot(draw);
+    // the next path's windings and colours below are loaded while the
+    // current one is processed (colours below also where they are unused)
+    float nW[4];
+    rzp::C4 nB[4];
+    auto prefetch = [&](uint64_t mm) {
+      if (!mm) return;
+      const int s2 = 63 - __clzll((long long)mm);
+      const size_t base = (size_t)(uint32_t)__builtin_amdgcn_readlane((int)lt, s2) * 256 + lane;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        nW[k] = b.wbuf[base + 64 * k];
+        nB[k] = b.pbuf[base + 64 * k];
+      }
+    };
+    prefetch(m);
     while (m) {
       const int src = 63 - __clzll((long long)m);
       m &= ~(1ull << src);
+      float cW[4];
+      rzp::C4 cB[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        cW[k] = nW[k];
+        cB[k] = nB[k];
+      }
+      prefetch(m);
       const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)pid, src);
       const PathInfo& P = a.pi[q];
       const int txl = TX - P.tx0, tyl = TY - P.ty0;
@@ -209,11 +232,11 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
         const size_t e = (size_t)t * 256 + (size_t)(r * 16 + col);
         float gW = 0.0f;
         if (xs >= 0 && xs < P.g.W && ys >= 0 && ys < P.g.H) {
-          const float W = b.wbuf[e];
+          const float W = cW[k];
           const float alpha = alpha_of(P.rule, W);
           if (alpha >= 1e-6f) {
             const rzp::C4 cp = rzp::paint_at(P.paint, xs, ys);
-            const rzp::C4 below = b.pbuf[e];
+            const rzp::C4 below = cB[k];
             const float gdot = g[k].r * below.r + g[k].g * below.g + g[k].b * below.b + g[k].a * below.a;
             const rzp::C4 ds = rzp::c4(T[k] * g[k].r, T[k] * g[k].g, T[k] * g[k].b, T[k] * (g[k].a - gdot));
             const float da = cp.r * ds.r + cp.g * ds.g + cp.b * ds.b + cp.a * ds.a;
