@@ -941,9 +941,19 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b, const 
     }
     if (r.y != 0) {
       const size_t region = 2 * (size_t)r.x;
-      for (uint32_t j = threadIdx.x & 63; j < used; j += 64) {
+      // two groups of 64 entries in flight at once
+      for (uint32_t j = threadIdx.x & 63; j < used; j += 128) {
+        const bool h2 = j + 64 < used;
         const uint2 mt = b.list_meta[region + j];
-        place_record(b, mt.x, mt.y, w, b.list_rec[region + j]);
+        const float4 rc = b.list_rec[region + j];
+        uint2 mt2 = make_uint2(0u, 0u);
+        float4 rc2 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (h2) {
+          mt2 = b.list_meta[region + j + 64];
+          rc2 = b.list_rec[region + j + 64];
+        }
+        place_record(b, mt.x, mt.y, w, rc);
+        if (h2) place_record(b, mt2.x, mt2.y, w, rc2);
       }
     }
     w = wn;
