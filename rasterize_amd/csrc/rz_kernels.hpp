@@ -1285,12 +1285,16 @@ __device__ inline void fine_row(int* acc, const RowStage& R, int y) {
 // wave's 64-entry LDS scratch. Every lane of the wave takes part.
 __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re, int* acc,
                                     int* rowd, uint32_t& row0, RowStage* rs, int* marks, int lane) {
+  // each round's records are loaded while the previous round is deposited
+  float4 nx = rb + (uint32_t)lane < re ? recs[rb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint32_t r0 = rb; r0 < re; r0 += 64) {
     const uint32_t i = r0 + (uint32_t)lane;
     int nrows = 0;
     RowStage R{};
+    const float4 cur = nx;
+    if (r0 + 64 + (uint32_t)lane < re) nx = recs[r0 + 64 + lane];
     if (i < re) {
-      const float4 r = recs[i];
+      const float4 r = cur;
       float x0 = r.x, y0 = r.y, x1 = r.z, y1 = r.w;
       // records are clipped to the cell; anything else is skipped rather than
       // allowed to index outside the tile
