@@ -80,6 +80,18 @@ RZ_HD bool has_nans(const Node<NP>& n) {
   return r;
 }
 
+// True when every coordinate is below 1e300 in magnitude (false for NaN).
+// Every point of a split is a combination of its parent's points with
+// non-negative power-of-two weights summing to 1, so the descendants of such
+// a node stay below 1e300 as well (no overflow to infinity, hence no NaN):
+// the per-node NaN test of the traversal is needed only for other roots.
+template <int NP>
+RZ_HD bool bounded(const Node<NP>& n) {
+  bool r = true;
+  for (int i = 0; i < NP; ++i) r = r && fabs(n.p[i].x) < 1e300 && fabs(n.p[i].y) < 1e300;
+  return r;
+}
+
 // Curve::split_at(0.5), the split PathFlattenIter uses (src/curve.rs:43-45,
 // 1186-1201). With t = 0.5 every coefficient the reference computes
 // (1-t, t*t, 2*t*(1-t), 3*t*(1-t)^2, ...) is an exact power-of-two multiple,
@@ -151,8 +163,9 @@ RZ_HD int traverse_db(const Node<NP>& root, double fl, Leaf&& leaf) {
   Tail<NP> st[S];
   int nst = 0, depth = 0;
   uint64_t bits = 0;
+  const bool check = !bounded(root);
   while (true) {
-    if (has_nans(cur)) return ST_NAN;
+    if (check && has_nans(cur)) return ST_NAN;
     if (flatness(cur) < fl) {
       leaf(cur.p[0], cur.p[NP - 1], depth, bits);
       while (depth > 0 && (bits & 1)) {
