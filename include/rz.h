@@ -14,7 +14,10 @@
  *     `&self` and single-threaded).
  *   - Blocking entry points take host buffers and return when the result is
  *     in them. `_device` entry points take device pointers, are asynchronous
- *     on the context's stream and report errors at the next rz_sync().
+ *     on the context's stream and report errors at the next rz_sync(): the
+ *     status of every call queued since the previous sync is merged (first
+ *     error, every capacity miss), so RZ_E_CAPACITY there means the queued
+ *     calls must all be re-run (the buffers have been grown).
  *   - Coordinates and transforms are f64 as in the reference (Scalar = f64).
  *     Masks are produced in f32 and widened where the reference's image is
  *     f64. Colours are premultiplied linear RGBA f32 (LinColor).
@@ -120,6 +123,15 @@ const char* rz_last_error(const rz_ctx* ctx);
 rz_status rz_sync(rz_ctx* ctx);
 /* The context's hipStream_t (as void*), for callers that order their own work. */
 void* rz_stream(rz_ctx* ctx);
+/* Stream ordering with a caller's stream (hipStream_t as void*, NULL = the
+ * null stream). rz_wait_stream: work the context enqueues from now on starts
+ * after everything already enqueued on `stream` (e.g. the kernels that wrote
+ * a device input, or the last use of a recycled output buffer).
+ * rz_stream_wait_ctx: work enqueued on `stream` from now on starts after
+ * everything the context has enqueued. No reference counterpart (the
+ * reference is single-threaded host code). */
+rz_status rz_wait_stream(rz_ctx* ctx, void* stream);
+rz_status rz_stream_wait_ctx(rz_ctx* ctx, void* stream);
 
 /* ---- stage hooks (bit-exact parity surfaces) ------------------------- */
 

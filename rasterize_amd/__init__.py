@@ -666,45 +666,74 @@ class DeviceScene:
 def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLATNESS):
     """Differentiable Scene::render (batched form) as a torch.autograd.Function:
     `points` (N, 2) float tensor of every path's stored control points in scene
-    order, `colors` (n_paths, 4) solid premultiplied linear RGBA; returns the
-    (H, W, 4) float32 layer of `view`. Gradients flow to points and colors
-    through rz_render_backward_device."""
+    order, `colors` (n_paths, 4) solid premultiplied linear RGBA, `tr` a
+    Transform or a 6-element tensor [m00, m01, m02, m10, m11, m12]; returns the
+    (H, W, 4) float32 layer of `view`. Gradients flow to points, colors and (a
+    tensor) tr through rz_render_backward_device.
+
+    The library's context runs on its own HIP stream: before it reads the
+    inputs (or writes a buffer torch just recycled) it waits for everything
+    already enqueued on torch's current stream (rz_wait_stream), and each
+    call returns only after the context's work is done. The backward pass
+    re-uploads the points and colours its forward used, so other renders of
+    the same scene in between do not change the gradients."""
     import torch
+
+    tr_is_tensor = isinstance(tr, torch.Tensor)
+
+    def _tr_of(t):
+        if tr_is_tensor:
+            return Transform(*[float(v) for v in t.detach().to("cpu", torch.float64).reshape(6)])
+        return tr
+
+    def _wait(dev):
+        s = torch.cuda.current_stream(dev)
+        dscene.ctx.check(lib().rz_wait_stream(dscene.ctx.h, C.c_void_p(s.cuda_stream)))
 
     class _Render(torch.autograd.Function):
         @staticmethod
-        def forward(fctx, pts, cols):
+        def forward(fctx, pts, cols, trt):
             x0, y0 = int(np.floor(view[0])), int(np.floor(view[1]))
             w, h = int(np.ceil(view[2])) - x0, int(np.ceil(view[3])) - y0
+            T = _tr_of(trt)
             p64 = pts.detach().to(torch.float64).contiguous()
             c32 = cols.detach().to(torch.float32).contiguous()
-            dscene.update_device(p64.data_ptr(), c32.data_ptr())
             out = torch.empty((h, w, 4), dtype=torch.float32, device=pts.device)
+            _wait(pts.device)
+            dscene.update_device(p64.data_ptr(), c32.data_ptr())
             L = lib()
             for _ in range(4):
-                dscene.render_device(tr, view, out.data_ptr(), w, bg=bg, batched=True,
+                dscene.render_device(T, view, out.data_ptr(), w, bg=bg, batched=True,
                                      flatness=flatness)
                 s = L.rz_sync(dscene.ctx.h)
                 if s != _abi.RZ_E_CAPACITY:
                     break
             dscene.ctx.check(s)
             fctx.keep = (p64, c32)
+            fctx.T = T
             fctx.shape = (h, w)
-            fctx.dtypes = (pts.dtype, cols.dtype)
+            fctx.dtypes = (pts.dtype, cols.dtype, trt.dtype if tr_is_tensor else None)
             return out
 
         @staticmethod
         def backward(fctx, grad):
             h, w = fctx.shape
+            p64, c32 = fctx.keep
             g = grad.detach().to(torch.float32).contiguous()
             gp = torch.zeros((dscene.n_points(), 2), dtype=torch.float32, device=g.device)
             gc = torch.zeros((dscene.n, 4), dtype=torch.float32, device=g.device)
-            dscene.backward_device(tr, view, g.data_ptr(), w, gp.data_ptr(), gc.data_ptr(), bg=bg,
+            gt = torch.zeros(6, dtype=torch.float64, device=g.device) if tr_is_tensor else None
+            _wait(g.device)
+            # the state the forward rendered (the scene may have been updated since)
+            dscene.update_device(p64.data_ptr(), c32.data_ptr())
+            dscene.backward_device(fctx.T, view, g.data_ptr(), w, gp.data_ptr(), gc.data_ptr(),
+                                   gt.data_ptr() if gt is not None else None, bg=bg,
                                    flatness=flatness)
             dscene.ctx.check(lib().rz_sync(dscene.ctx.h))
-            return gp.to(fctx.dtypes[0]), gc.to(fctx.dtypes[1])
+            gtr = gt.to(fctx.dtypes[2]) if gt is not None else None
+            return gp.to(fctx.dtypes[0]), gc.to(fctx.dtypes[1]), gtr
 
-    return _Render.apply(points, colors)
+    return _Render.apply(points, colors, tr if tr_is_tensor else None)
 
 
 class DiffScene:

@@ -90,6 +90,8 @@ PROTOTYPES = [
     ("rz_last_error", C.c_char_p, [c_ctx]),
     ("rz_sync", C.c_int, [c_ctx]),
     ("rz_stream", C.c_void_p, [c_ctx]),
+    ("rz_wait_stream", C.c_int, [c_ctx, C.c_void_p]),
+    ("rz_stream_wait_ctx", C.c_int, [c_ctx, C.c_void_p]),
     ("rz_flatten", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int,
                              P(C.c_double), C.c_size_t, P(C.c_size_t)]),
     ("rz_bin", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_size_t,

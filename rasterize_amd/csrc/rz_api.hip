@@ -40,8 +40,6 @@ struct rz_ctx {
   DBuf lines;     // f64 lines for rz_flatten
   DBuf counts;    // per-slot counts for rz_flatten
   DBuf stops;     // gradient stops of the current fill: f64 positions, then C4 colours
-  DBuf sticky;    // status folded over the runs of a plain-form scene render
-  rzk::Status* sticky_st = nullptr;  // set while such a render is enqueued
   // batched scene render
   DBuf line_path;  // path of each line
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
@@ -53,6 +51,8 @@ struct rz_ctx {
   uint32_t rec_cap = 0, ev_cap = 0;
   uint32_t part_cap = 0;  // overflow record-list entries per part
   rzk::Status* host_status = nullptr;  // pinned
+  rzk::Status* acc = nullptr;          // device: status of the runs since the last sync, merged
+  hipEvent_t xev = nullptr;            // orders the context's stream with a caller's stream
   rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
   int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
@@ -632,6 +632,7 @@ rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   a.band_ev_off = P.ev_off;
   a.events = (const int4*)ctx->events.p;
   a.st = P.st;
+  a.acc = ctx->acc;  // the fine kernel merges the run's status (read at rz_sync)
   a.W = (int)W;
   a.H = (int)H;
   a.tiles_x = P.grid.g.tiles_x;
@@ -650,15 +651,7 @@ rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
     RZ_LAUNCH(ctx, name0, (rzk::k_fine<0, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
   else
     RZ_LAUNCH(ctx, name1, (rzk::k_fine<1, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
-  RZ_TRY(check_launch(ctx, name0));
-  if (ctx->sticky_st) {
-    RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), P.st, ctx->sticky_st);
-    return check_launch(ctx, "k_status_merge");
-  }
-  // the status block is read back at rz_sync
-  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, P.st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  return RZ_OK;
+  return check_launch(ctx, name0);
 }
 
 rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
@@ -704,8 +697,13 @@ rz_status enqueue_fill(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   return enqueue_fine(ctx, d, tr, flatness, rule, mode, W, H, o, "k_fine_fill<0>", "k_fine_fill<1>");
 }
 
-// Wait, then interpret the status block: errors, and capacity growth.
+// Wait, then interpret the status merged over every run since the last
+// finish: errors, and capacity growth. The accumulated block starts over.
 rz_status finish(rz_ctx* ctx) {
+  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->acc, sizeof(rzk::Status), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3(1), dim3(64), (uint32_t*)ctx->acc,
+            (size_t)rzk::kStatusWords, 0u);
   RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
   rzk::Status s = *ctx->host_status;
   std::memset(ctx->host_status, 0, sizeof(s));
@@ -1107,6 +1105,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.events = (const int4*)ctx->events.p;
   a.redge = redge;
   a.st = st;
+  a.acc = ctx->acc;
   a.bg = bg;
   a.out = d_out;
   a.row_stride = row_stride;
@@ -1116,10 +1115,10 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   // no layer (the adjoint's own tile pass recomposites): the binned state only
   if (d_out)
     RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
-  RZ_TRY(check_launch(ctx, "scene pass 1"));
-  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  return RZ_OK;
+  else
+    RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), (const rzk::Status*)st,
+              ctx->acc);
+  return check_launch(ctx, "scene pass 1");
 }
 
 // The plain form: Pipeline::render_rec's Fill arm (src/scene.rs:410-430) run
@@ -1131,13 +1130,7 @@ rz_status enqueue_scene_plain(rz_ctx* ctx, const rz_dscene* s, const rz_transfor
   layer_of(view, &lx, &ly, &lw, &lh);
   if (lw == 0 || lh == 0) return RZ_OK;
   RZ_LAUNCH(ctx, "k_fill_rgba", rzk::k_fill_rgba, dim3(1024), dim3(256), d_out, lw, lh, row_stride, bg);
-  RZ_TRY(ensure(ctx, ctx->sticky, sizeof(rzk::Status)));
-  RZ_HIP(ctx, hipMemsetAsync(ctx->sticky.p, 0, sizeof(rzk::Status), ctx->stream));
-  ctx->sticky_st = (rzk::Status*)ctx->sticky.p;
-  struct Unset {
-    rz_ctx* c;
-    ~Unset() { c->sticky_st = nullptr; }
-  } unset{ctx};
+  // each path's run merges its status into the context's (enqueue_fine)
   const rzh::Tr T = rzh::tr_of(tr.m);
   for (uint32_t i = 0; i < s->n; ++i) {
     const rz_path hp = s->host_path(i);
@@ -1171,8 +1164,6 @@ rz_status enqueue_scene_plain(rz_ctx* ctx, const rz_dscene* s, const rz_transfor
     RZ_TRY(enqueue_fine(ctx, s->per[i], tm, flatness, S.rule, mode, (size_t)W, (size_t)H, o,
                         "k_fine_fill<0>", "k_fine_fill<1>"));
   }
-  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->sticky_st, sizeof(rzk::Status),
-                             hipMemcpyDeviceToHost, ctx->stream));
   return RZ_OK;
 }
 
@@ -1257,16 +1248,14 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   a.events = (const int4*)ctx->events.p;
   a.redge = redge;
   a.st = st;
+  a.acc = ctx->acc;
   a.out = d_atlas;
   a.row_stride = row_stride;
   if (rule == RZ_NONZERO)
     RZ_LAUNCH(ctx, "k_atlas_fine<0>", rzk::k_atlas_fine<0>, dim3(ct), dim3(64), a);
   else
     RZ_LAUNCH(ctx, "k_atlas_fine<1>", rzk::k_atlas_fine<1>, dim3(ct), dim3(64), a);
-  RZ_TRY(check_launch(ctx, "atlas pass 1"));
-  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
-                             ctx->stream));
-  return RZ_OK;
+  return check_launch(ctx, "atlas pass 1");
 }
 
 // Adjoint of the batched render (see rz_backward.hpp): the forward pipeline
@@ -1342,7 +1331,7 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
               (const rzk::Status*)st);
     if (grad_tr)
       RZ_LAUNCH(ctx, "k_sum_tr", rzk::k_sum_tr, dim3(1), dim3(rzk::kBlock),
-                (const double*)ctx->bw_tr.p, nwaves, grad_tr);
+                (const double*)ctx->bw_tr.p, nwaves, grad_tr, (const rzk::Status*)st);
   }
   return check_launch(ctx, "scene backward");
 }
@@ -1496,7 +1485,13 @@ rz_status rz_ctx_create(rz_ctx** out, int device) {
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->host_status, sizeof(rzk::Status), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->acc, sizeof(rzk::Status));
+  if (e == hipSuccess) e = hipMemset(ctx->acc, 0, sizeof(rzk::Status));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->xev, hipEventDisableTiming);
   if (e != hipSuccess) {
+    if (ctx->acc) (void)hipFree(ctx->acc);
+    if (ctx->host_status) (void)hipHostFree(ctx->host_status);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RZ_E_HIP;
   }
@@ -1511,7 +1506,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   DBuf* bufs[] = {&ctx->meta,      &ctx->tile_off,  &ctx->ev_off,    &ctx->sums,
                   &ctx->recs,      &ctx->events,    &ctx->out,       &ctx->lines,
-                  &ctx->counts,    &ctx->stops,     &ctx->sticky,    &ctx->line_path,
+                  &ctx->counts,    &ctx->stops,     &ctx->line_path,
                   &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
                   &ctx->sc_pi,     &ctx->sc_counts,
                   &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
@@ -1523,6 +1518,8 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->host_status) (void)hipHostFree(ctx->host_status);
+  if (ctx->acc) (void)hipFree(ctx->acc);
+  if (ctx->xev) (void)hipEventDestroy(ctx->xev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RZ_OK;
@@ -1531,6 +1528,20 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
 const char* rz_last_error(const rz_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
 void* rz_stream(rz_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+rz_status rz_wait_stream(rz_ctx* ctx, void* stream) {
+  if (!ctx) return RZ_E_ARG;
+  RZ_HIP(ctx, hipEventRecord(ctx->xev, (hipStream_t)stream));
+  RZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->xev, 0));
+  return RZ_OK;
+}
+
+rz_status rz_stream_wait_ctx(rz_ctx* ctx, void* stream) {
+  if (!ctx) return RZ_E_ARG;
+  RZ_HIP(ctx, hipEventRecord(ctx->xev, ctx->stream));
+  RZ_HIP(ctx, hipStreamWaitEvent((hipStream_t)stream, ctx->xev, 0));
+  return RZ_OK;
+}
 
 rz_status rz_sync(rz_ctx* ctx) {
   if (!ctx) return RZ_E_ARG;
@@ -1732,8 +1743,8 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
   rz_status st = RZ_OK;
   for (int attempt = 0; attempt < 3; ++attempt) {
     RZ_TRY(enqueue_bin(ctx, d, tr, flatness, edge_mode, width, height, &P));
-    RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, P.st, sizeof(rzk::Status), hipMemcpyDeviceToHost,
-                               ctx->stream));
+    RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64),
+              (const rzk::Status*)P.st, ctx->acc);
     st = finish(ctx);
     if (st != RZ_E_CAPACITY) break;
   }

@@ -468,8 +468,11 @@ __global__ __launch_bounds__(kBlock) void k_scene_bwd_lines(DPath p, Xform x, do
   }
 }
 
-// Sum the per-wave transform partials (one block).
-__global__ __launch_bounds__(kBlock) void k_sum_tr(const double* part, size_t nwaves, double* out) {
+// Sum the per-wave transform partials (one block). A run that failed or
+// outgrew its buffers wrote no partials, and adds nothing.
+__global__ __launch_bounds__(kBlock) void k_sum_tr(const double* part, size_t nwaves, double* out,
+                                                    const Status* st) {
+  if (st->err || st->overflow) return;
   __shared__ double sh[kBlock];
   for (int k = 0; k < 6; ++k) {
     double v = 0.0;

@@ -972,17 +972,25 @@ __global__ __launch_bounds__(kScatterBlock) void k_bin_scatter(BinBufs b, const 
   }
 }
 
-// Fold one run's status block into a sticky one (first error, overflow bits,
-// largest counts), for a sequence of runs that is checked once at the end.
-__global__ void k_status_merge(const Status* src, Status* acc) {
-  if (threadIdx.x != 0) return;
+// Fold one run's status block into the context's accumulated one (first
+// error, overflow bits, largest counts), so that every run queued before a
+// sync is checked there: a later run that fits cannot hide an earlier one
+// that did not. Called by one thread after all of the run's other kernels
+// (stream order), usually by the first thread of its last kernel.
+__device__ inline void merge_status(const Status* src, Status* acc) {
   if (!acc->err) acc->err = src->err;
   acc->overflow |= src->overflow;
   acc->n_records = max(acc->n_records, src->n_records);
   acc->n_events = max(acc->n_events, src->n_events);
   acc->n_lines = max(acc->n_lines, src->n_lines);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) acc->pad[k] = max(acc->pad[k], src->pad[k]);
   acc->n_list = max(acc->n_list, src->n_list);
   acc->n_line_part = max(acc->n_line_part, src->n_line_part);
+}
+
+__global__ void k_status_merge(const Status* src, Status* acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) merge_status(src, acc);
 }
 
 // Fill a layer with its background colour (Layer::new, src/scene.rs:483-501).
@@ -1349,6 +1357,7 @@ struct FineArgs {
   const int4* events;
   const int32_t* redge;  // [H] right-edge column per row (2^-20 fixed point)
   const Status* st;
+  Status* acc;           // the context's accumulated status (merged by block 0)
   float* out;
   size_t row_stride;
   int W, H, tiles_x, tiles_y;
@@ -1404,6 +1413,7 @@ struct FillOut {
 // coalesced stores.
 template <int RULE, class Out>
 __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];

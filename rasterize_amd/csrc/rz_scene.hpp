@@ -287,6 +287,7 @@ struct SceneFineArgs {
   const int4* events;
   const int32_t* redge;
   const Status* st;
+  Status* acc;        // accumulated status (merged by block 0), or null
   rzp::C4 bg;
   rzp::C4* out;       // layer image
   size_t row_stride;  // pixels
@@ -308,6 +309,7 @@ struct SceneFineArgs {
 // of its right-edge column has zero winding on the whole tile (alpha 0
 // everywhere) and is skipped.
 __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];
@@ -507,6 +509,7 @@ struct AtlasFineArgs {
   const int4* events;
   const int32_t* redge;
   const Status* st;
+  Status* acc;        // accumulated status (merged by block 0)
   float* out;
   size_t row_stride;  // floats
 };
@@ -515,6 +518,7 @@ struct AtlasFineArgs {
 // written into the glyph's atlas view.
 template <int RULE>
 __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
   __shared__ int acc[256];
   __shared__ int rowd[18];

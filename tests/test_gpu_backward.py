@@ -134,3 +134,143 @@ def test_zero_upstream_gradient_gives_zero():
     ds.ctx.check(rz._abi.lib().rz_sync(ds.ctx.h))
     assert float(gp.abs().sum()) == 0.0 and float(gc.abs().sum()) == 0.0
     assert float(gtr.abs().sum()) == 0.0
+
+
+def _busy_torch_stream(torch):
+    """Keep torch's current stream busy for a few milliseconds; returns a
+    zero-valued scalar tensor that is ready only when that work is done."""
+    x = torch.ones(1 << 26, device="cuda")
+    for _ in range(40):
+        x = x * 1.0000001
+    return x[:1] * 0.0
+
+
+def test_render_torch_orders_after_torch_stream():
+    """Inputs produced by kernels still queued on torch's stream, and gradient
+    buffers torch has just cleared, are seen by the library's stream."""
+    import torch
+    W = H = 96
+    ds = blob_scene(17, 5, 6, W)
+    dev = DeviceScene(ds.scene())
+    idx = torch.tensor(ds.index, device="cuda")
+    view = (0.0, 0.0, float(W), float(H))
+    U0 = torch.tensor(ds.points, dtype=torch.float64, device="cuda")
+    C0 = torch.tensor(ds.colors, dtype=torch.float32, device="cuda")
+    # reference: everything settled before each call
+    Ur = U0.clone().requires_grad_(True)
+    Cr = C0.clone().requires_grad_(True)
+    ref = render_torch(dev, Ur[idx], Cr, Transform.identity(), view)
+    (ref ** 2).sum().backward()
+    torch.cuda.synchronize()
+    for trial in range(3):
+        z = _busy_torch_stream(torch)
+        U = (U0 + z.to(torch.float64)).requires_grad_(True)
+        Cc = (C0 + z).detach().requires_grad_(True)
+        out = render_torch(dev, U[idx], Cc, Transform.identity(), view)
+        z2 = _busy_torch_stream(torch)
+        ((out + z2) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), f"trial {trial}: forward differs"
+        assert torch.allclose(U.grad, Ur.grad, rtol=1e-4, atol=1e-4), f"trial {trial}: dL/dpoints"
+        assert torch.allclose(Cc.grad, Cr.grad, rtol=1e-4, atol=1e-4), f"trial {trial}: dL/dcolors"
+
+
+def test_backward_uses_its_forward_state():
+    """A second forward with other points before loss.backward() of the first
+    must not change the first's gradients."""
+    import torch
+    W = H = 96
+    ds = blob_scene(23, 4, 6, W)
+    dev = DeviceScene(ds.scene())
+    idx = torch.tensor(ds.index, device="cuda")
+    view = (0.0, 0.0, float(W), float(H))
+    rng = np.random.default_rng(2)
+    target = torch.tensor(rng.uniform(0, 0.4, (H, W, 4)), dtype=torch.float32, device="cuda")
+
+    def grads(second):
+        U = torch.tensor(ds.points, dtype=torch.float64, device="cuda", requires_grad=True)
+        Cc = torch.tensor(ds.colors, dtype=torch.float32, device="cuda", requires_grad=True)
+        out = render_torch(dev, U[idx], Cc, Transform.identity(), view)
+        loss = ((out - target) ** 2).sum()
+        if second:
+            with torch.no_grad():
+                moved = torch.tensor(ds.points + 7.0, dtype=torch.float64, device="cuda")
+                render_torch(dev, moved[idx], Cc * 0.5, Transform.identity(), view)
+        loss.backward()
+        return U.grad.cpu().numpy(), Cc.grad.cpu().numpy()
+
+    gu0, gc0 = grads(False)
+    gu1, gc1 = grads(True)
+    _say(f"max |dgrad| points {np.abs(gu1 - gu0).max():.3e} colours {np.abs(gc1 - gc0).max():.3e}")
+    assert np.allclose(gu1, gu0, rtol=1e-4, atol=1e-3 * np.abs(gu0).max())
+    assert np.allclose(gc1, gc0, rtol=1e-4, atol=1e-3 * np.abs(gc0).max())
+
+
+def test_transform_tensor_gradient_through_render_torch():
+    """render_torch with a 6-element transform tensor returns dL/d(transform),
+    the same values rz_render_backward_device adds to d_grad_tr."""
+    import torch
+    W = H = 80
+    ds = blob_scene(11, 4, 5, W)
+    dev = DeviceScene(ds.scene())
+    idx = torch.tensor(ds.index, device="cuda")
+    view = (0.0, 0.0, float(W), float(H))
+    m = [0.9, 0.15, 3.0, -0.1, 1.05, -2.0]
+    trt = torch.tensor(m, dtype=torch.float64, device="cuda", requires_grad=True)
+    U = torch.tensor(ds.points, dtype=torch.float64, device="cuda")
+    Cc = torch.tensor(ds.colors, dtype=torch.float32, device="cuda")
+    out = render_torch(dev, U[idx], Cc, trt, view)
+    target = torch.full_like(out, 0.2)
+    ((out - target) ** 2).sum().backward()
+    g = (2.0 * (out.detach() - target)).contiguous()
+    gp = torch.zeros((dev.n_points(), 2), dtype=torch.float32, device="cuda")
+    gc = torch.zeros((4, 4), dtype=torch.float32, device="cuda")
+    gtr = torch.zeros(6, dtype=torch.float64, device="cuda")
+    dev.update_device(U[idx].contiguous().data_ptr(), Cc.data_ptr())
+    dev.backward_device(Transform(*m), view, g.data_ptr(), W, gp.data_ptr(), gc.data_ptr(),
+                        gtr.data_ptr())
+    dev.ctx.check(rz._abi.lib().rz_sync(dev.ctx.h))
+    _say(f"render_torch dL/dtr {trt.grad.cpu().numpy()} raw {gtr.cpu().numpy()}")
+    assert torch.allclose(trt.grad, gtr, rtol=1e-5, atol=1e-6 * float(gtr.abs().max()))
+
+
+def test_backward_capacity_miss_leaves_transform_gradient():
+    """A backward on a fresh context outgrows its buffers (RZ_E_CAPACITY):
+    d_grad_tr must be left as it was; the re-run then adds the gradient once."""
+    import torch
+    W = H = 512
+    ds, _, _ = rz.workload_diff(9, 128, 32, 512.0)  # about 200k lines: more than a first call holds
+    ctx = rz.Context(0)
+    dev = DeviceScene(ds.scene(), ctx=ctx)
+    view = (0.0, 0.0, float(W), float(H))
+    g = torch.full((H, W, 4), 0.1, dtype=torch.float32, device="cuda")
+    gp = torch.zeros((dev.n_points(), 2), dtype=torch.float32, device="cuda")
+    gc = torch.zeros((128, 4), dtype=torch.float32, device="cuda")
+    gtr = torch.zeros(6, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L = rz._abi.lib()
+    dev.backward_device(Transform.identity(), view, g.data_ptr(), W, gp.data_ptr(), gc.data_ptr(),
+                        gtr.data_ptr())
+    s = L.rz_sync(ctx.h)
+    _say(f"first backward status {s}, |gtr| {float(gtr.abs().sum()):.3e}")
+    assert s == rz._abi.RZ_E_CAPACITY
+    assert float(gtr.abs().sum()) == 0.0
+    assert float(gp.abs().sum()) == 0.0 and float(gc.abs().sum()) == 0.0
+    for _ in range(3):
+        dev.backward_device(Transform.identity(), view, g.data_ptr(), W, gp.data_ptr(),
+                            gc.data_ptr(), gtr.data_ptr())
+        s = L.rz_sync(ctx.h)
+        if s != rz._abi.RZ_E_CAPACITY:
+            break
+    ctx.check(s)
+    once = gtr.clone()
+    gtr.zero_()
+    gp.zero_()
+    gc.zero_()
+    dev.backward_device(Transform.identity(), view, g.data_ptr(), W, gp.data_ptr(), gc.data_ptr(),
+                        gtr.data_ptr())
+    ctx.check(L.rz_sync(ctx.h))
+    assert float(once.abs().sum()) > 0.0
+    assert torch.allclose(once, gtr, rtol=1e-6, atol=1e-9 * float(gtr.abs().max()))
+    dev.close()
+    ctx.close()

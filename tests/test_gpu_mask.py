@@ -340,3 +340,44 @@ def test_error_vs_exact_area(c1):
         _say(f"rule={rule}: gpu vs exact max {dg.max():.3e} mean {dg.mean():.3e}; "
              f"reference vs exact max {dr.max():.3e} mean {dr.mean():.3e}")
         assert abs(dg.mean() - dr.mean()) < 1e-5
+
+
+def test_queued_capacity_miss_is_reported(c1, c2):
+    """Two asynchronous masks queued before one rz_sync on a fresh context: the
+    first (200k-segment path, 6.6 M lines) outgrows the first-call buffers,
+    the second (64 cubics) fits. rz_sync must still report RZ_E_CAPACITY
+    (the first call's mask was not written); after a re-run of both, both
+    masks are right."""
+    import torch
+    L = rz._abi.lib()
+    big, tr2 = c2
+    ctx = rz.Context(0)
+    d_big, d_small = C.c_void_p(), C.c_void_p()
+    try:
+        ctx.check(L.rz_path_upload(ctx.h, C.byref(big.abi()), C.byref(d_big)))
+        ctx.check(L.rz_path_upload(ctx.h, C.byref(c1.abi()), C.byref(d_small)))
+        ob = torch.zeros((4096, 4096), dtype=torch.float32, device="cuda")
+        os_ = torch.zeros((512, 512), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+
+        def run():
+            ctx.check(L.rz_mask_device(ctx.h, d_big, tr2.abi(), 0.05, 0, rz._abi.RZ_EDGE_MASK, 4096,
+                                       4096, C.c_void_p(ob.data_ptr()), 4096))
+            ctx.check(L.rz_mask_device(ctx.h, d_small, I.abi(), 0.05, 0, rz._abi.RZ_EDGE_MASK, 512,
+                                       512, C.c_void_p(os_.data_ptr()), 512))
+        _say("queued big + small masks on a fresh context")
+        run()
+        s = L.rz_sync(ctx.h)
+        assert s == rz._abi.RZ_E_CAPACITY, f"expected RZ_E_CAPACITY, got {s}"
+        for _ in range(3):
+            run()
+            s = L.rz_sync(ctx.h)
+            if s != rz._abi.RZ_E_CAPACITY:
+                break
+        ctx.check(s)
+        check_close(os_.cpu().numpy(), O.sd_mask(c1, I, 512, 512, 0), "small mask after re-run")
+        check_close(ob.cpu().numpy(), O.sd_mask(big, tr2, 4096, 4096, 0), "big mask after re-run")
+    finally:
+        L.rz_path_free(ctx.h, d_big)
+        L.rz_path_free(ctx.h, d_small)
+        ctx.close()
