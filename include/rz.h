@@ -186,6 +186,25 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr, do
                          int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
                          size_t row_stride);
 
+/* No-alloc form of rz_mask_device. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It calls no
+ * hipMalloc, hipFree or allocating library routine: the pipeline's buffers
+ * are carved from one 64 MiB array per GPU (a file-scope __device__ array
+ * of librz, held until the process ends) instead of the context's. The path
+ * must already be resident (rz_path_upload, done at set-up) and the caller
+ * passes the output. A size that cannot fit (the per-tile, per-band and
+ * per-flatten-wave parts of the call) is refused with RZ_E_CAPACITY before
+ * anything is enqueued; a path that flattens into more lines than the rest
+ * of the array holds (about 600k lines for a 4096x4096 mask) is reported at
+ * rz_sync as RZ_E_CAPACITY, with the output not written and nothing grown.
+ * All no-alloc calls on one GPU share its array: calls that overlap on one
+ * GPU (other contexts, other streams) race; calls on one context's stream
+ * are ordered. */
+rz_status rz_mask_device_noalloc(rz_ctx* ctx, const rz_dpath* dpath, rz_transform tr,
+                                 double flatness, int fill_rule, int edge_mode, size_t width,
+                                 size_t height, float* d_out, size_t row_stride);
+
 /* ---- Rasterizer::fill ------------------------------------------------------ */
 
 /* Rasterizer::fill (src/rasterize.rs:69-115): the pixels mask_iter yields

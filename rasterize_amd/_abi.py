@@ -105,6 +105,8 @@ PROTOTYPES = [
     ("rz_path_free", C.c_int, [c_ctx, c_dpath]),
     ("rz_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
                                  C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
+    ("rz_mask_device_noalloc", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int,
+                                         C.c_int, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t]),
     ("rz_fill", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_int,
                           P(rz_paint), P(C.c_float), C.c_size_t, rz_shape]),
     ("rz_fill_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,

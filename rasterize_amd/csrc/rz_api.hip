@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,35 @@ struct DBuf {
   size_t cap = 0;
 };
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-alloc entry point (rz_mask_device_noalloc): the pipeline
+// buffers of one call, carved from one fixed array. Each GPU the library runs
+// on holds its own copy, allocated with the code object and kept until the
+// process ends. Every no-alloc call on a GPU shares that GPU's array (calls
+// that overlap on one GPU race); each run clears what must start at zero.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on `device`, which must be current: looked
+// up once per device with hipGetSymbolAddress. Host code uses only this
+// address, never the array's name (a host-side stand-in).
+char* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = (char*)p;
+  }
+  return g_scratch_ptr[device];
+}
 }  // namespace
 
 struct rz_ctx {
@@ -50,8 +80,13 @@ struct rz_ctx {
   DBuf atlas_tiles;  // per atlas tile: what the fine stage needs (AtlasTile)
   uint32_t rec_cap = 0, ev_cap = 0;
   uint32_t part_cap = 0;  // overflow record-list entries per part
-  rzk::Status* host_status = nullptr;  // pinned
-  rzk::Status* acc = nullptr;          // device: status of the runs since the last sync, merged
+  rzk::Status* host_status = nullptr;  // pinned, two blocks (the copies of acc[0], acc[1])
+  // device, two blocks: the merged status of the runs since the last sync,
+  // [0] of runs on the context's own buffers, [1] of no-alloc runs
+  rzk::Status* acc = nullptr;
+  rzk::Status* acc_run = nullptr;      // where the runs being enqueued merge (acc or acc + 1)
+  bool noalloc = false;                // buffers are carved from `scratch`: ensure() never allocates
+  char* scratch_ptr = nullptr;         // device address of this device's `scratch`
   hipEvent_t xev = nullptr;            // orders the context's stream with a caller's stream
   rzk::Status last{};                  // status of the last run collected by finish()
   // launch parameters (rz_set_param)
@@ -105,6 +140,10 @@ rz_status fail(rz_ctx* ctx, rz_status s, const std::string& msg) {
 
 rz_status ensure(rz_ctx* ctx, DBuf& b, size_t bytes) {
   if (b.cap >= bytes && b.p) return RZ_OK;
+  if (ctx->noalloc)
+    return fail(ctx, RZ_E_CAPACITY, "no-alloc call: a buffer of " + std::to_string(bytes) +
+                                        " bytes does not fit its part of the scratch (" +
+                                        std::to_string(b.cap) + " bytes)");
   size_t want = std::max<size_t>(bytes, 256);
   if (b.p) {
     // a previous asynchronous call may still read the old buffer
@@ -632,7 +671,7 @@ rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   a.band_ev_off = P.ev_off;
   a.events = (const int4*)ctx->events.p;
   a.st = P.st;
-  a.acc = ctx->acc;  // the fine kernel merges the run's status (read at rz_sync)
+  a.acc = ctx->acc_run;  // the fine kernel merges the run's status (read at rz_sync)
   a.W = (int)W;
   a.H = (int)H;
   a.tiles_x = P.grid.g.tiles_x;
@@ -698,18 +737,27 @@ rz_status enqueue_fill(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
 }
 
 // Wait, then interpret the status merged over every run since the last
-// finish: errors, and capacity growth. The accumulated block starts over.
+// finish: errors, and capacity growth. The accumulated blocks start over.
+// No-alloc runs (their own block) never grow anything: a capacity miss there
+// means the input does not fit the scratch.
 rz_status finish(rz_ctx* ctx) {
-  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->acc, sizeof(rzk::Status), hipMemcpyDeviceToHost,
-                             ctx->stream));
+  RZ_HIP(ctx, hipMemcpyAsync(ctx->host_status, ctx->acc, 2 * sizeof(rzk::Status),
+                             hipMemcpyDeviceToHost, ctx->stream));
   RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3(1), dim3(64), (uint32_t*)ctx->acc,
-            (size_t)rzk::kStatusWords, 0u);
+            (size_t)(2 * rzk::kStatusWords), 0u);
   RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  rzk::Status s = *ctx->host_status;
-  std::memset(ctx->host_status, 0, sizeof(s));
-  ctx->last = s;
-  if (s.err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
-  if (s.err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
+  rzk::Status s = ctx->host_status[0];
+  const rzk::Status sn = ctx->host_status[1];
+  std::memset(ctx->host_status, 0, 2 * sizeof(s));
+  ctx->last = s.n_lines || !sn.n_lines ? s : sn;
+  const uint32_t err = s.err ? s.err : sn.err;
+  if (err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
+  if (err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
+  if (sn.overflow && !s.overflow)
+    return fail(ctx, RZ_E_CAPACITY,
+                "a no-alloc call outgrew the scratch (" + std::to_string(sn.n_lines) +
+                    " lines, " + std::to_string(sn.n_records) + " records); its output was not "
+                    "written");
   if (s.overflow) {
     if (s.n_records > ctx->rec_cap) {
       ctx->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_records * 5 / 4 + 1024, 0xffffffffu);
@@ -1105,7 +1153,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.events = (const int4*)ctx->events.p;
   a.redge = redge;
   a.st = st;
-  a.acc = ctx->acc;
+  a.acc = ctx->acc_run;
   a.bg = bg;
   a.out = d_out;
   a.row_stride = row_stride;
@@ -1117,7 +1165,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
     RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
   else
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), (const rzk::Status*)st,
-              ctx->acc);
+              ctx->acc_run);
   return check_launch(ctx, "scene pass 1");
 }
 
@@ -1248,7 +1296,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   a.events = (const int4*)ctx->events.p;
   a.redge = redge;
   a.st = st;
-  a.acc = ctx->acc;
+  a.acc = ctx->acc_run;
   a.out = d_atlas;
   a.row_stride = row_stride;
   if (rule == RZ_NONZERO)
@@ -1484,10 +1532,12 @@ rz_status rz_ctx_create(rz_ctx** out, int device) {
   ctx->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->host_status, sizeof(rzk::Status), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc((void**)&ctx->acc, sizeof(rzk::Status));
-  if (e == hipSuccess) e = hipMemset(ctx->acc, 0, sizeof(rzk::Status));
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&ctx->host_status, 2 * sizeof(rzk::Status), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&ctx->acc, 2 * sizeof(rzk::Status));
+  if (e == hipSuccess) e = hipMemset(ctx->acc, 0, 2 * sizeof(rzk::Status));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->xev, hipEventDisableTiming);
+  if (e == hipSuccess) ctx->scratch_ptr = noalloc_scratch(device);
   if (e != hipSuccess) {
     if (ctx->acc) (void)hipFree(ctx->acc);
     if (ctx->host_status) (void)hipHostFree(ctx->host_status);
@@ -1495,7 +1545,8 @@ rz_status rz_ctx_create(rz_ctx** out, int device) {
     delete ctx;
     return RZ_E_HIP;
   }
-  std::memset(ctx->host_status, 0, sizeof(rzk::Status));
+  std::memset(ctx->host_status, 0, 2 * sizeof(rzk::Status));
+  ctx->acc_run = ctx->acc;
   *out = ctx;
   return RZ_OK;
 }
@@ -1630,6 +1681,98 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double
   return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width, height, d_out, row_stride);
 }
 
+// The no-alloc form of rz_mask_device: the same pipeline on buffers carved
+// from this GPU's `scratch` instead of the context's own (which are left as
+// they are). The fixed parts (per tile, per band, per row, per flatten wave,
+// events, overflow list) are sized from the call; the rest is shared out
+// per line (f64 line, record, two list entries). Inputs that cannot fit are
+// refused here; a path that flattens into more lines than fit is reported at
+// rz_sync (RZ_E_CAPACITY, nothing written, nothing grown).
+rz_status rz_mask_device_noalloc(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
+                                 int fill_rule, int edge_mode, size_t width, size_t height,
+                                 float* d_out, size_t row_stride) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, width, height));
+  if (!d || (!d_out && width && height)) return fail(ctx, RZ_E_ARG, "null argument");
+  if (row_stride < width) return fail(ctx, RZ_E_ARG, "row_stride < width");
+  if (width == 0 || height == 0) return RZ_OK;
+  if (!ctx->scratch_ptr)
+    return fail(ctx, RZ_E_HIP, "no-alloc scratch: hipGetSymbolAddress failed at context creation");
+  // the sizes enqueue_mask will ask for, as its helpers compute them
+  const Grid gr = make_grid(edge_mode, width, height);
+  const size_t tiles = gr.tiles, ty = (size_t)gr.g.tiles_y;
+  const size_t meta_u32 =
+      rzk::kStatusWords + 2 * tiles + ty + (height + 2) + rzk::kParts + 1 + rzk::kLineParts;
+  int dq, dc;
+  split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
+  const size_t nblk = (size_t)d->view.blocks_line + ((size_t)d->view.blocks_quad << dq) +
+                      ((size_t)d->view.blocks_cubic << dc);
+  const size_t nw = nblk * (rzk::kBlock / 64);
+  const uint32_t ev_cap = 1u << 14, part_cap = 1u << 12;
+  auto scan_blocks = [](size_t n) {
+    return std::max<size_t>(1, (n + rzk::kScanTile - 1) / rzk::kScanTile);
+  };
+  const size_t sums = std::max(scan_blocks(tiles), scan_blocks(ty)) * 4;
+  const size_t no = (size_t)rzk::kParts * part_cap;
+  const size_t list_fixed = no * 24 + (size_t)ev_cap * 16 +
+                            (no + ev_cap + nw + nw * rzk::kMap) * 4;  // + 48 per line
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t fixed = al(meta_u32 * 4) + al((tiles + 1) * 4) + al((ty + 1) * 4) + al(sums) +
+                       al(std::max<size_t>(nw, 1) * 8) + al((size_t)ev_cap * 16) +
+                       al(list_fixed) + al(tiles * 16) + 4 * 256;
+  const size_t unit = (size_t)rzk::kLineParts * 64;  // line capacity comes in whole parts
+  const size_t per_line = 32 + 16 + 48;
+  if (fixed + unit * per_line > kNoAllocBytes || tiles + unit > 0xffffffffull)
+    return fail(ctx, RZ_E_CAPACITY,
+                "no-alloc call: a " + std::to_string(width) + "x" + std::to_string(height) +
+                    " mask of this path needs more than the " +
+                    std::to_string(kNoAllocBytes >> 20) + " MiB scratch");
+  const size_t line_cap = std::min<size_t>((kNoAllocBytes - fixed) / per_line / unit * unit,
+                                           (size_t)1 << 30);
+  const size_t rec_cap = line_cap + tiles;
+  // carve, in order; every part starts 256-byte aligned
+  char* p = ctx->scratch_ptr;
+  auto take = [&](size_t bytes) {
+    DBuf b;
+    b.p = p;
+    b.cap = al(bytes);
+    p += b.cap;
+    return b;
+  };
+  struct Saved {
+    rz_ctx* c;
+    DBuf meta, tile_off, ev_off, sums, lines, recs, events, list_buf, wave_lines;
+    uint32_t line_cap, rec_cap, ev_cap, part_cap;
+    ~Saved() {
+      c->meta = meta; c->tile_off = tile_off; c->ev_off = ev_off; c->sums = sums;
+      c->lines = lines; c->recs = recs; c->events = events; c->list_buf = list_buf;
+      c->wave_lines = wave_lines;
+      c->line_cap = line_cap; c->rec_cap = rec_cap; c->ev_cap = ev_cap; c->part_cap = part_cap;
+      c->noalloc = false;
+      c->acc_run = c->acc;
+    }
+  } saved{ctx,          ctx->meta,   ctx->tile_off, ctx->ev_off,   ctx->sums,
+          ctx->lines,   ctx->recs,   ctx->events,   ctx->list_buf, ctx->wave_lines,
+          ctx->line_cap, ctx->rec_cap, ctx->ev_cap, ctx->part_cap};
+  ctx->noalloc = true;
+  ctx->acc_run = ctx->acc + 1;
+  ctx->meta = take(meta_u32 * 4);
+  ctx->tile_off = take((tiles + 1) * 4);
+  ctx->ev_off = take((ty + 1) * 4);
+  ctx->sums = take(sums);
+  ctx->wave_lines = take(std::max<size_t>(nw, 1) * 8);
+  ctx->events = take((size_t)ev_cap * 16);
+  ctx->list_buf = take(list_fixed + 48 * line_cap);
+  ctx->recs = take(rec_cap * 16);
+  ctx->lines = take(line_cap * 32);
+  ctx->line_cap = (uint32_t)line_cap;
+  ctx->rec_cap = (uint32_t)rec_cap;
+  ctx->ev_cap = ev_cap;
+  ctx->part_cap = part_cap;
+  if ((size_t)(p - ctx->scratch_ptr) > kNoAllocBytes)
+    return fail(ctx, RZ_E_CAPACITY, "no-alloc call: scratch layout overflow");
+  return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width, height, d_out, row_stride);
+}
+
 rz_status rz_fill_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
                          int fill_rule, int edge_mode, const rz_paint* paint, size_t width,
                          size_t height, float* d_rgba, size_t row_stride) {
@@ -1744,7 +1887,7 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
   for (int attempt = 0; attempt < 3; ++attempt) {
     RZ_TRY(enqueue_bin(ctx, d, tr, flatness, edge_mode, width, height, &P));
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64),
-              (const rzk::Status*)P.st, ctx->acc);
+              (const rzk::Status*)P.st, ctx->acc_run);
     st = finish(ctx);
     if (st != RZ_E_CAPACITY) break;
   }
