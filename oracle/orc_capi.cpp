@@ -4,9 +4,11 @@
 //
 // The path argument has the same memory layout as the product's rz_path
 // (include/rz.h) so the same host arrays feed both.
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include "orc_dual.hpp"
 #include "orc_exact.hpp"
 #include "orc_paint.hpp"
 
@@ -267,6 +269,99 @@ int orc_scene_render(size_t n, const orc_path* paths, const orc_paint* paints, c
   layer_xywh[3] = (int64_t)layer.shape.height;
   if (layer.image.size() > cap_px) return ORC_E_CAP;
   std::memcpy(out, layer.image.data(), layer.image.size() * sizeof(F4));
+  return ORC_OK;
+}
+
+// orc_scene_render on `threads` host threads (bands of layer rows; the same
+// pixels and values). Used for the full-size scene checks and as the
+// multi-core CPU baseline (an upper bound: the reference is single-threaded).
+int orc_scene_render_mt(size_t n, const orc_path* paths, const orc_paint* paints,
+                        const int32_t* rules, const double* tr, const double* view,
+                        const float* bg, double flatness, float* out, size_t cap_px,
+                        int64_t* layer_xywh, int threads) {
+  std::vector<SceneRef> fills;
+  for (size_t i = 0; i < n; ++i) {
+    auto node = std::make_shared<SceneNode>();
+    node->kind = SceneNode::FILL;
+    node->path = std::make_shared<Path>(to_path(&paths[i]));
+    node->paint = std::make_shared<Paint>(to_paint(&paints[i]));
+    node->rule = (FillRule)rules[i];
+    fills.push_back(node);
+  }
+  auto root = std::make_shared<SceneNode>();
+  root->kind = SceneNode::GROUP;
+  root->children = fills;
+  BBox vb;
+  if (view) vb = BBox::make(P(view[0], view[1]), P(view[2], view[3]));
+  LinColor bgc;
+  if (bg) bgc = LinColor{f4(bg[0], bg[1], bg[2], bg[3])};
+  Layer layer;
+  try {
+    layer = scene_render_mt(root.get(), to_tr(tr), view ? &vb : nullptr, bg ? &bgc : nullptr,
+                            flatness, threads);
+  } catch (const std::runtime_error&) {
+    return ORC_E_NAN;
+  }
+  layer_xywh[0] = layer.x;
+  layer_xywh[1] = layer.y;
+  layer_xywh[2] = (int64_t)layer.shape.width;
+  layer_xywh[3] = (int64_t)layer.shape.height;
+  if (layer.image.size() > cap_px) return ORC_E_CAP;
+  std::memcpy(out, layer.image.data(), layer.image.size() * sizeof(F4));
+  return ORC_OK;
+}
+
+// A glyph atlas: for each glyph i, SignedDifferenceRasterizer::mask of
+// paths[i] under tr[6i..6i+6] into a zeroed w x h image, copied into the
+// atlas (row-major f32, atlas_w wide) at (x, y) = xywh[4i..4i+4]. Glyphs are
+// shared out over `threads` host threads. Pixels outside the views are left.
+int orc_mask_atlas_mt(size_t n, const orc_path* paths, const double* tr, const uint32_t* xywh,
+                      double flatness, int rule, float* atlas, size_t atlas_w, size_t atlas_h,
+                      int threads) {
+  std::atomic<size_t> next{0};
+  std::atomic<int> rc{ORC_OK};
+  auto work = [&]() {
+    std::vector<double> img;
+    for (size_t i; (i = next.fetch_add(1)) < n;) {
+      const uint32_t* q = xywh + 4 * i;
+      const size_t x = q[0], y = q[1], w = q[2], h = q[3];
+      if (!w || !h) continue;
+      if (x + w > atlas_w || y + h > atlas_h) {
+        rc = ORC_E_ARG;
+        continue;
+      }
+      img.assign(w * h, 0.0);
+      try {
+        Path path = to_path(&paths[i]);
+        ImageF64 im{img.data(), img.size(), Shape::simple(h, w)};
+        sd_mask(path, to_tr(tr + 6 * i), flatness, (FillRule)rule, im);
+      } catch (const std::runtime_error&) {
+        rc = ORC_E_NAN;
+        continue;
+      }
+      for (size_t r = 0; r < h; ++r)
+        for (size_t c = 0; c < w; ++c) atlas[(y + r) * atlas_w + x + c] = (float)img[r * w + c];
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  return rc;
+}
+
+// SignedDifferenceRasterizer::mask_iter as a dense W x H image (pixels it
+// does not yield are 0) and its derivative along `tan` (per stored point of
+// the path: dx, dy), decisions fixed at the values (see orc_dual.hpp).
+int orc_sd_mask_iter_dual(const orc_path* p, const double* tan, const double* tr,
+                          double flatness, int rule, size_t W, size_t H, double* alpha,
+                          double* dalpha) {
+  Path path = to_path(p);
+  try {
+    sd_mask_iter_dual(path, tan, to_tr(tr), flatness, (FillRule)rule, W, H, alpha, dalpha);
+  } catch (const std::runtime_error&) {
+    return ORC_E_NAN;
+  }
   return ORC_OK;
 }
 

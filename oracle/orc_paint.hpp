@@ -6,6 +6,8 @@
 // default x86+simd build (features ["serde","simd"]): l2s/s2l are the SSE
 // polynomials applied to all four lanes, alpha included.
 #pragma once
+#include <atomic>
+#include <thread>
 #include <memory>
 #include "orc_raster.hpp"
 
@@ -239,17 +241,25 @@ struct ImageRGBA {
 
 // Rasterizer::fill default + fill_impl (src/rasterize.rs:69-115) with the
 // signed-difference mask_iter.
+// Rows [r0, r1) of the image only (all of it by default), through
+// sd_mask_iter_rows: the same pixels and values.
 inline void sd_fill(const Path& path, const Transform& tr, double flatness, FillRule rule,
-                    const Paint& paint, ImageRGBA img) {
+                    const Paint& paint, ImageRGBA img, size_t r0 = 0, size_t r1 = SIZE_MAX) {
   auto blend = [&](const Pixel& px, LinColor color) {
     F4& dst = img.data[img.shape.offset(px.y, px.x)];
     LinColor d{dst};
     dst = d.blend_over(color.with_alpha(px.alpha)).c;
   };
+  const bool whole = r0 == 0 && r1 >= img.shape.height;
+  auto mask_iter = [&](auto&& sink) {
+    if (whole)
+      sd_mask_iter(path, tr, flatness, rule, img.shape.width, img.shape.height, sink);
+    else
+      sd_mask_iter_rows(path, tr, flatness, rule, img.shape.width, img.shape.height, r0, r1, sink);
+  };
   if (paint.kind == PAINT_SOLID) {
     LinColor color = paint.at(P(0.0, 0.0));
-    sd_mask_iter(path, tr, flatness, rule, img.shape.width, img.shape.height,
-                 [&](const Pixel& px) { blend(px, color); });
+    mask_iter([&](const Pixel& px) { blend(px, color); });
     return;
   }
   Transform units_tr;
@@ -262,7 +272,7 @@ inline void sd_fill(const Path& path, const Transform& tr, double flatness, Fill
   }
   Transform pixel_tr;
   if (!units_tr.invert(&pixel_tr)) return;
-  sd_mask_iter(path, tr, flatness, rule, img.shape.width, img.shape.height, [&](const Pixel& px) {
+  mask_iter([&](const Pixel& px) {
     Point pt = P((double)px.x + 0.5, (double)px.y + 0.5);
     blend(px, paint.at(pixel_tr.apply(pt)));
   });
@@ -364,7 +374,62 @@ struct Pipeline {
     ImageRGBA img{layer.image.data(), layer.image.size(), vs};
     sd_fill(*n.fill->path, align * n.tr, flatness, n.fill->rule, *n.fill->paint, img);
   }
+  // render_rec for the layer rows [y0, y1) only: each fill runs over the rows
+  // of its sub-view inside the band (sd_fill's row window).
+  void render_rec_rows(size_t id, Layer& layer, double flatness, size_t y0, size_t y1) const {
+    const PipelineNode& n = nodes[id];
+    if (n.kind == SceneNode::GROUP) {
+      for (size_t c : n.children) render_rec_rows(c, layer, flatness, y0, y1);
+      return;
+    }
+    long col_min = (long)std::floor(n.bbox.min.x) - layer.x;
+    long col_max = (long)std::ceil(n.bbox.max.x) - layer.x + 1;
+    long row_min = (long)std::floor(n.bbox.min.y) - layer.y;
+    long row_max = (long)std::ceil(n.bbox.max.y) - layer.y + 1;
+    Shape vs = view_shape(layer.shape, (size_t)row_min, (size_t)row_max, (size_t)col_min,
+                          (size_t)col_max);
+    // the sub-view's first layer row (view_shape's clamped row_min)
+    const size_t first = std::min((size_t)row_min, layer.shape.height);
+    const size_t a = std::max(y0, first), b = std::min(y1, first + vs.height);
+    if (a >= b) return;
+    Transform align =
+        Transform::new_translate(-std::floor(n.bbox.min.x), -std::floor(n.bbox.min.y));
+    ImageRGBA img{layer.image.data(), layer.image.size(), vs};
+    sd_fill(*n.fill->path, align * n.tr, flatness, n.fill->rule, *n.fill->paint, img, a - first,
+            b - first);
+  }
 };
+
+// Scene::render as scene_render below, the layer split into bands of rows
+// rendered by `threads` host threads (each pixel belongs to one band, so the
+// threads never write the same pixel). Same pixels, same values.
+inline Layer scene_render_mt(const SceneNode* scene, const Transform& tr, const BBox* view,
+                             const LinColor* bg, double flatness, int threads, size_t band = 16) {
+  Pipeline p;
+  size_t root;
+  if (!p.build_rec(scene, view, tr, &root)) return Layer{};
+  (void)root;
+  size_t root_id = p.nodes.size() - 1;
+  Layer layer = Layer::make(view ? *view : p.nodes[root_id].bbox, bg);
+  const size_t H = layer.shape.height, nb = (H + band - 1) / band;
+  std::atomic<size_t> next{0};
+  std::atomic<bool> nan{false};
+  auto work = [&]() {
+    for (size_t b; (b = next.fetch_add(1)) < nb;) {
+      try {
+        p.render_rec_rows(root_id, layer, flatness, b * band, std::min(H, (b + 1) * band));
+      } catch (const std::runtime_error&) {
+        nan = true;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (nan) throw std::runtime_error("cannot flatten segment with NaN");
+  return layer;
+}
 
 // Scene::render (src/scene.rs:186-199) with the signed-difference rasterizer
 inline Layer scene_render(const SceneNode* scene, const Transform& tr, const BBox* view,

@@ -71,10 +71,14 @@ inline void split_at_zero_x(Line line, Line* out, Line* rest, bool* has_rest) {
   }
 }
 
-// src/rasterize.rs:365-470
-inline void signed_difference_line(ImageF64 img, Line line) {
+// src/rasterize.rs:365-470: the deposits of one line into an image of
+// `shape` over a buffer of `len` elements, each handed to add(offset, value)
+// with the offset the reference writes (the one guarded write included only
+// when it stays inside the buffer, as `offset < data.len()` does).
+template <typename Add>
+inline void sd_line_deposits(const Shape& shape, size_t len, Line line, Add&& add) {
   Point p0 = line.p0, p1 = line.p1;
-  const double width = (double)img.shape.width - 1.0;
+  const double width = (double)shape.width - 1.0;
   if (p0.x > width || p1.x > width) {
     if (p0.x > width && p1.x > width) {
       line = Line{P(width - 0.001, p0.y), P(width - 0.001, p1.y)};
@@ -88,12 +92,10 @@ inline void signed_difference_line(ImageF64 img, Line line) {
   Line rest;
   bool has_rest;
   split_at_zero_x(line, &line, &rest, &has_rest);
-  if (has_rest) signed_difference_line(img, rest);
+  if (has_rest) sd_line_deposits(shape, len, rest, add);
 
   p0 = line.p0;
   p1 = line.p1;
-  const Shape shape = img.shape;
-  double* data = img.data;
   const size_t stride = shape.col_stride;
 
   if (std::fabs(p0.y - p1.y) < EPSILON) return;
@@ -124,28 +126,33 @@ inline void signed_difference_line(ImageF64 img, Line line) {
     int x1i = (int)x1_ceil;
     if (x1i <= x0i + 1) {
       double xmf = 0.5 * (x + x_next) - x0_floor;
-      data[row_offset + (size_t)x0i * stride] += d * (1.0 - xmf);
+      add(row_offset + (size_t)x0i * stride, d * (1.0 - xmf));
       size_t offset = row_offset + (size_t)(x0i + 1) * stride;
-      if (offset < img.len) data[offset] += d * xmf;
+      if (offset < len) add(offset, d * xmf);
     } else {
       double s = 1.0 / (x1 - x0);
       double x0f = x0 - x0_floor;
       double x1f = x1 - x1_ceil + 1.0;
       double a0 = 0.5 * s * (1.0 - x0f) * (1.0 - x0f);
       double am = 0.5 * s * x1f * x1f;
-      data[row_offset + (size_t)x0i * stride] += d * a0;
+      add(row_offset + (size_t)x0i * stride, d * a0);
       if (x1i == x0i + 2) {
-        data[row_offset + (size_t)(x0i + 1) * stride] += d * (1.0 - a0 - am);
+        add(row_offset + (size_t)(x0i + 1) * stride, d * (1.0 - a0 - am));
       } else {
         double a1 = s * (1.5 - x0f);
-        data[row_offset + (size_t)(x0i + 1) * stride] += d * (a1 - a0);
-        for (int xi = x0i + 2; xi < x1i - 1; ++xi) data[row_offset + (size_t)xi * stride] += d * s;
+        add(row_offset + (size_t)(x0i + 1) * stride, d * (a1 - a0));
+        for (int xi = x0i + 2; xi < x1i - 1; ++xi) add(row_offset + (size_t)xi * stride, d * s);
         double a2 = a1 + (double)(x1i - x0i - 3) * s;
-        data[row_offset + (size_t)(x1i - 1) * stride] += d * (1.0 - a2 - am);
+        add(row_offset + (size_t)(x1i - 1) * stride, d * (1.0 - a2 - am));
       }
-      data[row_offset + (size_t)x1i * stride] += d * am;
+      add(row_offset + (size_t)x1i * stride, d * am);
     }
   }
+}
+
+inline void signed_difference_line(ImageF64 img, Line line) {
+  double* data = img.data;
+  sd_line_deposits(img.shape, img.len, line, [data](size_t off, double v) { data[off] += v; });
 }
 
 // src/rasterize.rs:473-507
@@ -200,6 +207,43 @@ inline void sd_mask_iter(const Path& path, const Transform& tr, double flatness,
     double alpha = alpha_from_winding(rule, winding);
     if (std::fabs(alpha) < 1e-6) continue;
     sink(Pixel{x, y, alpha});
+  }
+}
+
+// sd_mask_iter restricted to the rows [r0, r1) of the image: the same pixels
+// with the same values. Every line still runs its arithmetic over all of its
+// rows (x advances row by row), but only deposits that land in the window's
+// rows of the (W+1)-wide scratch are kept (a deposit one column past a row's
+// end lands in the next row, as in the reference's buffer). Pixel rows are
+// reported in image coordinates.
+template <typename Sink>
+inline void sd_mask_iter_rows(const Path& path, const Transform& tr, double flatness,
+                              FillRule rule, size_t width, size_t height, size_t r0, size_t r1,
+                              Sink&& sink) {
+  r1 = std::min(r1, height);
+  if (width == 0 || height == 0 || r0 >= r1) return;
+  const size_t w1 = width + 1;
+  const Shape full = Shape::simple(height, w1);
+  const size_t len = w1 * height, lo = r0 * w1, hi = r1 * w1;
+  std::vector<double> scratch(hi - lo, 0.0);
+  double* sp = scratch.data();
+  flatten(path, tr, flatness, true, [&](const Line& l) {
+    const double ylo = std::fmin(l.p0.y, l.p1.y), yhi = std::fmax(l.p0.y, l.p1.y);
+    if (yhi + 1.0 < (double)r0 || ylo >= (double)r1) return;  // no deposit in the window
+    sd_line_deposits(full, len, l, [&](size_t off, double v) {
+      if (off >= lo && off < hi) sp[off - lo] += v;
+    });
+  });
+  double winding = 0.0;
+  for (size_t index = 0; index < scratch.size(); ++index) {
+    size_t y = index / w1;
+    size_t x = index - y * w1;
+    if (x == 0) winding = 0.0;
+    else if (x >= width) continue;
+    winding += scratch[index];
+    double alpha = alpha_from_winding(rule, winding);
+    if (std::fabs(alpha) < 1e-6) continue;
+    sink(Pixel{x, r0 + y, alpha});
   }
 }
 

@@ -30,3 +30,19 @@ def render_f64(O, ds, points, colors, W, H, rule=0, flatness=0.05):
         s = c[None, None, :] * a[..., None]
         dst = s + dst * (1.0 - s[..., 3:4])
     return dst
+
+
+def path_of(ds, points, k):
+    """Path k of a DiffScene from unique points `points` (only that path)."""
+    from rasterize_amd import Path
+    nc = ds.n_cubics
+    stored = np.asarray(points, np.float64).reshape(-1, 2)[ds.index[4 * nc * k: 4 * nc * (k + 1)]]
+    return Path(np.full(nc, 2, np.uint8), np.arange(nc + 1, dtype=np.uint32) * 4,
+                stored.reshape(-1), np.array([0, nc], np.uint32), np.array([1], np.uint8))
+
+
+def local_index(ds, k):
+    """For path k's stored points, the index of their unique point within the
+    path's own unique points."""
+    nc = ds.n_cubics
+    return ds.index[4 * nc * k: 4 * nc * (k + 1)] - 3 * nc * k

@@ -74,6 +74,11 @@ def lib():
                              sz]),
             "orc_scene_render": (i, [sz, P(orc_path), P(orc_paint), P(C.c_int32), P(d), P(d),
                                      P(C.c_float), d, P(C.c_float), sz, P(C.c_int64)]),
+            "orc_scene_render_mt": (i, [sz, P(orc_path), P(orc_paint), P(C.c_int32), P(d), P(d),
+                                        P(C.c_float), d, P(C.c_float), sz, P(C.c_int64), i]),
+            "orc_mask_atlas_mt": (i, [sz, P(orc_path), P(d), P(C.c_uint32), d, i, P(C.c_float), sz,
+                                      sz, i]),
+            "orc_sd_mask_iter_dual": (i, [P(orc_path), P(d), P(d), d, i, sz, sz, P(d), P(d)]),
             "orc_bin_replay": (i, [P(orc_path), P(d), d, i, sz, sz, P(C.c_uint32), P(C.c_float),
                                    sz, P(C.c_int32), P(C.c_float), sz, P(C.c_int32), P(sz),
                                    P(sz), i, i]),
@@ -257,3 +262,64 @@ def scene_render(fills, tr, view=None, bg=None, flatness=0.05):
     elif rc not in (0, 2):
         _check(rc)
     return img, (x, y)
+
+
+def host_threads():
+    """Host threads for the oracle's parallel checks: this process's CPU share,
+    at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def scene_render_mt(fills, tr, view=None, bg=None, flatness=0.05, threads=None):
+    """scene_render on several host threads (bands of layer rows): the same
+    image, bit for bit."""
+    n = len(fills)
+    paths = (orc_path * max(n, 1))(*[opath(f[0]) for f in fills])
+    paints = (orc_paint * max(n, 1))(*[orc_paint.from_buffer_copy(f[1]) for f in fills])
+    rules = (C.c_int32 * max(n, 1))(*[int(f[2]) for f in fills])
+    t = _tr(tr)
+    v = (C.c_double * 4)(*view) if view is not None else None
+    b = (C.c_float * 4)(*bg) if bg is not None else None
+    xywh = (C.c_int64 * 4)()
+    th = threads or host_threads()
+    L = lib()
+    rc = L.orc_scene_render_mt(n, paths, paints, rules, _dp(t), v, b, flatness, None, 0, xywh, th)
+    x, y, w, h = list(xywh)
+    img = np.zeros((h, w, 4), np.float32)
+    if w and h:
+        _check(L.orc_scene_render_mt(n, paths, paints, rules, _dp(t), v, b, flatness,
+                                     img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh, th))
+    elif rc not in (0, 2):
+        _check(rc)
+    return img, (x, y)
+
+
+def mask_atlas(paths, trs, xywh, atlas_w, atlas_h, rule=0, flatness=0.05, threads=None, out=None):
+    """Per glyph SignedDifferenceRasterizer::mask into a zeroed w x h image,
+    copied into a float32 atlas (atlas_h, atlas_w) at (x, y)."""
+    n = len(paths)
+    keep = [opath(p) for p in paths]
+    arr = (orc_path * max(n, 1))(*keep)
+    t = np.ascontiguousarray(np.asarray(trs, np.float64).reshape(-1))
+    q = np.ascontiguousarray(np.asarray(xywh, np.uint32).reshape(-1))
+    atlas = out if out is not None else np.zeros((atlas_h, atlas_w), np.float32)
+    _check(lib().orc_mask_atlas_mt(n, arr, _dp(t), q.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   flatness, rule, atlas.ctypes.data_as(C.POINTER(C.c_float)),
+                                   atlas_w, atlas_h, threads or host_threads()))
+    return atlas
+
+
+def sd_mask_iter_dual(path, tan, tr, W, H, rule=0, flatness=0.05):
+    """(alpha, dalpha): mask_iter coverage (H, W) and its derivative along
+    `tan` ((n_points, 2): d(point)/d(parameter) per stored point)."""
+    p, t = opath(path), _tr(tr)
+    tn = np.ascontiguousarray(np.asarray(tan, np.float64).reshape(-1))
+    a = np.zeros((H, W))
+    da = np.zeros((H, W))
+    _check(lib().orc_sd_mask_iter_dual(C.byref(p), _dp(tn), _dp(t), flatness, rule, W, H, _dp(a),
+                                       _dp(da)))
+    return a, da

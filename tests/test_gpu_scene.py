@@ -146,3 +146,24 @@ def test_scene_empty_outside_and_clipped_paths(small_scene, batched):
     got = scene.render(SignedDifferenceRasterizer(), S, view, batched=batched)
     ref = O.scene_render([(p, q.abi(), r) for p, q, r in items], S, view)
     check_layer(got, ref, f"scene empty/outside/clipped batched={batched}")
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_scene_view_layer_geometry(batched):
+    """src/scene.rs:669-693 test_scene_view: the layer of a view is the view's
+    pixel box (1, 2, 22 x 21); without a view it is the fill's box
+    (6, 4, 16 x 15). The scene: transform "translate(7, 7) rotate(45, 7, 7)
+    scale(10)" of a fill of the unit square, ActiveEdgeRasterizer."""
+    from test_oracle_pins import _scene_view_fill
+    path, tr = _scene_view_fill()
+    sc = Scene.fill(path, LinColor(1.0, 0.2158605, 0.05126946, 1.0))
+    r = rz.ActiveEdgeRasterizer()
+    lay = sc.render(r, tr, view=(1.0, 2.0, 23.0, 23.0), batched=batched)
+    assert (lay.x, lay.y, lay.image.shape[1], lay.image.shape[0]) == (1, 2, 22, 21)
+    lay = sc.render(r, tr, batched=batched)
+    assert (lay.x, lay.y, lay.image.shape[1], lay.image.shape[0]) == (6, 4, 16, 15)
+    # and the pixels agree with the oracle's render of the same layer
+    ref, (x, y) = O.scene_render([(path, LinColor(1.0, 0.2158605, 0.05126946, 1.0).abi(), 0)], tr)
+    assert (x, y) == (6, 4)
+    d = np.abs(lay.image.astype(np.float64) - ref)
+    assert d.max() <= 2e-3
