@@ -300,31 +300,29 @@ def main():
     dom = max(per_step, key=per_step.get)
     dom_ms = per_step[dom] / kern[dom][0]
 
-    # algorithmic bytes per launch: inputs read once, outputs written once
-    seg_in = 16 * n_pts + 4 * n_seg
-    fine_bytes = (4 * W * H + 16 * n_recs, "4*W*H (f32 mask written) + 16*cell records (read)")
-    alg_def = {
-        "k_flat_lines": (seg_in + 32 * n_lines,
-                         "16*control points + 4*segments (read) + 32*lines (f64, written)"),
-        "k_bin_lines": (32 * n_lines + 16 * n_recs,
-                        "32*lines (f64, read) + 16*cell records (written, to the record list)"),
-        "k_bin_scatter": (8 * n_lines + 32 * n_recs,
-                          "8*lines (tile ids, read) + 16*cell records (read) + 16*cell records (written)"),
-        "k_fine_mask<0>": fine_bytes,
-        "k_fine_mask<1>": fine_bytes,
-    }
-    alg, alg_note = alg_def.get(dom, (0, "n/a"))
-    achieved = alg / (dom_ms * 1e-3) / 1e9 if alg else 0.0
-    per_kernel_gbs = {k: round(alg_def[k][0] / (per_step[k] / kern[k][0] * 1e-3) / 1e9, 1)
-                      for k in per_step if k in alg_def}
+    # Algorithmic bytes of a mask step: the f32 mask written once and each
+    # flattened line read once (16 B: f32 x0, y0, x1, y1), 4*W*H + 16*lines.
+    # Intermediate buffers are not counted. `achieved` divides them by the
+    # dominant kernel's average launch time (the roofline of the kernel that
+    # bounds the step); `frac_step` by the whole step's time.
+    alg = 4 * W * H + 16 * n_lines
+    alg_note = "4*W*H (f32 mask written) + 16*lines (f32 line read), per step"
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    achieved_step = alg / (ms_step * 1e-3) / 1e9
+    # the same bytes over each kernel's time (what each kernel alone would reach)
+    per_kernel_gbs = {k: round(alg / (per_step[k] / kern[k][0] * 1e-3) / 1e9, 1) for k in per_step}
 
-    traffic = None
+    traffic = traffic_step = None
     traffic_src = None
     if os.path.exists(TRAFFIC_FILE):
         tf = json.load(open(TRAFFIC_FILE))
-        if dom in tf.get("kernels", {}):
-            traffic = tf["kernels"][dom]["hbm_bytes"]
+        ks = tf.get("kernels", {})
+        if dom in ks:
+            traffic = ks[dom]["hbm_bytes"]
             traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT)
+        if ks:
+            traffic_step = int(sum(int(v["hbm_bytes"]) * int(v.get("launches_per_step", 1))
+                                   for v in ks.values()))
 
     copy_bw = copy_bandwidth_gbps(torch)
 
@@ -332,11 +330,13 @@ def main():
     msegs = n_seg * args.steps / (ms_total * 1e-3) / 1e6
 
     scenes = None
+    ws_c3 = None
     if not args.no_scene:
+        ws_c3 = rz.workload_scene(3, 10_000, 2048.0)
         scenes = [
             scene_measure(torch, rz, lib, _abi, "batched scene: 10,000 star/blob paths, 60% solid / "
                           "25% linear / 15% radial paint, 2048x2048 RGBA f32",
-                          rz.workload_scene(3, 10_000, 2048.0), 2048, 2048, 10, 2),
+                          ws_c3, 2048, 2048, 10, 2),
             scene_measure(torch, rz, lib, _abi, "reference many-circles scene bench: 1024 circles, "
                           "1024x1024 RGBA f32", rz.workload_circles(1024, 1024.0), 1024, 1024, 20, 2),
         ]
@@ -357,6 +357,9 @@ def main():
         t = np.array(tr.m)
         reps = args.cpu_reps
         secs = O.lib().orc_time_sd_mask(C.byref(p), O._dp(t), 0.05, rule, W, H, reps, 1)
+        nth = O.host_threads()
+        reps_n = 2 * nth
+        secs_n = O.lib().orc_time_sd_mask(C.byref(p), O._dp(t), 0.05, rule, W, H, reps_n, nth)
         cpu = {
             "value": round(W * H * reps / secs / 1e6, 3),
             "unit": "Mpix/s",
@@ -364,9 +367,39 @@ def main():
             "kind": "port",
             "sample": f"{reps} full masks of the same 200k-segment path at 4096x4096 "
                       f"({secs:.1f} s), the oracle's C++ restatement of "
-                      "SignedDifferenceRasterizer::mask, one host thread",
+                      "SignedDifferenceRasterizer::mask, one host thread (the reference is "
+                      "single-threaded)",
             "msegs_per_s": round(n_seg * reps / secs / 1e6, 4),
+            "host_cores": os.cpu_count(),
+            "threads": {
+                "cores": nth,
+                "value": round(W * H * reps_n / secs_n / 1e6, 3),
+                "unit": "Mpix/s",
+                "sample": f"{reps_n} masks spread over {nth} host threads ({secs_n:.1f} s); an "
+                          "upper bound for the reference, which has no threading",
+            },
         }
+        if ws_c3 is not None and scenes:
+            # the 10k-path scene on the host: one thread over the top 512 of the
+            # 2048 rows (a view of the same scene), scaled to the frame; then the
+            # whole frame on every host thread (bands of rows)
+            fills = [(q, pt.abi(), r) for q, pt, r in zip(ws_c3.paths, ws_c3.paints, ws_c3.rules)]
+            ident = rz.Transform.identity()
+            t1 = time.perf_counter()
+            O.scene_render_mt(fills, ident, (0.0, 0.0, 2048.0, 512.0), threads=1)
+            one = (time.perf_counter() - t1) * 4.0
+            t1 = time.perf_counter()
+            O.scene_render_mt(fills, ident, (0.0, 0.0, 2048.0, 2048.0), threads=nth)
+            many = time.perf_counter() - t1
+            gpu_ms = scenes[0]["batched_ms_per_frame"]
+            scenes[0]["cpu"] = {
+                "ms_per_frame_1thread": round(one * 1e3, 1),
+                "sample_1thread": "the oracle's Scene::render of rows 0-512 (a view), one thread, x4",
+                "ms_per_frame_threads": round(many * 1e3, 1),
+                "threads": nth,
+                "batched_gpu_over_1thread": round(one * 1e3 / gpu_ms, 1),
+                "batched_gpu_over_threads": round(many * 1e3 / gpu_ms, 1),
+            }
         if glyph_set is not None:
             # the glyph atlas on the host: the first 4096 glyphs, one at a time
             t1 = time.perf_counter()
@@ -413,10 +446,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
+            "traffic_step": traffic_step,
             "traffic_source": traffic_src,
             "algorithmic_bytes": alg,
             "algorithmic_bytes_def": alg_note,
             "kernel_ms": round(dom_ms, 4),
+            "achieved_step": round(achieved_step, 1),
+            "frac_step": round(achieved_step / HBM_PEAK_GBPS, 4),
             "copy_bw_measured": round(copy_bw, 1),
         },
         "scenes": scenes,

@@ -24,7 +24,7 @@ struct BinReplay {
   std::vector<std::vector<rzg::Rec>> recs;  // per tile
   std::vector<int32_t> bd;                  // per tile, prefix along each band
   std::vector<rzg::Event> events;
-  std::vector<int32_t> redge;               // [H]: right-edge column per row
+  std::vector<int64_t> redge;               // [H]: right-edge column per row
 };
 
 inline rzg::Pt rp(Point p) { return rzg::mkpt(p.x, p.y); }
@@ -47,7 +47,7 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
   out->recs.assign(tiles, {});
   out->bd.assign(tiles, 0);
   out->events.clear();
-  std::vector<int32_t> ed((size_t)H + 2, 0);
+  std::vector<int64_t> ed((size_t)H + 2, 0);
   const double fl = 16.0 * flatness * flatness;
   auto event = [&](rzg::Pt p, float sign) {
     rzg::Event ev;
@@ -109,7 +109,7 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
     slot(Segment::line(path.segments[e - 1].end(), path.segments[b].start()));
   }
   out->redge.assign((size_t)H, 0);
-  int32_t run = 0;
+  int64_t run = 0;
   for (int y = 0; y < H; ++y) out->redge[(size_t)y] = run += ed[(size_t)y];
   for (int band = 0; band < g.tiles_y; ++band) {
     int32_t acc = 0;

@@ -529,8 +529,9 @@ int orc_bin_replay(const orc_path* p, const double* tr, double flatness, int mod
     }
     bd[t] = r.bd[t];
   }
-  if (redge)
-    for (size_t y = 0; y < H; ++y) redge[y] = r.redge[y];
+  if (redge)  // int32 out, clamped (as the product's rz_bin reports it)
+    for (size_t y = 0; y < H; ++y)
+      redge[y] = (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, r.redge[y]));
   for (size_t i = 0; i < r.events.size(); ++i) {
     const auto& e = r.events[i];
     std::memcpy(&events[4 * i], &e.band, 4);

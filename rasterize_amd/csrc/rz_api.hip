@@ -480,6 +480,15 @@ rz_status check_args(rz_ctx* ctx, double flatness, int rule, int mode, size_t W,
   return RZ_OK;
 }
 
+// The meta block holds the status, per-tile counts and backdrops, per-band
+// event counts, the right-edge rows (int64, so at an even word: the block
+// itself is 256-byte aligned) and the counters of the binning stage.
+size_t redge_words(size_t before) { return (before + 1) & ~(size_t)1; }
+size_t mask_meta_words(size_t tiles, size_t ty, size_t H) {
+  return redge_words(rzk::kStatusWords + 2 * tiles + ty) + 2 * (H + 2) + rzk::kParts + 1 +
+         rzk::kLineParts;
+}
+
 struct Pipe {
   Grid grid;
   rzk::Status* st;
@@ -488,7 +497,7 @@ struct Pipe {
   uint32_t* band_ev;
   uint32_t* tile_off;
   uint32_t* ev_off;
-  int32_t* redge;  // [H + 2] right-edge row differences, then (scanned) per-row sums
+  long long* redge;  // [H + 2] right-edge row differences, then (scanned) per-row sums
 };
 
 // Flatten every slot of `d` into the line parts of ctx->lines (and, for a
@@ -593,7 +602,8 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + tiles + tiles + ty + (H + 2) + rzk::kParts + 1 + rzk::kLineParts;
+  const size_t ro = redge_words(sw + 2 * tiles + ty);
+  const size_t meta_u32 = mask_meta_words(tiles, ty, H);
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, (tiles + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, (ty + 1) * 4));
@@ -614,7 +624,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   P.tile_count = meta + sw;
   P.bd = (int32_t*)(meta + sw + tiles);
   P.band_ev = meta + sw + 2 * tiles;
-  P.redge = (int32_t*)(meta + sw + 2 * tiles + ty);
+  P.redge = (long long*)(meta + ro);
   P.tile_off = (uint32_t*)ctx->tile_off.p;
   P.ev_off = (uint32_t*)ctx->ev_off.p;
 
@@ -1065,7 +1075,8 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   const uint32_t ct = ctx->sc_cap_t, cb = ctx->sc_cap_b, cr = ctx->sc_cap_r;
   const size_t sw = rzk::kStatusWords;
   // per path four 64-bit box keys (k_scene_bbox) after the right-edge rows
-  const size_t kb = (sw + 2 * (size_t)ct + cb + cr + 1) & ~(size_t)1;
+  const size_t ro = redge_words(sw + 2 * (size_t)ct + cb);
+  const size_t kb = ro + 2 * (size_t)cr;
   const size_t meta_u32 = kb + 8 * (size_t)n + rzk::kParts + 1 + rzk::kLineParts;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
@@ -1081,7 +1092,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   uint32_t* tile_count = meta + sw;
   int32_t* bd = (int32_t*)(meta + sw + ct);
   uint32_t* band_ev = meta + sw + 2 * (size_t)ct;
-  int32_t* redge = (int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  long long* redge = (long long*)(meta + ro);
   uint32_t* tile_off = (uint32_t*)ctx->tile_off.p;
   uint32_t* ev_off = (uint32_t*)ctx->ev_off.p;
   rzk::PathInfo* pi = (rzk::PathInfo*)ctx->sc_pi.p;
@@ -1236,7 +1247,8 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   }
   const uint32_t ct = s->atlas_tiles, cb = s->atlas_bands, cr = s->atlas_rows;
   const size_t sw = rzk::kStatusWords;
-  const size_t meta_u32 = sw + 2 * (size_t)ct + cb + cr + rzk::kParts + 1 + rzk::kLineParts;
+  const size_t ro = redge_words(sw + 2 * (size_t)ct + cb);
+  const size_t meta_u32 = ro + 2 * (size_t)cr + rzk::kParts + 1 + rzk::kLineParts;
   RZ_TRY(ensure(ctx, ctx->meta, meta_u32 * 4));
   RZ_TRY(ensure(ctx, ctx->tile_off, ((size_t)ct + 1) * 4));
   RZ_TRY(ensure(ctx, ctx->ev_off, ((size_t)cb + 1) * 4));
@@ -1247,7 +1259,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   uint32_t* tile_count = meta + sw;
   int32_t* bd = (int32_t*)(meta + sw + ct);
   uint32_t* band_ev = meta + sw + 2 * (size_t)ct;
-  int32_t* redge = (int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  long long* redge = (long long*)(meta + ro);
   uint32_t* tile_off = (uint32_t*)ctx->tile_off.p;
   uint32_t* ev_off = (uint32_t*)ctx->ev_off.p;
   rzk::PathInfo* pi = (rzk::PathInfo*)ctx->sc_pi.p;
@@ -1338,7 +1350,7 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
   a.bd = (const int32_t*)(meta + sw + ct);
   a.band_ev_off = (const uint32_t*)ctx->ev_off.p;
   a.events = (const int4*)ctx->events.p;
-  a.redge = (const int32_t*)(meta + sw + 2 * (size_t)ct + cb);
+  a.redge = (const long long*)(meta + redge_words(sw + 2 * (size_t)ct + cb));
   a.st = st;
   a.bg = bg;
   a.out = nullptr;
@@ -1700,8 +1712,7 @@ rz_status rz_mask_device_noalloc(rz_ctx* ctx, const rz_dpath* d, rz_transform tr
   // the sizes enqueue_mask will ask for, as its helpers compute them
   const Grid gr = make_grid(edge_mode, width, height);
   const size_t tiles = gr.tiles, ty = (size_t)gr.g.tiles_y;
-  const size_t meta_u32 =
-      rzk::kStatusWords + 2 * tiles + ty + (height + 2) + rzk::kParts + 1 + rzk::kLineParts;
+  const size_t meta_u32 = mask_meta_words(tiles, ty, height);
   int dq, dc;
   split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
   const size_t nblk = (size_t)d->view.blocks_line + ((size_t)d->view.blocks_quad << dq) +
@@ -1902,7 +1913,14 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
   if (tile_counts)
     for (size_t t = 0; t < tiles; ++t) tile_counts[t] = off[t + 1] - off[t];
   if (backdrop) RZ_HIP(ctx, hipMemcpy(backdrop, P.bd, tiles * 4, hipMemcpyDeviceToHost));
-  if (right_edge) RZ_HIP(ctx, hipMemcpy(right_edge, P.redge, height * 4, hipMemcpyDeviceToHost));
+  if (right_edge) {
+    // kept in int64 on the device; the hook reports int32 (values beyond it
+    // are clamped)
+    std::vector<long long> re64(height);
+    RZ_HIP(ctx, hipMemcpy(re64.data(), P.redge, height * 8, hipMemcpyDeviceToHost));
+    for (size_t y = 0; y < height; ++y)
+      right_edge[y] = (int32_t)std::max<long long>(INT32_MIN, std::min<long long>(INT32_MAX, re64[y]));
+  }
   if (records) {
     if (records_cap < nr) return fail(ctx, RZ_E_CAPACITY, "records buffer too small");
     RZ_HIP(ctx, hipMemcpy(records, ctx->recs.p, nr * 16, hipMemcpyDeviceToHost));

@@ -78,10 +78,7 @@ __device__ inline bool path_tile(const SceneFineArgs& a, uint32_t q, int TX, int
 __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
   const SceneFineArgs& a = b.f;
   if (a.st->err || a.st->overflow) return;
-  __shared__ int acc[256];
-  __shared__ int rowd[18];
-  __shared__ RowStage rs[64];
-  __shared__ int marks[64];
+  __shared__ TileLDS L;
   const int lane = threadIdx.x;
   const int TX = (int)(blockIdx.x % (uint32_t)a.tiles_x), TY = (int)(blockIdx.x / (uint32_t)a.tiles_x);
   const int col = lane & 15;
@@ -117,29 +114,14 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
       const uint32_t eb = (uint32_t)__builtin_amdgcn_readlane((int)leb, src);
       const uint32_t ee = (uint32_t)__builtin_amdgcn_readlane((int)lee, src);
       const bool local = rb != re || eb != ee;
-      if (local) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
-        if (lane < 18) rowd[lane] = 0;
-        __syncthreads();
-        uint32_t r0 = 0;
-        tile_records(a.recs, rb, re, acc, rowd, r0, rs, marks, lane);
-        for (uint32_t i = eb + lane; i < ee; i += 64) {
-          const int4 ev = a.events[i];
-          if (ev.y < txl) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
-        }
-        flush_row0(rowd, r0, lane);
-        __syncthreads();
-        if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
-        __syncthreads();
-      }
       const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
+      int64_t wl[4] = {bd, bd, bd, bd};
+      if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, txl, lane, wl);
       const int xs = txl * 16 - P.g.phx + col;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * k + (lane >> 4);
-        int64_t w = bd;
-        if (local) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
+        int64_t w = wl[k];
         const int ys = tyl * 16 - P.g.phy + r;
         float W = 0.0f;
         const size_t e = (size_t)t * 256 + (size_t)(r * 16 + col);

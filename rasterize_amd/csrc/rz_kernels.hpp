@@ -396,7 +396,8 @@ struct BinBufs {
   uint32_t rec_cap;
   int4* events;           // (band, col, yrel bits, sign bits)
   uint32_t ev_cap;
-  int32_t* redge;         // [H + 2] right-edge row differences
+  long long* redge;       // [H + 2] right-edge row differences (int64: a row may be crossed
+                          // by any number of lines beyond the clip)
   // record list: two entries per line slot (each binning wave's region is
   // those of its lines), then [kParts][part_cap] overflow parts
   uint2* list_meta;       // tile (~0u: padding), map slot << 24 | rank within the wave's
@@ -762,7 +763,9 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
       if (proj) {
         const bool skip_end = lane < 63 && nproj && npid == pid && nx == p1.x && ny == p1.y;
         const bool skip_start = lane > 0 && pproj && ppid == pid && px == p0.x && py == p0.y;
-        auto add = [&](int k, int v) { atomicAdd(&b.redge[rb0 + k], v); };
+        auto add = [&](int k, int v) {
+          atomicAdd((unsigned long long*)&b.redge[rb0 + k], (unsigned long long)(long long)v);
+        };
         if (!skip_end) rzg::redge_endpoint(p1.y, 1, g, add);
         if (!skip_start) rzg::redge_endpoint(p0.y, -1, g, add);
       }
@@ -1097,21 +1100,60 @@ __device__ inline uint32_t block_sum(uint32_t v) {
   return tot;
 }
 
+// Exclusive scan of 64-bit values over the block; the total to *total.
+__device__ inline unsigned long long block_exclusive_scan64(unsigned long long v,
+                                                            unsigned long long* total) {
+  __shared__ unsigned long long wsum64[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = (unsigned long long)__shfl_up((long long)x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum64[wid] = x;
+  __syncthreads();
+  unsigned long long base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const unsigned long long q = wsum64[w];
+    if (w < wid) base += q;
+    tot += q;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + x - v;
+}
+
 // Inclusive prefix along rows of a signed array (two's-complement sums, so
-// the unsigned scan is exact): `rows` rows of `len` values from `p`, then, if
-// n_tail > 0, one more row of n_tail values from `tail`.
+// the unsigned scan is exact): `rows` rows of `len` int32 values from `p`,
+// then, if n_tail > 0, one more row of n_tail int64 values from `tail`.
 struct RowPrefix {
   int32_t* p;
   int len;
   uint32_t rows;
-  int32_t* tail;
+  long long* tail;
   int n_tail;
 };
 
 __device__ void row_prefix_block(const RowPrefix& r, uint32_t row) {
-  const bool tail = row == r.rows;
-  uint32_t* v = (uint32_t*)(tail ? r.tail : r.p + (size_t)row * r.len);
-  const int n = tail ? r.n_tail : r.len;
+  if (row == r.rows) {
+    unsigned long long* v = (unsigned long long*)r.tail;
+    const int n = r.n_tail;
+    const int per = (n + kBlock - 1) / kBlock;
+    const int i0 = (int)threadIdx.x * per, i1 = min(i0 + per, n);
+    unsigned long long sum = 0;
+    for (int i = i0; i < i1; ++i) sum += v[i];
+    unsigned long long tot;
+    unsigned long long run = block_exclusive_scan64(sum, &tot);
+    for (int i = i0; i < i1; ++i) {
+      run += v[i];
+      v[i] = run;
+    }
+    return;
+  }
+  uint32_t* v = (uint32_t*)(r.p + (size_t)row * r.len);
+  const int n = r.len;
   // each thread takes `per` consecutive values: one block scan of their sums
   const int per = (n + kBlock - 1) / kBlock;
   const int i0 = (int)threadIdx.x * per, i1 = min(i0 + per, n);
@@ -1349,13 +1391,79 @@ __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re
   }
 }
 
+// The LDS of one tile's coverage (2^-20 fixed point, int32).
+struct TileLDS {
+  int acc[256];
+  int rowd[18];
+  RowStage rs[64];
+  int marks[64];
+};
+
+// The int32 accumulation is exact while a tile's local winding (its records'
+// deposits, their V terms and its band's events) stays inside int32. Per
+// pixel, each record contributes at most 1 (its area in the row) plus 2 V
+// terms of at most 1, each event at most 1, each term with rounding below
+// 2^20 + 9: so |local| <= (3 records + events) (2^20 + 9) < 2^31 below 2000.
+// Larger tiles are accumulated in chunks of at most 600 records and 190
+// events, each chunk's int32 result folded into int64 per pixel.
+constexpr uint32_t kChunkRecords = 600, kChunkEvents = 190;
+__device__ inline bool tile_narrow(uint32_t n_records, uint32_t n_events) {
+  return 3ull * n_records + n_events < 2000ull;
+}
+
+// One narrow chunk into LDS: records [rb, re), and those of the events
+// [eb, ee) that lie left of tile column `tx` (in the path's grid); then the
+// per-row base prefix. Every lane takes part.
+__device__ inline void chunk_build(TileLDS& L, const float4* recs, uint32_t rb, uint32_t re,
+                                   const int4* events, uint32_t eb, uint32_t ee, int tx, int lane) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) L.acc[lane + 64 * k] = 0;
+  if (lane < 18) L.rowd[lane] = 0;
+  __syncthreads();
+  uint32_t r0 = 0;
+  tile_records(recs, rb, re, L.acc, L.rowd, r0, L.rs, L.marks, lane);
+  for (uint32_t i = eb + lane; i < ee; i += 64) {
+    const int4 ev = events[i];
+    if (ev.y < tx) add_v(L.rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
+  }
+  flush_row0(L.rowd, r0, lane);
+  __syncthreads();
+  // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
+  if (lane < 16) L.rowd[lane] = row_scan16(L.rowd[lane]);
+  __syncthreads();
+}
+
+// The local winding of one tile (records [rb, re), the band's events [eb, ee))
+// at the lane's four pixels, rows 4k + lane / 16, column lane % 16, into
+// w[0..3] (added). Every lane of the wave takes part (the row scans are DPP).
+__device__ inline void tile_coverage(TileLDS& L, const float4* recs, uint32_t rb, uint32_t re,
+                                     const int4* events, uint32_t eb, uint32_t ee, int tx,
+                                     int lane, int64_t w[4]) {
+  const int col = lane & 15;
+  const bool narrow = tile_narrow(re - rb, ee - eb);
+  uint32_t r0 = rb, e0 = eb;
+  do {
+    const uint32_t r1 = narrow ? re : min(re, r0 + kChunkRecords);
+    const uint32_t e1 = narrow ? ee : min(ee, e0 + kChunkEvents);
+    chunk_build(L, recs, r0, r1, events, e0, e1, tx, lane);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int row = 4 * k + (lane >> 4);
+      w[k] += (int64_t)(int32_t)(L.rowd[row] + row_scan16(L.acc[row * 16 + col]));
+    }
+    r0 = r1;
+    e0 = e1;
+    if (r0 < re || e0 < ee) __syncthreads();  // the next chunk clears the LDS
+  } while (r0 < re || e0 < ee);
+}
+
 struct FineArgs {
   const uint32_t* tile_off;
   const float4* recs;
   const int32_t* bd;
   const uint32_t* band_ev_off;
   const int4* events;
-  const int32_t* redge;  // [H] right-edge column per row (2^-20 fixed point)
+  const long long* redge;  // [H] right-edge column per row (2^-20 fixed point)
   const Status* st;
   Status* acc;           // the context's accumulated status (merged by block 0)
   float* out;
@@ -1415,10 +1523,7 @@ template <int RULE, class Out>
 __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
-  __shared__ int acc[256];
-  __shared__ int rowd[18];
-  __shared__ RowStage rs[64];
-  __shared__ int marks[64];
+  __shared__ TileLDS L;
   const int lane = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
@@ -1427,31 +1532,15 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   // a tile with no records and no events of its band has the winding of its
   // backdrop (plus the right-edge column) everywhere: no LDS work
   const bool local = rb != re || eb != ee;
-  if (local) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
-    if (lane < 18) rowd[lane] = 0;
-    __syncthreads();
-    uint32_t r0 = 0;
-    tile_records(a.recs, rb, re, acc, rowd, r0, rs, marks, lane);
-    for (uint32_t i = eb + lane; i < ee; i += kFineWave) {
-      const int4 ev = a.events[i];
-      if (ev.y < tx) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
-    }
-    flush_row0(rowd, r0, lane);
-    __syncthreads();
-    // per-row base: prefix of the row differences (lanes 0..15 scan, in place)
-    if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
-    __syncthreads();
-  }
   const int64_t bd = (int64_t)a.bd[tile] << rzg::kFixShift;
+  int64_t wl[4] = {bd, bd, bd, bd};
+  if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, tx, lane, wl);
   const int col = lane & 15;
   const int x = tx * 16 + col;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int row = 4 * k + (lane >> 4);
-    int64_t w = bd;
-    if (local) w += (int64_t)(int32_t)(rowd[row] + row_scan16(acc[row * 16 + col]));
+    int64_t w = wl[k];
     const int y = ty * 16 + row;
     if (x < a.W && y < a.H) {
       if (x >= a.redge_col) {

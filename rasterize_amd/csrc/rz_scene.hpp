@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_offsets(uint32_t n, const uint
 // rows (inclusive, two's-complement sums), one wave per path: lane r walks
 // tile row r; for the right-edge column each lane takes a run of consecutive
 // rows, and one wave scan of the run totals gives each run's carry.
-__global__ __launch_bounds__(64) void k_scene_bd(const PathInfo* pi, int32_t* bd, int32_t* redge,
+__global__ __launch_bounds__(64) void k_scene_bd(const PathInfo* pi, int32_t* bd, long long* redge,
                                                   const Status* st) {
   if (st->err || st->overflow) return;
   const PathInfo& P = pi[blockIdx.x];
@@ -232,13 +232,19 @@ __global__ __launch_bounds__(64) void k_scene_bd(const PathInfo* pi, int32_t* bd
       row[i] = carry;
     }
   }
-  uint32_t* e = (uint32_t*)(redge + P.redge_base);
+  unsigned long long* e = (unsigned long long*)(redge + P.redge_base);
   const int n = P.g.H + 2;
   const int per = (n + 63) / 64;
   const int i0 = lane * per, i1 = min(i0 + per, n);
-  uint32_t sum = 0;
+  unsigned long long sum = 0;
   for (int i = i0; i < i1; ++i) sum += e[i];
-  uint32_t run = (uint32_t)dpp_add_scan((int)sum) - sum;  // exclusive
+  unsigned long long incl = sum;  // inclusive wave scan, then exclusive
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long y = (unsigned long long)__shfl_up((long long)incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  unsigned long long run = incl - sum;
   for (int i = i0; i < i1; ++i) {
     run += e[i];
     e[i] = run;
@@ -285,7 +291,7 @@ struct SceneFineArgs {
   const int32_t* bd;
   const uint32_t* band_ev_off;
   const int4* events;
-  const int32_t* redge;
+  const long long* redge;
   const Status* st;
   Status* acc;        // accumulated status (merged by block 0), or null
   rzp::C4 bg;
@@ -311,10 +317,7 @@ struct SceneFineArgs {
 __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
-  __shared__ int acc[256];
-  __shared__ int rowd[18];
-  __shared__ RowStage rs[64];
-  __shared__ int marks[64];
+  __shared__ TileLDS L;
   const int lane = threadIdx.x;
   const int TX = (int)(blockIdx.x % (uint32_t)a.tiles_x), TY = (int)(blockIdx.x / (uint32_t)a.tiles_x);
   const int col = lane & 15;
@@ -359,29 +362,14 @@ __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
       const int qtyl = __builtin_amdgcn_readlane(tyl, src);
       const PathInfo& P = a.pi[q];
       const bool local = qrb != qre || qeb != qee;
-      if (local) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
-        if (lane < 18) rowd[lane] = 0;
-        __syncthreads();
-        uint32_t r0 = 0;
-        tile_records(a.recs, qrb, qre, acc, rowd, r0, rs, marks, lane);
-        for (uint32_t i = qeb + lane; i < qee; i += 64) {
-          const int4 ev = a.events[i];
-          if (ev.y < qtxl) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
-        }
-        flush_row0(rowd, r0, lane);
-        __syncthreads();
-        if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
-        __syncthreads();
-      }
       const int64_t bd = (int64_t)qbd << rzg::kFixShift;
+      int64_t wl[4] = {bd, bd, bd, bd};
+      if (local) tile_coverage(L, a.recs, qrb, qre, a.events, qeb, qee, qtxl, lane, wl);
       const int xs = qtxl * 16 - P.g.phx + col;  // sub-view pixel column
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * k + (lane >> 4);
-        int64_t w = bd;
-        if (local) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
+        int64_t w = wl[k];
         const int ys = qtyl * 16 - P.g.phy + r;
         if (xs >= 0 && xs < P.g.W && ys >= 0 && ys < P.g.H) {
           if (xs >= P.redge_col) {
@@ -507,7 +495,7 @@ struct AtlasFineArgs {
   uint32_t n;
   const float4* recs;
   const int4* events;
-  const int32_t* redge;
+  const long long* redge;
   const Status* st;
   Status* acc;        // accumulated status (merged by block 0)
   float* out;
@@ -520,10 +508,7 @@ template <int RULE>
 __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
-  __shared__ int acc[256];
-  __shared__ int rowd[18];
-  __shared__ RowStage rs[64];
-  __shared__ int marks[64];
+  __shared__ TileLDS L;
   const int lane = threadIdx.x;
   const uint32_t t = blockIdx.x;
   const AtlasTile T = a.tiles[t];
@@ -531,28 +516,13 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   const uint32_t rb = T.rb, re = T.re, eb = T.eb, ee = T.ee;
   const int64_t bd = (int64_t)T.bd << rzg::kFixShift;
   const bool local_work = rb != re || eb != ee;
-  if (local_work) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) acc[lane + 64 * k] = 0;
-    if (lane < 18) rowd[lane] = 0;
-    __syncthreads();
-    uint32_t r0 = 0;
-    tile_records(a.recs, rb, re, acc, rowd, r0, rs, marks, lane);
-    for (uint32_t i = eb + lane; i < ee; i += 64) {
-      const int4 ev = a.events[i];
-      if (ev.y < T.txl) add_v(rowd, r0, __int_as_float(ev.z), __int_as_float(ev.w));
-    }
-    flush_row0(rowd, r0, lane);
-    __syncthreads();
-    if (lane < 16) rowd[lane] = row_scan16(rowd[lane]);
-    __syncthreads();
-  }
+  int64_t wl[4] = {bd, bd, bd, bd};
+  if (local_work) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, T.txl, lane, wl);
   const int col = lane & 15;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int r = 4 * k + (lane >> 4);
-    int64_t w = bd;
-    if (local_work) w += (int64_t)(int32_t)(rowd[r] + row_scan16(acc[r * 16 + col]));
+    int64_t w = wl[k];
     if (col < T.w && r < T.h) {
       if (col >= T.rcol) {
         const int64_t rr = a.redge[T.redge + (uint32_t)r];

@@ -17,7 +17,11 @@ def V(y):
     return np.clip(y - np.arange(16), 0.0, 1.0)
 
 
-def fine_emulation(counts, recs, bd, events, redge, W, H, rule, mode, phase=(0, 0)):
+def fine_emulation(counts, recs, bd, events, redge, W, H, rule, mode, phase=(0, 0), arith="f64"):
+    """arith: "f64" accumulates the tile's local winding in f64; "int32" wraps
+    it to int32 in 2^-20 fixed point, as a fine stage that kept every tile in
+    int32 would. In both, every tile the fine stage keeps in int32 (3 records
+    + band events < 2000) is checked to stay inside int32."""
     phx, phy = phase
     tx_n, ty_n = (W + phx + 15) // 16, (H + phy + 15) // 16
     out = np.zeros((ty_n * 16, tx_n * 16))
@@ -80,7 +84,13 @@ def fine_emulation(counts, recs, bd, events, redge, W, H, rule, mode, phase=(0, 
         sel = (ev_band == ty) & (ev_col < tx)
         for e in events[sel]:
             base += e[3] * V(e[2])
-        w = bd[t] + base[:, None] + np.cumsum(acc[:, :16], axis=1)
+        local = base[:, None] + np.cumsum(acc[:, :16], axis=1)
+        lf = np.rint(local * float(1 << 20)).astype(np.int64)
+        if 3 * int(off[t + 1] - off[t]) + int((ev_band == ty).sum()) < 2000:
+            assert np.abs(lf).max() < 2 ** 31, "an int32 tile's local winding leaves int32"
+        if arith == "int32":
+            local = (((lf + 2 ** 31) % 2 ** 32) - 2 ** 31) / float(1 << 20)
+        w = bd[t] + local
         if mode != 2:
             # right-edge column: lines beyond the clip, as per-row sums R
             X = (W - 1.0 if mode == 0 else float(W)) - 0.001
@@ -159,3 +169,26 @@ def test_decomposition_with_grid_phase(phase, mode):
         d = np.abs(got - ref)
         assert d.max() <= 2e-4, f"rule {rule}: max {d.max()} at {np.unravel_index(d.argmax(), d.shape)}"
         assert d.mean() <= 2e-6
+
+
+def _rect_copies(n, x0, y0, x1, y1):
+    pb = PathBuilder()
+    for _ in range(n):
+        pb.move_to((x0, y0)).line_to((x1, y0)).line_to((x1, y1)).line_to((x0, y1)).close()
+    return pb.build()
+
+
+def test_many_coincident_edges_need_64_bit_tiles():
+    """2,100 copies of one rectangle: in the tile holding their right edges the
+    backdrop is +2100 and the local sum -2100 windings, -2100 * 2^20 in fixed
+    point, outside int32. Kept in int32 it wraps to +1996 and NonZero draws
+    coverage 1 right of the rectangle, where the reference has 0; the fine
+    stage keeps such a tile in int64 (3 x 2100 records >= 2000)."""
+    path = _rect_copies(2100, 10.5, 10.5, 40.5, 40.5)
+    W = H = 64
+    counts, recs, bd, ev, redge = O.bin_replay(path, I, W, H, 0)
+    ref = O.sd_mask(path, I, W, H, 0)
+    wrapped = fine_emulation(counts, recs.astype(np.float64), bd, ev, redge, W, H, 0, 0, arith="int32")
+    assert np.abs(wrapped - ref).max() > 0.5  # the wrap is real
+    exact = fine_emulation(counts, recs.astype(np.float64), bd, ev, redge, W, H, 0, 0)
+    assert np.abs(exact - ref).max() <= 2e-4

@@ -381,3 +381,35 @@ def test_queued_capacity_miss_is_reported(c1, c2):
         L.rz_path_free(ctx.h, d_big)
         L.rz_path_free(ctx.h, d_small)
         ctx.close()
+
+
+def _rect_copies(n, x0, y0, x1, y1, pb=None):
+    pb = pb or PathBuilder()
+    for _ in range(n):
+        pb.move_to((x0, y0)).line_to((x1, y0)).line_to((x1, y1)).line_to((x0, y1)).close()
+    return pb
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_many_coincident_edges(rule):
+    """2,100 copies of a rectangle inside the image and 2,100 of one that
+    crosses its right edge: windings of +-2100, beyond int32 in 2^-20 fixed
+    point, in the tiles holding their edges and in the right-edge column.
+    Mask, batched scene and atlas against the oracle."""
+    pb = _rect_copies(2100, 10.5, 10.5, 40.5, 40.5)
+    path = _rect_copies(2100, 20.25, 44.5, 100.0, 60.75, pb).build()
+    W = H = 64
+    for mode in (rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_ITER):
+        got = SignedDifferenceRasterizer().mask_f32(path, I, W, H, rule, mode)
+        ref = O.sd_mask(path, I, W, H, rule) if mode == 0 else O.sd_mask_iter(path, I, W, H, rule)
+        check_close(got, ref, f"2x2100 rectangles, mode {mode}, rule {rule}")
+    paint = rz.LinColor(0.2, 0.3, 0.1, 0.6)
+    lay = rz.Scene.fill(path, paint, rule).render(SignedDifferenceRasterizer(), I, (0.0, 0.0, 64.0, 64.0))
+    sref, _ = O.scene_render([(path, paint.abi(), rule)], I, (0.0, 0.0, 64.0, 64.0))
+    d = np.abs(lay.image.astype(np.float64) - sref)
+    _say(f"scene: max|d|={d.max():.3e}")
+    assert d.max() <= MAX_ABS and d.mean() <= MEAN_ABS
+    atlas = np.full((80, 80), -1.0, np.float32)
+    rz.mask_atlas([path], [rz.atlas_slot(7, 9, W, H)], atlas, rule)
+    check_close(atlas[9:9 + H, 7:7 + W], O.sd_mask(path, I, W, H, rule), "atlas view")
+    assert (atlas[:9] == -1.0).all() and (atlas[:, :7] == -1.0).all()
