@@ -72,6 +72,11 @@ def kernel_stats(lib, _abi, ctx, run, reps):
             for i in range(n.value)}
 
 
+def set_graph(lib, _abi, ctx, on):
+    """HIP-graph replay of repeated frames on the context (RZ_PARAM_GRAPH)."""
+    ctx.check(lib.rz_set_param(ctx.h, _abi.RZ_PARAM_GRAPH, int(on)))
+
+
 def settle(lib, _abi, ctx, run):
     """Run until the context's buffers fit (a first run may grow them)."""
     for _ in range(4):
@@ -98,10 +103,17 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
     stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
     ident = rz.Transform.identity()
     res = {}
-    for batched, frames in ((True, frames_b), (False, frames_p)):
+    for form, batched, frames in (("batched", True, frames_b), ("plain", False, frames_p),
+                                  ("graph", True, frames_b)):
+        set_graph(lib, _abi, ctx, form == "graph")
+
         def run():
             ds.render_device(ident, view, out.data_ptr(), W, batched=batched)
         settle(lib, _abi, ctx, run)
+        if form == "graph":  # the first clean run settles, the next one is captured
+            for _ in range(2):
+                run()
+                ctx.check(lib.rz_sync(ctx.h))
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
         for _ in range(frames):
@@ -109,7 +121,8 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
         b.record(stream)
         ctx.check(lib.rz_sync(ctx.h))
         torch.cuda.synchronize()
-        res[batched] = a.elapsed_time(b) / frames
+        res[form] = a.elapsed_time(b) / frames
+    set_graph(lib, _abi, ctx, False)
     k = kernel_stats(lib, _abi, ctx, lambda: ds.render_device(ident, view, out.data_ptr(), W), 3)
     ctx.check(lib.rz_sync(ctx.h))
     n_lines, n_recs = last_counts(lib, ctx)
@@ -122,11 +135,12 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
         "lines": n_lines,
         "cell_records": n_recs,
         "layer": [W, H],
-        "batched_ms_per_frame": round(res[True], 4),
-        "plain_ms_per_frame": round(res[False], 3),
-        "batched_over_plain": round(res[False] / res[True], 1),
-        "batched_mpix_s": round(W * H / res[True] / 1e3, 1),
-        "batched_msegs_s": round(segs / res[True] / 1e3, 2),
+        "batched_ms_per_frame": round(res["batched"], 4),
+        "batched_graph_ms_per_frame": round(res["graph"], 4),
+        "plain_ms_per_frame": round(res["plain"], 3),
+        "batched_over_plain": round(res["plain"] / res["batched"], 1),
+        "batched_mpix_s": round(W * H / res["batched"] / 1e3, 1),
+        "batched_msegs_s": round(segs / res["batched"] / 1e3, 2),
         "batched_kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
     }
 
@@ -141,16 +155,24 @@ def atlas_measure(torch, rz, lib, _abi, frames):
 
     def run():
         dg.mask_device(out.data_ptr(), W)
-    settle(lib, _abi, ctx, run)
     stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record(stream)
-    for _ in range(frames):
-        run()
-    b.record(stream)
-    ctx.check(lib.rz_sync(ctx.h))
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / frames
+    timed = {}
+    for graph in (False, True):
+        set_graph(lib, _abi, ctx, graph)
+        settle(lib, _abi, ctx, run)
+        for _ in range(2 if graph else 0):
+            run()
+            ctx.check(lib.rz_sync(ctx.h))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(frames):
+            run()
+        b.record(stream)
+        ctx.check(lib.rz_sync(ctx.h))
+        torch.cuda.synchronize()
+        timed[graph] = a.elapsed_time(b) / frames
+    set_graph(lib, _abi, ctx, False)
+    ms = timed[False]
     k = kernel_stats(lib, _abi, ctx, run, 2)
     ctx.check(lib.rz_sync(ctx.h))
     n_lines, n_recs = last_counts(lib, ctx)
@@ -167,6 +189,7 @@ def atlas_measure(torch, rz, lib, _abi, frames):
         "cell_records": n_recs,
         "mask_pixels": px,
         "ms_per_frame": round(ms, 4),
+        "graph_ms_per_frame": round(timed[True], 4),
         "mpix_s": round(px / ms / 1e3, 1),
         "msegs_s": round(segs / ms / 1e3, 1),
         "kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
@@ -175,51 +198,64 @@ def atlas_measure(torch, rz, lib, _abi, frames):
 
 def diff_measure(torch, rz, steps):
     """Differentiable render: 256 closed paths x 32 cubics, 1024x1024 RGBA,
-    L2 loss against a perturbed target, forward + adjoint backward + Adam."""
+    L2 loss against a perturbed target, forward + adjoint backward + Adam;
+    kernel by kernel, then with each step's forward and backward replayed
+    from HIP graphs (the points and colours they read change every step)."""
+    from rasterize_amd import _abi
     ds, tp, tc = rz.workload_diff(5, 256, 32, 1024.0)
     dev = rz.DeviceScene(ds.scene())
+    L = _abi.lib()
     view = (0.0, 0.0, 1024.0, 1024.0)
     ident = rz.Transform.identity()
     idx = torch.tensor(ds.index, device="cuda")
     with torch.no_grad():
         T = rz.render_torch(dev, torch.tensor(tp, device="cuda")[idx],
                             torch.tensor(tc, dtype=torch.float32, device="cuda"), ident, view).clone()
-    U = torch.tensor(ds.points, dtype=torch.float32, device="cuda", requires_grad=True)
-    Cc = torch.tensor(ds.colors, dtype=torch.float32, device="cuda", requires_grad=True)
-    opt = torch.optim.Adam([{"params": [U], "lr": 0.25}, {"params": [Cc], "lr": 0.005}])
 
-    def step():
-        out = rz.render_torch(dev, U[idx], Cc, ident, view)
-        loss = ((out - T) ** 2).sum()
-        opt.zero_grad()
-        loss.backward()
-        opt.step()
-        return loss
+    def run(graph):
+        dev.ctx.check(L.rz_set_param(dev.ctx.h, _abi.RZ_PARAM_GRAPH, int(graph)))
+        U = torch.tensor(ds.points, dtype=torch.float32, device="cuda", requires_grad=True)
+        Cc = torch.tensor(ds.colors, dtype=torch.float32, device="cuda", requires_grad=True)
+        opt = torch.optim.Adam([{"params": [U], "lr": 0.25}, {"params": [Cc], "lr": 0.005}])
 
-    loss0 = float(step())
-    for _ in range(2):
-        step()
-    torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    a.record()
-    for _ in range(steps):
-        loss = step()
-    b.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lossN = float(loss)
+        def step():
+            out = rz.render_torch(dev, U[idx], Cc, ident, view)
+            loss = ((out - T) ** 2).sum()
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+            return loss
+
+        loss0 = float(step().detach())
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record()
+        for _ in range(steps):
+            loss = step()
+        b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall * 1e3 / steps, a.elapsed_time(b) / steps, loss0, float(loss.detach())
+
+    eager = run(False)
+    graph = run(True)
+    dev.ctx.check(L.rz_set_param(dev.ctx.h, _abi.RZ_PARAM_GRAPH, 0))
     dev.close()
     return {
         "workload": "differentiable render: 256 closed paths x 32 cubics (control points U[0,1024)^2), "
                     "solid RGBA, 1024x1024, L2 loss vs target (points + N(0,2px), colours + N(0,0.05))",
-        "parameters": int(U.numel() + Cc.numel()),
+        "parameters": int(ds.points.size + ds.colors.size),
         "adam_steps": steps,
-        "ms_per_step": round(wall * 1e3 / steps, 3),
-        "gpu_ms_per_step": round(a.elapsed_time(b) / steps, 3),
-        "loss_first": round(loss0, 2),
-        "loss_last": round(lossN, 2),
-        "gradient_check": "tests/test_gpu_backward.py: adjoint vs f64 central finite differences",
+        "ms_per_step": round(eager[0], 3),
+        "gpu_ms_per_step": round(eager[1], 3),
+        "graph_ms_per_step": round(graph[0], 3),
+        "loss_first": round(eager[2], 2),
+        "loss_last": round(eager[3], 2),
+        "gradient_check": "tests/test_gpu_fullsize.py::test_c5_gradients_full_size: 512 sampled "
+                          "parameters vs FP64 central differences and the FP64 forward-mode gradient",
     }
 
 
@@ -279,18 +315,32 @@ def main():
     n_lines, n_recs = nl.value, nr.value
 
     stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
-    torch.cuda.synchronize()
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    t_start.record(stream)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+
+    def timed_steps():
+        torch.cuda.synchronize()
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        t_start.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        t_end.record(stream)
+        ctx.check(lib.rz_sync(ctx.h))
+        torch.cuda.synchronize()
+        return t_start.elapsed_time(t_end), time.perf_counter() - t0
+
+    # each step launched kernel by kernel, then the same step replayed from a
+    # HIP graph (captured once the step has run without growing a buffer);
+    # the faster of the two is the headline
+    ms_eager, wall_eager = timed_steps()
+    set_graph(lib, _abi, ctx, True)
+    for _ in range(3):
         step()
-    t_end.record(stream)
-    ctx.check(lib.rz_sync(ctx.h))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = t_start.elapsed_time(t_end)
+        ctx.check(lib.rz_sync(ctx.h))
+    ms_graph, wall_graph = timed_steps()
+    set_graph(lib, _abi, ctx, False)
+    launch = "hip graph replay" if ms_graph < ms_eager else "kernel by kernel"
+    ms_total, wall = (ms_graph, wall_graph) if ms_graph < ms_eager else (ms_eager, wall_eager)
     ms_step = ms_total / args.steps
 
     # per-kernel device time over a few profiled steps (events on the stream)
@@ -435,6 +485,9 @@ def main():
             "flatness": 0.05,
         },
         "msegs_per_s": round(msegs, 3),
+        "launch": launch,
+        "ms_per_step_kernel_by_kernel": round(ms_eager / args.steps, 4),
+        "ms_per_step_graph": round(ms_graph / args.steps, 4),
         "wall_s": round(wall, 3),
         "kernels_ms_per_step": {k: round(v, 4) for k, v in sorted(per_step.items())},
         "kernels_gb_s": per_kernel_gbs,

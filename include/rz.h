@@ -285,6 +285,33 @@ rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_tran
                                     const float* bg, const float* d_grad_rgba, size_t grad_stride,
                                     float* d_grad_points, float* d_grad_colors, double* d_grad_tr);
 
+/* ---- output encoders ---------------------------------------------------------- */
+
+/* RGBA::from(LinColor) (src/color.rs:164-175) for every pixel of a device
+ * LinColor image d_rgba (width x height, row_stride pixels per row): l2s of
+ * the unmultiplied colour, each channel (v * 255 + 0.5) as u8 (saturating),
+ * alpha from the premultiplied alpha; 4 bytes per pixel, R, G, B, A, into
+ * d_out (out_stride bytes per row). bottom_up = 1 writes the last row first
+ * (the BMP row order). Asynchronous on the context's stream. */
+rz_status rz_encode_rgba8_device(rz_ctx* ctx, const float* d_rgba, size_t width, size_t height,
+                                 size_t row_stride, uint8_t* d_out, size_t out_stride,
+                                 int bottom_up);
+/* Blocking form over host buffers: rgba (row_stride pixels per row) to out. */
+rz_status rz_encode_rgba8(rz_ctx* ctx, const float* rgba, size_t width, size_t height,
+                          size_t row_stride, uint8_t* out, size_t out_stride, int bottom_up);
+/* ImageRead::write_bmp (src/image.rs:164-213): the 122-byte header (file
+ * header + BITMAPV4HEADER, 32-bit BI_BITFIELDS, R G B A byte masks, sRGB) of
+ * a width x height image; the file is this header followed by the pixel rows
+ * bottom-up (rz_encode_rgba8_device with bottom_up = 1). Host only. */
+rz_status rz_bmp_header(size_t width, size_t height, uint8_t* out /* [122] */);
+/* ImageRead::write_png (src/image.rs:215-235): a PNG (8-bit RGBA) of the
+ * rows rgba8 (top-down, stride bytes apart) into out[cap]; *len receives the
+ * file size (pass out = NULL to query it). The image data is stored in
+ * uncompressed deflate blocks (the reference's png crate compresses; every
+ * decoder reads both). Host only. */
+rz_status rz_encode_png(const uint8_t* rgba8, size_t width, size_t height, size_t stride,
+                        uint8_t* out, size_t cap, size_t* len);
+
 /* ---- glyph atlas ------------------------------------------------------------ */
 
 /* Where glyph i goes: Rasterizer::mask of its path under `tr` into the atlas
@@ -320,9 +347,15 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
  * the segments' expected line counts. RZ_PARAM_SCAN_DIRECT (0..16, default
  * 12): a scan of up to 2^value blocks of 2048 counts has each block sum the
  * totals before its own; a longer one scans the totals in a kernel of their
- * own. Results do not depend on any of them. */
+ * own. RZ_PARAM_GRAPH (0 or 1, default 0): rz_mask_device,
+ * rz_render_device (batched), rz_mask_atlas_device and
+ * rz_render_backward_device replay a frame whose arguments repeat from a
+ * HIP graph captured after the same call completed once without growing the
+ * context's buffers (the device data they read, e.g. updated scene points,
+ * may change between frames). RZ_PARAM_GRAPH_REPLAYS (read only): frames
+ * replayed so far. Results do not depend on any of them. */
 enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2,
-       RZ_PARAM_SCAN_DIRECT = 3 };
+       RZ_PARAM_SCAN_DIRECT = 3, RZ_PARAM_GRAPH = 4, RZ_PARAM_GRAPH_REPLAYS = 5 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 

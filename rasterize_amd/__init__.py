@@ -554,12 +554,65 @@ class RawPaint:
         return self.p
 
 
+def encode_rgba8(image, bottom_up=False, ctx=None):
+    """RGBA::from(LinColor) (src/color.rs:164-175) of every pixel of an
+    (H, W, 4) float32 premultiplied linear image, on the GPU: (H, W, 4) uint8
+    sRGB with straight alpha (rows last to first when bottom_up)."""
+    ctx = ctx or default_context()
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise TypeError("image must be (H, W, 4)")
+    h, w = img.shape[:2]
+    out = np.zeros((h, w, 4), np.uint8)
+    if w and h:
+        ctx.check(lib().rz_encode_rgba8(ctx.h, img.ctypes.data_as(C.POINTER(C.c_float)), w, h, w,
+                                        out.ctypes.data_as(C.POINTER(C.c_uint8)), 4 * w,
+                                        int(bottom_up)))
+    return out
+
+
+def bmp_bytes(image, ctx=None):
+    """ImageRead::write_bmp (src/image.rs:164-213) of a LinColor image."""
+    h, w = image.shape[:2]
+    hdr = (C.c_uint8 * 122)()
+    if lib().rz_bmp_header(w, h, hdr) != _abi.RZ_OK:
+        raise RzError(_abi.RZ_E_ARG, "image too large for BMP")
+    return bytes(hdr) + encode_rgba8(image, bottom_up=True, ctx=ctx).tobytes()
+
+
+def png_bytes(image, ctx=None):
+    """ImageRead::write_png (src/image.rs:215-235) of a LinColor image (stored
+    deflate blocks)."""
+    px = encode_rgba8(image, ctx=ctx)
+    h, w = px.shape[:2]
+    n = C.c_size_t()
+    L = lib()
+    src = px.ctypes.data_as(C.POINTER(C.c_uint8))
+    if L.rz_encode_png(src, w, h, 4 * w, None, 0, C.byref(n)) != _abi.RZ_OK:
+        raise RzError(_abi.RZ_E_ARG, "image too large for PNG")
+    buf = (C.c_uint8 * n.value)()
+    if L.rz_encode_png(src, w, h, 4 * w, buf, n.value, C.byref(n)) != _abi.RZ_OK:
+        raise RzError(_abi.RZ_E_ARG, "PNG encoding failed")
+    return bytes(buf)
+
+
 class Layer:
     """Rendered layer (src/scene.rs:463-520): `image` (H, W, 4) float32
     premultiplied linear RGBA whose pixel (0, 0) is at (x, y)."""
 
     def __init__(self, image, x, y):
         self.image, self.x, self.y = image, x, y
+
+    def rgba8(self):
+        return encode_rgba8(self.image)
+
+    def write_bmp(self, path):
+        with open(path, "wb") as f:
+            f.write(bmp_bytes(self.image))
+
+    def write_png(self, path):
+        with open(path, "wb") as f:
+            f.write(png_bytes(self.image))
 
 
 class Scene:

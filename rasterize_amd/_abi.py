@@ -17,6 +17,7 @@ STATUS_NAMES = {
 }
 RZ_NONZERO, RZ_EVENODD = 0, 1
 RZ_PARAM_SPLIT_QUAD, RZ_PARAM_SPLIT_CUBIC, RZ_PARAM_SPLIT_AUTO, RZ_PARAM_SCAN_DIRECT = 0, 1, 2, 3
+RZ_PARAM_GRAPH, RZ_PARAM_GRAPH_REPLAYS = 4, 5
 RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
 RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
 RZ_PAINT_SOLID, RZ_PAINT_LINEAR, RZ_PAINT_RADIAL = 0, 1, 2
@@ -124,6 +125,13 @@ PROTOTYPES = [
     ("rz_render_backward_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,
                                             P(C.c_double), P(C.c_float), C.c_void_p, C.c_size_t,
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rz_encode_rgba8_device", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                         C.c_void_p, C.c_size_t, C.c_int]),
+    ("rz_encode_rgba8", C.c_int, [c_ctx, P(C.c_float), C.c_size_t, C.c_size_t, C.c_size_t,
+                                  P(C.c_uint8), C.c_size_t, C.c_int]),
+    ("rz_bmp_header", C.c_int, [C.c_size_t, C.c_size_t, P(C.c_uint8)]),
+    ("rz_encode_png", C.c_int, [P(C.c_uint8), C.c_size_t, C.c_size_t, C.c_size_t, P(C.c_uint8),
+                                C.c_size_t, P(C.c_size_t)]),
     ("rz_atlas_set", C.c_int, [c_ctx, C.c_void_p, P(rz_atlas_slot)]),
     ("rz_mask_atlas_device", C.c_int, [c_ctx, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p,
                                        C.c_size_t]),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -15,6 +16,7 @@
 #include "rz_kernels.hpp"
 #include "rz_scene.hpp"
 #include "rz_backward.hpp"
+#include "rz_encode.hpp"
 
 namespace {
 
@@ -54,6 +56,27 @@ char* noalloc_scratch(int device) {
 }
 }  // namespace
 
+// Arguments that fix the work a device entry point enqueues, for replaying a
+// captured frame (all bytes compared; zero-initialised, padding included).
+struct GraphKey {
+  int kind;                 // 0 mask, 1 scene render, 2 glyph atlas, 3 scene backward
+  int rule, mode, has_bg;
+  uint64_t obj;             // serial of the resident path / scene
+  uint64_t gen;             // the context's buffer and parameter generation
+  double tr[6], flatness, view[4];
+  float bg[4];
+  size_t w, h, stride, stride2;
+  const void* ptr[4];       // device outputs / inputs
+};
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+  uint64_t last_use;
+};
+namespace {
+std::atomic<uint64_t> g_serial{1};  // resident paths and scenes
+}
+
 struct rz_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -86,6 +109,15 @@ struct rz_ctx {
   rzk::Status* acc = nullptr;
   rzk::Status* acc_run = nullptr;      // where the runs being enqueued merge (acc or acc + 1)
   bool noalloc = false;                // buffers are carved from `scratch`: ensure() never allocates
+  // hipGraph replay of repeated frames (RZ_PARAM_GRAPH)
+  bool use_graphs = false;
+  bool capturing = false;              // a stream capture is open: ensure() never allocates
+  uint64_t gen = 1;                    // bumped when a buffer or a launch parameter changes
+  uint64_t tick = 0;
+  uint64_t replays = 0;                // frames replayed from a captured graph
+  std::vector<GraphEntry> graphs;      // captured frames, least recently used evicted
+  GraphKey graph_pending{}, graph_settled{};  // the last eager run; the last one that completed
+  bool pending_valid = false, settled_valid = false;
   char* scratch_ptr = nullptr;         // device address of this device's `scratch`
   hipEvent_t xev = nullptr;            // orders the context's stream with a caller's stream
   rzk::Status last{};                  // status of the last run collected by finish()
@@ -115,6 +147,7 @@ struct SplitEst {
 };
 
 struct rz_dpath {
+  uint64_t serial = g_serial.fetch_add(1);
   rzk::DPath view{};
   std::vector<void*> allocs;
   uint32_t n_order = 0;  // padded length of `order`
@@ -140,6 +173,9 @@ rz_status fail(rz_ctx* ctx, rz_status s, const std::string& msg) {
 
 rz_status ensure(rz_ctx* ctx, DBuf& b, size_t bytes) {
   if (b.cap >= bytes && b.p) return RZ_OK;
+  if (ctx->capturing)
+    return fail(ctx, RZ_E_CAPACITY, "a buffer would grow while a frame is being captured");
+  ++ctx->gen;
   if (ctx->noalloc)
     return fail(ctx, RZ_E_CAPACITY, "no-alloc call: a buffer of " + std::to_string(bytes) +
                                         " bytes does not fit its part of the scratch (" +
@@ -763,12 +799,19 @@ rz_status finish(rz_ctx* ctx) {
   const uint32_t err = s.err ? s.err : sn.err;
   if (err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
   if (err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
+  // a run that completed with its buffers as they are can be captured next time
+  if (ctx->pending_valid && !s.overflow && ctx->graph_pending.gen == ctx->gen) {
+    ctx->graph_settled = ctx->graph_pending;
+    ctx->settled_valid = true;
+  }
+  ctx->pending_valid = false;
   if (sn.overflow && !s.overflow)
     return fail(ctx, RZ_E_CAPACITY,
                 "a no-alloc call outgrew the scratch (" + std::to_string(sn.n_lines) +
                     " lines, " + std::to_string(sn.n_records) + " records); its output was not "
                     "written");
   if (s.overflow) {
+    ++ctx->gen;
     if (s.n_records > ctx->rec_cap) {
       ctx->rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)s.n_records * 5 / 4 + 1024, 0xffffffffu);
       RZ_TRY(ensure(ctx, ctx->recs, (size_t)ctx->rec_cap * 16));
@@ -862,6 +905,7 @@ rz_status collect_prof(rz_ctx* ctx) {
 // Scenes: Scene::group(fills).render(rasterizer, tr, view, bg)
 // ---------------------------------------------------------------------------
 struct rz_dscene {
+  uint64_t serial = g_serial.fetch_add(1);  // renewed by rz_atlas_set
   uint32_t n = 0;
   rz_dpath* all = nullptr;             // every path's segments, for the batched form
   std::vector<rz_dpath*> per;          // one resident path each, for the plain form
@@ -1462,7 +1506,85 @@ rz_status atlas_set_impl(rz_ctx* ctx, rz_dscene* s, const rz_atlas_slot* slots) 
   s->atlas_bands = (uint32_t)b;
   s->atlas_rows = (uint32_t)r;
   s->atlas_ready = true;
+  s->serial = g_serial.fetch_add(1);  // frames captured with the old placement do not match
   return RZ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hipGraph replay of repeated frames. With RZ_PARAM_GRAPH on, a device entry
+// point whose arguments (GraphKey) match a frame that already completed once
+// with the context's buffers as they are now is captured (one stream capture
+// of exactly the kernels an eager call enqueues: fills instead of memsets, no
+// host copies) and replayed from then on with one hipGraphLaunch. Anything
+// that changes a buffer or a launch parameter renews the generation, so older
+// captures no longer match. Data the kernels read (resident points and
+// colours, rz_scene_update_device) may change between replays.
+// ---------------------------------------------------------------------------
+GraphKey graph_key(rz_ctx* ctx, int kind, uint64_t obj) {
+  GraphKey k;
+  std::memset(&k, 0, sizeof(k));
+  k.kind = kind;
+  k.obj = obj;
+  k.gen = ctx->gen;
+  return k;
+}
+
+bool same_key(const GraphKey& a, const GraphKey& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+constexpr size_t kMaxGraphs = 8;
+
+void drop_graphs(rz_ctx* ctx) {
+  for (auto& e : ctx->graphs) (void)hipGraphExecDestroy(e.exec);
+  ctx->graphs.clear();
+}
+
+template <class F>
+rz_status run_graphed(rz_ctx* ctx, const GraphKey& k, F&& enqueue) {
+  if (!ctx->use_graphs || ctx->prof) return enqueue();
+  for (auto& e : ctx->graphs)
+    if (same_key(e.key, k)) {
+      e.last_use = ++ctx->tick;
+      RZ_HIP(ctx, hipGraphLaunch(e.exec, ctx->stream));
+      ++ctx->replays;
+      return RZ_OK;
+    }
+  if (ctx->settled_valid && same_key(ctx->graph_settled, k)) {
+    hipGraph_t g = nullptr;
+    RZ_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
+    const rz_status st = enqueue();
+    ctx->capturing = false;
+    const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+    hipGraphExec_t ex = nullptr;
+    hipError_t ei = hipErrorUnknown;
+    if (st == RZ_OK && ee == hipSuccess && g)
+      ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    ctx->settled_valid = false;
+    if (ei == hipSuccess) {
+      if (ctx->graphs.size() >= kMaxGraphs) {
+        auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
+                                    [](const GraphEntry& a, const GraphEntry& b) {
+                                      return a.last_use < b.last_use;
+                                    });
+        (void)hipGraphExecDestroy(lru->exec);
+        ctx->graphs.erase(lru);
+      }
+      ctx->graphs.push_back(GraphEntry{k, ex, ++ctx->tick});
+      RZ_HIP(ctx, hipGraphLaunch(ex, ctx->stream));
+      ++ctx->replays;
+      return RZ_OK;
+    }
+    // the capture did not work out: run this frame eagerly
+  }
+  const rz_status st = enqueue();
+  // a run that grew nothing is a candidate for capture once it has completed
+  if (st == RZ_OK && ctx->gen == k.gen) {
+    ctx->graph_pending = k;
+    ctx->pending_valid = true;
+  }
+  return st;
 }
 
 rz_status check_scene_args(rz_ctx* ctx, double flatness, int mode, const double* view) {
@@ -1483,6 +1605,14 @@ extern "C" {
 
 rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   if (!ctx) return RZ_E_ARG;
+  if (param == RZ_PARAM_GRAPH_REPLAYS) return fail(ctx, RZ_E_ARG, "read-only parameter");
+  ++ctx->gen;  // frames captured with the old value no longer match
+  if (param == RZ_PARAM_GRAPH) {
+    if (value != 0 && value != 1) return fail(ctx, RZ_E_ARG, "parameter out of range 0..1");
+    ctx->use_graphs = value != 0;
+    if (!ctx->use_graphs) drop_graphs(ctx);
+    return RZ_OK;
+  }
   if (param == RZ_PARAM_SCAN_DIRECT) {
     if (value < 0 || value > 16) return fail(ctx, RZ_E_ARG, "parameter out of range 0..16");
     ctx->scan_direct = 1u << value;
@@ -1509,6 +1639,8 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
   else if (param == RZ_PARAM_SPLIT_AUTO) *value = ctx->split_auto ? 1 : 0;
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
+  else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
+  else if (param == RZ_PARAM_GRAPH_REPLAYS) *value = (int)std::min<uint64_t>(ctx->replays, INT32_MAX);
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
   return RZ_OK;
 }
@@ -1574,6 +1706,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                   &ctx->sc_pi,     &ctx->sc_counts,
                   &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
                   &ctx->wave_lines, &ctx->atlas_tiles};
+  drop_graphs(ctx);
   for (DBuf* b : bufs) free_buf(*b);
   for (auto& r : ctx->pending) {
     (void)hipEventDestroy(r.a);
@@ -1690,7 +1823,18 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double
   RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, width, height));
   if (!d || (!d_out && width && height)) return fail(ctx, RZ_E_ARG, "null argument");
   if (row_stride < width) return fail(ctx, RZ_E_ARG, "row_stride < width");
-  return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width, height, d_out, row_stride);
+  GraphKey k = graph_key(ctx, 0, d->serial);
+  std::memcpy(k.tr, tr.m, sizeof(k.tr));
+  k.flatness = flatness;
+  k.rule = fill_rule;
+  k.mode = edge_mode;
+  k.w = width;
+  k.h = height;
+  k.stride = row_stride;
+  k.ptr[0] = d_out;
+  return run_graphed(ctx, k, [&]() {
+    return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width, height, d_out, row_stride);
+  });
 }
 
 // The no-alloc form of rz_mask_device: the same pipeline on buffers carved
@@ -1955,9 +2099,21 @@ rz_status rz_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
   if (lw && lh && !d_rgba) return fail(ctx, RZ_E_ARG, "d_rgba is NULL");
   if (row_stride < (size_t)lw) return fail(ctx, RZ_E_ARG, "row_stride < layer width");
   const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
-  if (batched)
-    return enqueue_scene_batched(ctx, scene, tr, view, c, edge_mode, flatness, (rzp::C4*)d_rgba,
-                                 row_stride);
+  if (batched) {
+    GraphKey k = graph_key(ctx, 1, scene->serial);
+    std::memcpy(k.tr, tr.m, sizeof(k.tr));
+    std::memcpy(k.view, view, sizeof(k.view));
+    if (bg) std::memcpy(k.bg, bg, sizeof(k.bg));
+    k.has_bg = bg != nullptr;
+    k.flatness = flatness;
+    k.mode = edge_mode;
+    k.stride = row_stride;
+    k.ptr[0] = d_rgba;
+    return run_graphed(ctx, k, [&]() {
+      return enqueue_scene_batched(ctx, scene, tr, view, c, edge_mode, flatness, (rzp::C4*)d_rgba,
+                                   row_stride);
+    });
+  }
   return enqueue_scene_plain(ctx, scene, tr, view, c, edge_mode, flatness, (rzp::C4*)d_rgba,
                              row_stride);
 }
@@ -2037,7 +2193,15 @@ rz_status rz_mask_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flat
   RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, 0, 0));
   if (!glyphs || !glyphs->atlas_ready) return fail(ctx, RZ_E_ARG, "glyph placement not set");
   if (glyphs->n && !d_atlas) return fail(ctx, RZ_E_ARG, "d_atlas is NULL");
-  return enqueue_atlas(ctx, glyphs, fill_rule, edge_mode, flatness, d_atlas, row_stride);
+  GraphKey k = graph_key(ctx, 2, glyphs->serial);
+  k.flatness = flatness;
+  k.rule = fill_rule;
+  k.mode = edge_mode;
+  k.stride = row_stride;
+  k.ptr[0] = d_atlas;
+  return run_graphed(ctx, k, [&]() {
+    return enqueue_atlas(ctx, glyphs, fill_rule, edge_mode, flatness, d_atlas, row_stride);
+  });
 }
 
 rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas_slot* slots,
@@ -2086,6 +2250,63 @@ rz_status rz_scene_update_device(rz_ctx* ctx, rz_dscene* scene, const double* d_
   return check_launch(ctx, "scene update");
 }
 
+rz_status rz_encode_rgba8_device(rz_ctx* ctx, const float* d_rgba, size_t width, size_t height,
+                                 size_t row_stride, uint8_t* d_out, size_t out_stride,
+                                 int bottom_up) {
+  if (!ctx) return RZ_E_ARG;
+  if (width == 0 || height == 0) return RZ_OK;
+  if (!d_rgba || !d_out) return fail(ctx, RZ_E_ARG, "null argument");
+  if (row_stride < width || out_stride < 4 * width) return fail(ctx, RZ_E_ARG, "stride too small");
+  if (width > (1u << 24) || height > (1u << 24)) return fail(ctx, RZ_E_ARG, "image too large");
+  const size_t n = width * height;
+  const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
+  RZ_LAUNCH(ctx, "k_encode_rgba8", rze::k_encode_rgba8, dim3(grid), dim3(256),
+            (const rzp::C4*)d_rgba, row_stride, (int)width, (int)height, d_out, out_stride,
+            bottom_up ? 1 : 0);
+  return check_launch(ctx, "k_encode_rgba8");
+}
+
+rz_status rz_encode_rgba8(rz_ctx* ctx, const float* rgba, size_t width, size_t height,
+                          size_t row_stride, uint8_t* out, size_t out_stride, int bottom_up) {
+  if (!ctx) return RZ_E_ARG;
+  if (width == 0 || height == 0) return RZ_OK;
+  if (!rgba || !out) return fail(ctx, RZ_E_ARG, "null argument");
+  if (row_stride < width || out_stride < 4 * width) return fail(ctx, RZ_E_ARG, "stride too small");
+  if (width > (1u << 24) || height > (1u << 24)) return fail(ctx, RZ_E_ARG, "image too large");
+  // the image goes to the device densely, the bytes come back densely
+  const size_t in_bytes = width * height * 16, out_bytes = width * height * 4;
+  RZ_TRY(ensure(ctx, ctx->out, in_bytes + out_bytes));
+  char* d = (char*)ctx->out.p;
+  RZ_HIP(ctx, hipMemcpy2DAsync(d, width * 16, rgba, row_stride * 16, width * 16, height,
+                               hipMemcpyHostToDevice, ctx->stream));
+  RZ_TRY(rz_encode_rgba8_device(ctx, (const float*)d, width, height, width,
+                                (uint8_t*)(d + in_bytes), width * 4, bottom_up));
+  RZ_HIP(ctx, hipMemcpy2DAsync(out, out_stride, d + in_bytes, width * 4, width * 4, height,
+                               hipMemcpyDeviceToHost, ctx->stream));
+  RZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RZ_OK;
+}
+
+rz_status rz_bmp_header(size_t width, size_t height, uint8_t* out) {
+  if (!out || width > 0xffffu * 0xffffu || height > 0x7fffffffu ||
+      width * height > (0xffffffffu - rze::kBmpHeader) / 4)
+    return RZ_E_ARG;
+  rze::bmp_header((uint32_t)width, (uint32_t)height, out);
+  return RZ_OK;
+}
+
+rz_status rz_encode_png(const uint8_t* rgba8, size_t width, size_t height, size_t stride,
+                        uint8_t* out, size_t cap, size_t* len) {
+  if (!len || width > 0x7fffffffu || height > 0x7fffffffu) return RZ_E_ARG;
+  if (height && (width * 4 + 1) * height > 0x7fffffffu) return RZ_E_ARG;  // one IDAT chunk
+  *len = rze::png_size(width, height);
+  if (!out) return RZ_OK;
+  if (cap < *len) return RZ_E_CAPACITY;
+  if (width && height && (!rgba8 || stride < 4 * width)) return RZ_E_ARG;
+  rze::png_encode(rgba8, width, height, stride, out);
+  return RZ_OK;
+}
+
 rz_status rz_scene_points(rz_ctx* ctx, const rz_dscene* scene, size_t* n_points) {
   if (!ctx || !scene || !n_points) return RZ_E_ARG;
   *n_points = scene->n_points;
@@ -2100,8 +2321,22 @@ rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_tran
   if (!scene || !d_grad_rgba || !d_grad_points || !d_grad_colors)
     return fail(ctx, RZ_E_ARG, "null argument");
   const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
-  return enqueue_scene_backward(ctx, scene, tr, view, c, edge_mode, flatness, d_grad_rgba,
-                                grad_stride, d_grad_points, d_grad_colors, d_grad_tr);
+  GraphKey k = graph_key(ctx, 3, scene->serial);
+  std::memcpy(k.tr, tr.m, sizeof(k.tr));
+  std::memcpy(k.view, view, sizeof(k.view));
+  if (bg) std::memcpy(k.bg, bg, sizeof(k.bg));
+  k.has_bg = bg != nullptr;
+  k.flatness = flatness;
+  k.mode = edge_mode;
+  k.stride = grad_stride;
+  k.ptr[0] = d_grad_rgba;
+  k.ptr[1] = d_grad_points;
+  k.ptr[2] = d_grad_colors;
+  k.ptr[3] = d_grad_tr;
+  return run_graphed(ctx, k, [&]() {
+    return enqueue_scene_backward(ctx, scene, tr, view, c, edge_mode, flatness, d_grad_rgba,
+                                  grad_stride, d_grad_points, d_grad_colors, d_grad_tr);
+  });
 }
 
 }  // extern "C"
