@@ -164,6 +164,10 @@ struct EllipArc {
     Point v0(1.0, 0.0), v1((x1 - cx) / rx, (y1 - cy) / ry), v2((-x1 - cx) / rx, (-y1 - cy) / ry);
     Scalar eta, ed;
     if (!angle_between(v0, v1, &eta) || !angle_between(v1, v2, &ed)) return false;
+    // A zero or non-finite radius gives NaN angles, from which the reference's
+    // cubic iterator never terminates; such an arc is drawn as the straight
+    // line the SVG implementation notes prescribe for a zero radius.
+    if (!std::isfinite(eta) || !std::isfinite(ed)) return false;
     ed = std::fmod(ed, 2.0 * PI);
     if (ed < 0.0) ed += 2.0 * PI;  // rem_euclid
     if (!sweep && ed > 0.0) ed -= 2.0 * PI;

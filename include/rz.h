@@ -133,6 +133,25 @@ void* rz_stream(rz_ctx* ctx);
 rz_status rz_wait_stream(rz_ctx* ctx, void* stream);
 rz_status rz_stream_wait_ctx(rz_ctx* ctx, void* stream);
 
+/* ---- SVG path data ------------------------------------------------------ */
+
+/* Path::read_svg_path / FromStr for Path (src/path.rs:528-534, 1058-1064;
+ * the parser of src/svg.rs:250-425): SVG path data (M L H V C S Q T A Z and
+ * their relative forms, implicit repeats) into an owned path, built with
+ * PathBuilder's rules (zero-length lines dropped, arcs expanded to cubics).
+ * On a syntax error returns RZ_E_ARG and, if err_offset is not NULL, the
+ * byte offset where parsing stopped. Host only. */
+typedef struct rz_path_buf rz_path_buf;
+rz_status rz_svg_path_parse(const char* text, size_t len, rz_path_buf** out, size_t* err_offset);
+/* FromStr for Transform (src/svg.rs:431-600): an SVG transform list
+ * (matrix, translate[X|Y], scale[X|Y], rotate with an optional centre, skewX,
+ * skewY; deg or rad angles) composed left to right. Host only. */
+rz_status rz_svg_transform_parse(const char* text, size_t len, rz_transform* out,
+                                 size_t* err_offset);
+/* The rz_path view of an owned path (valid until rz_path_buf_free). */
+rz_status rz_path_buf_view(const rz_path_buf* path, rz_path* view);
+rz_status rz_path_buf_free(rz_path_buf* path);
+
 /* ---- stage hooks (bit-exact parity surfaces) ------------------------- */
 
 /* Path::flatten(tr, flatness, close) (src/path.rs:418-425, 734-796): lines as

@@ -10,6 +10,7 @@
 #include <thread>
 #include "orc_dual.hpp"
 #include "orc_exact.hpp"
+#include "orc_svg.hpp"
 #include "orc_paint.hpp"
 
 using namespace orc;
@@ -361,6 +362,55 @@ int orc_sd_mask_iter_dual(const orc_path* p, const double* tan, const double* tr
     sd_mask_iter_dual(path, tan, to_tr(tr), flatness, (FillRule)rule, W, H, alpha, dalpha);
   } catch (const std::runtime_error&) {
     return ORC_E_NAN;
+  }
+  return ORC_OK;
+}
+
+// Path::read_svg_path (src/path.rs:528-534) of `text`: the path's arrays
+// (layout of orc_path) into caller buffers of the given capacities; the sizes
+// always go to *n_seg / *n_pts / *n_sub (subs holds n_sub + 1 offsets).
+// Returns ORC_E_ARG on a syntax error
+// (offset in *n_seg), ORC_E_CAP when a buffer is too small.
+int orc_svg_parse(const char* text, size_t len, uint8_t* kinds, uint32_t* offs, double* pts,
+                  uint32_t* subs, uint8_t* closed, size_t cap_seg, size_t cap_pts,
+                  size_t cap_sub, size_t* n_seg, size_t* n_pts, size_t* n_sub) {
+  Path path;
+  try {
+    path = read_svg_path(std::string(text, len));
+  } catch (const SvgError& e) {
+    *n_seg = e.offset;
+    return ORC_E_ARG;
+  }
+  size_t np = 0;
+  for (const auto& sg : path.segments) np += (size_t)sg.npts();
+  *n_seg = path.segments.size();
+  *n_pts = np;
+  *n_sub = path.nsubpaths();
+  if (!kinds) return ORC_OK;
+  if (cap_seg < *n_seg || cap_pts < np || cap_sub < path.subpaths.size()) return ORC_E_CAP;
+  size_t o = 0;
+  offs[0] = 0;
+  for (size_t k = 0; k < path.segments.size(); ++k) {
+    const Segment& sg = path.segments[k];
+    kinds[k] = (uint8_t)sg.kind;
+    for (int i = 0; i < sg.npts(); ++i, ++o) {
+      pts[2 * o] = sg.p[i].x;
+      pts[2 * o + 1] = sg.p[i].y;
+    }
+    offs[k + 1] = (uint32_t)o;
+  }
+  for (size_t i = 0; i < path.subpaths.size(); ++i) subs[i] = (uint32_t)path.subpaths[i];
+  for (size_t i = 0; i < path.closed.size(); ++i) closed[i] = path.closed[i];
+  return ORC_OK;
+}
+
+// FromStr for Transform (src/svg.rs:592-601); ORC_E_ARG on a syntax error.
+int orc_svg_transform(const char* text, size_t len, double* out6) {
+  try {
+    const Transform t = read_svg_transform(std::string(text, len));
+    for (int i = 0; i < 6; ++i) out6[i] = t.m[i];
+  } catch (const SvgError&) {
+    return ORC_E_ARG;
   }
   return ORC_OK;
 }

@@ -414,6 +414,9 @@ struct EllipArc {
     double eta, ed;
     if (!angle_between(v0, v1, &eta)) return false;
     if (!angle_between(v1, v2, &ed)) return false;
+    // deviation: NaN angles (a zero or non-finite radius) make the reference's
+    // EllipArcCubicIter yield forever; a line, as the SVG notes prescribe
+    if (!std::isfinite(eta) || !std::isfinite(ed)) return false;
     ed = rem_euclid(ed, 2.0 * PI);
     if (!sweep_flag && ed > 0.0) ed = ed - 2.0 * PI;
     else if (sweep_flag && ed < 0.0) ed = ed + 2.0 * PI;

@@ -123,6 +123,26 @@ class Path:
         closed = np.ctypeslib.as_array(p.closed, (nsub,)).copy() if nsub else np.zeros(0, np.uint8)
         return Path(kinds, offs, pts, subs, closed)
 
+    @staticmethod
+    def from_svg(text):
+        """FromStr for Path (src/path.rs:1058-1064): SVG path data to a Path
+        (rz_svg_path_parse). Raises ValueError with the byte offset on a
+        syntax error."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        err = C.c_size_t()
+        L = lib()
+        if L.rz_svg_path_parse(data, len(data), C.byref(h), C.byref(err)) != _abi.RZ_OK:
+            raise ValueError(f"invalid SVG path data at byte {err.value}")
+        try:
+            v = _abi.rz_path()
+            L.rz_path_buf_view(h, C.byref(v))
+            return Path.from_abi(v) if v.n_segments else Path(
+                np.zeros(0, np.uint8), np.zeros(1, np.uint32), np.zeros(0), np.zeros(0, np.uint32),
+                np.zeros(0, np.uint8))
+        finally:
+            L.rz_path_buf_free(h)
+
     def segments_count(self):
         return int(self.kinds.size)
 

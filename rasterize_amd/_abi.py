@@ -93,6 +93,10 @@ PROTOTYPES = [
     ("rz_stream", C.c_void_p, [c_ctx]),
     ("rz_wait_stream", C.c_int, [c_ctx, C.c_void_p]),
     ("rz_stream_wait_ctx", C.c_int, [c_ctx, C.c_void_p]),
+    ("rz_svg_path_parse", C.c_int, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(C.c_size_t)]),
+    ("rz_path_buf_view", C.c_int, [C.c_void_p, P(rz_path)]),
+    ("rz_svg_transform_parse", C.c_int, [C.c_char_p, C.c_size_t, P(rz_transform), P(C.c_size_t)]),
+    ("rz_path_buf_free", C.c_int, [C.c_void_p]),
     ("rz_flatten", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int,
                              P(C.c_double), C.c_size_t, P(C.c_size_t)]),
     ("rz_bin", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int, C.c_size_t,

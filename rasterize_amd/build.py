@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librz.so")
 
-SOURCES = ["rz_api.hip", "rz_workloads.cpp"]
+SOURCES = ["rz_api.hip", "rz_workloads.cpp", "rz_svg.cpp"]
 
 # -ffp-contract=off: flattening must reproduce the reference's f64 rounding
 # (Rust never fuses multiply-add). Packed FP32 instructions are disabled.

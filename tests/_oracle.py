@@ -79,6 +79,9 @@ def lib():
             "orc_mask_atlas_mt": (i, [sz, P(orc_path), P(d), P(C.c_uint32), d, i, P(C.c_float), sz,
                                       sz, i]),
             "orc_sd_mask_iter_dual": (i, [P(orc_path), P(d), P(d), d, i, sz, sz, P(d), P(d)]),
+            "orc_svg_transform": (i, [C.c_char_p, sz, P(d)]),
+            "orc_svg_parse": (i, [C.c_char_p, sz, P(C.c_uint8), P(C.c_uint32), P(d), P(C.c_uint32),
+                                  P(C.c_uint8), sz, sz, sz, P(sz), P(sz), P(sz)]),
             "orc_bin_replay": (i, [P(orc_path), P(d), d, i, sz, sz, P(C.c_uint32), P(C.c_float),
                                    sz, P(C.c_int32), P(C.c_float), sz, P(C.c_int32), P(sz),
                                    P(sz), i, i]),
@@ -323,3 +326,36 @@ def sd_mask_iter_dual(path, tan, tr, W, H, rule=0, flatness=0.05):
     _check(lib().orc_sd_mask_iter_dual(C.byref(p), _dp(tn), _dp(t), flatness, rule, W, H, _dp(a),
                                        _dp(da)))
     return a, da
+
+
+def svg_parse(text):
+    """Path::read_svg_path (src/path.rs:528-534) as the oracle restates it:
+    (kinds, point_offsets, points, subpaths, closed) arrays, or raises
+    OracleError with the byte offset on a syntax error."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    L = lib()
+    ns, npt, nsub = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    rc = L.orc_svg_parse(data, len(data), None, None, None, None, None, 0, 0, 0, C.byref(ns),
+                         C.byref(npt), C.byref(nsub))
+    if rc == 3:
+        raise OracleError(f"invalid SVG path data at byte {ns.value}")
+    kinds = np.zeros(ns.value, np.uint8)
+    offs = np.zeros(ns.value + 1, np.uint32)
+    pts = np.zeros(2 * npt.value)
+    subs = np.zeros(nsub.value + 1 if nsub.value else 0, np.uint32)
+    closed = np.zeros(nsub.value, np.uint8)
+    u8, u32 = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)
+    _check(L.orc_svg_parse(data, len(data), kinds.ctypes.data_as(u8), offs.ctypes.data_as(u32),
+                           _dp(pts), subs.ctypes.data_as(u32), closed.ctypes.data_as(u8),
+                           ns.value, npt.value, len(subs), C.byref(ns), C.byref(npt),
+                           C.byref(nsub)))
+    return kinds, offs, pts, subs, closed
+
+
+def svg_transform(text):
+    """FromStr for Transform as the oracle restates it: the 6 entries."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    out = np.zeros(6)
+    if lib().orc_svg_transform(data, len(data), _dp(out)) != 0:
+        raise OracleError("invalid SVG transform")
+    return out

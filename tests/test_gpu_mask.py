@@ -413,3 +413,23 @@ def test_many_coincident_edges(rule):
     rz.mask_atlas([path], [rz.atlas_slot(7, 9, W, H)], atlas, rule)
     check_close(atlas[9:9 + H, 7:7 + W], O.sd_mask(path, I, W, H, rule), "atlas view")
     assert (atlas[:9] == -1.0).all() and (atlas[:, :7] == -1.0).all()
+
+
+@pytest.mark.parametrize("name", ["squirrel.path", "tv.path", "rust.path"])
+def test_real_world_paths_from_svg(name):
+    """Path files the reference ships (data/*.path, copied under
+    tests/golden), parsed by rz_svg_path_parse and fitted into 512 px as the
+    reference's rasterize example fits them (Transform::fit_size, src/
+    geometry.rs:470-516), masked with both rules against the oracle."""
+    import os
+    text = open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)).read()
+    path = rz.Path.from_svg(text)
+    bb = O.path_bbox(path, I)
+    x0, y0, x1, y1 = np.floor(bb[0]), np.floor(bb[1]), np.ceil(bb[2]), np.ceil(bb[3])
+    W = 512
+    H = int(np.ceil((y1 - y0) * W / (x1 - x0)))
+    s = min(H / (y1 - y0), W / (x1 - x0))
+    tr = Transform(s, 0.0, (W - (x1 - x0) * s) / 2.0 - x0 * s, 0.0, s, (H - (y1 - y0) * s) / 2.0 - y0 * s)
+    for rule in (0, 1):
+        got = SignedDifferenceRasterizer().mask_f32(path, tr, W, H, rule)
+        check_close(got, O.sd_mask(path, tr, W, H, rule), f"{name} {W}x{H} rule {rule}")
