@@ -279,6 +279,41 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
                     int edge_mode, const double* view, const float* bg, int batched, float* rgba,
                     size_t cap_px, int32_t* layer_xywh);
 
+/* A scene tree (SceneInner, src/scene.rs:19-63) as a node array: Fill (path,
+ * paint, fill_rule), Group (children[first .. first + count)), Transform
+ * (tr, child), Opacity (opacity, used as given: Scene::opacity's clamp to
+ * [0, 1] is the caller's, as in the reference's deserialized scenes; child) and Clip (the clip
+ * path, fill_rule, units: RZ_UNITS_USER_SPACE or RZ_UNITS_BBOX of the child,
+ * child). Indices refer to the paths / paints / nodes / children arrays. */
+enum { RZ_NODE_FILL = 0, RZ_NODE_GROUP = 1, RZ_NODE_TRANSFORM = 2, RZ_NODE_OPACITY = 3,
+       RZ_NODE_CLIP = 4 };
+typedef struct rz_scene_node {
+  int32_t kind;
+  int32_t fill_rule;
+  int32_t units;
+  uint32_t path;
+  uint32_t paint;
+  uint32_t child;
+  uint32_t first, count;
+  double tr[6];
+  double opacity;
+} rz_scene_node;
+
+/* Blocking Scene::render(rasterizer, tr, view, bg) (src/scene.rs:186-199) of
+ * the tree rooted at nodes[root], with Pipeline::build_rec's bboxes and views
+ * (src/scene.rs:268-357) and render_rec's layers (src/scene.rs:397-459): the
+ * fills drawn into one layer go through one batched render (each under its
+ * own transform), an Opacity or Clip node into a layer of its own, composed
+ * into its parent (Layer::compose, src/scene.rs:528-560). Output as rz_render:
+ * layer_xywh always, the pixels when rgba holds them (cap_px). edge_mode
+ * selects the rasterizer (RZ_EDGE_ITER: SignedDifference, RZ_EDGE_NONE:
+ * ActiveEdge); clip masks use its Rasterizer::mask. */
+rz_status rz_render_tree(rz_ctx* ctx, const rz_path* const* paths, size_t n_paths,
+                         const rz_paint* paints, size_t n_paints, const rz_scene_node* nodes,
+                         size_t n_nodes, const uint32_t* children, uint32_t root, rz_transform tr,
+                         double flatness, int edge_mode, const double* view, const float* bg,
+                         float* rgba, size_t cap_px, int32_t* layer_xywh);
+
 /* ---- differentiable render --------------------------------------------------- */
 
 /* Replace a resident scene's control points (device f64 x,y pairs of every

@@ -415,6 +415,84 @@ int orc_svg_transform(const char* text, size_t len, double* out6) {
   return ORC_OK;
 }
 
+// Scene::render of a scene tree given as rz_scene_node's layout (kind,
+// fill_rule, units, path, paint, child, first, count, tr[6], opacity).
+typedef struct {
+  int32_t kind, fill_rule, units;
+  uint32_t path, paint, child, first, count;
+  double tr[6];
+  double opacity;
+} orc_scene_node;
+
+static SceneRef tree_node(const orc_scene_node* nodes, const uint32_t* children, uint32_t id,
+                          std::vector<std::shared_ptr<Path>>& paths,
+                          std::vector<std::shared_ptr<Paint>>& paints, int depth) {
+  if (depth > 256) throw std::runtime_error("scene tree too deep");
+  const orc_scene_node& n = nodes[id];
+  auto s = std::make_shared<SceneNode>();
+  switch (n.kind) {
+    case 0:
+      s->kind = SceneNode::FILL;
+      s->path = paths[n.path];
+      s->paint = paints[n.paint];
+      s->rule = (FillRule)n.fill_rule;
+      break;
+    case 1:
+      s->kind = SceneNode::GROUP;
+      for (uint32_t k = 0; k < n.count; ++k)
+        s->children.push_back(tree_node(nodes, children, children[n.first + k], paths, paints, depth + 1));
+      break;
+    case 2:
+      s->kind = SceneNode::TRANSFORM;
+      s->tr = to_tr(n.tr);
+      s->child = tree_node(nodes, children, n.child, paths, paints, depth + 1);
+      break;
+    case 3:
+      s->kind = SceneNode::OPACITY;
+      s->opacity = n.opacity;  // as given (SceneInner::Opacity); Scene::opacity clamps
+      s->child = tree_node(nodes, children, n.child, paths, paints, depth + 1);
+      break;
+    default:
+      s->kind = SceneNode::CLIP;
+      s->path = paths[n.path];
+      s->rule = (FillRule)n.fill_rule;
+      s->units = (Units)n.units;
+      s->child = tree_node(nodes, children, n.child, paths, paints, depth + 1);
+      break;
+  }
+  return s;
+}
+
+int orc_scene_tree_render(const orc_path* paths, size_t n_paths, const orc_paint* paints,
+                          size_t n_paints, const orc_scene_node* nodes, const uint32_t* children,
+                          uint32_t root, const double* tr, const double* view, const float* bg,
+                          double flatness, int active_edge, float* out, size_t cap_px,
+                          int64_t* layer_xywh) {
+  std::vector<std::shared_ptr<Path>> ps;
+  std::vector<std::shared_ptr<Paint>> pt;
+  for (size_t i = 0; i < n_paths; ++i) ps.push_back(std::make_shared<Path>(to_path(&paths[i])));
+  for (size_t i = 0; i < n_paints; ++i) pt.push_back(std::make_shared<Paint>(to_paint(&paints[i])));
+  BBox vb;
+  if (view) vb = BBox::make(P(view[0], view[1]), P(view[2], view[3]));
+  LinColor bgc;
+  if (bg) bgc = LinColor{f4(bg[0], bg[1], bg[2], bg[3])};
+  Layer layer;
+  try {
+    SceneRef root_node = tree_node(nodes, children, root, ps, pt, 0);
+    layer = scene_render(root_node.get(), to_tr(tr), view ? &vb : nullptr, bg ? &bgc : nullptr,
+                         flatness, active_edge != 0);
+  } catch (const std::runtime_error&) {
+    return ORC_E_NAN;
+  }
+  layer_xywh[0] = layer.x;
+  layer_xywh[1] = layer.y;
+  layer_xywh[2] = (int64_t)layer.shape.width;
+  layer_xywh[3] = (int64_t)layer.shape.height;
+  if (layer.image.size() > cap_px) return ORC_E_CAP;
+  std::memcpy(out, layer.image.data(), layer.image.size() * sizeof(F4));
+  return ORC_OK;
+}
+
 // Paint::at for a single point (gradient and colour known-answer tests)
 int orc_paint_at(const orc_paint* paint, double x, double y, float* out4) {
   Paint pt = to_paint(paint);

@@ -240,11 +240,13 @@ struct ImageRGBA {
 };
 
 // Rasterizer::fill default + fill_impl (src/rasterize.rs:69-115) with the
-// signed-difference mask_iter.
+// signed-difference mask_iter, or with ActiveEdgeRasterizer's (active_edge:
+// ActiveEdgeIter over the flattened lines, src/rasterize.rs:558-576).
 // Rows [r0, r1) of the image only (all of it by default), through
-// sd_mask_iter_rows: the same pixels and values.
+// sd_mask_iter_rows: the same pixels and values (signed difference only).
 inline void sd_fill(const Path& path, const Transform& tr, double flatness, FillRule rule,
-                    const Paint& paint, ImageRGBA img, size_t r0 = 0, size_t r1 = SIZE_MAX) {
+                    const Paint& paint, ImageRGBA img, size_t r0 = 0, size_t r1 = SIZE_MAX,
+                    bool active_edge = false) {
   auto blend = [&](const Pixel& px, LinColor color) {
     F4& dst = img.data[img.shape.offset(px.y, px.x)];
     LinColor d{dst};
@@ -252,7 +254,11 @@ inline void sd_fill(const Path& path, const Transform& tr, double flatness, Fill
   };
   const bool whole = r0 == 0 && r1 >= img.shape.height;
   auto mask_iter = [&](auto&& sink) {
-    if (whole)
+    if (active_edge) {
+      std::vector<Line> lines;
+      flatten(path, tr, flatness, true, [&](const Line& l) { lines.push_back(l); });
+      ae_iter(img.shape.width, img.shape.height, rule, lines, sink);
+    } else if (whole)
       sd_mask_iter(path, tr, flatness, rule, img.shape.width, img.shape.height, sink);
     else
       sd_mask_iter_rows(path, tr, flatness, rule, img.shape.width, img.shape.height, r0, r1, sink);
@@ -279,20 +285,55 @@ inline void sd_fill(const Path& path, const Transform& tr, double flatness, Fill
 }
 
 // ---------------------------------------------------------------------------
-// Scene: Group / Transform / Fill (src/scene.rs:87-200, 256-501). Clip,
-// Opacity and Stroke nodes are not part of this checker.
+// Scene: Fill / Group / Transform / Opacity / Clip (src/scene.rs:19-63,
+// 87-200, 256-566). Stroke nodes are not part of this checker.
 // ---------------------------------------------------------------------------
 struct SceneNode;
 typedef std::shared_ptr<SceneNode> SceneRef;
 struct SceneNode {
-  enum Kind { FILL, GROUP, TRANSFORM } kind;
-  std::shared_ptr<Path> path;
+  enum Kind { FILL, GROUP, TRANSFORM, OPACITY, CLIP } kind;
+  std::shared_ptr<Path> path;  // fill; the clip path of a clip
   std::shared_ptr<Paint> paint;
   FillRule rule = NONZERO;
   std::vector<SceneRef> children;
   Transform tr = Transform::identity();
   SceneRef child;
+  double opacity = 1.0;
+  Units units = USER_SPACE_ON_USE;  // clip
 };
+
+// Scene::bbox (src/scene.rs:154-183)
+inline bool scene_bbox(const SceneNode* s, const Transform& tr, BBox* out) {
+  switch (s->kind) {
+    case SceneNode::FILL:
+      return s->path->bbox(tr, out);
+    case SceneNode::GROUP: {
+      bool have = false;
+      for (auto& c : s->children) {
+        BBox b;
+        if (!scene_bbox(c.get(), tr, &b)) continue;
+        *out = have ? out->unite(b) : b;
+        have = true;
+      }
+      return have;
+    }
+    case SceneNode::TRANSFORM:
+      return scene_bbox(s->child.get(), tr * s->tr, out);
+    case SceneNode::OPACITY:
+      return scene_bbox(s->child.get(), tr, out);
+    default: {
+      Transform ctr = tr;
+      if (s->units == BOUNDING_BOX) {
+        BBox cb;
+        if (!scene_bbox(s->child.get(), Transform::identity(), &cb)) return false;
+        ctr = tr * cb.unit_transform();
+      }
+      BBox a, b;
+      if (!scene_bbox(s->child.get(), tr, &a) || !s->path->bbox(ctr, &b)) return false;
+      return a.intersect(b, out);
+    }
+  }
+}
 
 struct Layer {
   std::vector<F4> image;
@@ -314,11 +355,26 @@ struct Layer {
 
 struct PipelineNode {
   SceneNode::Kind kind;
-  const SceneNode* fill;
+  const SceneNode* fill;  // the scene node (fill, opacity, clip)
   std::vector<size_t> children;
   BBox bbox;
   Transform tr;
+  Transform clip_tr;
 };
+
+// Layer::compose (src/scene.rs:528-560): f(dst, src) over the overlap
+template <class F>
+inline void layer_compose(Layer& dst, int sx, int sy, size_t sw, size_t sh, F&& f) {
+  const long x0 = std::max<long>(dst.x, sx);
+  const long x1 = std::min<long>(dst.x + (long)dst.shape.width, sx + (long)sw);
+  const long y0 = std::max<long>(dst.y, sy);
+  const long y1 = std::min<long>(dst.y + (long)dst.shape.height, sy + (long)sh);
+  for (long x = x0; x < x1; ++x)
+    for (long y = y0; y < y1; ++y) {
+      F4& d = dst.image[dst.shape.offset((size_t)(y - dst.y), (size_t)(x - dst.x))];
+      d = f(d, (size_t)(y - sy), (size_t)(x - sx));
+    }
+}
 
 struct Pipeline {
   std::vector<PipelineNode> nodes;
@@ -333,7 +389,7 @@ struct Pipeline {
       BBox pb, vb;
       bool have = s->path->bbox(tr, &pb);
       if (!view_apply(have, pb, &vb)) return false;
-      nodes.push_back(PipelineNode{SceneNode::FILL, s, {}, vb, tr});
+      nodes.push_back(PipelineNode{SceneNode::FILL, s, {}, vb, tr, tr});
       *id = nodes.size() - 1;
       return true;
     }
@@ -350,17 +406,74 @@ struct Pipeline {
         bb = have ? nodes[cid].bbox.unite(bb) : nodes[cid].bbox;
         have = true;
       }
-      nodes.push_back(PipelineNode{SceneNode::GROUP, nullptr, ch, bb, tr});
+      nodes.push_back(PipelineNode{SceneNode::GROUP, nullptr, ch, bb, tr, tr});
+      *id = nodes.size() - 1;
+      return true;
+    }
+    if (s->kind == SceneNode::OPACITY) {
+      size_t cid;
+      if (!build_rec(s->child.get(), view, tr, &cid)) return false;
+      nodes.push_back(PipelineNode{SceneNode::OPACITY, s, {cid}, nodes[cid].bbox, tr, tr});
+      *id = nodes.size() - 1;
+      return true;
+    }
+    if (s->kind == SceneNode::CLIP) {
+      Transform clip_tr = tr;
+      if (s->units == BOUNDING_BOX) {
+        BBox cb;
+        if (!scene_bbox(s->child.get(), Transform::identity(), &cb)) return false;
+        clip_tr = tr * cb.unit_transform();
+      }
+      BBox pb, clip_bbox;
+      if (!s->path->bbox(clip_tr, &pb) || !view_apply(true, pb, &clip_bbox)) return false;
+      size_t cid;
+      if (!build_rec(s->child.get(), &clip_bbox, tr, &cid)) return false;
+      BBox bb;
+      if (!clip_bbox.intersect(nodes[cid].bbox, &bb)) return false;
+      nodes.push_back(PipelineNode{SceneNode::CLIP, s, {cid}, bb, tr, clip_tr});
       *id = nodes.size() - 1;
       return true;
     }
     return build_rec(s->child.get(), view, tr * s->tr, id);
   }
   // src/scene.rs:397-435
+  bool active_edge = false;  // ActiveEdgeRasterizer for fills and clip masks
   void render_rec(size_t id, Layer& layer, double flatness) const {
     const PipelineNode& n = nodes[id];
     if (n.kind == SceneNode::GROUP) {
       for (size_t c : n.children) render_rec(c, layer, flatness);
+      return;
+    }
+    if (n.kind == SceneNode::OPACITY) {
+      Layer child = Layer::make(nodes[n.children[0]].bbox, nullptr);
+      render_rec(n.children[0], child, flatness);
+      const float op = (float)n.fill->opacity;
+      layer_compose(layer, child.x, child.y, child.shape.width, child.shape.height,
+                    [&](F4 d, size_t r, size_t c) {
+                      const LinColor s{child.image[child.shape.offset(r, c)] * op};
+                      return LinColor{d}.blend_over(s).c;
+                    });
+      return;
+    }
+    if (n.kind == SceneNode::CLIP) {
+      Layer mask = Layer::make(n.bbox, nullptr);
+      std::vector<double> m(mask.shape.width * mask.shape.height, 0.0);
+      Layer child = Layer::make(nodes[n.children[0]].bbox, nullptr);
+      render_rec(n.children[0], child, flatness);
+      const Transform align = Transform::new_translate(-(double)mask.x, -(double)mask.y);
+      ImageF64 mi{m.data(), m.size(), Shape::simple(mask.shape.height, mask.shape.width)};
+      if (active_edge)
+        ae_mask(*n.fill->path, align * n.clip_tr, flatness, n.fill->rule, mi);
+      else
+        sd_mask(*n.fill->path, align * n.clip_tr, flatness, n.fill->rule, mi);
+      layer_compose(child, mask.x, mask.y, mask.shape.width, mask.shape.height,
+                    [&](F4 d, size_t r, size_t c) {
+                      return d * (float)m[r * mask.shape.width + c];
+                    });
+      layer_compose(layer, child.x, child.y, child.shape.width, child.shape.height,
+                    [&](F4 d, size_t r, size_t c) {
+                      return LinColor{d}.blend_over(LinColor{child.image[child.shape.offset(r, c)]}).c;
+                    });
       return;
     }
     long col_min = (long)std::floor(n.bbox.min.x) - layer.x;
@@ -372,7 +485,8 @@ struct Pipeline {
     Transform align =
         Transform::new_translate(-std::floor(n.bbox.min.x), -std::floor(n.bbox.min.y));
     ImageRGBA img{layer.image.data(), layer.image.size(), vs};
-    sd_fill(*n.fill->path, align * n.tr, flatness, n.fill->rule, *n.fill->paint, img);
+    sd_fill(*n.fill->path, align * n.tr, flatness, n.fill->rule, *n.fill->paint, img, 0, SIZE_MAX,
+            active_edge);
   }
   // render_rec for the layer rows [y0, y1) only: each fill runs over the rows
   // of its sub-view inside the band (sd_fill's row window).
@@ -432,9 +546,11 @@ inline Layer scene_render_mt(const SceneNode* scene, const Transform& tr, const 
 }
 
 // Scene::render (src/scene.rs:186-199) with the signed-difference rasterizer
+// (or the active-edge one)
 inline Layer scene_render(const SceneNode* scene, const Transform& tr, const BBox* view,
-                          const LinColor* bg, double flatness) {
+                          const LinColor* bg, double flatness, bool active_edge = false) {
   Pipeline p;
+  p.active_edge = active_edge;
   size_t root;
   if (!p.build_rec(scene, view, tr, &root)) return Layer{};
   (void)root;

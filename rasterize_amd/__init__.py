@@ -81,6 +81,18 @@ class Transform:
     def pre_rotate(self, a):
         return self * Transform.new_rotate(a)
 
+    @staticmethod
+    def from_svg(text):
+        """FromStr for Transform (src/svg.rs:591-601): an SVG transform list
+        (rz_svg_transform_parse). Raises ValueError with the byte offset on a
+        syntax error."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        t = _abi.rz_transform()
+        err = C.c_size_t()
+        if lib().rz_svg_transform_parse(data, len(data), C.byref(t), C.byref(err)) != _abi.RZ_OK:
+            raise ValueError(f"invalid SVG transform at byte {err.value}")
+        return Transform(*t.m)
+
     def apply(self, p):
         m = self.m
         return (p[0] * m[0] + p[1] * m[1] + m[2], p[0] * m[3] + p[1] * m[4] + m[5])
@@ -636,50 +648,258 @@ class Layer:
 
 
 class Scene:
-    """Scene::fill / Scene::group (src/scene.rs:64-110). Groups render their
-    children in order, so a tree of groups renders as its fills in order."""
+    """A scene tree (SceneInner, src/scene.rs:19-63): Fill, Group, Transform,
+    Opacity and Clip nodes, built with the constructors of src/scene.rs:87-151
+    or read from the reference's JSON scene format (Scene.from_json)."""
 
-    def __init__(self, fills):
-        self._fills = fills
+    FILL, GROUP, TRANSFORM, OPACITY, CLIP = (_abi.RZ_NODE_FILL, _abi.RZ_NODE_GROUP,
+                                             _abi.RZ_NODE_TRANSFORM, _abi.RZ_NODE_OPACITY,
+                                             _abi.RZ_NODE_CLIP)
+
+    def __init__(self, kind, path=None, paint=None, fill_rule=FillRule.NonZero, children=(),
+                 child=None, tr=None, opacity=1.0, units=0):
+        self.kind = kind
+        self.path, self.paint, self.fill_rule = path, paint, int(fill_rule)
+        self.children = list(children)
+        self.child = child
+        self.tr = tr
+        self.opacity = float(opacity)
+        self.units = int(units)
 
     @staticmethod
     def fill(path, paint, fill_rule=FillRule.NonZero):
-        return Scene([(path, paint, int(fill_rule))])
+        return Scene(Scene.FILL, path=path, paint=paint, fill_rule=fill_rule)
 
     @staticmethod
     def group(children):
-        return Scene([f for c in children for f in c._fills])
+        """A group of one child is that child (src/scene.rs:104-109)."""
+        children = list(children)
+        if len(children) == 1:
+            return children[0]
+        return Scene(Scene.GROUP, children=children)
+
+    def with_opacity(self, opacity):
+        """Scene::opacity (src/scene.rs:112-125): clamped to [0, 1]; nested
+        opacities multiply."""
+        o = min(max(float(opacity), 0.0), 1.0)
+        if self.kind == Scene.OPACITY:
+            return self.child.with_opacity(o * self.opacity)
+        return Scene(Scene.OPACITY, child=self, opacity=o)
+
+    def with_clip(self, clip, units=0, fill_rule=FillRule.NonZero):
+        """Scene::clip (src/scene.rs:128-136): clip path in user space or in
+        the child's bounding box (Units.BoundingBox)."""
+        return Scene(Scene.CLIP, path=clip, fill_rule=fill_rule, units=units, child=self)
+
+    def with_transform(self, tr):
+        """Scene::transform (src/scene.rs:139-151): nested transforms compose."""
+        if self.kind == Scene.TRANSFORM:
+            return self.child.with_transform(tr * self.tr)
+        return Scene(Scene.TRANSFORM, child=self, tr=tr)
+
+    # the reference's method names
+    opacity_of = with_opacity
+    clip_by = with_clip
+    transform_by = with_transform
+
+    @staticmethod
+    def from_json(doc, colors=None):
+        """Deserialize for Scene (src/scene.rs:13-63, 587-640): a JSON text
+        (or its parsed value) of {"type": "fill" | "group" | "transform" |
+        "opacity" | "clip", ...} nodes. Paths and transforms are SVG strings,
+        colours "#rrggbb[aa][/alpha]" or names (`colors` adds names),
+        gradients {"type": "linear-gradient" | "radial-gradient", ...}.
+        Stroke nodes are refused (NotImplementedError)."""
+        import json
+        if isinstance(doc, (str, bytes)):
+            doc = json.loads(doc)
+        try:
+            return _scene_of_json(doc, colors)
+        except KeyError as e:
+            raise ValueError(f"missing field {e.args[0]!r}") from None
+
+    def is_flat(self):
+        """Only fills and groups: renders as its fills in order."""
+        if self.kind == Scene.FILL:
+            return True
+        return self.kind == Scene.GROUP and all(c.is_flat() for c in self.children)
 
     def fills(self):
-        return list(self._fills)
+        """The fills of a flat scene in drawing order, (path, paint, rule)."""
+        if self.kind == Scene.FILL:
+            return [(self.path, self.paint, self.fill_rule)]
+        if self.kind != Scene.GROUP:
+            raise ValueError("fills(): the scene has transform, opacity or clip nodes")
+        return [f for c in self.children for f in c.fills()]
 
     def _abi_arrays(self):
-        n = len(self._fills)
-        keep = [f[0].abi() for f in self._fills]
+        fl = self.fills()
+        n = len(fl)
+        keep = [f[0].abi() for f in fl]
         paths = (C.POINTER(_abi.rz_path) * max(n, 1))(*[C.pointer(p) for p in keep])
-        pk = [f[1].abi() for f in self._fills]
+        pk = [f[1].abi() for f in fl]
         paints = (_abi.rz_paint * max(n, 1))(*pk)
-        rules = (C.c_int32 * max(n, 1))(*[f[2] for f in self._fills])
+        rules = (C.c_int32 * max(n, 1))(*[f[2] for f in fl])
         return paths, paints, rules, (keep, pk)
 
+    def node_arrays(self):
+        """The tree as rz_render_tree's arrays: (paths, paints, nodes,
+        children, root). A path or paint object used by several nodes is
+        stored once."""
+        paths, paints, nodes, children = [], [], [], []
+        pidx, qidx = {}, {}
+
+        def path_id(p):
+            if id(p) not in pidx:
+                pidx[id(p)] = len(paths)
+                paths.append(p)
+            return pidx[id(p)]
+
+        def paint_id(q):
+            if id(q) not in qidx:
+                qidx[id(q)] = len(paints)
+                paints.append(q)
+            return qidx[id(q)]
+
+        def visit(sc, depth):
+            if depth > 256:
+                raise ValueError("scene tree deeper than 256")
+            n = _abi.rz_scene_node()
+            n.kind = sc.kind
+            n.fill_rule = sc.fill_rule
+            n.units = sc.units
+            n.opacity = sc.opacity
+            if sc.kind == Scene.FILL:
+                n.path, n.paint = path_id(sc.path), paint_id(sc.paint)
+            elif sc.kind == Scene.GROUP:
+                ids = [visit(c, depth + 1) for c in sc.children]
+                n.first, n.count = len(children), len(ids)
+                children.extend(ids)
+            else:
+                if sc.kind == Scene.TRANSFORM:
+                    for i, v in enumerate(sc.tr.m):
+                        n.tr[i] = v
+                if sc.kind == Scene.CLIP:
+                    n.path = path_id(sc.path)
+                n.child = visit(sc.child, depth + 1)
+            nodes.append(n)
+            return len(nodes) - 1
+
+        root = visit(self, 0)
+        return paths, paints, nodes, children, root
+
     def render(self, rasterizer, tr, view=None, bg=None, batched=True):
-        """Scene::render (src/scene.rs:186-199) -> Layer. view = (x0, y0, x1, y1)."""
+        """Scene::render (src/scene.rs:186-199) -> Layer. view = (x0, y0, x1, y1).
+        A flat scene renders through rz_render (`batched` picks its form),
+        any other tree through rz_render_tree."""
         ctx = rasterizer._ctx()
-        paths, paints, rules, keep = self._abi_arrays()
-        n = len(self._fills)
         v = (C.c_double * 4)(*view) if view is not None else None
         b = (C.c_float * 4)(*bg) if bg is not None else None
         xywh = (C.c_int32 * 4)()
         L = lib()
-        ctx.check(L.rz_render(ctx.h, paths, paints, rules, n, tr.abi(), rasterizer.flatness,
-                              rasterizer.iter_mode, v, b, int(batched), None, 0, xywh))
+        if self.is_flat():
+            paths, paints, rules, keep = self._abi_arrays()
+            n = len(keep[0])
+
+            def call(img, cap):
+                return L.rz_render(ctx.h, paths, paints, rules, n, tr.abi(), rasterizer.flatness,
+                                   rasterizer.iter_mode, v, b, int(batched), img, cap, xywh)
+        else:
+            ps, qs, nodes, children, root = self.node_arrays()
+            keep = [p.abi() for p in ps]
+            pk = [q.abi() for q in qs]
+            paths = (C.POINTER(_abi.rz_path) * max(len(ps), 1))(*[C.pointer(p) for p in keep])
+            paints = (_abi.rz_paint * max(len(qs), 1))(*pk)
+            na = (_abi.rz_scene_node * len(nodes))(*nodes)
+            ca = (C.c_uint32 * max(len(children), 1))(*children)
+
+            def call(img, cap):
+                return L.rz_render_tree(ctx.h, paths, len(ps), paints, len(qs), na, len(nodes), ca,
+                                        root, tr.abi(), rasterizer.flatness, rasterizer.iter_mode,
+                                        v, b, img, cap, xywh)
+        ctx.check(call(None, 0))
         x, y, w, h = list(xywh)
         img = np.zeros((h, w, 4), np.float32)
         if w and h:
-            ctx.check(L.rz_render(ctx.h, paths, paints, rules, n, tr.abi(), rasterizer.flatness,
-                                  rasterizer.iter_mode, v, b, int(batched),
-                                  img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh))
+            ctx.check(call(img.ctypes.data_as(C.POINTER(C.c_float)), w * h))
         return Layer(img, x, y)
+
+
+def _json_units(v):
+    if v is None or v == "userSpaceOnUse":
+        return Units.UserSpaceOnUse
+    if v == "objectBoundingBox":
+        return Units.BoundingBox
+    raise ValueError(f"unknown units: {v!r}")
+
+
+def _json_rule(v):
+    if v is None or v == "nonzero":
+        return FillRule.NonZero
+    if v == "evenodd":
+        return FillRule.EvenOdd
+    raise ValueError(f"unknown fill rule: {v!r}")
+
+
+def _json_point(v):
+    if not isinstance(v, (list, tuple)) or len(v) != 2:
+        raise ValueError(f"point [x, y] expected, got {v!r}")
+    return (float(v[0]), float(v[1]))
+
+
+def _json_color(text, colors):
+    from .color import parse_rgba
+    return LinColor.from_rgba8(*parse_rgba(text, colors))
+
+
+def _json_paint(v, colors):
+    """serde_paint::deserialize (src/scene.rs:604-640), GradLinearSerialize /
+    GradRadialSerialize (src/grad.rs:249-290, 449-492)."""
+    if isinstance(v, str):
+        return _json_color(v, colors)
+    if not isinstance(v, dict) or "type" not in v:
+        raise ValueError(f"failed to parse paint: {v!r}")
+    if v["type"] not in ("linear-gradient", "radial-gradient"):
+        raise ValueError(f"unknown paint type: {v['type']}")
+    spreads = {"pad": GradSpread.Pad, "repeat": GradSpread.Repeat, "reflect": GradSpread.Reflect}
+    if v.get("spread", "pad") not in spreads:
+        raise ValueError(f"unknown spread: {v['spread']!r}")
+    spread = spreads[v.get("spread", "pad")]
+    stops = [GradStop(float(p), _json_color(c, colors)) for p, c in v["stops"]]
+    common = dict(units=_json_units(v.get("units")), linear_colors=bool(v.get("linear_colors", False)),
+                  spread=spread)
+    if v["type"] == "linear-gradient":
+        tr = Transform.from_svg(v["tr"]) if "tr" in v else Transform.identity()
+        return GradLinear(stops, tr=tr, start=_json_point(v["start"]), end=_json_point(v["end"]),
+                          **common)
+    if v["type"] == "radial-gradient":
+        fc = v.get("fcenter")
+        return GradRadial(stops, tr=Transform.from_svg(v["tr"]), center=_json_point(v["center"]),
+                          radius=float(v["radius"]), fcenter=_json_point(fc) if fc is not None else None,
+                          fradius=float(v.get("fradius", 0.0)), **common)
+
+
+def _scene_of_json(v, colors, depth=0):
+    if depth > 256:
+        raise ValueError("scene tree deeper than 256")
+    t = v.get("type")
+    if t == "fill":
+        return Scene(Scene.FILL, path=Path.from_svg(v["path"]), paint=_json_paint(v["paint"], colors),
+                     fill_rule=_json_rule(v.get("fill_rule")))
+    if t == "group":
+        return Scene(Scene.GROUP, children=[_scene_of_json(c, colors, depth + 1) for c in v["children"]])
+    if t == "transform":
+        return Scene(Scene.TRANSFORM, tr=Transform.from_svg(v["tr"]),
+                     child=_scene_of_json(v["child"], colors, depth + 1))
+    if t == "opacity":
+        return Scene(Scene.OPACITY, opacity=float(v["opacity"]),
+                     child=_scene_of_json(v["child"], colors, depth + 1))
+    if t == "clip":
+        return Scene(Scene.CLIP, path=Path.from_svg(v["clip"]), fill_rule=_json_rule(v.get("fill_rule")),
+                     units=_json_units(v.get("units")), child=_scene_of_json(v["child"], colors, depth + 1))
+    if t == "stroke":
+        raise NotImplementedError("stroke nodes are not supported")
+    raise ValueError(f"unknown scene node type: {t!r}")
 
 
 class DeviceScene:

@@ -74,6 +74,16 @@ class rz_atlas_slot(C.Structure):
                 ("height", C.c_uint32)]
 
 
+RZ_NODE_FILL, RZ_NODE_GROUP, RZ_NODE_TRANSFORM, RZ_NODE_OPACITY, RZ_NODE_CLIP = 0, 1, 2, 3, 4
+
+
+class rz_scene_node(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("fill_rule", C.c_int32), ("units", C.c_int32),
+                ("path", C.c_uint32), ("paint", C.c_uint32), ("child", C.c_uint32),
+                ("first", C.c_uint32), ("count", C.c_uint32), ("tr", C.c_double * 6),
+                ("opacity", C.c_double)]
+
+
 class rz_kernel_stat(C.Structure):
     _fields_ = [("name", C.c_char * 40), ("calls", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -124,6 +134,10 @@ PROTOTYPES = [
     ("rz_render", C.c_int, [c_ctx, P(P(rz_path)), P(rz_paint), P(C.c_int32), C.c_size_t,
                             rz_transform, C.c_double, C.c_int, P(C.c_double), P(C.c_float), C.c_int,
                             P(C.c_float), C.c_size_t, P(C.c_int32)]),
+    ("rz_render_tree", C.c_int, [c_ctx, P(P(rz_path)), C.c_size_t, P(rz_paint), C.c_size_t,
+                                 P(rz_scene_node), C.c_size_t, P(C.c_uint32), C.c_uint32,
+                                 rz_transform, C.c_double, C.c_int, P(C.c_double), P(C.c_float),
+                                 P(C.c_float), C.c_size_t, P(C.c_int32)]),
     ("rz_scene_update_device", C.c_int, [c_ctx, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("rz_scene_points", C.c_int, [c_ctx, C.c_void_p, P(C.c_size_t)]),
     ("rz_render_backward_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,

@@ -17,6 +17,7 @@
 #include "rz_scene.hpp"
 #include "rz_backward.hpp"
 #include "rz_encode.hpp"
+#include "rz_tree.hpp"
 
 namespace {
 
@@ -75,6 +76,8 @@ struct GraphEntry {
 };
 namespace {
 std::atomic<uint64_t> g_serial{1};  // resident paths and scenes
+bool same_key(const GraphKey& a, const GraphKey& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+constexpr size_t kMaxGraphs = 8;  // captured frames kept, and settled keys remembered
 }
 
 struct rz_ctx {
@@ -116,8 +119,9 @@ struct rz_ctx {
   uint64_t tick = 0;
   uint64_t replays = 0;                // frames replayed from a captured graph
   std::vector<GraphEntry> graphs;      // captured frames, least recently used evicted
-  GraphKey graph_pending{}, graph_settled{};  // the last eager run; the last one that completed
-  bool pending_valid = false, settled_valid = false;
+  GraphKey graph_pending{};            // the last eager run
+  bool pending_valid = false;
+  std::vector<GraphKey> graph_settled;  // eager runs that completed, oldest first
   char* scratch_ptr = nullptr;         // device address of this device's `scratch`
   hipEvent_t xev = nullptr;            // orders the context's stream with a caller's stream
   rzk::Status last{};                  // status of the last run collected by finish()
@@ -801,8 +805,13 @@ rz_status finish(rz_ctx* ctx) {
   if (err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
   // a run that completed with its buffers as they are can be captured next time
   if (ctx->pending_valid && !s.overflow && ctx->graph_pending.gen == ctx->gen) {
-    ctx->graph_settled = ctx->graph_pending;
-    ctx->settled_valid = true;
+    auto& st = ctx->graph_settled;
+    bool known = false;
+    for (const auto& e : st) known = known || same_key(e, ctx->graph_pending);
+    if (!known) {
+      if (st.size() >= kMaxGraphs) st.erase(st.begin());
+      st.push_back(ctx->graph_pending);
+    }
   }
   ctx->pending_valid = false;
   if (sn.overflow && !s.overflow)
@@ -971,8 +980,11 @@ void layer_of(const double* v, int* lx, int* ly, int* lw, int* lh) {
   *lh = std::max(0, y1 - y0);
 }
 
+// `trs` (6 doubles per path, or NULL): each fill's own transform, drawn
+// under render tr * trs[i] (a Transform node above it).
 rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* paints,
-                            const int32_t* rules, size_t n, rz_dscene** out) {
+                            const int32_t* rules, size_t n, rz_dscene** out,
+                            const double* trs = nullptr) {
   if (!ctx || !out) return RZ_E_ARG;
   if (n && !paths) return fail(ctx, RZ_E_ARG, "null scene arrays");
   if (n > (1u << 24)) return fail(ctx, RZ_E_ARG, "too many paths");
@@ -1026,6 +1038,10 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
     if (!rzh::prepare_paint_static(paint_i, &pp)) return fail(ctx, RZ_E_ARG, "invalid paint");
     rzk::ScenePath& S = s->host_sp[i];
     S = rzk::ScenePath{};
+    if (trs) {
+      S.has_tr = 1;
+      for (int k = 0; k < 6; ++k) S.tr[k] = trs[6 * i + k];
+    }
     S.seg_begin = seg0;
     S.seg_end = seg0 + ns;
     S.rule = rule_i;
@@ -1085,15 +1101,28 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
 // The batched form: one pipeline for every path of the scene. With d_out
 // null it stops before the fine stage (the adjoint needs only the binned
 // state).
+// `layer` (x, y, w, h; NULL: the layer of the view, Layer::new) and `load`
+// (draw over the layer's pixels rather than start from bg) serve the scene
+// tree, whose fills between two Opacity / Clip nodes are drawn in batches
+// into layers that are not the view's.
 rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
                                 const double* view, rzp::C4 bg, int mode, double flatness,
-                                rzp::C4* d_out, size_t row_stride) {
+                                rzp::C4* d_out, size_t row_stride, const int* layer = nullptr,
+                                int load = 0) {
   int lx, ly, lw, lh;
-  layer_of(view, &lx, &ly, &lw, &lh);
+  if (layer) {
+    lx = layer[0];
+    ly = layer[1];
+    lw = layer[2];
+    lh = layer[3];
+  } else {
+    layer_of(view, &lx, &ly, &lw, &lh);
+  }
   const uint32_t n = s->n;
   if (lw == 0 || lh == 0) return RZ_OK;
   const int ctx_x = (lw + 15) / 16, cty = (lh + 15) / 16;
   if (n == 0) {
+    if (load) return RZ_OK;
     RZ_LAUNCH(ctx, "k_fill_rgba", rzk::k_fill_rgba, dim3(1024), dim3(256), d_out, lw, lh, row_stride, bg);
     return check_launch(ctx, "k_fill_rgba");
   }
@@ -1157,7 +1186,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   unsigned long long* bkeys = (unsigned long long*)(meta + kb);
   if (d->n_slots)
     RZ_LAUNCH(ctx, "k_scene_bbox", rzk::k_scene_bbox, dim3((d->n_slots + 63) / 64), dim3(64),
-              d->view, s->slot_path, v, bkeys);
+              d->view, s->slot_path, v, s->sp, bkeys);
   RZ_LAUNCH(ctx, "k_scene_setup", rzk::k_scene_setup, dim3(blocks_for(n)), dim3(rzk::kBlock), s->sp,
             n, v, (const unsigned long long*)bkeys, pi, (uint32_t*)ctx->sc_counts.p,
             (int4*)ctx->sc_rects.p);
@@ -1210,6 +1239,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.st = st;
   a.acc = ctx->acc_run;
   a.bg = bg;
+  a.load_dst = load;
   a.out = d_out;
   a.row_stride = row_stride;
   a.lw = lw;
@@ -1396,7 +1426,9 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
   a.events = (const int4*)ctx->events.p;
   a.redge = (const long long*)(meta + redge_words(sw + 2 * (size_t)ct + cb));
   a.st = st;
+  a.acc = nullptr;
   a.bg = bg;
+  a.load_dst = 0;
   a.out = nullptr;
   a.row_stride = 0;
   a.lw = lw;
@@ -1529,10 +1561,6 @@ GraphKey graph_key(rz_ctx* ctx, int kind, uint64_t obj) {
   return k;
 }
 
-bool same_key(const GraphKey& a, const GraphKey& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
-
-constexpr size_t kMaxGraphs = 8;
-
 void drop_graphs(rz_ctx* ctx) {
   for (auto& e : ctx->graphs) (void)hipGraphExecDestroy(e.exec);
   ctx->graphs.clear();
@@ -1548,7 +1576,10 @@ rz_status run_graphed(rz_ctx* ctx, const GraphKey& k, F&& enqueue) {
       ++ctx->replays;
       return RZ_OK;
     }
-  if (ctx->settled_valid && same_key(ctx->graph_settled, k)) {
+  auto settled = std::find_if(ctx->graph_settled.begin(), ctx->graph_settled.end(),
+                              [&](const GraphKey& e) { return same_key(e, k); });
+  if (settled != ctx->graph_settled.end()) {
+    ctx->graph_settled.erase(settled);
     hipGraph_t g = nullptr;
     RZ_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
     ctx->capturing = true;
@@ -1561,7 +1592,6 @@ rz_status run_graphed(rz_ctx* ctx, const GraphKey& k, F&& enqueue) {
       ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
     (void)hipGetLastError();
-    ctx->settled_valid = false;
     if (ei == hipSuccess) {
       if (ctx->graphs.size() >= kMaxGraphs) {
         auto lru = std::min_element(ctx->graphs.begin(), ctx->graphs.end(),
@@ -1586,6 +1616,353 @@ rz_status run_graphed(rz_ctx* ctx, const GraphKey& k, F&& enqueue) {
   }
   return st;
 }
+
+// ---------------------------------------------------------------------------
+// Scene trees: Scene::render of Fill / Group / Transform / Opacity / Clip
+// nodes (src/scene.rs:186-199, 268-459). Pipeline::build_rec runs on the host
+// (bboxes, views, transforms); render_rec draws every run of fills that goes
+// into one layer as one batched render (each fill under its own transform),
+// and an Opacity or Clip node into a layer of its own, composed into its
+// parent by k_compose_over / k_compose_mask.
+// ---------------------------------------------------------------------------
+struct TreeNode {
+  int kind;            // RZ_NODE_*
+  rzh::BBox bbox;
+  rzh::Tr tr;
+  uint32_t path = 0, paint = 0;
+  int rule = 0;
+  std::vector<int> children;
+  int child = -1;
+  float opacity = 1.0f;
+  rzh::Tr clip_tr;
+  bool has_view = false;  // the view the node was built under (fills: their batch's)
+  rzh::BBox view;
+};
+
+struct TreeBuild {
+  const rz_scene_node* nodes;
+  size_t n_nodes;
+  const uint32_t* children;
+  const rz_path* const* paths;
+  size_t n_paths;
+  size_t n_paints;
+  std::vector<TreeNode> out;
+  int depth = 0;
+  bool bad = false;
+
+  const rz_scene_node* node(uint32_t id) {
+    if (id >= n_nodes || depth > 256) {
+      bad = true;
+      return nullptr;
+    }
+    return &nodes[id];
+  }
+  static rzh::BBox unite(const rzh::BBox& a, const rzh::BBox& b) { return rzh::bbox_union_opt(a, &b); }
+  // range_intersect of both axes (src/geometry.rs:648-657, 677-688)
+  static bool intersect(const rzh::BBox& a, const rzh::BBox& b, rzh::BBox* out) {
+    if (a.mn.x > b.mx.x || b.mn.x > a.mx.x || a.mn.y > b.mx.y || b.mn.y > a.mx.y) return false;
+    *out = rzh::bbox_new(rzg::mkpt(std::fmax(a.mn.x, b.mn.x), std::fmax(a.mn.y, b.mn.y)),
+                         rzg::mkpt(std::fmin(a.mx.x, b.mx.x), std::fmin(a.mx.y, b.mx.y)));
+    return true;
+  }
+  // Scene::bbox (src/scene.rs:154-183)
+  bool scene_bbox(uint32_t id, const rzh::Tr& tr, rzh::BBox* res) {
+    const rz_scene_node* n = node(id);
+    if (!n) return false;
+    ++depth;
+    bool have = false;
+    switch (n->kind) {
+      case RZ_NODE_FILL:
+        have = n->path < n_paths && rzh::path_bbox(paths[n->path], tr, res);
+        break;
+      case RZ_NODE_GROUP:
+        for (uint32_t k = 0; k < n->count; ++k) {
+          rzh::BBox b;
+          if (!scene_bbox(children[n->first + k], tr, &b)) continue;
+          *res = have ? unite(*res, b) : b;
+          have = true;
+        }
+        break;
+      case RZ_NODE_TRANSFORM:
+        have = scene_bbox(n->child, rzh::tr_mul(tr, rzh::tr_of(n->tr)), res);
+        break;
+      case RZ_NODE_OPACITY:
+        have = scene_bbox(n->child, tr, res);
+        break;
+      case RZ_NODE_CLIP: {
+        rzh::Tr ctr = tr;
+        if (n->units == RZ_UNITS_BBOX) {
+          rzh::BBox cb;
+          const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
+          if (!scene_bbox(n->child, ident, &cb)) break;
+          ctr = rzh::tr_mul(tr, rzh::bbox_unit_transform(cb));
+        }
+        rzh::BBox a, b;
+        have = scene_bbox(n->child, tr, &a) && n->path < n_paths &&
+               rzh::path_bbox(paths[n->path], ctr, &b) && intersect(a, b, res);
+        break;
+      }
+      default:
+        bad = true;
+    }
+    --depth;
+    return have;
+  }
+  // Pipeline::build_rec (src/scene.rs:268-357): the node's index in `out`, or -1
+  int build(uint32_t id, const rzh::BBox* view, const rzh::Tr& tr) {
+    const rz_scene_node* n = node(id);
+    if (!n) return -1;
+    ++depth;
+    int r = -1;
+    auto view_apply = [&](const rzh::BBox& b, rzh::BBox* o) {
+      if (!view) {
+        *o = b;
+        return true;
+      }
+      return intersect(*view, b, o);
+    };
+    TreeNode t;
+    t.kind = n->kind;
+    t.tr = tr;
+    t.has_view = view != nullptr;
+    if (view) t.view = *view;
+    switch (n->kind) {
+      case RZ_NODE_FILL: {
+        rzh::BBox pb;
+        if (n->path >= n_paths || n->paint >= n_paints) {
+          bad = true;
+          break;
+        }
+        if (!rzh::path_bbox(paths[n->path], tr, &pb) || !view_apply(pb, &t.bbox)) break;
+        t.path = n->path;
+        t.paint = n->paint;
+        t.rule = n->fill_rule;
+        out.push_back(t);
+        r = (int)out.size() - 1;
+        break;
+      }
+      case RZ_NODE_GROUP: {
+        bool have = false;
+        for (uint32_t k = 0; k < n->count; ++k) {
+          const int c = build(children[n->first + k], view, tr);
+          if (c < 0) continue;
+          t.children.push_back(c);
+          t.bbox = have ? unite(out[c].bbox, t.bbox) : out[c].bbox;
+          have = true;
+        }
+        if (!have) break;
+        out.push_back(t);
+        r = (int)out.size() - 1;
+        break;
+      }
+      case RZ_NODE_TRANSFORM:
+        r = build(n->child, view, rzh::tr_mul(tr, rzh::tr_of(n->tr)));
+        break;
+      case RZ_NODE_OPACITY: {
+        const int c = build(n->child, view, tr);
+        if (c < 0) break;
+        t.child = c;
+        t.bbox = out[c].bbox;
+        t.opacity = (float)n->opacity;  // as given: Scene::opacity clamps, the tree does not
+        out.push_back(t);
+        r = (int)out.size() - 1;
+        break;
+      }
+      case RZ_NODE_CLIP: {
+        if (n->path >= n_paths) {
+          bad = true;
+          break;
+        }
+        t.clip_tr = tr;
+        if (n->units == RZ_UNITS_BBOX) {
+          rzh::BBox cb;
+          const rzh::Tr ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
+          if (!scene_bbox(n->child, ident, &cb)) break;
+          t.clip_tr = rzh::tr_mul(tr, rzh::bbox_unit_transform(cb));
+        }
+        rzh::BBox pb, clip_bbox;
+        if (!rzh::path_bbox(paths[n->path], t.clip_tr, &pb) || !view_apply(pb, &clip_bbox)) break;
+        const int c = build(n->child, &clip_bbox, tr);
+        if (c < 0 || !intersect(clip_bbox, out[c].bbox, &t.bbox)) break;
+        t.child = c;
+        t.path = n->path;
+        t.rule = n->fill_rule;
+        out.push_back(t);
+        r = (int)out.size() - 1;
+        break;
+      }
+      default:
+        bad = true;
+    }
+    --depth;
+    return r;
+  }
+};
+
+// A layer on the device (Layer::new: the pixel box around a bbox).
+struct DLayer {
+  int x = 0, y = 0, w = 0, h = 0;
+  rzp::C4* d = nullptr;
+};
+
+struct TreeRender {
+  rz_ctx* ctx;
+  TreeBuild* tb;
+  const rz_paint* paints;
+  int mode;
+  double flatness;
+  std::vector<void*> allocs;            // layers and masks, freed at the end
+  std::vector<rz_dscene*> scenes;       // batches
+  std::vector<rz_dpath*> dpaths;        // clip paths
+  std::vector<int> batch;               // fills waiting for their layer
+
+  ~TreeRender() {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : allocs) (void)hipFree(p);
+    for (rz_dscene* s : scenes) {
+      scene_release(s);
+      delete s;
+    }
+    for (rz_dpath* d : dpaths) {
+      dpath_release(d);
+      delete d;
+    }
+  }
+  rz_status alloc(size_t bytes, void** p) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return fail(ctx, RZ_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return RZ_OK;
+  }
+  rz_status new_layer(const rzh::BBox& b, rzp::C4 fill, DLayer* L) {
+    const double v[4] = {b.mn.x, b.mn.y, b.mx.x, b.mx.y};
+    layer_of(v, &L->x, &L->y, &L->w, &L->h);
+    void* p = nullptr;
+    RZ_TRY(alloc((size_t)L->w * L->h * sizeof(rzp::C4), &p));
+    L->d = (rzp::C4*)p;
+    if (L->w && L->h)
+      RZ_LAUNCH(ctx, "k_fill_rgba", rzk::k_fill_rgba, dim3(1024), dim3(256), L->d, L->w, L->h,
+                (size_t)L->w, fill);
+    return check_launch(ctx, "k_fill_rgba");
+  }
+  // the fills collected for `L`, as one batched render over its pixels
+  rz_status flush(const DLayer& L) {
+    if (batch.empty()) return RZ_OK;
+    std::vector<int> b;
+    b.swap(batch);
+    if (!L.w || !L.h) return RZ_OK;
+    const size_t n = b.size();
+    std::vector<const rz_path*> ps(n);
+    std::vector<rz_paint> pt(n);
+    std::vector<int32_t> rules(n);
+    std::vector<double> trs(6 * n);
+    for (size_t i = 0; i < n; ++i) {
+      const TreeNode& f = tb->out[b[i]];
+      ps[i] = tb->paths[f.path];
+      pt[i] = paints[f.paint];
+      rules[i] = f.rule;
+      for (int k = 0; k < 6; ++k) trs[6 * i + k] = f.tr.m[k];
+    }
+    rz_dscene* s = nullptr;
+    RZ_TRY(scene_upload_impl(ctx, ps.data(), pt.data(), rules.data(), n, &s, trs.data()));
+    scenes.push_back(s);
+    const TreeNode& f0 = tb->out[b[0]];
+    const double inf = INFINITY;
+    double view[4] = {-inf, -inf, inf, inf};
+    if (f0.has_view) {
+      view[0] = f0.view.mn.x;
+      view[1] = f0.view.mn.y;
+      view[2] = f0.view.mx.x;
+      view[3] = f0.view.mx.y;
+    }
+    const int layer[4] = {L.x, L.y, L.w, L.h};
+    rz_transform ident = {{1.0, 0.0, 0.0, 0.0, 1.0, 0.0}};
+    return enqueue_scene_batched(ctx, s, ident, view, rzp::splat(0.0f), mode, flatness, L.d,
+                                 (size_t)L.w, layer, 1);
+  }
+  // Pipeline::render_rec (src/scene.rs:397-459)
+  rz_status render(int id, const DLayer& L) {
+    const TreeNode& n = tb->out[id];
+    switch (n.kind) {
+      case RZ_NODE_FILL: {
+        // fills of one batch were built under one view (a new view only
+        // comes with a Clip, whose child draws into its own layer)
+        if (!batch.empty()) {
+          const TreeNode& f0 = tb->out[batch[0]];
+          const bool same = f0.has_view == n.has_view &&
+                            (!n.has_view || std::memcmp(&f0.view, &n.view, sizeof(n.view)) == 0);
+          if (!same) RZ_TRY(flush(L));
+        }
+        batch.push_back(id);
+        return RZ_OK;
+      }
+      case RZ_NODE_GROUP:
+        for (int c : n.children) RZ_TRY(render(c, L));
+        return RZ_OK;
+      case RZ_NODE_OPACITY: {
+        RZ_TRY(flush(L));
+        DLayer C;
+        RZ_TRY(new_layer(tb->out[n.child].bbox, rzp::splat(0.0f), &C));
+        RZ_TRY(render(n.child, C));
+        RZ_TRY(flush(C));
+        return compose_over(L, C, n.opacity);
+      }
+      case RZ_NODE_CLIP: {
+        RZ_TRY(flush(L));
+        DLayer C;
+        RZ_TRY(new_layer(tb->out[n.child].bbox, rzp::splat(0.0f), &C));
+        RZ_TRY(render(n.child, C));
+        RZ_TRY(flush(C));
+        // the clip's mask over the node's box: Rasterizer::mask of the clip
+        // path under translate(-mask.x, -mask.y) * clip_tr
+        int mx, my, mw, mh;
+        const double v[4] = {n.bbox.mn.x, n.bbox.mn.y, n.bbox.mx.x, n.bbox.mx.y};
+        layer_of(v, &mx, &my, &mw, &mh);
+        if (mw && mh) {
+          void* m = nullptr;
+          RZ_TRY(alloc((size_t)mw * mh * sizeof(float), &m));
+          rz_dpath* dp = nullptr;
+          RZ_TRY(path_upload_impl(ctx, tb->paths[n.path], &dp));
+          dpaths.push_back(dp);
+          const rzh::Tr align = {{1.0, 0.0, -(double)mx, 0.0, 1.0, -(double)my}};
+          const rzh::Tr mt = rzh::tr_mul(align, n.clip_tr);
+          rz_transform t;
+          for (int k = 0; k < 6; ++k) t.m[k] = mt.m[k];
+          const int mmode = mode == RZ_EDGE_NONE ? RZ_EDGE_NONE : RZ_EDGE_MASK;
+          RZ_TRY(enqueue_mask(ctx, dp, t, flatness, n.rule, mmode, (size_t)mw, (size_t)mh, (float*)m,
+                              (size_t)mw));
+          // child * mask over their overlap
+          rzk::ComposeRect r;
+          if (overlap(C.x, C.y, C.w, C.h, mx, my, mw, mh, &r)) {
+            const uint32_t grid = (uint32_t)std::min<size_t>(((size_t)r.w * r.h + 255) / 256, 4096);
+            RZ_LAUNCH(ctx, "k_compose_mask", rzk::k_compose_mask, dim3(grid), dim3(256), C.d,
+                      (size_t)C.w, (const float*)m, (size_t)mw, r);
+          }
+        }
+        return compose_over(L, C, 1.0f);
+      }
+      default:
+        return fail(ctx, RZ_E_ARG, "bad scene node");
+    }
+  }
+  // the overlap of layer a (dst) and layer b (src), as a ComposeRect
+  static bool overlap(int ax, int ay, int aw, int ah, int bx, int by, int bw, int bh,
+                      rzk::ComposeRect* r) {
+    const int x0 = std::max(ax, bx), x1 = std::min(ax + aw, bx + bw);
+    const int y0 = std::max(ay, by), y1 = std::min(ay + ah, by + bh);
+    if (x1 <= x0 || y1 <= y0) return false;
+    *r = rzk::ComposeRect{x1 - x0, y1 - y0, x0 - ax, y0 - ay, x0 - bx, y0 - by};
+    return true;
+  }
+  rz_status compose_over(const DLayer& dst, const DLayer& src, float opacity) {
+    rzk::ComposeRect r;
+    if (!overlap(dst.x, dst.y, dst.w, dst.h, src.x, src.y, src.w, src.h, &r)) return RZ_OK;
+    const uint32_t grid = (uint32_t)std::min<size_t>(((size_t)r.w * r.h + 255) / 256, 4096);
+    RZ_LAUNCH(ctx, "k_compose_over", rzk::k_compose_over, dim3(grid), dim3(256), dst.d,
+              (size_t)dst.w, (const rzp::C4*)src.d, (size_t)src.w, r, opacity);
+    return check_launch(ctx, "k_compose_over");
+  }
+};
 
 rz_status check_scene_args(rz_ctx* ctx, double flatness, int mode, const double* view) {
   if (!ctx) return RZ_E_ARG;
@@ -2182,6 +2559,55 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
   if (st != RZ_OK) return st;
   RZ_HIP(ctx, hipMemcpy(rgba, ctx->out.p, px * 16, hipMemcpyDeviceToHost));
   return RZ_OK;
+}
+
+rz_status rz_render_tree(rz_ctx* ctx, const rz_path* const* paths, size_t n_paths,
+                         const rz_paint* paints, size_t n_paints, const rz_scene_node* nodes,
+                         size_t n_nodes, const uint32_t* children, uint32_t root, rz_transform tr,
+                         double flatness, int edge_mode, const double* view, const float* bg,
+                         float* rgba, size_t cap_px, int32_t* layer_xywh) {
+  if (!ctx || !layer_xywh || (n_nodes && !nodes)) return RZ_E_ARG;
+  if (!(flatness > 0.0) || !std::isfinite(flatness)) return fail(ctx, RZ_E_ARG, "flatness must be > 0");
+  if (edge_mode < RZ_EDGE_MASK || edge_mode > RZ_EDGE_NONE) return fail(ctx, RZ_E_ARG, "bad edge mode");
+  for (size_t i = 0; i < n_paths; ++i) RZ_TRY(validate_path(ctx, paths ? paths[i] : nullptr));
+  for (int k = 0; k < 4; ++k) layer_xywh[k] = 0;
+  TreeBuild tb{nodes, n_nodes, children, paths, n_paths, n_paints, {}};
+  rzh::BBox vb;
+  if (view) {
+    for (int k = 0; k < 4; ++k)
+      if (!std::isfinite(view[k])) return fail(ctx, RZ_E_ARG, "view must be finite");
+    vb = rzh::bbox_new(rzg::mkpt(view[0], view[1]), rzg::mkpt(view[2], view[3]));
+  }
+  const int root_id = tb.build(root, view ? &vb : nullptr, rzh::tr_of(tr.m));
+  if (tb.bad) return fail(ctx, RZ_E_ARG, "malformed scene tree (index out of range, or deeper than 256)");
+  if (root_id < 0) return RZ_OK;  // nothing in view: the reference's empty layer
+  const rzh::BBox lb = view ? vb : tb.out[root_id].bbox;
+  double v[4] = {lb.mn.x, lb.mn.y, lb.mx.x, lb.mx.y};
+  RZ_TRY(check_scene_args(ctx, flatness, edge_mode, v));
+  int lx, ly, lw, lh;
+  layer_of(v, &lx, &ly, &lw, &lh);
+  layer_xywh[0] = lx;
+  layer_xywh[1] = ly;
+  layer_xywh[2] = lw;
+  layer_xywh[3] = lh;
+  if (!rgba || cap_px < (size_t)lw * lh || !lw || !lh) return RZ_OK;
+  const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
+  rz_status st = RZ_OK;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    TreeRender R{ctx, &tb, paints, edge_mode, flatness, {}, {}, {}, {}};
+    DLayer L;
+    st = R.new_layer(lb, c, &L);
+    if (st == RZ_OK) st = R.render(root_id, L);
+    if (st == RZ_OK) st = R.flush(L);
+    if (st != RZ_OK) return st;
+    st = finish(ctx);
+    if (st == RZ_OK) {
+      RZ_HIP(ctx, hipMemcpy(rgba, L.d, (size_t)lw * lh * 16, hipMemcpyDeviceToHost));
+      return RZ_OK;
+    }
+    if (st != RZ_E_CAPACITY) return st;
+  }
+  return st;
 }
 
 rz_status rz_atlas_set(rz_ctx* ctx, rz_dscene* glyphs, const rz_atlas_slot* slots) {

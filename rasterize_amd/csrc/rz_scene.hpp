@@ -21,6 +21,8 @@ namespace rzk {
 struct ScenePath {
   uint32_t seg_begin, seg_end;  // segments in the combined path
   int rule;
+  int has_tr;                   // a Transform node above the fill: draw under render tr * tr
+  double tr[6];
   int has_bbox_i;               // Path::bbox(identity) exists
   rzh::BBox bbox_i;             // for BoundingBox paints
   int units;
@@ -50,8 +52,14 @@ __device__ inline double okey_value(unsigned long long k) {
   return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
 }
 
+// The transform a fill is drawn under: the render's, times the fill's own
+// (Pipeline::build_rec's tr * child_tr, src/scene.rs:341-344) if it has one.
+__device__ inline rzh::Tr fill_tr(const SceneView& v, const ScenePath& S) {
+  return S.has_tr ? rzh::tr_mul(rzh::tr_of(v.tr), rzh::tr_of(S.tr)) : rzh::tr_of(v.tr);
+}
+
 __global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const uint32_t* slot_path, SceneView v,
-                                                    unsigned long long* keys) {
+                                                    const ScenePath* sp, unsigned long long* keys) {
   const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
   const int lane = threadIdx.x;
   bool have = false;
@@ -64,8 +72,9 @@ __global__ __launch_bounds__(64) void k_scene_bbox(DPath p, const uint32_t* slot
       pid = slot_path[slot];
       const int kind = p.kinds[k];
       const double* q = p.pts + 2 * (size_t)p.pt_off[k];
+      const rzh::Tr m = fill_tr(v, sp[pid]);
       Pt c[4];
-      for (int j = 0; j < kind + 2; ++j) c[j] = rzg::tr_apply(v.tr, rzg::mkpt(q[2 * j], q[2 * j + 1]));
+      for (int j = 0; j < kind + 2; ++j) c[j] = rzg::tr_apply(m.m, rzg::mkpt(q[2 * j], q[2 * j + 1]));
       const rzh::BBox b = rzh::segment_bbox(kind, c, nullptr);
       mnx = fmin(mnx, b.mn.x);
       mny = fmin(mny, b.mn.y);
@@ -139,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_scene_setup(const ScenePath* sp, uin
     rzh::Tr align;
     const double av[6] = {1.0, 0.0, -fx, 0.0, 1.0, -fy};
     for (int k = 0; k < 6; ++k) align.m[k] = av[k];
-    const rzh::Tr m = rzh::tr_mul(align, rzh::tr_of(v.tr));
+    const rzh::Tr m = rzh::tr_mul(align, fill_tr(v, S));
     for (int k = 0; k < 6; ++k) P.m[k] = m.m[k];
     P.paint = S.paint;
     bool draw = W > 0 && H > 0;
@@ -295,6 +304,7 @@ struct SceneFineArgs {
   const Status* st;
   Status* acc;        // accumulated status (merged by block 0), or null
   rzp::C4 bg;
+  int load_dst;       // start from the layer's pixels (a layer already drawn into) instead of bg
   rzp::C4* out;       // layer image
   size_t row_stride;  // pixels
   int lw, lh;         // layer size
@@ -323,7 +333,10 @@ __global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
   const int col = lane & 15;
   rzp::C4 c[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) c[k] = a.bg;
+  for (int k = 0; k < 4; ++k) {
+    const int x = TX * 16 + col, y = TY * 16 + 4 * k + (lane >> 4);
+    c[k] = a.load_dst && x < a.lw && y < a.lh ? a.out[(size_t)y * a.row_stride + x] : a.bg;
+  }
   const uint32_t nrow = a.row_count[TY];
   const uint32_t* row = a.rows + (size_t)TY * a.n_paths;
   for (uint32_t start = 0; start < nrow; start += 64) {

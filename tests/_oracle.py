@@ -76,6 +76,9 @@ def lib():
                                      P(C.c_float), d, P(C.c_float), sz, P(C.c_int64)]),
             "orc_scene_render_mt": (i, [sz, P(orc_path), P(orc_paint), P(C.c_int32), P(d), P(d),
                                         P(C.c_float), d, P(C.c_float), sz, P(C.c_int64), i]),
+            "orc_scene_tree_render": (i, [P(orc_path), sz, P(orc_paint), sz, C.c_void_p,
+                                          P(C.c_uint32), C.c_uint32, P(d), P(d), P(C.c_float), d,
+                                          i, P(C.c_float), sz, P(C.c_int64)]),
             "orc_mask_atlas_mt": (i, [sz, P(orc_path), P(d), P(C.c_uint32), d, i, P(C.c_float), sz,
                                       sz, i]),
             "orc_sd_mask_iter_dual": (i, [P(orc_path), P(d), P(d), d, i, sz, sz, P(d), P(d)]),
@@ -264,6 +267,34 @@ def scene_render(fills, tr, view=None, bg=None, flatness=0.05):
                                       img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh))
     elif rc not in (0, 2):
         _check(rc)
+    return img, (x, y)
+
+
+def scene_tree_render(scene, tr, view=None, bg=None, flatness=0.05, active_edge=False):
+    """Scene::render(SignedDifferenceRasterizer, tr, view, bg) of a
+    rasterize_amd.Scene tree (its node arrays, the rz_scene_node layout), or
+    with ActiveEdgeRasterizer: (image (H, W, 4) float32, (x, y))."""
+    ps, qs, nodes, children, root = scene.node_arrays()
+    paths = (orc_path * max(len(ps), 1))(*[opath(p) for p in ps])
+    keep = [q.abi() for q in qs]  # owns the stop arrays the copies point at
+    paints = (orc_paint * max(len(qs), 1))(*[orc_paint.from_buffer_copy(q) for q in keep])
+    na = (type(nodes[0]) * len(nodes))(*nodes)
+    ca = (C.c_uint32 * max(len(children), 1))(*children)
+    t = _tr(tr)
+    v = (C.c_double * 4)(*view) if view is not None else None
+    b = (C.c_float * 4)(*bg) if bg is not None else None
+    xywh = (C.c_int64 * 4)()
+    L = lib()
+    args = (paths, len(ps), paints, len(qs), C.cast(na, C.c_void_p), ca, root, _dp(t), v, b, flatness,
+            int(active_edge))
+    rc = L.orc_scene_tree_render(*args, None, 0, xywh)
+    x, y, w, h = list(xywh)
+    img = np.zeros((h, w, 4), np.float32)
+    if w and h:
+        _check(L.orc_scene_tree_render(*args, img.ctypes.data_as(C.POINTER(C.c_float)), w * h, xywh))
+    elif rc not in (0, 2):
+        _check(rc)
+    del keep
     return img, (x, y)
 
 

@@ -1,0 +1,246 @@
+"""Scene trees on the CPU: the JSON scene format (Deserialize for Scene,
+src/scene.rs:13-63, 587-640; colours src/color.rs:96-141), and pins of the
+oracle's Transform / Opacity / Clip nodes (src/scene.rs:268-357, 397-459)
+against renders of flat scenes and masks composed independently here."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _paths import rect
+from rasterize_amd import (FillRule, GradLinear, GradRadial, LinColor, Path, PathBuilder, Scene,
+                           Transform, Units)
+from rasterize_amd.color import SVG_COLORS, ColorError, parse_rgba
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+I = Transform.identity()
+
+# the scene of src/scene.rs:654-665 (test_scene_view)
+SCENE = """
+{
+    "type": "transform",
+    "tr": "translate(7, 7) rotate(45, 7, 7) scale(10)",
+    "child": {
+        "type": "fill",
+        "paint": "#ff8040",
+        "path": "M0,0 h1 v1 h-1 z"
+    }
+}
+"""
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return f.read()
+
+
+def count(sc, kind):
+    n = int(sc.kind == kind)
+    for c in sc.children:
+        n += count(c, kind)
+    if sc.child is not None:
+        n += count(sc.child, kind)
+    return n
+
+
+def test_color_strings():
+    # src/color.rs:520-535 (test_color_parse)
+    assert parse_rgba("#01020304") == (1, 2, 3, 4)
+    assert parse_rgba("#aabbcc") == (170, 187, 204, 255)
+    assert parse_rgba("#AABBCC") == (170, 187, 204, 255)
+    assert parse_rgba("#000000") == (0, 0, 0, 255)
+    assert parse_rgba("#010203/.25") == (1, 2, 3, 63)
+    assert parse_rgba("coral") == (0xff, 0x7f, 0x50, 0xff)
+    # src/color.rs:577-580 (test_serde): "aqua/.5"
+    assert parse_rgba("aqua/.5") == (0, 255, 255, 127)
+    assert len(SVG_COLORS) == 147
+    # the alpha factor saturates like Rust's `f32 as u8`
+    assert parse_rgba("#01020380/4")[3] == 255
+    assert parse_rgba("#010203/-1")[3] == 0
+    assert parse_rgba("#010203/NaN")[3] == 0
+    # names beyond the SVG keywords come from a caller's table
+    assert parse_rgba("gruv-orange-2", {"gruv-orange-2": 0xfe8019}) == (0xfe, 0x80, 0x19, 0xff)
+    for bad in ("#12345", "#gg0000", "nosuchcolor", "#010203/x", "#010203/ 1"):
+        with pytest.raises(ColorError):
+            parse_rgba(bad)
+
+
+def test_linear_color_of_string():
+    # src/color.rs:394-406: exact sRGB -> linear, premultiplied
+    c = LinColor.from_rgba8(*parse_rgba("#ff8040"))
+    assert c[0] == 1.0 and c[3] == 1.0
+    assert abs(c[1] - 0.21586050) < 1e-7 and abs(c[2] - 0.05126946) < 1e-7
+
+
+def test_json_scene_shapes():
+    sc = Scene.from_json(SCENE)
+    assert sc.kind == Scene.TRANSFORM and sc.child.kind == Scene.FILL
+    # translate(7,7) rotate(45,7,7) scale(10), src/svg.rs transform list order
+    T = (Transform.new_translate(7, 7) * Transform.new_translate(7, 7) *
+         Transform.new_rotate(np.pi / 4) * Transform.new_translate(-7, -7) *
+         Transform.new_scale(10, 10))
+    assert np.allclose(sc.tr.m, T.m, rtol=0, atol=1e-12)
+
+    g = Scene.from_json(load("grad.scene"))
+    assert [g.kind, g.child.kind, g.child.child.kind, g.child.child.child.kind] == [
+        Scene.TRANSFORM, Scene.CLIP, Scene.TRANSFORM, Scene.FILL]
+    fill = g.child.child.child
+    assert isinstance(fill.paint, GradRadial) and fill.paint.linear_colors
+    assert fill.paint.spread == 2 and fill.paint.fcenter == (50.0, 50.0)
+    assert [s.position for s in fill.paint.stops] == [0, 0.5, 0.7, 1]
+
+    f = Scene.from_json(load("firefox.scene"))
+    assert f.kind == Scene.TRANSFORM and f.child.kind == Scene.GROUP
+    n_fill = count(f, Scene.FILL)
+    assert n_fill == len(f.child.children) and n_fill > 10
+    radial = [c for c in f.child.children if isinstance(c.paint, GradRadial)]
+    assert radial and radial[0].paint.fcenter == (-14519.88, 9293.72)
+
+    lc = Scene.from_json(load("linear-colors.scene"))
+    kinds = [c.kind for c in lc.children]
+    assert kinds == [Scene.FILL, Scene.FILL, Scene.TRANSFORM]
+    assert isinstance(lc.children[1].paint, GradLinear)
+    assert lc.children[1].paint.units == Units.BoundingBox
+    assert lc.children[2].child.paint.linear_colors is True
+
+
+def test_json_scene_errors():
+    with pytest.raises(NotImplementedError):
+        Scene.from_json({"type": "stroke", "width": 1, "paint": "#000000", "path": "M0,0 1,1"})
+    with pytest.raises(ValueError):
+        Scene.from_json({"type": "fill", "paint": {"type": "conic"}, "path": "M0,0 1,1"})
+    with pytest.raises(ValueError):
+        Scene.from_json({"type": "fill", "paint": "#000000", "path": "M0,0 1,1", "fill_rule": "odd"})
+    with pytest.raises(ValueError):
+        Scene.from_json({"type": "blur", "child": {}})
+
+
+def test_constructors_fold_like_the_reference():
+    p = rect(0, 0, 4, 4)
+    f = Scene.fill(p, LinColor(1, 0, 0, 1))
+    assert Scene.group([f]) is f
+    o = f.with_opacity(0.5).with_opacity(0.5)
+    assert o.kind == Scene.OPACITY and o.child is f and o.opacity == 0.25
+    assert f.with_opacity(2.0).opacity == 1.0 and f.with_opacity(-1.0).opacity == 0.0
+    a, b = Transform.new_translate(1, 2), Transform.new_scale(3, 3)
+    t = f.with_transform(a).with_transform(b)
+    assert t.kind == Scene.TRANSFORM and t.child is f and t.tr.m == (b * a).m
+
+
+def test_oracle_transform_node_is_the_composed_transform():
+    """Transform(T, fills) renders as the fills under tr * T."""
+    pb = PathBuilder().move_to((2.5, 3.25)).cubic_to((30, 1), (40, 35), (8, 30)).close()
+    path = pb.build()
+    paint = LinColor(0.2, 0.4, 0.6, 0.8)
+    T = Transform.new_rotate(0.3).pre_scale(1.5, 0.75)
+    tr = Transform.new_translate(4.0, 2.0)
+    sc = Scene.fill(path, paint).with_transform(T)
+    got, org = O.scene_tree_render(sc, tr)
+    ref, rorg = O.scene_render([(path, paint.abi(), 0)], tr * T)
+    assert org == rorg and got.shape == ref.shape and got.size
+    assert np.abs(got - ref).max() <= 1e-7
+
+
+def test_oracle_opacity_node_composes_the_child_layer():
+    """Opacity(o, child) = bg blend_over (child layer * o), the child drawn on
+    its own transparent layer over its bbox (Layer::compose)."""
+    a, b = rect(2, 2, 20, 14), rect(10, 6, 30, 26)
+    ca, cb = LinColor(0.5, 0.1, 0.0, 0.5), LinColor(0.0, 0.3, 0.6, 0.75)
+    child = Scene.group([Scene.fill(a, ca), Scene.fill(b, cb)])
+    bg = (0.1, 0.1, 0.1, 1.0)
+    view = (0.0, 0.0, 32.0, 32.0)
+    fl = [(a, ca.abi(), 0), (b, cb.abi(), 0)]
+    lay, (lx, ly) = O.scene_render(fl, I)  # the child layer over its own bbox
+    for op in (0.0, 0.37, 1.0):
+        got, org = O.scene_tree_render(child.with_opacity(op), I, view, bg)
+        assert org == (0, 0) and got.shape == (32, 32, 4)
+        exp = np.empty((32, 32, 4), np.float32)
+        exp[:] = np.float32(bg)
+        h, w = lay.shape[:2]
+        src = lay * np.float32(op)
+        dst = exp[ly:ly + h, lx:lx + w]
+        exp[ly:ly + h, lx:lx + w] = src + dst * (np.float32(1) - src[..., 3:4])
+        assert np.abs(got - exp).max() <= 1e-6, op
+        if op == 0.0:
+            assert np.array_equal(got, np.broadcast_to(np.float32(bg), got.shape))
+    # opacity 1 over nothing is the child itself
+    got, org = O.scene_tree_render(child.with_opacity(1.0), I)
+    assert org == (lx, ly) and np.array_equal(got, lay)
+
+
+def test_oracle_clip_node_multiplies_by_the_clip_mask():
+    """Clip(q, fill p) = fill p's pixels times q's mask (Rasterizer::mask),
+    over the clip bbox intersected with the child's."""
+    p = PathBuilder().move_to((3, 3)).cubic_to((40, 0), (44, 40), (6, 37)).close().build()
+    q = PathBuilder().move_to((0, 20)).line_to((20, 0)).line_to((40, 20)).line_to((20, 40)).close().build()
+    col = LinColor(0.9, 0.5, 0.25, 1.0)
+    W = H = 48
+    cov = O.sd_mask_iter(p, I, W, H, 0).astype(np.float32)
+    m = O.sd_mask(q, I, W, H, 0).astype(np.float32)
+    for rule in (FillRule.NonZero, FillRule.EvenOdd):
+        sc = Scene.fill(p, col).with_clip(q, Units.UserSpaceOnUse, rule)
+        got, (x, y) = O.scene_tree_render(sc, I)
+        h, w = got.shape[:2]
+        exp = (np.float32(col) * cov[..., None]) * m[..., None]
+        assert np.abs(got - exp[y:y + h, x:x + w]).max() <= 1e-6
+        # nothing drawn outside the layer
+        outside = np.ones((H, W), bool)
+        outside[y:y + h, x:x + w] = False
+        assert np.abs(exp[outside]).max() <= 1e-6
+
+
+def test_oracle_clip_units_bbox_is_the_unit_square_of_the_child():
+    """Units::BoundingBox: the clip's unit square maps onto the child's bbox
+    (tr * bbox.unit_transform())."""
+    p = rect(8, 8, 40, 40)
+    col = LinColor(0.3, 0.6, 0.9, 1.0)
+    unit_clip = rect(0.25, 0.25, 0.75, 0.75)
+    user_clip = rect(16, 16, 32, 32)
+    a, oa = O.scene_tree_render(Scene.fill(p, col).with_clip(unit_clip, Units.BoundingBox), I)
+    b, ob = O.scene_tree_render(Scene.fill(p, col).with_clip(user_clip, Units.UserSpaceOnUse), I)
+    assert oa == ob and a.shape == b.shape and a.size
+    assert np.abs(a - b).max() <= 1e-6
+    # the clip square's 16 x 16 pixels; Rasterizer::mask leaves the column at
+    # its right edge (the layer's last) empty, as the reference does
+    assert a.shape[:2] == (16, 16) and oa == (16, 16)
+    assert a[:, :15, 3].min() > 0.99 and a[:, 15, 3].max() == 0.0
+
+
+def test_oracle_disjoint_clip_is_the_empty_layer():
+    sc = Scene.fill(rect(0, 0, 8, 8), LinColor(1, 1, 1, 1)).with_clip(rect(20, 20, 30, 30))
+    img, org = O.scene_tree_render(sc, I)
+    assert img.size == 0 and org == (0, 0)
+
+
+def test_oracle_reference_scene_files_render():
+    """The reference's scene files render through the oracle with the sizes
+    their transforms give (the GPU tests compare pixels)."""
+    g, (gx, gy) = O.scene_tree_render(Scene.from_json(load("grad.scene")), I)
+    # scale(3) of the clip square 0..80: at most 240 x 240, cut by the rotated fill
+    assert 0 < g.shape[0] <= 240 and 0 < g.shape[1] <= 240 and gx >= 0 and gy >= 0
+    assert np.isfinite(g).all() and g[..., 3].max() <= 1.0 + 1e-6
+    lc, _ = O.scene_tree_render(Scene.from_json(load("linear-colors.scene")), I)
+    assert lc.shape[:2] == (90, 520)
+    # the linear_colors bar and the sRGB-interpolated bar differ in the middle
+    mid = lc[:, 260]
+    assert abs(float(mid[25, 1]) - float(mid[65, 1])) > 0.05
+
+
+def test_oracle_active_edge_tree():
+    """The oracle's tree with ActiveEdgeRasterizer: fills and clip masks both
+    from ActiveEdgeIter (ActiveEdgeRasterizer::mask of each path)."""
+    p = PathBuilder().move_to((3, 3)).cubic_to((40, 0), (44, 40), (6, 37)).close().build()
+    q = PathBuilder().move_to((0, 20)).line_to((20, 0)).line_to((40, 20)).line_to((20, 40)).close().build()
+    col = LinColor(0.9, 0.5, 0.25, 1.0)
+    W = H = 48
+    cov = O.ae_mask(p, I, W, H, 0).astype(np.float32)
+    m = O.ae_mask(q, I, W, H, 0).astype(np.float32)
+    sc = Scene.fill(p, col).with_clip(q)
+    got, (x, y) = O.scene_tree_render(sc, I, active_edge=True)
+    h, w = got.shape[:2]
+    exp = (np.float32(col) * cov[..., None]) * m[..., None]
+    assert np.abs(got - exp[y:y + h, x:x + w]).max() <= 1e-6
+    sd, org = O.scene_tree_render(sc, I)
+    assert org == (x, y) and sd.shape == got.shape and np.abs(sd - got).max() <= 1e-3
