@@ -31,10 +31,13 @@ def parse():
     ap.add_argument("--rule", choices=["nonzero", "evenodd"], default="nonzero")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scene", action="store_true", help="skip the scene measurements")
-    ap.add_argument("--scene-only", action="store_true",
-                    help="only the scene measurements (for per-kernel profiles of them)")
+    ap.add_argument("--workload", choices=["scene", "circles", "atlas", "diff"],
+                    help="only this measurement, printed as one JSON line (for per-kernel "
+                         "profiles of one workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 256 MiB device buffer 3 times (calibrates HBM counters)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="with --workload scene/circles: only the batched form")
     ap.add_argument("--diff-steps", type=int, default=200, help="Adam steps of the differentiable render")
     ap.add_argument("--cpu-reps", type=int, default=16)
     return ap.parse_args()
@@ -94,7 +97,7 @@ def last_counts(lib, ctx):
     return nl.value, nr.value
 
 
-def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
+def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p, forms=None):
     """Batched and plain Scene::render of a generated scene into a W x H layer."""
     ds = rz.DeviceScene(ws.scene())
     ctx = ds.ctx
@@ -105,6 +108,8 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
     res = {}
     for form, batched, frames in (("batched", True, frames_b), ("plain", False, frames_p),
                                   ("graph", True, frames_b)):
+        if forms is not None and form not in forms:
+            continue
         set_graph(lib, _abi, ctx, form == "graph")
 
         def run():
@@ -136,9 +141,9 @@ def scene_measure(torch, rz, lib, _abi, name, ws, W, H, frames_b, frames_p):
         "cell_records": n_recs,
         "layer": [W, H],
         "batched_ms_per_frame": round(res["batched"], 4),
-        "batched_graph_ms_per_frame": round(res["graph"], 4),
-        "plain_ms_per_frame": round(res["plain"], 3),
-        "batched_over_plain": round(res["plain"] / res["batched"], 1),
+        "batched_graph_ms_per_frame": round(res["graph"], 4) if "graph" in res else None,
+        "plain_ms_per_frame": round(res["plain"], 3) if "plain" in res else None,
+        "batched_over_plain": round(res["plain"] / res["batched"], 1) if "plain" in res else None,
         "batched_mpix_s": round(W * H / res["batched"] / 1e3, 1),
         "batched_msegs_s": round(segs / res["batched"] / 1e3, 2),
         "batched_kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
@@ -281,15 +286,22 @@ def main():
         print(json.dumps({"copy_calibration_bytes": 4 * n}))
         return
     lib = _abi.lib()
-    if args.scene_only:
-        scenes = [
-            scene_measure(torch, rz, lib, _abi, "scene-c3", rz.workload_scene(3, 10_000, 2048.0),
-                          2048, 2048, 5, 1),
-            scene_measure(torch, rz, lib, _abi, "circles", rz.workload_circles(1024, 1024.0),
-                          1024, 1024, 5, 1),
-        ]
-        diff = diff_measure(torch, rz, 5) if args.diff_steps > 0 else None
-        print(json.dumps({"scenes": scenes, "differentiable": diff}))
+    if args.workload == "scene":
+        print(json.dumps(scene_measure(torch, rz, lib, _abi, "batched scene: 10,000 paths, 2048x2048",
+                                       rz.workload_scene(3, 10_000, 2048.0), 2048, 2048,
+                                       args.steps, 1, ["batched"] if args.batched_only else None)),
+              flush=True)
+        return
+    if args.workload == "circles":
+        print(json.dumps(scene_measure(torch, rz, lib, _abi, "1024 circles, 1024x1024",
+                                       rz.workload_circles(1024, 1024.0), 1024, 1024, args.steps,
+                                       1, ["batched"] if args.batched_only else None)), flush=True)
+        return
+    if args.workload == "atlas":
+        print(json.dumps(atlas_measure(torch, rz, lib, _abi, args.steps)[0]), flush=True)
+        return
+    if args.workload == "diff":
+        print(json.dumps(diff_measure(torch, rz, args.diff_steps)), flush=True)
         return
 
     W = H = 4096

@@ -407,11 +407,62 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
  * HIP graph captured after the same call completed once without growing the
  * context's buffers (the device data they read, e.g. updated scene points,
  * may change between frames). RZ_PARAM_GRAPH_REPLAYS (read only): frames
- * replayed so far. Results do not depend on any of them. */
+ * replayed so far. RZ_PARAM_FLAT_LDS (-1..32, default -1: 4 for one path, 0
+ * for scenes and atlases), RZ_PARAM_BIN_LDS, RZ_PARAM_FINE_LDS (0..32,
+ * default 0): KB of extra LDS per wave of the flatten, the count pass and the
+ * fine stage (fewer waves per SIMD). RZ_PARAM_SCATTER_WAVES (8..18, default
+ * 15): the scatter's grid holds at most 2^value waves. RZ_PARAM_TUNED (0 or
+ * 1, default 1): a call whose workload has a tuned entry (rz_tune_*) uses that
+ * entry's choices instead of the context's. Results do not depend on any of
+ * them. */
 enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2,
-       RZ_PARAM_SCAN_DIRECT = 3, RZ_PARAM_GRAPH = 4, RZ_PARAM_GRAPH_REPLAYS = 5 };
+       RZ_PARAM_SCAN_DIRECT = 3, RZ_PARAM_GRAPH = 4, RZ_PARAM_GRAPH_REPLAYS = 5,
+       RZ_PARAM_FLAT_LDS = 6, RZ_PARAM_BIN_LDS = 7, RZ_PARAM_FINE_LDS = 8,
+       RZ_PARAM_SCATTER_WAVES = 9, RZ_PARAM_TUNED = 10 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
+
+/* ---- launch tuning ---------------------------------------------------------
+ * No reference counterpart (the reference's only tunable is the flatness,
+ * src/rasterize.rs:279-296, which changes results and is not tuned here).
+ * rz_tune_mask_device / rz_tune_render_device (batched form) /
+ * rz_tune_atlas_device run the call they are given repeatedly (so they
+ * write its output, as the call would), searching the launch parameters
+ * above one at a time (virtual-subtree depths 0..6, extra LDS per wave for
+ * the flatten, the count pass and the fine stage, scatter grid size), each
+ * candidate timed with HIP events over about 3 ms of frames, for at most
+ * about budget_ms. The fastest becomes the entry of the call's workload
+ * (keyed by entry point, edge mode and log2 buckets of the pixels, the
+ * path's slots and its curves' expected line counts), which later calls with
+ * that workload use while RZ_PARAM_TUNED is on. Every choice gives the same
+ * result bit for bit. rz_tune_save / rz_tune_load keep the entries in a text
+ * file, per GPU (arch and CU count; other GPUs' lines are kept by a save and
+ * skipped by a load). Not to be called while a capture is open. */
+typedef struct rz_tune_result {
+  int32_t split_auto;    /* 1: depths from the per-call estimate (the two below) */
+  int32_t split_quad, split_cubic;
+  int32_t flat_lds;      /* bytes of extra LDS per flatten wave, -1: the built-in choice */
+  int32_t bin_lds, fine_lds;
+  int32_t scatter_log2;
+  uint32_t trials;       /* candidates timed */
+  double ms_default;     /* per frame with the context's parameters */
+  double ms_tuned;       /* per frame with the chosen ones */
+} rz_tune_result;
+rz_status rz_tune_mask_device(rz_ctx* ctx, const rz_dpath* path, rz_transform tr, double flatness,
+                              int fill_rule, int edge_mode, size_t width, size_t height,
+                              float* d_out, size_t row_stride, double budget_ms,
+                              rz_tune_result* result);
+rz_status rz_tune_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                double flatness, int edge_mode, const double* view,
+                                const float* bg, float* d_rgba, size_t row_stride,
+                                double budget_ms, rz_tune_result* result);
+rz_status rz_tune_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flatness,
+                               int fill_rule, int edge_mode, float* d_atlas, size_t row_stride,
+                               double budget_ms, rz_tune_result* result);
+rz_status rz_tune_save(rz_ctx* ctx, const char* path);
+rz_status rz_tune_load(rz_ctx* ctx, const char* path, size_t* n_loaded);
+rz_status rz_tune_clear(rz_ctx* ctx);
+rz_status rz_tune_entries(rz_ctx* ctx, size_t* n_entries);
 
 /* ---- kernel timing -------------------------------------------------------
  * No reference counterpart (the reference only has `tracing` spans in its

@@ -90,7 +90,13 @@ inline void bin_replay(const Path& path, const Transform& tr, double flatness, i
       stack.pop_back();
       if (s.has_nans()) throw std::runtime_error("cannot flatten segment with NaN");
       if (s.flatness() < fl) {
-        line(s.start(), s.end());
+        if (mode == rzg::EDGE_NONE) {
+          rzg::ae_chain(rp(s.start()), rp(s.end()), (double)H, [&](rzg::Pt a, rzg::Pt b) {
+            line(P(a.x, a.y), P(b.x, b.y));
+          });
+        } else {
+          line(s.start(), s.end());
+        }
         continue;
       }
       Segment s0, s1;

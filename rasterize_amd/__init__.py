@@ -8,6 +8,7 @@ coverage on the CPU.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -290,6 +291,38 @@ class Context:
     def check(self, s):
         if s != _abi.RZ_OK:
             raise RzError(s, lib().rz_last_error(self.h).decode())
+
+    def sync(self):
+        """rz_sync: wait for the context's work; raises the first error of the
+        calls enqueued since the last sync."""
+        self.check(lib().rz_sync(self.h))
+
+    def set_param(self, param, value):
+        """rz_set_param: launch choices (RZ_PARAM_*); results never depend on them."""
+        self.check(lib().rz_set_param(self.h, int(param), int(value)))
+
+    def get_param(self, param):
+        v = C.c_int()
+        self.check(lib().rz_get_param(self.h, int(param), C.byref(v)))
+        return v.value
+
+    def tune_save(self, path):
+        """Write this context's tuned launch choices to a cache file (per GPU)."""
+        self.check(lib().rz_tune_save(self.h, os.fsencode(path)))
+
+    def tune_load(self, path):
+        """Read the entries of this GPU from a cache file; returns how many."""
+        n = C.c_size_t()
+        self.check(lib().rz_tune_load(self.h, os.fsencode(path), C.byref(n)))
+        return n.value
+
+    def tune_clear(self):
+        self.check(lib().rz_tune_clear(self.h))
+
+    def tune_entries(self):
+        n = C.c_size_t()
+        self.check(lib().rz_tune_entries(self.h, C.byref(n)))
+        return n.value
 
     def close(self):
         if self.h:
@@ -902,6 +935,50 @@ def _scene_of_json(v, colors, depth=0):
     raise ValueError(f"unknown scene node type: {t!r}")
 
 
+def _tune_dict(r):
+    return {f: getattr(r, f) for f, _ in _abi.rz_tune_result._fields_}
+
+
+class DevicePath:
+    """A path uploaded once (rz_path_upload) for repeated masks into device
+    memory (rz_mask_device)."""
+
+    def __init__(self, path, ctx=None):
+        self.ctx = ctx or default_context()
+        self._abi_path = path.abi()
+        h = C.c_void_p()
+        self.ctx.check(lib().rz_path_upload(self.ctx.h, C.byref(self._abi_path), C.byref(h)))
+        self.h = h
+
+    def mask_device(self, tr, width, height, d_out, row_stride, fill_rule=FillRule.NonZero,
+                    flatness=DEFAULT_FLATNESS, edge_mode=_abi.RZ_EDGE_MASK):
+        """Asynchronous; errors surface at rz_sync (Context.sync)."""
+        self.ctx.check(lib().rz_mask_device(self.ctx.h, self.h, tr.abi(), flatness, int(fill_rule),
+                                            edge_mode, width, height, C.c_void_p(d_out), row_stride))
+
+    def tune(self, tr, width, height, d_out, row_stride, fill_rule=FillRule.NonZero,
+             flatness=DEFAULT_FLATNESS, edge_mode=_abi.RZ_EDGE_MASK, budget_ms=2000.0):
+        """rz_tune_mask_device: search the launch choices for this workload
+        (writing the mask meanwhile); returns the choice and both timings."""
+        r = _abi.rz_tune_result()
+        self.ctx.check(lib().rz_tune_mask_device(self.ctx.h, self.h, tr.abi(), flatness,
+                                                 int(fill_rule), edge_mode, width, height,
+                                                 C.c_void_p(d_out), row_stride, float(budget_ms),
+                                                 C.byref(r)))
+        return _tune_dict(r)
+
+    def close(self):
+        if self.h:
+            lib().rz_path_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceScene:
     """A scene uploaded once (rz_scene_upload) for repeated renders into device
     memory (rz_render_device)."""
@@ -920,6 +997,18 @@ class DeviceScene:
         b = (C.c_float * 4)(*bg) if bg is not None else None
         self.ctx.check(lib().rz_render_device(self.ctx.h, self.h, tr.abi(), flatness, edge_mode, v, b,
                                               int(batched), C.c_void_p(d_rgba), row_stride))
+
+    def tune(self, tr, view, d_rgba, row_stride, bg=None, flatness=DEFAULT_FLATNESS,
+             edge_mode=_abi.RZ_EDGE_ITER, budget_ms=2000.0):
+        """rz_tune_render_device (batched form): search the launch choices for
+        this scene and view; returns the choice and both timings."""
+        v = (C.c_double * 4)(*view)
+        b = (C.c_float * 4)(*bg) if bg is not None else None
+        r = _abi.rz_tune_result()
+        self.ctx.check(lib().rz_tune_render_device(self.ctx.h, self.h, tr.abi(), flatness, edge_mode,
+                                                   v, b, C.c_void_p(d_rgba), row_stride,
+                                                   float(budget_ms), C.byref(r)))
+        return _tune_dict(r)
 
     def n_points(self):
         n = C.c_size_t()
@@ -1181,6 +1270,15 @@ class DeviceGlyphs:
                     flatness=DEFAULT_FLATNESS, edge_mode=_abi.RZ_EDGE_MASK):
         self.ctx.check(lib().rz_mask_atlas_device(self.ctx.h, self.h, flatness, int(fill_rule),
                                                   edge_mode, C.c_void_p(d_atlas), row_stride))
+
+    def tune(self, d_atlas, row_stride, fill_rule=FillRule.NonZero, flatness=DEFAULT_FLATNESS,
+             edge_mode=_abi.RZ_EDGE_MASK, budget_ms=2000.0):
+        """rz_tune_atlas_device: search the launch choices for this glyph set."""
+        r = _abi.rz_tune_result()
+        self.ctx.check(lib().rz_tune_atlas_device(self.ctx.h, self.h, flatness, int(fill_rule),
+                                                  edge_mode, C.c_void_p(d_atlas), row_stride,
+                                                  float(budget_ms), C.byref(r)))
+        return _tune_dict(r)
 
     def close(self):
         if self.h:

@@ -18,6 +18,7 @@ STATUS_NAMES = {
 RZ_NONZERO, RZ_EVENODD = 0, 1
 RZ_PARAM_SPLIT_QUAD, RZ_PARAM_SPLIT_CUBIC, RZ_PARAM_SPLIT_AUTO, RZ_PARAM_SCAN_DIRECT = 0, 1, 2, 3
 RZ_PARAM_GRAPH, RZ_PARAM_GRAPH_REPLAYS = 4, 5
+RZ_PARAM_FLAT_LDS, RZ_PARAM_BIN_LDS, RZ_PARAM_FINE_LDS, RZ_PARAM_SCATTER_WAVES, RZ_PARAM_TUNED = 6, 7, 8, 9, 10
 RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
 RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
 RZ_PAINT_SOLID, RZ_PAINT_LINEAR, RZ_PAINT_RADIAL = 0, 1, 2
@@ -82,6 +83,13 @@ class rz_scene_node(C.Structure):
                 ("path", C.c_uint32), ("paint", C.c_uint32), ("child", C.c_uint32),
                 ("first", C.c_uint32), ("count", C.c_uint32), ("tr", C.c_double * 6),
                 ("opacity", C.c_double)]
+
+
+class rz_tune_result(C.Structure):
+    _fields_ = [("split_auto", C.c_int32), ("split_quad", C.c_int32), ("split_cubic", C.c_int32),
+                ("flat_lds", C.c_int32), ("bin_lds", C.c_int32), ("fine_lds", C.c_int32),
+                ("scatter_log2", C.c_int32), ("trials", C.c_uint32), ("ms_default", C.c_double),
+                ("ms_tuned", C.c_double)]
 
 
 class rz_kernel_stat(C.Structure):
@@ -157,6 +165,18 @@ PROTOTYPES = [
                                 C.c_int, C.c_int, P(C.c_float), C.c_size_t, C.c_size_t]),
     ("rz_set_param", C.c_int, [c_ctx, C.c_int, C.c_int]),
     ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
+    ("rz_tune_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
+                                      C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_double,
+                                      P(rz_tune_result)]),
+    ("rz_tune_render_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,
+                                        P(C.c_double), P(C.c_float), C.c_void_p, C.c_size_t,
+                                        C.c_double, P(rz_tune_result)]),
+    ("rz_tune_atlas_device", C.c_int, [c_ctx, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_size_t, C.c_double, P(rz_tune_result)]),
+    ("rz_tune_save", C.c_int, [c_ctx, C.c_char_p]),
+    ("rz_tune_load", C.c_int, [c_ctx, C.c_char_p, P(C.c_size_t)]),
+    ("rz_tune_clear", C.c_int, [c_ctx]),
+    ("rz_tune_entries", C.c_int, [c_ctx, P(C.c_size_t)]),
     ("rz_profile", C.c_int, [c_ctx, C.c_int]),
     ("rz_profile_stats", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, P(C.c_size_t)]),
     ("rz_last_counts", C.c_int, [c_ctx, P(C.c_size_t), P(C.c_size_t), P(C.c_size_t)]),

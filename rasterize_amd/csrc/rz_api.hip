@@ -7,7 +7,10 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <chrono>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -57,6 +60,37 @@ char* noalloc_scratch(int device) {
 }
 }  // namespace
 
+// Launch choices that change only the speed of a run, never its result: the
+// flatten's virtual-subtree depths, extra LDS per wave (so fewer waves per
+// SIMD) for the flatten, the count pass and the fine stage, and the size of
+// the scatter's grid. Set per context (rz_set_param) or per workload by the
+// tuner (rz_tune_*).
+struct LaunchParams {
+  int32_t split_auto = 1;     // depths from the path's line estimate (split_depths)
+  int32_t dq = 3, dc = 4;     // else these
+  int32_t flat_lds = -1;      // bytes; -1: 4096 for one path (3 waves per SIMD), 0 for batches
+  int32_t bin_lds = 0;        // bytes
+  int32_t fine_lds = 0;       // bytes
+  int32_t scatter_log2 = 15;  // the scatter grid holds at most 2^v waves
+};
+bool same_params(const LaunchParams& a, const LaunchParams& b) {
+  return std::memcmp(&a, &b, sizeof(a)) == 0;
+}
+
+// What the tuner keys its choices by: the entry point, the edge mode, and
+// log2 buckets of the output's pixels, the path's slots and its quads' and
+// cubics' expected line counts under the call's transform and flatness.
+struct TuneKey {
+  int32_t kind;  // 0 mask / fill, 1 batched scene, 2 glyph atlas
+  int32_t mode, px_log2, slots_log2, quad_log2, cubic_log2;
+};
+bool same_tune_key(const TuneKey& a, const TuneKey& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+struct TuneEntry {
+  TuneKey key;
+  LaunchParams p;
+  double ms_default, ms_tuned;
+};
+
 // Arguments that fix the work a device entry point enqueues, for replaying a
 // captured frame (all bytes compared; zero-initialised, padding included).
 struct GraphKey {
@@ -68,6 +102,7 @@ struct GraphKey {
   float bg[4];
   size_t w, h, stride, stride2;
   const void* ptr[4];       // device outputs / inputs
+  LaunchParams lp;          // the launch choices the frame was enqueued with
 };
 struct GraphEntry {
   GraphKey key;
@@ -125,9 +160,13 @@ struct rz_ctx {
   char* scratch_ptr = nullptr;         // device address of this device's `scratch`
   hipEvent_t xev = nullptr;            // orders the context's stream with a caller's stream
   rzk::Status last{};                  // status of the last run collected by finish()
-  // launch parameters (rz_set_param)
-  int split_quad = 3, split_cubic = 4;  // virtual-subtree depth of quad / cubic slots
-  bool split_auto = true;               // choose them per call (split_depths)
+  // launch parameters: the context's (rz_set_param), and those of the call
+  // being enqueued (the context's, or the tuned entry of the call's workload)
+  LaunchParams lp, cur;
+  bool use_tuned = true;               // apply tuned entries (RZ_PARAM_TUNED)
+  bool tune_forced = false;            // the tuner has set `cur` itself
+  std::vector<TuneEntry> tuned;
+  std::string gpu_id;                  // arch and CU count: tuning cache entries are per GPU
   uint32_t scan_direct = rzk::kScanDirect;  // block totals summed by the add pass itself
   uint32_t line_cap = 0;
   // kernel timing (rz_profile)
@@ -276,20 +315,66 @@ void split_sum(const rz_path* p, const double* m, SplitSum* s) {
 // two for cubics (measured best on the 200k-segment path and the glyph atlas).
 // `scale` is the transform's area scale (|det|), applied on top of the
 // estimate's own transform.
-void split_depths(rz_ctx* ctx, const SplitEst& e, double scale, double flatness, int* dq,
-                  int* dc) {
-  if (!ctx->split_auto) {
-    *dq = ctx->split_quad;
-    *dc = ctx->split_cubic;
-    return;
-  }
-  const double f = 1.41 * std::sqrt(std::sqrt(scale > 0.0 ? scale : 1.0) * 0.05 / flatness);
+// Expected lines per quad / cubic of a path under a transform of determinant
+// `scale` at `flatness` (the estimate is made at tol 0.05 under identity).
+double line_factor(double scale, double flatness) {
+  return 1.41 * std::sqrt(std::sqrt(scale > 0.0 ? scale : 1.0) * 0.05 / flatness);
+}
+
+void auto_depths(const SplitEst& e, double scale, double flatness, int* dq, int* dc) {
+  const double f = line_factor(scale, flatness);
   auto depth = [](double lines, double per_thread) {
     const double d = std::round(std::log2(std::max(lines, 1.0) / per_thread));
     return (int)std::min(6.0, std::max(0.0, d));
   };
   *dq = depth(e.quad * f, 4.0);
   *dc = depth(e.cubic * f, 2.0);
+}
+
+void split_depths(rz_ctx* ctx, const SplitEst& e, double scale, double flatness, int* dq,
+                  int* dc) {
+  if (!ctx->cur.split_auto) {
+    *dq = ctx->cur.dq;
+    *dc = ctx->cur.dc;
+    return;
+  }
+  auto_depths(e, scale, flatness, dq, dc);
+}
+
+int log2_bucket(double v) { return v >= 1.0 ? (int)std::lround(std::log2(v)) : 0; }
+
+TuneKey tune_key(int kind, int mode, size_t px, uint32_t slots, const SplitEst& e, double scale,
+                 double flatness) {
+  const double f = line_factor(scale, flatness);
+  TuneKey k;
+  std::memset(&k, 0, sizeof(k));
+  k.kind = kind;
+  k.mode = mode;
+  k.px_log2 = log2_bucket((double)px);
+  k.slots_log2 = log2_bucket((double)slots);
+  k.quad_log2 = log2_bucket(e.quad * f);
+  k.cubic_log2 = log2_bucket(e.cubic * f);
+  return k;
+}
+
+// The launch choices of the call about to be enqueued: its workload's tuned
+// entry if there is one (and RZ_PARAM_TUNED is on), else the context's.
+void select_params(rz_ctx* ctx, const TuneKey& key) {
+  if (ctx->tune_forced) return;
+  ctx->cur = ctx->lp;
+  if (!ctx->use_tuned) return;
+  for (const TuneEntry& t : ctx->tuned)
+    if (same_tune_key(t.key, key)) {
+      ctx->cur = t.p;
+      return;
+    }
+}
+
+double det_of(const rz_transform& tr) { return std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]); }
+
+TuneKey mask_tune_key(const rz_dpath* d, const rz_transform& tr, double flatness, int mode,
+                      size_t W, size_t H) {
+  return tune_key(0, mode, W * H, d->n_slots, d->est, det_of(tr), flatness);
 }
 
 rz_status check_launch(rz_ctx* ctx, const char* what) {
@@ -545,7 +630,8 @@ struct Pipe {
 // (kLineParts words) must be zero; the number of lines goes to st->n_lines.
 rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, double fl,
                           const rzk::PathInfo* pi, const uint32_t* slot_path, rzk::Status* st,
-                          uint32_t* line_fill, int dq, int dc, rzk::LineBufs* lb) {
+                          uint32_t* line_fill, int dq, int dc, int mode, double H,
+                          rzk::LineBufs* lb) {
   const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
                  bc = d->view.blocks_cubic << dc;
   const uint32_t nblk = bl + bq + bc;
@@ -555,20 +641,31 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
   lb->fill = line_fill;
   lb->st = st;
   lb->n_waves = nblk * (rzk::kBlock / 64);
+  lb->ae_h = H;
   RZ_TRY(ensure(ctx, ctx->wave_lines, (size_t)std::max<uint32_t>(lb->n_waves, 1) * sizeof(uint2)));
   lb->wave_lines = (uint2*)ctx->wave_lines.p;
   const uint32_t fsub = rzk::kBlock / rzk::kFlatBlock;
   if (nblk) {
-    if (pi) {
-      RZ_LAUNCH(ctx, "k_flat_lines", rzk::k_flat_lines<true>, dim3(nblk * fsub),
-                dim3(rzk::kFlatBlock), d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    // By default a single path's flatten runs at three waves per SIMD rather
+    // than the four its registers allow: 4 KB more LDS per wave. Measured on
+    // the 200k-segment path: 169 us at three, 181 us at four (the batched
+    // forms are faster at four).
+    // ActiveEdgeRasterizer's lines come near x = 0 as chains (rzg::ae_chain)
+    const int32_t lds = ctx->cur.flat_lds >= 0 ? ctx->cur.flat_lds : (pi ? 0 : 4096);
+    const bool ae = mode == RZ_EDGE_NONE;
+    const dim3 grid(nblk * fsub), block(rzk::kFlatBlock);
+    if (pi && ae) {
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, true>), grid, block, lds, d->view,
+                    x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    } else if (pi) {
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, false>), grid, block, lds,
+                    d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    } else if (ae) {
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, true>), grid, block, lds,
+                    d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     } else {
-      // A single path's flatten runs at three waves per SIMD rather than the
-      // four its registers allow: 4 KB more LDS per wave. Measured on the
-      // 200k-segment path: 169 us at three, 181 us at four (the batched forms
-      // are faster at four).
-      RZ_LAUNCH_LDS(ctx, "k_flat_lines", rzk::k_flat_lines<false>, dim3(nblk * fsub),
-                    dim3(rzk::kFlatBlock), 4096, d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, false>), grid, block, lds,
+                    d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     }
   }
   return check_launch(ctx, "flatten");
@@ -615,8 +712,8 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   if (nw == 0) return RZ_OK;
   // then the gap slots, 64 per wave
   const uint32_t gap_blocks = (ga.p.n_gaps + rzk::kBinBlock - 1) / rzk::kBinBlock;
-  RZ_LAUNCH(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb + gap_blocks),
-            dim3(rzk::kBinBlock), (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi,
+  RZ_LAUNCH_LDS(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb + gap_blocks),
+            dim3(rzk::kBinBlock), ctx->cur.bin_lds, (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi,
             *b, ga);
   return RZ_OK;
 }
@@ -626,7 +723,8 @@ rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* 
   // many waves, each over regions far apart (neighbouring regions hold
   // records of the same tiles)
   const uint32_t wpb = rzk::kScatterBlock / 64;
-  const uint32_t grid = (std::min(b.n_waves, rzk::kScatterWaves) + wpb - 1) / wpb;
+  const uint32_t cap = 1u << ctx->cur.scatter_log2;
+  const uint32_t grid = (std::min(b.n_waves, cap) + wpb - 1) / wpb;
   // then 64 blocks for the events
   RZ_LAUNCH(ctx, "k_bin_scatter", rzk::k_bin_scatter, dim3(grid + 64), dim3(rzk::kScatterBlock), b,
             line_fill, grid);
@@ -638,6 +736,7 @@ rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* 
 // device; finish() grows them and asks for a re-run).
 rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
                       int mode, size_t W, size_t H, Pipe* pp) {
+  select_params(ctx, mask_tune_key(d, tr, flatness, mode, W, H));
   Pipe& P = *pp;
   P.grid = make_grid(mode, W, H);
   const size_t tiles = P.grid.tiles, ty = (size_t)P.grid.g.tiles_y;
@@ -680,7 +779,7 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   int dq, dc;
   split_depths(ctx, d->est, std::fabs(x.m[0] * x.m[4] - x.m[1] * x.m[3]), flatness, &dq, &dc);
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, bin_fill + rzk::kParts + 1, dq, dc,
-                         &lb));
+                         mode, (double)H, &lb));
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
   b.tile_off = P.tile_off;
@@ -737,9 +836,11 @@ rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   }
   const uint32_t gf = (uint32_t)P.grid.tiles;
   if (rule == RZ_NONZERO)
-    RZ_LAUNCH(ctx, name0, (rzk::k_fine<0, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
+    RZ_LAUNCH_LDS(ctx, name0, (rzk::k_fine<0, Out>), dim3(gf), dim3(rzk::kFineWave),
+                  ctx->cur.fine_lds, a, o);
   else
-    RZ_LAUNCH(ctx, name1, (rzk::k_fine<1, Out>), dim3(gf), dim3(rzk::kFineWave), a, o);
+    RZ_LAUNCH_LDS(ctx, name1, (rzk::k_fine<1, Out>), dim3(gf), dim3(rzk::kFineWave),
+                  ctx->cur.fine_lds, a, o);
   return check_launch(ctx, name0);
 }
 
@@ -1105,10 +1206,24 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
 // (draw over the layer's pixels rather than start from bg) serve the scene
 // tree, whose fills between two Opacity / Clip nodes are drawn in batches
 // into layers that are not the view's.
+TuneKey scene_tune_key(const rz_dscene* s, const rz_transform& tr, double flatness, int mode,
+                       const double* view) {
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  return tune_key(1, mode, (size_t)lw * lh, s->all ? s->all->n_slots : 0,
+                  s->all ? s->all->est : SplitEst{}, det_of(tr), flatness);
+}
+
+TuneKey atlas_tune_key(const rz_dscene* s, double flatness, int mode) {
+  return tune_key(2, mode, (size_t)s->atlas_tiles * 256, s->all ? s->all->n_slots : 0,
+                  s->atlas_est, 1.0, flatness);
+}
+
 rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
                                 const double* view, rzp::C4 bg, int mode, double flatness,
                                 rzp::C4* d_out, size_t row_stride, const int* layer = nullptr,
                                 int load = 0) {
+  select_params(ctx, scene_tune_key(s, tr, flatness, mode, view));
   int lx, ly, lw, lh;
   if (layer) {
     lx = layer[0];
@@ -1200,7 +1315,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   int dq, dc;
   split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
-                         &lb));
+                         mode, 0.0, &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -1247,7 +1362,8 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.tiles_x = ctx_x;
   // no layer (the adjoint's own tile pass recomposites): the binned state only
   if (d_out)
-    RZ_LAUNCH(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64), a);
+    RZ_LAUNCH_LDS(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64),
+                  ctx->cur.fine_lds, a);
   else
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), (const rzk::Status*)st,
               ctx->acc_run);
@@ -1306,6 +1422,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
                         float* d_atlas, size_t row_stride) {
   const uint32_t n = s->n;
   if (n == 0 || s->atlas_tiles == 0) return RZ_OK;
+  select_params(ctx, atlas_tune_key(s, flatness, mode));
   const rz_dpath* d = s->all;
   if (ctx->line_cap == 0) {
     ctx->line_cap = line_cap_for(std::max<size_t>((size_t)d->n_slots * 16, 1 << 16));
@@ -1349,7 +1466,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   int dq, dc;
   split_depths(ctx, s->atlas_est, 1.0, flatness, &dq, &dc);
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
-                         &lb));
+                         mode, 0.0, &lb));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -1386,9 +1503,9 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   a.out = d_atlas;
   a.row_stride = row_stride;
   if (rule == RZ_NONZERO)
-    RZ_LAUNCH(ctx, "k_atlas_fine<0>", rzk::k_atlas_fine<0>, dim3(ct), dim3(64), a);
+    RZ_LAUNCH_LDS(ctx, "k_atlas_fine<0>", rzk::k_atlas_fine<0>, dim3(ct), dim3(64), ctx->cur.fine_lds, a);
   else
-    RZ_LAUNCH(ctx, "k_atlas_fine<1>", rzk::k_atlas_fine<1>, dim3(ct), dim3(64), a);
+    RZ_LAUNCH_LDS(ctx, "k_atlas_fine<1>", rzk::k_atlas_fine<1>, dim3(ct), dim3(64), ctx->cur.fine_lds, a);
   return check_launch(ctx, "atlas pass 1");
 }
 
@@ -1558,6 +1675,7 @@ GraphKey graph_key(rz_ctx* ctx, int kind, uint64_t obj) {
   k.kind = kind;
   k.obj = obj;
   k.gen = ctx->gen;
+  k.lp = ctx->cur;
   return k;
 }
 
@@ -1976,6 +2094,187 @@ rz_status check_scene_args(rz_ctx* ctx, double flatness, int mode, const double*
   return RZ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Tuning: a coordinate search over the launch choices of one workload. Each
+// candidate is timed with HIP events over repeated frames on the context's
+// stream (kernel by kernel, no graph replay); the fastest becomes the
+// workload's entry, which select_params applies to later calls. Every choice
+// gives the same result bit for bit, so the search only ever trades speed.
+// ---------------------------------------------------------------------------
+template <typename F>
+rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc, F&& frame,
+                      double budget_ms, rz_tune_result* res) {
+  if (ctx->capturing) return fail(ctx, RZ_E_ARG, "tuning inside a stream capture");
+  hipEvent_t ea = nullptr, eb = nullptr;
+  RZ_HIP(ctx, hipEventCreate(&ea));
+  if (hipEventCreate(&eb) != hipSuccess) {
+    (void)hipEventDestroy(ea);
+    return fail(ctx, RZ_E_HIP, "hipEventCreate failed");
+  }
+  const bool graphs = ctx->use_graphs;
+  ctx->use_graphs = false;
+  ctx->tune_forced = true;
+  struct Trial {
+    LaunchParams p;
+    double ms;
+  };
+  std::vector<Trial> seen;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto elapsed_ms = [&]() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  auto timed = [&](int reps, double* ms) -> rz_status {
+    RZ_HIP(ctx, hipEventRecord(ea, ctx->stream));
+    for (int r = 0; r < reps; ++r) RZ_TRY(frame());
+    RZ_HIP(ctx, hipEventRecord(eb, ctx->stream));
+    RZ_TRY(finish(ctx));
+    float t = 0.0f;
+    RZ_HIP(ctx, hipEventElapsedTime(&t, ea, eb));
+    *ms = (double)t / reps;
+    return RZ_OK;
+  };
+  auto measure = [&](const LaunchParams& p, double* ms) -> rz_status {
+    for (const Trial& t : seen)
+      if (same_params(t.p, p)) {
+        *ms = t.ms;
+        return RZ_OK;
+      }
+    ctx->cur = p;
+    // a frame that grew a buffer is run again (as a caller would)
+    rz_status st = RZ_E_CAPACITY;
+    for (int a = 0; a < 4 && st == RZ_E_CAPACITY; ++a) {
+      st = frame();
+      if (st == RZ_OK) st = finish(ctx);
+    }
+    RZ_TRY(st);
+    double one = 0.0;
+    RZ_TRY(timed(1, &one));
+    // about 3 ms of frames, 3 to 50 of them
+    const int reps = (int)std::min(50.0, std::max(3.0, 3.0 / std::max(one, 0.01)));
+    RZ_TRY(timed(reps, ms));
+    seen.push_back(Trial{p, *ms});
+    return RZ_OK;
+  };
+  auto explicit_depths = [&](LaunchParams p) {
+    if (p.split_auto) {
+      p.split_auto = 0;
+      p.dq = auto_dq;
+      p.dc = auto_dc;
+    }
+    return p;
+  };
+  const LaunchParams base = ctx->lp;
+  LaunchParams best = base;
+  double ms_default = 0.0, best_ms = 0.0;
+  rz_status st = measure(base, &ms_default);
+  best_ms = ms_default;
+  auto consider = [&](const LaunchParams& p) -> rz_status {
+    if (elapsed_ms() > budget_ms) return RZ_OK;
+    double ms = 0.0;
+    RZ_TRY(measure(p, &ms));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = p;
+    }
+    return RZ_OK;
+  };
+  auto sweep_depths = [&]() -> rz_status {
+    for (int v = 0; v <= 6; ++v) {
+      LaunchParams p = explicit_depths(best);
+      p.dq = v;
+      RZ_TRY(consider(p));
+    }
+    for (int v = 0; v <= 6; ++v) {
+      LaunchParams p = explicit_depths(best);
+      p.dc = v;
+      RZ_TRY(consider(p));
+    }
+    return RZ_OK;
+  };
+  if (st == RZ_OK) st = sweep_depths();
+  for (int32_t v : {-1, 0, 4096, 8192, 16384}) {
+    if (st != RZ_OK) break;
+    LaunchParams p = best;
+    p.flat_lds = v;
+    st = consider(p);
+  }
+  for (int32_t v : {0, 4096, 8192, 16384}) {
+    if (st != RZ_OK) break;
+    LaunchParams p = best;
+    p.bin_lds = v;
+    st = consider(p);
+  }
+  for (int32_t v : {0, 8192, 16384, 24576}) {
+    if (st != RZ_OK) break;
+    LaunchParams p = best;
+    p.fine_lds = v;
+    st = consider(p);
+  }
+  for (int32_t v = 12; v <= 17 && st == RZ_OK; ++v) {
+    LaunchParams p = best;
+    p.scatter_log2 = v;
+    st = consider(p);
+  }
+  if (st == RZ_OK) st = sweep_depths();
+  ctx->tune_forced = false;
+  ctx->use_graphs = graphs;
+  ++ctx->gen;
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  RZ_TRY(st);
+  bool found = false;
+  for (TuneEntry& t : ctx->tuned)
+    if (same_tune_key(t.key, key)) {
+      t = TuneEntry{key, best, ms_default, best_ms};
+      found = true;
+    }
+  if (!found) ctx->tuned.push_back(TuneEntry{key, best, ms_default, best_ms});
+  if (res) {
+    res->split_auto = best.split_auto;
+    res->split_quad = best.split_auto ? auto_dq : best.dq;
+    res->split_cubic = best.split_auto ? auto_dc : best.dc;
+    res->flat_lds = best.flat_lds;
+    res->bin_lds = best.bin_lds;
+    res->fine_lds = best.fine_lds;
+    res->scatter_log2 = best.scatter_log2;
+    res->trials = (uint32_t)seen.size();
+    res->ms_default = ms_default;
+    res->ms_tuned = best_ms;
+  }
+  return RZ_OK;
+}
+
+// The tuning cache file: lines "<gpu> <kind> <mode> <px> <slots> <quad>
+// <cubic> <split_auto> <dq> <dc> <flat_lds> <bin_lds> <fine_lds> <scatter>
+// <ms_default> <ms_tuned>"; '#' starts a comment. Entries of other GPUs are
+// kept by a save and skipped by a load.
+std::string tune_line(const std::string& gpu, const TuneEntry& t) {
+  std::ostringstream o;
+  o << gpu << ' ' << t.key.kind << ' ' << t.key.mode << ' ' << t.key.px_log2 << ' '
+    << t.key.slots_log2 << ' ' << t.key.quad_log2 << ' ' << t.key.cubic_log2 << ' '
+    << t.p.split_auto << ' ' << t.p.dq << ' ' << t.p.dc << ' ' << t.p.flat_lds << ' '
+    << t.p.bin_lds << ' ' << t.p.fine_lds << ' ' << t.p.scatter_log2 << ' ' << t.ms_default << ' '
+    << t.ms_tuned;
+  return o.str();
+}
+
+bool parse_tune_line(const std::string& line, std::string* gpu, TuneEntry* t) {
+  std::istringstream in(line);
+  std::memset(&t->key, 0, sizeof(t->key));
+  LaunchParams p;
+  if (!(in >> *gpu >> t->key.kind >> t->key.mode >> t->key.px_log2 >> t->key.slots_log2 >>
+        t->key.quad_log2 >> t->key.cubic_log2 >> p.split_auto >> p.dq >> p.dc >> p.flat_lds >>
+        p.bin_lds >> p.fine_lds >> p.scatter_log2 >> t->ms_default >> t->ms_tuned))
+    return false;
+  // only choices that rz_set_param could make
+  if (p.split_auto < 0 || p.split_auto > 1 || p.dq < 0 || p.dq > 8 || p.dc < 0 || p.dc > 8 ||
+      p.flat_lds < -1 || p.flat_lds > 32768 || p.bin_lds < 0 || p.bin_lds > 32768 ||
+      p.fine_lds < 0 || p.fine_lds > 32768 || p.scatter_log2 < 8 || p.scatter_log2 > 18)
+    return false;
+  t->p = p;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1995,15 +2294,33 @@ rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
     ctx->scan_direct = 1u << value;
     return RZ_OK;
   }
-  if (value < 0 || value > 8) return fail(ctx, RZ_E_ARG, "parameter out of range 0..8");
-  if (param == RZ_PARAM_SPLIT_QUAD) {
-    ctx->split_quad = value;
-    ctx->split_auto = false;
-  } else if (param == RZ_PARAM_SPLIT_CUBIC) {
-    ctx->split_cubic = value;
-    ctx->split_auto = false;
+  auto range = [&](int lo, int hi) {
+    if (value >= lo && value <= hi) return RZ_OK;
+    return fail(ctx, RZ_E_ARG, "parameter out of range " + std::to_string(lo) + ".." + std::to_string(hi));
+  };
+  LaunchParams& lp = ctx->lp;
+  if (param == RZ_PARAM_SPLIT_QUAD || param == RZ_PARAM_SPLIT_CUBIC) {
+    RZ_TRY(range(0, 8));
+    (param == RZ_PARAM_SPLIT_QUAD ? lp.dq : lp.dc) = value;
+    lp.split_auto = 0;
   } else if (param == RZ_PARAM_SPLIT_AUTO) {
-    ctx->split_auto = value != 0;
+    RZ_TRY(range(0, 1));
+    lp.split_auto = value;
+  } else if (param == RZ_PARAM_FLAT_LDS) {
+    RZ_TRY(range(-1, 32));
+    lp.flat_lds = value < 0 ? -1 : value * 1024;
+  } else if (param == RZ_PARAM_BIN_LDS) {
+    RZ_TRY(range(0, 32));
+    lp.bin_lds = value * 1024;
+  } else if (param == RZ_PARAM_FINE_LDS) {
+    RZ_TRY(range(0, 32));
+    lp.fine_lds = value * 1024;
+  } else if (param == RZ_PARAM_SCATTER_WAVES) {
+    RZ_TRY(range(8, 18));
+    lp.scatter_log2 = value;
+  } else if (param == RZ_PARAM_TUNED) {
+    RZ_TRY(range(0, 1));
+    ctx->use_tuned = value != 0;
   } else {
     return fail(ctx, RZ_E_ARG, "unknown parameter");
   }
@@ -2012,13 +2329,136 @@ rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
 
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   if (!ctx || !value) return RZ_E_ARG;
-  if (param == RZ_PARAM_SPLIT_QUAD) *value = ctx->split_quad;
-  else if (param == RZ_PARAM_SPLIT_CUBIC) *value = ctx->split_cubic;
-  else if (param == RZ_PARAM_SPLIT_AUTO) *value = ctx->split_auto ? 1 : 0;
+  const LaunchParams& lp = ctx->lp;
+  if (param == RZ_PARAM_SPLIT_QUAD) *value = lp.dq;
+  else if (param == RZ_PARAM_SPLIT_CUBIC) *value = lp.dc;
+  else if (param == RZ_PARAM_SPLIT_AUTO) *value = lp.split_auto;
+  else if (param == RZ_PARAM_FLAT_LDS) *value = lp.flat_lds < 0 ? -1 : lp.flat_lds / 1024;
+  else if (param == RZ_PARAM_BIN_LDS) *value = lp.bin_lds / 1024;
+  else if (param == RZ_PARAM_FINE_LDS) *value = lp.fine_lds / 1024;
+  else if (param == RZ_PARAM_SCATTER_WAVES) *value = lp.scatter_log2;
+  else if (param == RZ_PARAM_TUNED) *value = ctx->use_tuned ? 1 : 0;
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
   else if (param == RZ_PARAM_GRAPH_REPLAYS) *value = (int)std::min<uint64_t>(ctx->replays, INT32_MAX);
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
+  return RZ_OK;
+}
+
+rz_status rz_tune_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
+                              int fill_rule, int edge_mode, size_t width, size_t height,
+                              float* d_out, size_t row_stride, double budget_ms,
+                              rz_tune_result* res) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, width, height));
+  if (!d || !d_out || !width || !height) return fail(ctx, RZ_E_ARG, "null argument or empty mask");
+  if (row_stride < width) return fail(ctx, RZ_E_ARG, "row_stride < width");
+  int adq, adc;
+  auto_depths(d->est, det_of(tr), flatness, &adq, &adc);
+  return tune_search(ctx, mask_tune_key(d, tr, flatness, edge_mode, width, height), adq, adc,
+                     [&]() {
+                       return enqueue_mask(ctx, d, tr, flatness, fill_rule, edge_mode, width,
+                                           height, d_out, row_stride);
+                     },
+                     budget_ms, res);
+}
+
+rz_status rz_tune_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                double flatness, int edge_mode, const double* view,
+                                const float* bg, float* d_rgba, size_t row_stride,
+                                double budget_ms, rz_tune_result* res) {
+  RZ_TRY(check_scene_args(ctx, flatness, edge_mode, view));
+  if (!scene || !d_rgba) return fail(ctx, RZ_E_ARG, "null argument");
+  int lx, ly, lw, lh;
+  layer_of(view, &lx, &ly, &lw, &lh);
+  if (row_stride < (size_t)lw) return fail(ctx, RZ_E_ARG, "row_stride < layer width");
+  const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
+  int adq = 0, adc = 0;
+  if (scene->all) auto_depths(scene->all->est, det_of(tr), flatness, &adq, &adc);
+  return tune_search(ctx, scene_tune_key(scene, tr, flatness, edge_mode, view), adq, adc,
+                     [&]() {
+                       return enqueue_scene_batched(ctx, scene, tr, view, c, edge_mode, flatness,
+                                                    (rzp::C4*)d_rgba, row_stride);
+                     },
+                     budget_ms, res);
+}
+
+rz_status rz_tune_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flatness,
+                               int fill_rule, int edge_mode, float* d_atlas, size_t row_stride,
+                               double budget_ms, rz_tune_result* res) {
+  RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, 0, 0));
+  if (!glyphs || !glyphs->atlas_ready) return fail(ctx, RZ_E_ARG, "glyph placement not set");
+  if (!d_atlas) return fail(ctx, RZ_E_ARG, "d_atlas is NULL");
+  int adq, adc;
+  auto_depths(glyphs->atlas_est, 1.0, flatness, &adq, &adc);
+  return tune_search(ctx, atlas_tune_key(glyphs, flatness, edge_mode), adq, adc,
+                     [&]() {
+                       return enqueue_atlas(ctx, glyphs, fill_rule, edge_mode, flatness, d_atlas,
+                                            row_stride);
+                     },
+                     budget_ms, res);
+}
+
+rz_status rz_tune_save(rz_ctx* ctx, const char* path) {
+  if (!ctx || !path) return RZ_E_ARG;
+  std::vector<std::string> keep;
+  {
+    std::ifstream in(path);
+    std::string line, gpu;
+    TuneEntry t;
+    while (std::getline(in, line))
+      if (!line.empty() && line[0] != '#' && parse_tune_line(line, &gpu, &t) && gpu != ctx->gpu_id)
+        keep.push_back(line);
+  }
+  const std::string tmp = std::string(path) + ".tmp";
+  {
+    std::ofstream out(tmp, std::ios::trunc);
+    if (!out) return fail(ctx, RZ_E_ARG, std::string("cannot write ") + tmp);
+    out << "# rasterize_amd launch tuning: gpu kind mode px slots quad cubic split_auto dq dc "
+           "flat_lds bin_lds fine_lds scatter ms_default ms_tuned\n";
+    for (const std::string& l : keep) out << l << '\n';
+    out.precision(6);
+    for (const TuneEntry& t : ctx->tuned) out << tune_line(ctx->gpu_id, t) << '\n';
+    if (!out) return fail(ctx, RZ_E_ARG, std::string("cannot write ") + tmp);
+  }
+  if (std::rename(tmp.c_str(), path) != 0)
+    return fail(ctx, RZ_E_ARG, std::string("cannot replace ") + path);
+  return RZ_OK;
+}
+
+rz_status rz_tune_load(rz_ctx* ctx, const char* path, size_t* n_loaded) {
+  if (!ctx || !path) return RZ_E_ARG;
+  std::ifstream in(path);
+  if (!in) return fail(ctx, RZ_E_ARG, std::string("cannot read ") + path);
+  std::string line, gpu;
+  size_t n = 0;
+  while (std::getline(in, line)) {
+    TuneEntry t;
+    if (line.empty() || line[0] == '#' || !parse_tune_line(line, &gpu, &t) || gpu != ctx->gpu_id)
+      continue;
+    bool found = false;
+    for (TuneEntry& e : ctx->tuned)
+      if (same_tune_key(e.key, t.key)) {
+        e = t;
+        found = true;
+      }
+    if (!found) ctx->tuned.push_back(t);
+    ++n;
+  }
+  ++ctx->gen;
+  if (n_loaded) *n_loaded = n;
+  return RZ_OK;
+}
+
+rz_status rz_tune_clear(rz_ctx* ctx) {
+  if (!ctx) return RZ_E_ARG;
+  ctx->tuned.clear();
+  ++ctx->gen;
+  return RZ_OK;
+}
+
+rz_status rz_tune_entries(rz_ctx* ctx, size_t* n) {
+  if (!ctx || !n) return RZ_E_ARG;
+  *n = ctx->tuned.size();
   return RZ_OK;
 }
 
@@ -2059,6 +2499,15 @@ rz_status rz_ctx_create(rz_ctx** out, int device) {
   if (e == hipSuccess) e = hipMemset(ctx->acc, 0, 2 * sizeof(rzk::Status));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->xev, hipEventDisableTiming);
   if (e == hipSuccess) ctx->scratch_ptr = noalloc_scratch(device);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e == hipSuccess) {
+      std::string arch = prop.gcnArchName;
+      arch = arch.substr(0, arch.find(':'));
+      ctx->gpu_id = arch + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+    }
+  }
   if (e != hipSuccess) {
     if (ctx->acc) (void)hipFree(ctx->acc);
     if (ctx->host_status) (void)hipHostFree(ctx->host_status);
@@ -2200,6 +2649,7 @@ rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double
   RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, width, height));
   if (!d || (!d_out && width && height)) return fail(ctx, RZ_E_ARG, "null argument");
   if (row_stride < width) return fail(ctx, RZ_E_ARG, "row_stride < width");
+  select_params(ctx, mask_tune_key(d, tr, flatness, edge_mode, width, height));
   GraphKey k = graph_key(ctx, 0, d->serial);
   std::memcpy(k.tr, tr.m, sizeof(k.tr));
   k.flatness = flatness;
@@ -2235,6 +2685,7 @@ rz_status rz_mask_device_noalloc(rz_ctx* ctx, const rz_dpath* d, rz_transform tr
   const size_t tiles = gr.tiles, ty = (size_t)gr.g.tiles_y;
   const size_t meta_u32 = mask_meta_words(tiles, ty, height);
   int dq, dc;
+  select_params(ctx, mask_tune_key(d, tr, flatness, edge_mode, width, height));
   split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
   const size_t nblk = (size_t)d->view.blocks_line + ((size_t)d->view.blocks_quad << dq) +
                       ((size_t)d->view.blocks_cubic << dc);
@@ -2477,6 +2928,7 @@ rz_status rz_render_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
   if (row_stride < (size_t)lw) return fail(ctx, RZ_E_ARG, "row_stride < layer width");
   const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
   if (batched) {
+    select_params(ctx, scene_tune_key(scene, tr, flatness, edge_mode, view));
     GraphKey k = graph_key(ctx, 1, scene->serial);
     std::memcpy(k.tr, tr.m, sizeof(k.tr));
     std::memcpy(k.view, view, sizeof(k.view));
@@ -2619,6 +3071,7 @@ rz_status rz_mask_atlas_device(rz_ctx* ctx, const rz_dscene* glyphs, double flat
   RZ_TRY(check_args(ctx, flatness, fill_rule, edge_mode, 0, 0));
   if (!glyphs || !glyphs->atlas_ready) return fail(ctx, RZ_E_ARG, "glyph placement not set");
   if (glyphs->n && !d_atlas) return fail(ctx, RZ_E_ARG, "d_atlas is NULL");
+  select_params(ctx, atlas_tune_key(glyphs, flatness, edge_mode));
   GraphKey k = graph_key(ctx, 2, glyphs->serial);
   k.flatness = flatness;
   k.rule = fill_rule;
@@ -2747,6 +3200,7 @@ rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_tran
   if (!scene || !d_grad_rgba || !d_grad_points || !d_grad_colors)
     return fail(ctx, RZ_E_ARG, "null argument");
   const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
+  select_params(ctx, scene_tune_key(scene, tr, flatness, edge_mode, view));
   GraphKey k = graph_key(ctx, 3, scene->serial);
   std::memcpy(k.tr, tr.m, sizeof(k.tr));
   std::memcpy(k.view, view, sizeof(k.view));

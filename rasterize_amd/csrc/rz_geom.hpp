@@ -310,6 +310,132 @@ RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {
   return e;
 }
 
+// ActiveEdgeRasterizer's left-edge quirk (src/rasterize.rs:733-737, 760-871,
+// 923-937). ActiveEdgeIter walks each edge row by row from its lower end,
+// computing the x where it leaves each row incrementally (x += dxdy * dy). A
+// row piece whose leftmost x comes out negative (a left-split point at
+// x = -1e-17, or the walk arriving at x = 0 a rounding below it) starts its
+// column walk with next_ceil(x) = 0, so all of that row piece's deposits
+// land one column right of their exact place. Those deposits are the exact-
+// area deposits of the row piece moved by x + 1, so an edge line is replaced
+// by an equivalent chain: the rows of each piece of its left split as runs,
+// those with a negative leftmost x moved by + 1, joined by horizontal lines
+// (which deposit nothing). Only pieces whose x comes within kAeNear of 0 are
+// walked (elsewhere the incremental x stays far from 0 for any image below
+// about 10^5 px); `line` receives the chain's lines in order.
+constexpr double kAeNear = 1e-6;
+constexpr int kAeRuns = 4;  // runs recorded per piece (more: the rest keeps the last state)
+constexpr double kEps = 2.220446049250313e-16;  // f64::EPSILON (src/geometry.rs:14)
+
+// The rows of the piece a -> b (one ActiveEdgeIter edge) below image row H:
+// vertices after a up to (not including) b, through `to`.
+template <typename To>
+RZ_HD void ae_piece(Pt a, Pt b, double H, To&& to) {
+  if (fabs(a.y - b.y) < kEps) return;  // Edge::new: no rows
+  const bool up = a.y <= b.y;
+  const Pt lo = up ? a : b, hi = up ? b : a;
+  const double dxdy = (hi.x - lo.x) / (hi.y - lo.y);
+  Pt p = lo;
+  if (p.y < 0.0) p = mkpt(p.x - p.y * dxdy, 0.0);
+  if (!(fmin(p.x, hi.x) < kAeNear)) return;
+  Pt rs[kAeRuns];
+  bool rf[kAeRuns];
+  int nr = 0;
+  bool any = false;
+  while (hi.y - p.y >= kEps && floor(p.y) < H) {
+    const double ys = fmin(floor(p.y) + 1.0, hi.y);
+    const Pt q = mkpt(p.x + dxdy * (ys - p.y), ys);
+    const bool f = fmin(p.x, q.x) < 0.0;
+    if ((nr == 0 || f != rf[nr - 1]) && nr < kAeRuns) {
+      rs[nr] = p;
+      rf[nr] = f;
+      ++nr;
+    }
+    any = any || f;
+    p = q;
+  }
+  if (!any) return;
+  const Pt e = p;  // where the walk ended
+  auto sh = [](Pt v) { return mkpt(v.x + 1.0, v.y); };
+  if (up) {
+    bool st = false;
+    for (int k = 0; k < nr; ++k) {
+      if (rf[k] == st) continue;
+      if (rf[k]) {
+        to(rs[k]);
+        to(sh(rs[k]));
+      } else {
+        to(sh(rs[k]));
+        to(rs[k]);
+      }
+      st = rf[k];
+    }
+    if (st) {
+      to(sh(e));
+      to(e);
+    }
+  } else {
+    bool st = false;
+    if (rf[nr - 1]) {
+      to(e);
+      to(sh(e));
+      st = true;
+    }
+    for (int k = nr - 1; k >= 0; --k) {
+      const bool below = k > 0 ? rf[k - 1] : false;
+      if (rf[k] == below) continue;
+      if (rf[k]) {
+        to(sh(rs[k]));
+        to(rs[k]);
+      } else {
+        to(rs[k]);
+        to(sh(rs[k]));
+      }
+      st = below;
+    }
+    (void)st;
+  }
+}
+
+// The chain that replaces the line p0 -> p1 under ActiveEdgeRasterizer
+// (H: image rows); `line(a, b)` receives its lines.
+template <typename LineF>
+RZ_HD void ae_chain(Pt p0, Pt p1, double H, LineF&& line) {
+  if (!(fmin(p0.x, p1.x) < kAeNear) || (p0.x <= 0.0 && p1.x <= 0.0)) {
+    line(p0, p1);  // never near x = 0, or all projected onto x = 0 exactly
+    return;
+  }
+  Pt prev = p0;
+  auto to = [&](Pt v) {
+    if (v.x == prev.x && v.y == prev.y) return;
+    line(prev, v);
+    prev = v;
+  };
+  if (p0.x >= 0.0 && p1.x >= 0.0) {
+    ae_piece(p0, p1, H, to);
+    to(p1);
+    return;
+  }
+  // the left split, as effective_line computes it (src/rasterize.rs:923-937)
+  const double t = p0.x / (p0.x - p1.x);
+  const Pt mid = mkpt((1.0 - t) * p0.x + t * p1.x, (1.0 - t) * p0.y + t * p1.y);
+  if (p0.x < 0.0) {
+    const Pt v0 = mkpt(0.0, p0.y);
+    to(v0);
+    ae_piece(v0, mid, H, to);
+    to(mid);
+    ae_piece(mid, p1, H, to);
+    to(p1);
+  } else {
+    const Pt v1 = mkpt(0.0, p1.y);
+    ae_piece(p0, mid, H, to);
+    to(mid);
+    ae_piece(mid, v1, H, to);
+    to(v1);
+    to(p1);
+  }
+}
+
 RZ_HD int tile_col(double x, const BinGrid& g) {
   double f = floor((x + (double)g.phx) / (double)kTile);
   if (!(f >= 0.0)) return 0;

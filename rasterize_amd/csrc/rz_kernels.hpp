@@ -205,6 +205,7 @@ struct LineBufs {
   uint32_t* fill;       // [kLineParts] lines per part
   uint2* wave_lines;    // per flatten wave: first line, line count
   uint32_t n_waves;     // flatten waves
+  double ae_h;          // ActiveEdgeRasterizer: image rows of a single path (rzg::ae_chain)
   Status* st;
 };
 
@@ -253,7 +254,9 @@ __device__ inline void flat_subtree(const DPath& p, uint32_t slot, uint32_t j, i
 
 // One flatten thread: count (keeping the first lines), take the wave's range,
 // write. Every lane of the wave takes part.
-template <int NP, bool MULTI>
+// AE: ActiveEdgeRasterizer, each leaf line becomes the chain rzg::ae_chain
+// makes of it (the line itself unless it comes near x = 0).
+template <int NP, bool MULTI, bool AE>
 __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
                           const Xform& x, double fl, const PathInfo* pi, const uint32_t* slot_path,
                           const LineBufs& lb, double2* keep, uint32_t part, uint32_t wave) {
@@ -262,20 +265,24 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   int err;
   uint32_t pid;
   flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
+  double ae_h = 0.0;
+  if (AE) ae_h = MULTI ? (double)(active ? pi[pid].g.H : 0) : lb.ae_h;
+  // each line of the subtree, as it is or as its chain
+  auto lines_of = [&](Pt a, Pt c, auto&& f) {
+    if (AE) rzg::ae_chain(a, c, ae_h, f);
+    else f(a, c);
+  };
   uint32_t cnt = 0;
-  if (leaf) {
-    keep[0] = make_double2(n.p[0].x, n.p[0].y);
-    keep[1] = make_double2(n.p[NP - 1].x, n.p[NP - 1].y);
-    cnt = 1;
-  }
+  auto count = [&](Pt a, Pt c) {
+    if (cnt < (uint32_t)kLineKeep) {
+      if (cnt == 0) keep[0] = make_double2(a.x, a.y);
+      keep[cnt + 1] = make_double2(c.x, c.y);
+    }
+    ++cnt;
+  };
+  if (leaf) lines_of(n.p[0], n.p[NP - 1], count);
   if (walk) {
-    const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) {
-      if (cnt < (uint32_t)kLineKeep) {
-        if (cnt == 0) keep[0] = make_double2(a.x, a.y);
-        keep[cnt + 1] = make_double2(c.x, c.y);
-      }
-      ++cnt;
-    });
+    const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, count); });
     if (r != rzg::ST_OK) err = r;
   }
   if (err != rzg::ST_OK) {
@@ -310,15 +317,15 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
     return;
   }
   // more lines than were kept: walk the subtree again
-  if (leaf) {
-    lb.lines[o] = make_double4(n.p[0].x, n.p[0].y, n.p[NP - 1].x, n.p[NP - 1].y);
-    if (MULTI) lb.line_path[o] = pid;
-    return;
-  }
-  rzg::traverse(n, fl, [&](Pt a, Pt c) {
+  auto put = [&](Pt a, Pt c) {
     if (MULTI) lb.line_path[o] = pid;
     lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y);
-  });
+  };
+  if (leaf) {
+    lines_of(n.p[0], n.p[NP - 1], put);
+    return;
+  }
+  rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, put); });
 }
 
 // Blocks [0, bl) handle line slots, [bl, bl + bq) quad virtual items, the rest
@@ -328,17 +335,17 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
                  t = (blockIdx.x % (kBlock / kFlatBlock)) * kFlatBlock + threadIdx.x;      \
   if (blk < bl) {                                                                          \
     const uint32_t slot = p.order[blk * kBlock + t];                                       \
-    ITEM<2, MULTI>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);         \
+    ITEM<2, MULTI, AE>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);     \
   } else if (blk < bl + bq) {                                                              \
     const uint32_t v = (blk - bl) * kBlock + t;                                            \
     const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];                     \
-    ITEM<3, MULTI>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi, slot_path,  \
-                   __VA_ARGS__);                                                           \
+    ITEM<3, MULTI, AE>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi,        \
+                       slot_path, __VA_ARGS__);                                            \
   } else {                                                                                 \
     const uint32_t v = (blk - bl - bq) * kBlock + t;                                       \
     const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];   \
-    ITEM<4, MULTI>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi, slot_path,  \
-                   __VA_ARGS__);                                                           \
+    ITEM<4, MULTI, AE>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi,        \
+                       slot_path, __VA_ARGS__);                                            \
   }
 
 // Workgroup size of the flatten: the slot blocks above are kBlock threads,
@@ -346,7 +353,7 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
 // with long subtrees then holds only its own LDS and wave slot.
 constexpr uint32_t kFlatBlock = 64;
 
-template <bool MULTI>
+template <bool MULTI, bool AE>
 __global__ __launch_bounds__(kFlatBlock) void k_flat_lines(DPath p, Xform x, double fl, uint32_t bl,
                                                             uint32_t bq, int dq, int dc,
                                                             const PathInfo* pi,

@@ -139,12 +139,8 @@ def test_decomposition_reproduces_reference(name, mode, rule):
         ref = O.sd_mask_iter(path, tr, W, H, rule)
         got = np.where(np.abs(got) >= 1e-6, got, 0.0)
     else:
-        if O.path_bbox(path, tr)[0] < 0.0:
-            pytest.skip(
-                "reference ActiveEdgeIter quirk: a left-edge split point that rounds to "
-                "x = -1e-16 makes next_ceil() return 0, which shifts the whole row piece of "
-                "that edge one column right (src/rasterize.rs:735-737, 857-871); the GPU keeps "
-                "exact-area semantics, so active-edge parity uses paths with x >= 0")
+        # ActiveEdgeIter's left-split quirk (a split point at x = -1e-16 moves
+        # row pieces one column right) is reproduced by rzg::ae_left_shift
         ref = O.ae_mask(path, tr, W, H, rule)
     d = np.abs(got - ref)
     assert d.max() <= 2e-4, f"max {d.max()} at {np.unravel_index(d.argmax(), d.shape)}"
@@ -192,3 +188,38 @@ def test_many_coincident_edges_need_64_bit_tiles():
     assert np.abs(wrapped - ref).max() > 0.5  # the wrap is real
     exact = fine_emulation(counts, recs.astype(np.float64), bd, ev, redge, W, H, 0, 0)
     assert np.abs(exact - ref).max() <= 2e-4
+
+
+def _zero_crossing_path(seed, n=400, W=64, H=96):
+    """Closed random walks that cross x = 0 and touch it: lines with
+    non-dyadic ends left and right of the image's left edge, and vertices
+    exactly on x = 0."""
+    rng = np.random.default_rng(seed)
+    pb = PathBuilder()
+    for sub in range(4):
+        pts = [(float(rng.uniform(-3.0, 6.0)), float(rng.uniform(0.0, H)))]
+        for k in range(n // 4):
+            x = 0.0 if rng.uniform() < 0.15 else float(rng.uniform(-3.0, 6.0)) / 3.0
+            pts.append((x, float(np.clip(pts[-1][1] + rng.normal(0.0, 7.3), -4.0, H + 4.0))))
+        pb.move_to(pts[0])
+        for p in pts[1:]:
+            pb.line_to(p)
+        pb.close()
+    return pb.build(), W, H
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("rule", [0, 1])
+def test_active_edge_left_edge_chains(seed, rule):
+    """The replay with ActiveEdgeRasterizer's chains reproduces the oracle's
+    ActiveEdgeRasterizer::mask on paths that cross and touch x = 0, where
+    the exact-area mask differs from it by whole row pieces."""
+    path, W, H = _zero_crossing_path(seed)
+    counts, recs, bd, ev, redge = O.bin_replay(path, I, W, H, 2)
+    got = fine_emulation(counts, recs.astype(np.float64), bd, ev, redge, W, H, rule, 2)
+    ref = O.ae_mask(path, I, W, H, rule)
+    d = np.abs(got - ref)
+    assert d.max() <= 2e-4, f"max {d.max()} at {np.unravel_index(d.argmax(), d.shape)}"
+    # the quirk is there to reproduce: the exact-area mask differs
+    exact = O.sd_mask_iter(path, I, W, H, rule)
+    assert np.abs(exact - ref).max() > 0.01

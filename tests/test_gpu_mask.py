@@ -99,7 +99,7 @@ def _sort_events(ev):
 
 
 @pytest.mark.parametrize("case", ["c1", "squirrel", "edges", "c2"])
-@pytest.mark.parametrize("mode", [rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_ITER])
+@pytest.mark.parametrize("mode", [rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_ITER, rz._abi.RZ_EDGE_NONE])
 def test_bin_bit_exact(case, mode, c1, c2):
     if case == "c1":
         path, tr, W, H = c1, I, 512, 512
@@ -277,16 +277,32 @@ def test_mask_iter_matches_oracle(rule):
 
 @pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
 def test_active_edge_matches_oracle(rule):
-    # crosses the right and bottom edges but not x = 0: the reference's
-    # ActiveEdgeIter shifts column-0 coverage into column 1 when a left-edge
-    # split point rounds to a tiny negative x (src/rasterize.rs:857-871),
-    # which the GPU does not reproduce
+    # crosses the right and bottom edges
     path = workload_cubics64(9, 300.0).paths[0]
     tr = Transform.new_translate(2.0, -10.0)
     W, H = 200, 230
     g = ActiveEdgeRasterizer().mask_f32(path, tr, W, H, rule)
     r = O.ae_mask(path, tr, W, H, rule)
     check_close(g, r, f"active-edge rule={rule}")
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("rule", [FillRule.NonZero, FillRule.EvenOdd])
+def test_active_edge_left_edge(seed, rule):
+    """Paths that cross and touch x = 0, where ActiveEdgeIter moves row pieces
+    one column right (a split point or a row's walk a rounding below x = 0,
+    src/rasterize.rs:733-737, 812-871): the GPU's chains reproduce it."""
+    from test_decomposition_cpu import _zero_crossing_path
+    path, W, H = _zero_crossing_path(seed)
+    _say(f"active-edge left edge seed={seed} rule={rule}")
+    g = ActiveEdgeRasterizer().mask_f32(path, I, W, H, rule)
+    r = O.ae_mask(path, I, W, H, rule)
+    check_close(g, r, f"active-edge left edge seed={seed} rule={rule}")
+    # and the binning, bit for bit
+    gc, gr, gb, ge, gx = rz.bin_stage(path, I, W, H, rz._abi.RZ_EDGE_NONE)
+    rc, rr, rb, re, rx = O.bin_replay(path, I, W, H, rz._abi.RZ_EDGE_NONE)
+    assert np.array_equal(gc, rc) and np.array_equal(gb, rb)
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr))
 
 
 def test_strided_view_output():
