@@ -422,6 +422,11 @@ enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 
+/* Path::bbox(tr) (src/path.rs:428-431): the exact box of the transformed
+ * segments (curve extrema, not control points) as x0, y0, x1, y1 into bbox;
+ * *has_bbox = 0 for a path without segments. Host only. */
+rz_status rz_path_bbox(const rz_path* path, rz_transform tr, int* has_bbox, double* bbox);
+
 /* ---- launch tuning ---------------------------------------------------------
  * No reference counterpart (the reference's only tunable is the flatness,
  * src/rasterize.rs:279-296, which changes results and is not tuned here).

@@ -94,6 +94,44 @@ class Transform:
             raise ValueError(f"invalid SVG transform at byte {err.value}")
         return Transform(*t.m)
 
+    @staticmethod
+    def fit_bbox(src, dst, align=1):
+        """Transform::fit_bbox (src/geometry.rs:470-487); boxes are (x0, y0,
+        x1, y1), align 0 Min, 1 Mid, 2 Max."""
+        sw, sh = src[2] - src[0], src[3] - src[1]
+        dw, dh = dst[2] - dst[0], dst[3] - dst[1]
+        scale = min(dh / sh, dw / sw)
+        base = (Transform.new_translate(dst[0], dst[1]).pre_scale(scale, scale)
+                .pre_translate(-src[0], -src[1]))
+        if align == 0:
+            a = Transform.identity()
+        elif align == 1:
+            a = Transform.new_translate((dw - sw * scale) / 2.0, (dh - sh * scale) / 2.0)
+        else:
+            a = Transform.new_translate(dw - sw * scale, dh - sh * scale)
+        return a * base
+
+    @staticmethod
+    def fit_size(src, width, height, align=1):
+        """Transform::fit_size (src/geometry.rs:490-516): ((width, height),
+        transform) fitting the whole-pixel box of src into an image of that
+        size; a 0 dimension follows the other (both 0: src's size)."""
+        x0, y0 = math.floor(src[0]), math.floor(src[1])
+        x1, y1 = math.ceil(src[2]), math.ceil(src[3])
+        box = (float(x0), float(y0), float(x1), float(y1))
+        sw, sh = box[2] - box[0], box[3] - box[1]
+        if height == 0 and width == 0:
+            h, w = sh, sw
+        elif width == 0:
+            h = float(height)
+            w = float(math.ceil(sw * h / sh))
+        elif height == 0:
+            w = float(width)
+            h = float(math.ceil(sh * w / sw))
+        else:
+            h, w = float(height), float(width)
+        return (int(w), int(h)), Transform.fit_bbox(box, (0.0, 0.0, w, h), align)
+
     def apply(self, p):
         m = self.m
         return (p[0] * m[0] + p[1] * m[1] + m[2], p[0] * m[3] + p[1] * m[4] + m[5])
@@ -174,6 +212,16 @@ class Path:
             p.closed = self.closed.ctypes.data_as(C.POINTER(C.c_uint8))
             self._abi = p
         return self._abi
+
+    def bbox(self, tr=None):
+        """Path::bbox (src/path.rs:428-431): (x0, y0, x1, y1) of the segments
+        under tr (curve extrema), or None for an empty path."""
+        has = C.c_int()
+        out = (C.c_double * 4)()
+        s = lib().rz_path_bbox(C.byref(self.abi()), (tr or Transform.identity()).abi(), C.byref(has), out)
+        if s != _abi.RZ_OK:
+            raise RzError(s, "rz_path_bbox failed")
+        return tuple(out) if has.value else None
 
     def transform_points(self, tr):
         """Path::transform (src/path.rs:359-363) on a copy."""
@@ -265,6 +313,25 @@ class PathBuilder:
                 .cubic_to((p3[0] - off, p3[1]), (p0[0], p0[1] + off), p0)
                 .close()
                 .move_to((cx, cy)))
+
+    def checkerboard(self, bbox, cell_size):
+        """PathBuilder::checkerboard (src/path.rs:1031-1050): cells of the
+        checkerboard over bbox = (x0, y0, x1, y1), to be filled EvenOdd."""
+        x0, y0, x1, y1 = bbox
+        y = y0
+        while y < y1:
+            x = x0
+            while x < x1:
+                self.move_to((x, y))
+                self.line_to((x + cell_size, y))
+                self.line_to((x + cell_size, y + 2.0 * cell_size))
+                self.line_to((x + 2.0 * cell_size, y + 2.0 * cell_size))
+                self.line_to((x + 2.0 * cell_size, y + cell_size))
+                self.line_to((x, y + cell_size))
+                self.close()
+                x += 2.0 * cell_size
+            y += 2.0 * cell_size
+        return self.move_to((x0, y0))
 
     def build(self):
         self._finish(False)

@@ -1,0 +1,174 @@
+"""Command line renderer of an SVG path, after the reference's
+examples/rasterize.rs (argument handling :47-160, rendering :232-326):
+
+    python -m rasterize_amd [-h <height>] [-w <width>] [-v <view_box>]
+        [-t <transform>] [-fg <color>] [-bg <color>] [-f bmp|png|rgba]
+        [-rf <flatness>] [-ra] <input_file> <output_file>
+
+The path (SVG path data; '-' reads stdin) is transformed by -t, fitted into
+the image (Transform::fit_size, centred), filled with -fg (default opaque
+black) over -bg, or over a 16 px #d0d0d0 checkerboard on #f0f0f0, and
+written as BMP (default), PNG or raw RGBA (u32 LE height, width, then RGBA
+bytes; '-' writes stdout). -ra uses ActiveEdgeRasterizer. Stroking (-s) and
+outlines (-ro) are not part of this build and are refused."""
+import struct
+import sys
+
+USAGE = """Very simple tool that accepts SVG path as an input and produces rasterized image
+
+USAGE:
+    python -m rasterize_amd [-h <height>] [-w <width>] [-v <view_box>] [-t <transform>]
+              [-fg <color>] [-bg <color>] [-f <format>] [-rf <flatness>] [-ra]
+              <input_file> <output_file>
+
+ARGS:
+    -h <height>        height in pixels of the output image
+    -w <width>         width in pixels of the output image
+    -v <view_box>      view box "x y width height"
+    -t <transform>     apply transform (SVG transform list)
+    -f <format>        output file format (bmp, png, rgba)
+    -fg <color>        foreground color
+    -bg <color>        background color
+    -rf <flatness>     flatness used by rasterizer (default: 0.05)
+    -ra                rasterize active-edge instead of signed-difference rasterizer
+    <input_file>       file containing SVG path ('-' means stdin)
+    <output_file>      image rendered in the BMP format ('-' means stdout)
+"""
+
+
+class ArgError(ValueError):
+    pass
+
+
+def parse_view_box(text):
+    """FromStr for BBox (src/geometry.rs:690-701): "minx miny width height",
+    separated by spaces or commas."""
+    vals = []
+    for part in text.replace(",", " ").split(" "):
+        if len(vals) == 4:
+            break
+        try:
+            vals.append(float(part.strip()))
+        except ValueError:
+            raise ArgError(f"invalid view box: {text!r}") from None
+    if len(vals) < 4:
+        raise ArgError(f"invalid view box: {text!r}")
+    x, y, w, h = vals
+    return (min(x, x + w), min(y, y + h), max(x, x + w), max(y, y + h))
+
+
+def parse_args(argv):
+    a = {"input": None, "output": None, "format": "bmp", "width": 0, "height": 0, "view_box": None,
+         "tr": None, "fg": None, "bg": None, "flatness": 0.05, "active_edge": False}
+    it = iter(argv)
+    positional = []
+
+    def value(flag):
+        try:
+            return next(it)
+        except StopIteration:
+            raise ArgError(f"{flag} requires argument") from None
+    for arg in it:
+        if arg == "-h":
+            a["height"] = int(value(arg))
+        elif arg == "-w":
+            a["width"] = int(value(arg))
+        elif arg == "-v":
+            a["view_box"] = parse_view_box(value(arg))
+        elif arg == "-t":
+            a["tr"] = value(arg)
+        elif arg == "-f":
+            f = value(arg)
+            if f not in ("bmp", "png", "rgba"):
+                raise ArgError(f"Invalid output format: {f}")
+            a["format"] = f
+        elif arg == "-fg":
+            a["fg"] = value(arg)
+        elif arg == "-bg":
+            a["bg"] = value(arg)
+        elif arg == "-rf":
+            fl = float(value(arg))
+            if not fl >= 2.220446049250313e-16:
+                raise ArgError("flatness is too small")
+            a["flatness"] = fl
+        elif arg == "-ra":
+            a["active_edge"] = True
+        elif arg in ("-s", "-ro"):
+            raise ArgError(f"{arg}: stroking is not part of this build")
+        else:
+            positional.append(arg)
+    if len(positional) > 2:
+        raise ArgError("unexpected positional argment")
+    if len(positional) < 2:
+        return None
+    a["input"], a["output"] = positional
+    return a
+
+
+def build_scene(path, a):
+    """The scene examples/rasterize.rs renders (:262-308): ((width, height),
+    scene, background)."""
+    import rasterize_amd as rz
+    from rasterize_amd.color import parse_rgba
+    if a["tr"] is not None:
+        path = path.transform_points(rz.Transform.from_svg(a["tr"]))
+    view_box = a["view_box"] or path.bbox()
+    if view_box is None:
+        raise ArgError("nothing to render")
+    (w, h), tr = rz.Transform.fit_size(view_box, a["width"], a["height"])
+    fg = (rz.LinColor.from_rgba8(*parse_rgba(a["fg"])) if a["fg"] is not None
+          else rz.LinColor(0.0, 0.0, 0.0, 1.0))
+    scene = rz.Scene.group([rz.Scene.fill(path, fg).with_transform(tr)])
+    if a["bg"] is None:
+        board = rz.PathBuilder().checkerboard((0.0, 0.0, float(w), float(h)), 16.0).build()
+        scene = rz.Scene.group([
+            rz.Scene.fill(board, rz.LinColor.from_rgba8(*parse_rgba("#d0d0d0")), rz.FillRule.EvenOdd),
+            scene])
+        bg = rz.LinColor.from_rgba8(*parse_rgba("#f0f0f0"))
+    else:
+        bg = rz.LinColor.from_rgba8(*parse_rgba(a["bg"]))
+    return (w, h), scene, bg
+
+
+def encode(image, fmt):
+    import rasterize_amd as rz
+    if fmt == "bmp":
+        return rz.bmp_bytes(image)
+    if fmt == "png":
+        return rz.png_bytes(image)
+    h, w = image.shape[:2]
+    return struct.pack("<II", h, w) + rz.encode_rgba8(image).tobytes()
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    try:
+        a = parse_args(argv)
+    except (ArgError, ValueError) as e:
+        print(f"Error: {e}", file=sys.stderr)
+        return 1
+    if a is None:
+        print(USAGE, file=sys.stderr)
+        return 1
+    import rasterize_amd as rz
+    data = sys.stdin.buffer.read() if a["input"] == "-" else open(a["input"], "rb").read()
+    try:
+        path = rz.Path.from_svg(data)
+        (w, h), scene, bg = build_scene(path, a)
+    except (ArgError, ValueError) as e:
+        print(f"Error: {e}", file=sys.stderr)
+        return 1
+    rast = (rz.ActiveEdgeRasterizer(a["flatness"]) if a["active_edge"]
+            else rz.SignedDifferenceRasterizer(a["flatness"]))
+    layer = scene.render(rast, rz.Transform.identity(), (0.0, 0.0, float(w), float(h)), bg)
+    out = encode(layer.image, a["format"])
+    if a["output"] == "-":
+        sys.stdout.buffer.write(out)
+    else:
+        with open(a["output"], "wb") as f:
+            f.write(out)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -165,6 +165,7 @@ PROTOTYPES = [
                                 C.c_int, C.c_int, P(C.c_float), C.c_size_t, C.c_size_t]),
     ("rz_set_param", C.c_int, [c_ctx, C.c_int, C.c_int]),
     ("rz_get_param", C.c_int, [c_ctx, C.c_int, P(C.c_int)]),
+    ("rz_path_bbox", C.c_int, [P(rz_path), rz_transform, P(C.c_int), P(C.c_double)]),
     ("rz_tune_mask_device", C.c_int, [c_ctx, c_dpath, rz_transform, C.c_double, C.c_int, C.c_int,
                                       C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_double,
                                       P(rz_tune_result)]),

@@ -2345,6 +2345,20 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   return RZ_OK;
 }
 
+rz_status rz_path_bbox(const rz_path* path, rz_transform tr, int* has_bbox, double* bbox) {
+  if (!path || !has_bbox || !bbox) return RZ_E_ARG;
+  if (path->n_segments && (!path->kinds || !path->point_offsets || !path->points)) return RZ_E_ARG;
+  rzh::BBox b;
+  *has_bbox = rzh::path_bbox(path, rzh::tr_of(tr.m), &b) ? 1 : 0;
+  if (*has_bbox) {
+    bbox[0] = b.mn.x;
+    bbox[1] = b.mn.y;
+    bbox[2] = b.mx.x;
+    bbox[3] = b.mx.y;
+  }
+  return RZ_OK;
+}
+
 rz_status rz_tune_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
                               int fill_rule, int edge_mode, size_t width, size_t height,
                               float* d_out, size_t row_stride, double budget_ms,

@@ -1,0 +1,86 @@
+"""The command line renderer (python -m rasterize_amd, after the reference's
+examples/rasterize.rs): argument handling and the fitted scene on the CPU;
+the rendered file against the oracle's render of the same scene on the GPU."""
+import ctypes as C
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import rasterize_amd as rz
+from rasterize_amd import __main__ as cli
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_arguments():
+    a = cli.parse_args(["-w", "64", "-h", "32", "-v", "1,2 30 40", "-t", "scale(2)", "-f", "png",
+                        "-fg", "#ff0000", "-bg", "aqua", "-rf", "0.1", "-ra", "in.path", "out.png"])
+    assert a["width"] == 64 and a["height"] == 32 and a["format"] == "png"
+    assert a["view_box"] == (1.0, 2.0, 31.0, 42.0)
+    assert a["tr"] == "scale(2)" and a["flatness"] == 0.1 and a["active_edge"]
+    assert (a["input"], a["output"]) == ("in.path", "out.png")
+    assert cli.parse_args(["in.path"]) is None  # usage
+    for bad in (["-w"], ["-f", "gif", "a", "b"], ["-rf", "0", "a", "b"], ["a", "b", "c"],
+                ["-s", "2", "a", "b"], ["-v", "1 2 3", "a", "b"]):
+        with pytest.raises((cli.ArgError, ValueError)):
+            cli.parse_args(bad)
+    assert cli.main(["-s", "1", "a", "b"]) == 1
+
+
+@pytest.mark.parametrize("w,h", [(0, 0), (64, 0), (0, 48), (100, 40)])
+def test_fit_size(w, h):
+    """Transform::fit_size: the whole-pixel box of the source, scaled by one
+    factor and centred, fills the image along one dimension."""
+    src = (3.25, -1.5, 19.75, 20.0)
+    (W, H), tr = rz.Transform.fit_size(src, w, h)
+    box = (3.0, -2.0, 20.0, 20.0)
+    if w == 0 and h == 0:
+        assert (W, H) == (17, 22)
+    if w and h:
+        assert (W, H) == (w, h)
+    if w and not h:
+        assert W == w and H == int(np.ceil(22 * w / 17))
+    if h and not w:
+        assert H == h and W == int(np.ceil(17 * h / 22))
+    (ax, ay), (bx, by) = tr.apply(box[:2]), tr.apply(box[2:])
+    assert ax >= -1e-9 and ay >= -1e-9 and bx <= W + 1e-9 and by <= H + 1e-9
+    assert abs(bx - ax - W) < 1e-9 or abs(by - ay - H) < 1e-9
+    assert abs((ax + bx) / 2 - W / 2) < 1e-9 and abs((ay + by) / 2 - H / 2) < 1e-9
+
+
+def test_checkerboard_cells():
+    """PathBuilder::checkerboard: one 6-line L-shaped cell pair per 2x2 block."""
+    p = rz.PathBuilder().checkerboard((0.0, 0.0, 64.0, 32.0), 16.0).build()
+    assert p.segments_count() == 2 * 5  # two closed cells of 5 lines each
+    assert p.bbox() == (0.0, 0.0, 64.0, 32.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [["-w", "96"], ["-h", "80", "-bg", "#203040", "-fg", "#ffcc00/0.5"],
+                                  ["-w", "64", "-t", "rotate(30) scale(1, 0.5)", "-ra"],
+                                  ["-w", "72", "-v", "2 0 12 14"]])
+def test_rendered_file_matches_oracle(tmp_path, args):
+    src = os.path.join(GOLDEN, "squirrel.path")
+    out = str(tmp_path / "out.rgba")
+    assert cli.main(args + ["-f", "rgba", src, out]) == 0
+    data = open(out, "rb").read()
+    h, w = struct.unpack("<II", data[:8])
+    got = np.frombuffer(data[8:], np.uint8).reshape(h, w, 4)
+    a = cli.parse_args(args + [src, out])
+    path = rz.Path.from_svg(open(src).read())
+    (W, H), scene, bg = cli.build_scene(path, a)
+    assert (W, H) == (w, h)
+    img, org = O.scene_tree_render(scene, rz.Transform.identity(), (0.0, 0.0, float(W), float(H)),
+                                   tuple(bg), flatness=a["flatness"], active_edge=a["active_edge"])
+    assert org == (0, 0) and img.shape[:2] == (h, w)
+    L = O.lib()
+    ref = np.zeros((h * w, 4), np.uint8)
+    for i, c in enumerate(img.reshape(-1, 4)):
+        o = (C.c_uint8 * 4)()
+        L.orc_lin_to_rgba((C.c_float * 4)(*[float(v) for v in c]), o)
+        ref[i] = list(o)
+    d = np.abs(got.reshape(-1, 4).astype(int) - ref.astype(int))
+    assert d.max() <= 1, f"max 8-bit difference {d.max()}"
