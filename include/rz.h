@@ -339,6 +339,25 @@ rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_tran
                                     const float* bg, const float* d_grad_rgba, size_t grad_stride,
                                     float* d_grad_points, float* d_grad_colors, double* d_grad_tr);
 
+/* Gradient paints' stops: the scene's given stops are those of its paints, in
+ * path order, each paint's stops in the order given (rz_paint.stop_colors;
+ * a solid paint has none), rz_scene_stops of them. rz_scene_update_stops_device
+ * replaces their colours (device f32 premultiplied linear RGBA per stop; the
+ * batched form reads them, converted to sRGB where the gradient interpolates
+ * sRGB colours, as GradLinear::new / GradRadial::new do, src/grad.rs:163-194).
+ * rz_render_backward_stops_device is rz_render_backward_device that also adds
+ * dL/d(stop colour) to d_grad_stops (f32 RGBA per given stop), through
+ * GradStops::at's interpolation and the sRGB conversions. No reference
+ * counterpart. */
+rz_status rz_scene_stops(rz_ctx* ctx, const rz_dscene* scene, size_t* n_stops);
+rz_status rz_scene_update_stops_device(rz_ctx* ctx, rz_dscene* scene, const float* d_stop_colors);
+rz_status rz_render_backward_stops_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                          double flatness, int edge_mode, const double* view,
+                                          const float* bg, const float* d_grad_rgba,
+                                          size_t grad_stride, float* d_grad_points,
+                                          float* d_grad_colors, float* d_grad_stops,
+                                          double* d_grad_tr);
+
 /* ---- output encoders ---------------------------------------------------------- */
 
 /* RGBA::from(LinColor) (src/color.rs:164-175) for every pixel of a device

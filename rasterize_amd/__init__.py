@@ -1082,6 +1082,18 @@ class DeviceScene:
         self.ctx.check(lib().rz_scene_points(self.ctx.h, self.h, C.byref(n)))
         return n.value
 
+    def n_stops(self):
+        """Gradient stops of the scene's paints, in path order, each paint's
+        stops as given (solid paints have none)."""
+        n = C.c_size_t()
+        self.ctx.check(lib().rz_scene_stops(self.ctx.h, self.h, C.byref(n)))
+        return n.value
+
+    def update_stops_device(self, d_stops):
+        """Replace the gradient stops' colours (device f32 premultiplied
+        linear RGBA per stop, n_stops of them)."""
+        self.ctx.check(lib().rz_scene_update_stops_device(self.ctx.h, self.h, C.c_void_p(d_stops)))
+
     def update_device(self, d_points=None, d_colors=None):
         """Replace the control points (device f64 x,y per point) and/or solid
         colours (device f32 RGBA per path) of the batched form."""
@@ -1091,13 +1103,15 @@ class DeviceScene:
 
     def backward_device(self, tr, view, d_grad_rgba, grad_stride, d_grad_points, d_grad_colors,
                         d_grad_tr=None, bg=None, flatness=DEFAULT_FLATNESS,
-                        edge_mode=_abi.RZ_EDGE_ITER):
-        """Adjoint of render_device (batched): gradients are added to the buffers."""
+                        edge_mode=_abi.RZ_EDGE_ITER, d_grad_stops=None):
+        """Adjoint of render_device (batched): gradients are added to the
+        buffers (d_grad_stops: f32 RGBA per gradient stop, or None)."""
         v = (C.c_double * 4)(*view)
         b = (C.c_float * 4)(*bg) if bg is not None else None
-        self.ctx.check(lib().rz_render_backward_device(
+        self.ctx.check(lib().rz_render_backward_stops_device(
             self.ctx.h, self.h, tr.abi(), flatness, edge_mode, v, b, C.c_void_p(d_grad_rgba),
             grad_stride, C.c_void_p(d_grad_points), C.c_void_p(d_grad_colors),
+            C.c_void_p(d_grad_stops) if d_grad_stops else None,
             C.c_void_p(d_grad_tr) if d_grad_tr else None))
 
     def close(self):
@@ -1112,13 +1126,15 @@ class DeviceScene:
             pass
 
 
-def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLATNESS):
+def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLATNESS, stops=None):
     """Differentiable Scene::render (batched form) as a torch.autograd.Function:
     `points` (N, 2) float tensor of every path's stored control points in scene
     order, `colors` (n_paths, 4) solid premultiplied linear RGBA, `tr` a
-    Transform or a 6-element tensor [m00, m01, m02, m10, m11, m12]; returns the
-    (H, W, 4) float32 layer of `view`. Gradients flow to points, colors and (a
-    tensor) tr through rz_render_backward_device.
+    Transform or a 6-element tensor [m00, m01, m02, m10, m11, m12], `stops`
+    None or a (n_stops, 4) tensor of the gradient paints' stop colours
+    (premultiplied linear RGBA, DeviceScene.n_stops order); returns the
+    (H, W, 4) float32 layer of `view`. Gradients flow to points, colors, (a
+    tensor) tr and stops through rz_render_backward_stops_device.
 
     The library's context runs on its own HIP stream: before it reads the
     inputs (or writes a buffer torch just recycled) it waits for everything
@@ -1141,15 +1157,18 @@ def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLA
 
     class _Render(torch.autograd.Function):
         @staticmethod
-        def forward(fctx, pts, cols, trt):
+        def forward(fctx, pts, cols, trt, sts):
             x0, y0 = int(np.floor(view[0])), int(np.floor(view[1]))
             w, h = int(np.ceil(view[2])) - x0, int(np.ceil(view[3])) - y0
             T = _tr_of(trt)
             p64 = pts.detach().to(torch.float64).contiguous()
             c32 = cols.detach().to(torch.float32).contiguous()
+            s32 = sts.detach().to(torch.float32).contiguous() if sts is not None else None
             out = torch.empty((h, w, 4), dtype=torch.float32, device=pts.device)
             _wait(pts.device)
             dscene.update_device(p64.data_ptr(), c32.data_ptr())
+            if s32 is not None:
+                dscene.update_stops_device(s32.data_ptr())
             L = lib()
             for _ in range(4):
                 dscene.render_device(T, view, out.data_ptr(), w, bg=bg, batched=True,
@@ -1158,31 +1177,37 @@ def render_torch(dscene, points, colors, tr, view, bg=None, flatness=DEFAULT_FLA
                 if s != _abi.RZ_E_CAPACITY:
                     break
             dscene.ctx.check(s)
-            fctx.keep = (p64, c32)
+            fctx.keep = (p64, c32, s32)
             fctx.T = T
             fctx.shape = (h, w)
-            fctx.dtypes = (pts.dtype, cols.dtype, trt.dtype if tr_is_tensor else None)
+            fctx.dtypes = (pts.dtype, cols.dtype, trt.dtype if tr_is_tensor else None,
+                           sts.dtype if sts is not None else None)
             return out
 
         @staticmethod
         def backward(fctx, grad):
             h, w = fctx.shape
-            p64, c32 = fctx.keep
+            p64, c32, s32 = fctx.keep
             g = grad.detach().to(torch.float32).contiguous()
             gp = torch.zeros((dscene.n_points(), 2), dtype=torch.float32, device=g.device)
             gc = torch.zeros((dscene.n, 4), dtype=torch.float32, device=g.device)
             gt = torch.zeros(6, dtype=torch.float64, device=g.device) if tr_is_tensor else None
+            gs = torch.zeros_like(s32) if s32 is not None else None
             _wait(g.device)
             # the state the forward rendered (the scene may have been updated since)
             dscene.update_device(p64.data_ptr(), c32.data_ptr())
+            if s32 is not None:
+                dscene.update_stops_device(s32.data_ptr())
             dscene.backward_device(fctx.T, view, g.data_ptr(), w, gp.data_ptr(), gc.data_ptr(),
                                    gt.data_ptr() if gt is not None else None, bg=bg,
-                                   flatness=flatness)
+                                   flatness=flatness,
+                                   d_grad_stops=gs.data_ptr() if gs is not None else None)
             dscene.ctx.check(lib().rz_sync(dscene.ctx.h))
             gtr = gt.to(fctx.dtypes[2]) if gt is not None else None
-            return gp.to(fctx.dtypes[0]), gc.to(fctx.dtypes[1]), gtr
+            gst = gs.to(fctx.dtypes[3]) if gs is not None else None
+            return gp.to(fctx.dtypes[0]), gc.to(fctx.dtypes[1]), gtr, gst
 
-    return _Render.apply(points, colors, tr if tr_is_tensor else None)
+    return _Render.apply(points, colors, tr if tr_is_tensor else None, stops)
 
 
 class DiffScene:

@@ -151,6 +151,12 @@ PROTOTYPES = [
     ("rz_render_backward_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double, C.c_int,
                                             P(C.c_double), P(C.c_float), C.c_void_p, C.c_size_t,
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rz_scene_stops", C.c_int, [c_ctx, C.c_void_p, P(C.c_size_t)]),
+    ("rz_scene_update_stops_device", C.c_int, [c_ctx, C.c_void_p, C.c_void_p]),
+    ("rz_render_backward_stops_device", C.c_int, [c_ctx, C.c_void_p, rz_transform, C.c_double,
+                                                  C.c_int, P(C.c_double), P(C.c_float), C.c_void_p,
+                                                  C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p]),
     ("rz_encode_rgba8_device", C.c_int, [c_ctx, C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t,
                                          C.c_void_p, C.c_size_t, C.c_int]),
     ("rz_encode_rgba8", C.c_int, [c_ctx, P(C.c_float), C.c_size_t, C.c_size_t, C.c_size_t,

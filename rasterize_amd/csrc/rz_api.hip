@@ -101,7 +101,7 @@ struct GraphKey {
   double tr[6], flatness, view[4];
   float bg[4];
   size_t w, h, stride, stride2;
-  const void* ptr[4];       // device outputs / inputs
+  const void* ptr[5];       // device outputs / inputs
   LaunchParams lp;          // the launch choices the frame was enqueued with
 };
 struct GraphEntry {
@@ -134,6 +134,7 @@ struct rz_ctx {
   // batched scene render
   DBuf line_path;  // path of each line
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
+  DBuf bw_stops;           // scene backward: dL/d(stop as read) per stop of the scene
   DBuf sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   DBuf list_buf;  // binning: record list (records, tiles) and event list
@@ -1026,6 +1027,14 @@ struct rz_dscene {
   const uint32_t* subpaths = nullptr;  // combined path: subpath -> first segment
   uint32_t n_points = 0;               // points of the combined path
   std::vector<rzk::ScenePath> host_sp;  // same table on the host (plain form)
+  // gradient stops: as the paints read them (sorted, in the space they are
+  // interpolated in), their colours as given, and their index among the
+  // scene's given stops (paths in order, each paint's stops as given)
+  uint32_t n_stops = 0, n_user_stops = 0;
+  rzp::C4* stop_col_dev = nullptr;
+  rzp::C4* stop_lin_dev = nullptr;
+  int32_t* stop_user_dev = nullptr;   // -1: a gradient's default stop
+  uint8_t* stop_flag_dev = nullptr;   // 1: its gradient interpolates linear colours
   // glyph atlas placement (rz_atlas_set): device slots and slice totals
   rzk::AtlasSlot* atlas_slots = nullptr;
   uint32_t* atlas_tile_glyph = nullptr;  // glyph of each atlas tile
@@ -1107,7 +1116,10 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
   std::vector<double> pts;
   std::vector<uint8_t> closed;
   std::vector<double> stop_pos;
-  std::vector<rzp::C4> stop_col;
+  std::vector<rzp::C4> stop_col, stop_lin;
+  std::vector<int32_t> stop_user;
+  std::vector<uint8_t> stop_flag;
+  uint32_t user_stops = 0;
   std::vector<size_t> stop_first(n);
   s->host_sp.resize(n);
   for (size_t i = 0; i < n; ++i) {
@@ -1154,6 +1166,12 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
     stop_first[i] = stop_pos.size();
     stop_pos.insert(stop_pos.end(), pp.pos.begin(), pp.pos.end());
     stop_col.insert(stop_col.end(), pp.col.begin(), pp.col.end());
+    stop_lin.insert(stop_lin.end(), pp.lin.begin(), pp.lin.end());
+    for (int32_t k : pp.src) {
+      stop_user.push_back(k < 0 ? -1 : (int32_t)user_stops + k);
+      stop_flag.push_back(pp.p.linear_colors ? 1 : 0);
+    }
+    if (paint_i.kind != RZ_PAINT_SOLID) user_stops += paint_i.n_stops;
     rz_dpath* d = nullptr;
     RZ_TRY(path_upload_impl(ctx, p, &d));
     s->per.push_back(d);
@@ -1181,6 +1199,16 @@ rz_status scene_upload_impl(rz_ctx* ctx, const rz_path* const* paths, const rz_p
   void *dpos = nullptr, *dcol = nullptr, *dsp = nullptr, *dslot = nullptr, *dsubs = nullptr;
   RZ_TRY(up(stop_pos.data(), stop_pos.size() * sizeof(double), &dpos));
   RZ_TRY(up(stop_col.data(), stop_col.size() * sizeof(rzp::C4), &dcol));
+  void *dlin = nullptr, *duser = nullptr, *dflag = nullptr;
+  RZ_TRY(up(stop_lin.data(), stop_lin.size() * sizeof(rzp::C4), &dlin));
+  RZ_TRY(up(stop_user.data(), stop_user.size() * sizeof(int32_t), &duser));
+  RZ_TRY(up(stop_flag.data(), stop_flag.size(), &dflag));
+  s->n_stops = (uint32_t)stop_col.size();
+  s->n_user_stops = user_stops;
+  s->stop_col_dev = (rzp::C4*)dcol;
+  s->stop_lin_dev = (rzp::C4*)dlin;
+  s->stop_user_dev = (int32_t*)duser;
+  s->stop_flag_dev = (uint8_t*)dflag;
   for (size_t i = 0; i < n; ++i) {
     rzk::ScenePath& S = s->host_sp[i];
     S.paint.stop_pos = (const double*)dpos + stop_first[i];
@@ -1509,6 +1537,35 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   return check_launch(ctx, "atlas pass 1");
 }
 
+// New colours for a scene's given gradient stops (f32 RGBA each): the stops
+// the paints read, converted to sRGB unless their gradient interpolates
+// linear colours (GradLinear::new / GradRadial::new, src/grad.rs:163-194).
+__global__ void k_set_stops(const float* colors, const int32_t* user, const uint8_t* flag,
+                            rzp::C4* lin, rzp::C4* col, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || user[i] < 0) return;
+  const float* c = colors + 4 * (size_t)user[i];
+  const rzp::C4 v = rzp::c4(c[0], c[1], c[2], c[3]);
+  lin[i] = v;
+  col[i] = flag[i] ? v : rzp::into_srgb(v);
+}
+
+// dL/d(given stop colour) += J^T dL/d(stop as read), through into_srgb where
+// the gradient interpolates sRGB colours.
+__global__ void k_stop_grads(const float* acc, const int32_t* user, const uint8_t* flag,
+                             const rzp::C4* lin, float* grad, uint32_t n, const rzk::Status* st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st->err || st->overflow) return;  // the run was not completed: nothing to add
+  if (i >= n || user[i] < 0) return;
+  rzp::C4 g = rzp::c4(acc[4 * i], acc[4 * i + 1], acc[4 * i + 2], acc[4 * i + 3]);
+  if (!flag[i]) g = rzp::into_srgb_vjp(lin[i], g);
+  float* o = grad + 4 * (size_t)user[i];
+  o[0] += g.r;
+  o[1] += g.g;
+  o[2] += g.b;
+  o[3] += g.a;
+}
+
 // Adjoint of the batched render (see rz_backward.hpp): the forward pipeline
 // again (its tile lists and records are recomputed, not kept), then the tile
 // pass (compositing adjoint) and the line pass (coverage and subdivision
@@ -1516,7 +1573,7 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
 rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_transform& tr,
                                  const double* view, rzp::C4 bg, int mode, double flatness,
                                  const float* grad_out, size_t grad_stride, float* grad_points,
-                                 float* grad_colors, double* grad_tr) {
+                                 float* grad_colors, float* grad_stops, double* grad_tr) {
   int lx, ly, lw, lh;
   layer_of(view, &lx, &ly, &lw, &lh);
   if (lw == 0 || lh == 0 || s->n == 0) return RZ_OK;
@@ -1558,8 +1615,22 @@ rz_status enqueue_scene_backward(rz_ctx* ctx, const rz_dscene* s, const rz_trans
   bt.wbuf = (float*)ctx->bw_w.p;
   bt.pbuf = (rzp::C4*)ctx->bw_p.p;
   bt.grad_colors = grad_colors;
+  bt.stop_acc = nullptr;
+  bt.stop_base = s->stop_col_dev;
+  const bool stops = grad_stops && s->n_user_stops;
+  if (stops) {
+    RZ_TRY(ensure(ctx, ctx->bw_stops, (size_t)s->n_stops * 4 * sizeof(float)));
+    bt.stop_acc = (float*)ctx->bw_stops.p;
+    RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3(blocks_for((size_t)s->n_stops * 4)),
+              dim3(rzk::kBlock), (uint32_t*)ctx->bw_stops.p, (size_t)s->n_stops * 4, 0u);
+  }
   const uint32_t ntiles = (uint32_t)(((lw + 15) / 16) * ((lh + 15) / 16));
   RZ_LAUNCH(ctx, "k_scene_bwd_tiles", rzk::k_scene_bwd_tiles, dim3(ntiles), dim3(64), bt);
+  if (stops)
+    RZ_LAUNCH(ctx, "k_stop_grads", k_stop_grads, dim3((s->n_stops + 255) / 256), dim3(256),
+              (const float*)ctx->bw_stops.p, (const int32_t*)s->stop_user_dev,
+              (const uint8_t*)s->stop_flag_dev, (const rzp::C4*)s->stop_lin_dev, grad_stops,
+              s->n_stops, (const rzk::Status*)st);
   // line pass over the same flatten items as the forward
   const rz_dpath* d = s->all;
   int dq, dc;
@@ -2542,7 +2613,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
   DBuf* bufs[] = {&ctx->meta,      &ctx->tile_off,  &ctx->ev_off,    &ctx->sums,
                   &ctx->recs,      &ctx->events,    &ctx->out,       &ctx->lines,
                   &ctx->counts,    &ctx->stops,     &ctx->line_path,
-                  &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,
+                  &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,     &ctx->bw_stops,
                   &ctx->sc_pi,     &ctx->sc_counts,
                   &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
                   &ctx->wave_lines, &ctx->atlas_tiles};
@@ -3143,6 +3214,21 @@ rz_status rz_scene_update_device(rz_ctx* ctx, rz_dscene* scene, const double* d_
   return check_launch(ctx, "scene update");
 }
 
+rz_status rz_scene_stops(rz_ctx* ctx, const rz_dscene* scene, size_t* n_stops) {
+  if (!ctx || !scene || !n_stops) return RZ_E_ARG;
+  *n_stops = scene->n_user_stops;
+  return RZ_OK;
+}
+
+rz_status rz_scene_update_stops_device(rz_ctx* ctx, rz_dscene* scene, const float* d_stop_colors) {
+  if (!ctx || !scene) return RZ_E_ARG;
+  if (!d_stop_colors || !scene->n_user_stops) return RZ_OK;
+  RZ_LAUNCH(ctx, "k_set_stops", k_set_stops, dim3((scene->n_stops + 255) / 256), dim3(256),
+            d_stop_colors, (const int32_t*)scene->stop_user_dev, (const uint8_t*)scene->stop_flag_dev,
+            scene->stop_lin_dev, scene->stop_col_dev, scene->n_stops);
+  return check_launch(ctx, "stop update");
+}
+
 rz_status rz_encode_rgba8_device(rz_ctx* ctx, const float* d_rgba, size_t width, size_t height,
                                  size_t row_stride, uint8_t* d_out, size_t out_stride,
                                  int bottom_up) {
@@ -3206,10 +3292,12 @@ rz_status rz_scene_points(rz_ctx* ctx, const rz_dscene* scene, size_t* n_points)
   return RZ_OK;
 }
 
-rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
-                                    double flatness, int edge_mode, const double* view,
-                                    const float* bg, const float* d_grad_rgba, size_t grad_stride,
-                                    float* d_grad_points, float* d_grad_colors, double* d_grad_tr) {
+rz_status rz_render_backward_stops_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                          double flatness, int edge_mode, const double* view,
+                                          const float* bg, const float* d_grad_rgba,
+                                          size_t grad_stride, float* d_grad_points,
+                                          float* d_grad_colors, float* d_grad_stops,
+                                          double* d_grad_tr) {
   RZ_TRY(check_scene_args(ctx, flatness, edge_mode, view));
   if (!scene || !d_grad_rgba || !d_grad_points || !d_grad_colors)
     return fail(ctx, RZ_E_ARG, "null argument");
@@ -3227,10 +3315,21 @@ rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_tran
   k.ptr[1] = d_grad_points;
   k.ptr[2] = d_grad_colors;
   k.ptr[3] = d_grad_tr;
+  k.ptr[4] = d_grad_stops;
   return run_graphed(ctx, k, [&]() {
     return enqueue_scene_backward(ctx, scene, tr, view, c, edge_mode, flatness, d_grad_rgba,
-                                  grad_stride, d_grad_points, d_grad_colors, d_grad_tr);
+                                  grad_stride, d_grad_points, d_grad_colors, d_grad_stops,
+                                  d_grad_tr);
   });
+}
+
+rz_status rz_render_backward_device(rz_ctx* ctx, const rz_dscene* scene, rz_transform tr,
+                                    double flatness, int edge_mode, const double* view,
+                                    const float* bg, const float* d_grad_rgba, size_t grad_stride,
+                                    float* d_grad_points, float* d_grad_colors, double* d_grad_tr) {
+  return rz_render_backward_stops_device(ctx, scene, tr, flatness, edge_mode, view, bg, d_grad_rgba,
+                                         grad_stride, d_grad_points, d_grad_colors, nullptr,
+                                         d_grad_tr);
 }
 
 }  // extern "C"

@@ -1,5 +1,6 @@
 // Adjoint of the batched scene render: dL/d(layer RGBA) -> dL/d(control
-// points, solid paint colours, transform). No counterpart in the reference.
+// points, solid paint colours, gradient stop colours, transform). No
+// counterpart in the reference.
 //
 // Forward per pixel, paths k in order (Scene::render, src/scene.rs:410-430,
 // Rasterizer::fill, src/rasterize.rs:102-115):
@@ -32,6 +33,8 @@ struct BwdTileArgs {
   float* wbuf;                  // [entries * 256]: winding W, then dL/dW
   rzp::C4* pbuf;                // [entries * 256]: dst below the path
   float* grad_colors;           // [n_paths * 4]
+  float* stop_acc;              // [stops * 4] dL/d(stop as the paint reads it), or null
+  const rzp::C4* stop_base;     // the scene's stop table (a paint's stops are a range of it)
 };
 
 __device__ inline float alpha_of(int rule, float W) {
@@ -224,6 +227,26 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
             const float da = cp.r * ds.r + cp.g * ds.g + cp.b * ds.b + cp.a * ds.a;
             dc = rzp::add(dc, rzp::mul(ds, alpha));
             gW = da * dalpha_of(P.rule, W);
+            int i0, i1;
+            float w1;
+            rzp::C4 mixed;
+            if (b.stop_acc && rzp::gradient_stops_at(P.paint, xs, ys, &i0, &i1, &w1, &mixed)) {
+              // through into_linear and the stops' lerp (GradStops::at)
+              rzp::C4 gm = rzp::mul(ds, alpha);
+              if (!P.paint.linear_colors) gm = rzp::into_linear_vjp(mixed, gm);
+              float* o = b.stop_acc + 4 * (size_t)(P.paint.stop_col - b.stop_base);
+              const float w0 = 1.0f - w1;
+              atomicAdd(o + 4 * i0 + 0, gm.r * w0);
+              atomicAdd(o + 4 * i0 + 1, gm.g * w0);
+              atomicAdd(o + 4 * i0 + 2, gm.b * w0);
+              atomicAdd(o + 4 * i0 + 3, gm.a * w0);
+              if (w1 != 0.0f) {
+                atomicAdd(o + 4 * i1 + 0, gm.r * w1);
+                atomicAdd(o + 4 * i1 + 1, gm.g * w1);
+                atomicAdd(o + 4 * i1 + 2, gm.b * w1);
+                atomicAdd(o + 4 * i1 + 3, gm.a * w1);
+              }
+            }
             T[k] *= 1.0f - cp.a * alpha;
           }
         }

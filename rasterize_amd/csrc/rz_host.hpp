@@ -183,6 +183,8 @@ struct PaintPrep {
   rzp::DPaint p;
   std::vector<double> pos;
   std::vector<rzp::C4> col;
+  std::vector<rzp::C4> lin;   // the stops' colours as given (before the sRGB conversion)
+  std::vector<int32_t> src;   // index of each stop in the paint as given; -1: the default stop
   bool draw = true;
 };
 
@@ -197,6 +199,8 @@ inline bool prepare_paint_static(const rz_paint& in, PaintPrep* out) {
   out->draw = true;
   out->pos.clear();
   out->col.clear();
+  out->lin.clear();
+  out->src.clear();
   if (in.kind == RZ_PAINT_SOLID) {
     p.n_stops = 0;
     return true;
@@ -209,19 +213,23 @@ inline bool prepare_paint_static(const rz_paint& in, PaintPrep* out) {
   p.linear_colors = in.linear_colors != 0;
   // stops: stable sort by position, a default opaque black stop if empty,
   // converted to sRGB unless the gradient interpolates linear colours
-  std::vector<std::pair<double, rzp::C4>> st;
+  struct Stop {
+    double pos;
+    rzp::C4 col;
+    int32_t src;
+  };
+  std::vector<Stop> st;
   for (uint32_t i = 0; i < in.n_stops; ++i) {
     const float* c = in.stop_colors + 4 * i;
-    st.push_back({in.stop_positions[i], rzp::c4(c[0], c[1], c[2], c[3])});
+    st.push_back({in.stop_positions[i], rzp::c4(c[0], c[1], c[2], c[3]), (int32_t)i});
   }
-  std::stable_sort(st.begin(), st.end(),
-                   [](const std::pair<double, rzp::C4>& a, const std::pair<double, rzp::C4>& b) {
-                     return a.first < b.first;
-                   });
-  if (st.empty()) st.push_back({0.0, rzp::c4(0.0f, 0.0f, 0.0f, 1.0f)});
+  std::stable_sort(st.begin(), st.end(), [](const Stop& a, const Stop& b) { return a.pos < b.pos; });
+  if (st.empty()) st.push_back({0.0, rzp::c4(0.0f, 0.0f, 0.0f, 1.0f), -1});
   for (auto& s : st) {
-    out->pos.push_back(s.first);
-    out->col.push_back(p.linear_colors ? s.second : rzp::into_srgb(s.second));
+    out->pos.push_back(s.pos);
+    out->col.push_back(p.linear_colors ? s.col : rzp::into_srgb(s.col));
+    out->lin.push_back(s.col);
+    out->src.push_back(s.src);
   }
   p.n_stops = (int)st.size();
   if (in.kind == RZ_PAINT_LINEAR) {

@@ -68,6 +68,43 @@ RZ_HD C4 into_linear(C4 c) {
   const C4 u = zero ? splat(0.0f) : c4(c.r / inv, c.g / inv, c.b / inv, 0.0f);
   return mul(c4(s2l1(u.r), s2l1(u.g), s2l1(u.b), la), splat(c.a));
 }
+// Derivatives of the two polynomials (where they are differentiable; the
+// side of each threshold the evaluation takes).
+RZ_HD float l2s1_d(float x0) {
+  if (x0 <= 0.0031308f) return 12.92f;
+  const float x1 = sqrtf(x0), x2 = sqrtf(x1), x3 = sqrtf(x2);
+  return -0.01848558f + 0.6445592f * 0.5f / x1 + 0.70994765f * 0.25f / (x1 * x2) -
+         0.33605254f * 0.125f / (x1 * x2 * x3);
+}
+RZ_HD float s2l1_d(float vs) {
+  if (vs <= 0.04045f) return 0.07739938080495357f;
+  const float x1 = 2.0843103538116825f * vs - 1.0843103538116827f;
+  return 2.0843103538116825f *
+         (0.4665843122387033f + 2.0f * 0.26901741378006355f * x1 + 3.0f * 0.031661580753065945f * x1 * x1);
+}
+// g^T d(f(c / a) * a)/dc for f = s2l1 / l2s1 on r, g, b and f(1) * a on
+// alpha (into_linear / into_srgb above a > 1e-6; nothing flows below it).
+template <typename F, typename D>
+RZ_HD C4 unmultiplied_vjp(C4 c, C4 g, F&& f, D&& df) {
+  if (!(c.a > 1e-6f)) return splat(0.0f);
+  const float u[3] = {c.r / c.a, c.g / c.a, c.b / c.a};
+  const float gc[3] = {g.r, g.g, g.b};
+  float out[3];
+  float ga = g.a * f(1.0f);
+  for (int k = 0; k < 3; ++k) {
+    const float d = df(u[k]);
+    out[k] = gc[k] * d;
+    ga += gc[k] * (f(u[k]) - d * u[k]);
+  }
+  return c4(out[0], out[1], out[2], ga);
+}
+RZ_HD C4 into_linear_vjp(C4 c, C4 g) {
+  return unmultiplied_vjp(c, g, [](float v) { return s2l1(v); }, [](float v) { return s2l1_d(v); });
+}
+RZ_HD C4 into_srgb_vjp(C4 c, C4 g) {
+  return unmultiplied_vjp(c, g, [](float v) { return l2s1(v); }, [](float v) { return l2s1_d(v); });
+}
+
 // Color::lerp for LinColor: other * t + self * (1 - t) (src/color.rs:352-354)
 RZ_HD C4 lerp(C4 self, C4 other, float t) { return add(mul(other, t), mul(self, 1.0f - t)); }
 // Color::blend_over for LinColor: other + self * (1 - other.alpha)
@@ -162,6 +199,37 @@ RZ_HD bool radial_offset(const DPaint& p, Pt q, double* out) {
   const int n = quadratic_solve(a, b, c, &r0, &r1);
   if (n == 0) return false;
   *out = n == 2 ? fmax(r0, r1) : r0;
+  return true;
+}
+
+// The stops a gradient's colour at pixel (x, y) interpolates: indices i0,
+// i1 and weights (1 - ratio, ratio) as GradStops::at takes them (one stop
+// with weight 1 outside the first and last), and the interpolated colour
+// before into_linear. False where the paint is transparent (a radial
+// gradient without a root) or solid.
+RZ_HD bool gradient_stops_at(const DPaint& p, int x, int y, int* i0, int* i1, float* w1, C4* mixed) {
+  if (p.kind == PAINT_SOLID) return false;
+  const Pt q = rzg::tr_apply(p.ptr, rzg::mkpt((double)x + 0.5, (double)y + 0.5));
+  double t;
+  if (p.kind == PAINT_LINEAR) {
+    t = (q.x - p.start.x) * p.dir.x + (q.y - p.start.y) * p.dir.y;
+  } else if (!radial_offset(p, q, &t)) {
+    return false;
+  }
+  t = spread_at(p.spread, t);
+  int index = 0;
+  while (index < p.n_stops && p.stop_pos[index] < t) ++index;
+  if (index == 0 || index == p.n_stops) {
+    *i0 = *i1 = index == 0 ? 0 : p.n_stops - 1;
+    *w1 = 0.0f;
+    *mixed = p.stop_col[*i0];
+    return true;
+  }
+  const double p0 = p.stop_pos[index - 1], p1 = p.stop_pos[index];
+  *i0 = index - 1;
+  *i1 = index;
+  *w1 = (float)((t - p0) / (p1 - p0));
+  *mixed = lerp(p.stop_col[index - 1], p.stop_col[index], *w1);
   return true;
 }
 

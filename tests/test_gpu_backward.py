@@ -274,3 +274,73 @@ def test_backward_capacity_miss_leaves_transform_gradient():
     assert torch.allclose(once, gtr, rtol=1e-6, atol=1e-9 * float(gtr.abs().max()))
     dev.close()
     ctx.close()
+
+
+def _gradient_scene(stops):
+    """Three gradient fills (linear with linear colours and unsorted stops,
+    radial and reflected linear interpolating sRGB colours) under a solid
+    one; `stops` (7, 4) premultiplied linear RGBA in the scene's stop order."""
+    from rasterize_amd import (GradLinear, GradRadial, GradSpread, LinColor, PathBuilder, Scene,
+                               Units)
+    from _paths import rect
+    s = [tuple(float(v) for v in c) for c in stops]
+    p1 = rect(8, 8, 70, 60)
+    g1 = GradLinear([(0.8, s[0]), (0.1, s[1]), (0.5, s[2])], units=Units.BoundingBox,
+                    linear_colors=True, start=(0.0, 0.0), end=(1.0, 0.3))
+    p2 = PathBuilder().move_to((50, 50)).circle(30).build()
+    g2 = GradRadial([(0.0, s[3]), (1.0, s[4])], center=(50, 45), radius=35, fcenter=(45, 40))
+    p3 = rect(30, 20, 90, 80)
+    g3 = GradLinear([(0.2, s[5]), (0.9, s[6])], spread=GradSpread.Reflect, start=(30, 20),
+                    end=(50, 40))
+    p4 = rect(20, 50, 60, 90)
+    return Scene.group([Scene.fill(p1, g1), Scene.fill(p2, g2), Scene.fill(p3, g3),
+                        Scene.fill(p4, LinColor(0.2, 0.3, 0.1, 0.5))])
+
+
+def test_gradient_stop_colours_match_finite_differences():
+    """dL/d(stop colour) through GradStops::at and the sRGB conversions, for
+    stops given out of order, against central differences of the oracle's
+    render (loss linear in the layer, so the differences see only the paint)."""
+    import torch
+    W = H = 96
+    rng = np.random.default_rng(8)
+    a = rng.uniform(0.35, 0.95, 7)
+    stops = np.concatenate([rng.uniform(0.1, 0.9, (7, 3)) * a[:, None], a[:, None]], 1)
+    stops = stops.astype(np.float32)
+    weights = rng.uniform(-1.0, 1.0, (H, W, 4)).astype(np.float32)
+    sc = _gradient_scene(stops)
+    dev = DeviceScene(sc)
+    assert dev.n_stops() == 7
+    view = (0.0, 0.0, float(W), float(H))
+    fl = sc.fills()
+    pts = torch.tensor(np.concatenate([f[0].points for f in fl]).reshape(-1, 2), dtype=torch.float64,
+                       device="cuda")
+    cols = torch.tensor([f[1] if isinstance(f[1], tuple) else (0.0, 0.0, 0.0, 0.0) for f in fl],
+                        dtype=torch.float32, device="cuda")
+    S = torch.tensor(stops, device="cuda", requires_grad=True)
+    out = render_torch(dev, pts, cols, Transform.identity(), view, stops=S)
+    ref, _ = O.scene_render([(f[0], f[1].abi(), f[2]) for f in fl], Transform.identity(), view)
+    assert np.abs(out.detach().cpu().numpy() - ref).max() <= 2e-3
+    loss = (out * torch.tensor(weights, device="cuda")).sum()
+    loss.backward()
+    g = S.grad.cpu().numpy()
+
+    def loss_ref(st):
+        f2 = _gradient_scene(st).fills()
+        img, _ = O.scene_render([(f[0], f[1].abi(), f[2]) for f in f2], Transform.identity(), view)
+        return float((img.astype(np.float64) * weights).sum())
+
+    errs = []
+    for i in range(7):
+        for c in range(4):
+            h = 1e-2 if i < 3 else 2e-3  # the first paint interpolates linear colours: exactly linear
+            sp, sm = stops.copy(), stops.copy()
+            sp[i, c] += h
+            sm[i, c] -= h
+            fd = (loss_ref(sp) - loss_ref(sm)) / (2 * h)
+            errs.append((fd, float(g[i, c]), f"stop {i} channel {c}"))
+    scale = max(abs(e[0]) for e in errs)
+    bad = [(fd, ad, w) for fd, ad, w in errs if abs(fd - ad) > 2e-3 * max(abs(fd), 0.05 * scale)]
+    _say("stop gradients: " + ", ".join(f"{w}: fd {fd:.4g} adj {ad:.4g}" for fd, ad, w in errs[:8]))
+    assert not bad, bad
+    assert np.count_nonzero(g) >= 20
