@@ -72,6 +72,7 @@ struct LaunchParams {
   int32_t bin_lds = 0;        // bytes
   int32_t fine_lds = 0;       // bytes
   int32_t scatter_log2 = 15;  // the scatter grid holds at most 2^v waves
+  int32_t scene_waves = 6;    // k_scene_fine built for 1 (no bound), 6 or 8 waves per SIMD
 };
 bool same_params(const LaunchParams& a, const LaunchParams& b) {
   return std::memcmp(&a, &b, sizeof(a)) == 0;
@@ -1389,12 +1390,19 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.lh = lh;
   a.tiles_x = ctx_x;
   // no layer (the adjoint's own tile pass recomposites): the binned state only
-  if (d_out)
-    RZ_LAUNCH_LDS(ctx, "k_scene_fine", rzk::k_scene_fine, dim3((uint32_t)(ctx_x * cty)), dim3(64),
-                  ctx->cur.fine_lds, a);
-  else
+  if (d_out) {
+    const dim3 fgrid((uint32_t)(ctx_x * cty));
+    const int waves = ctx->cur.scene_waves;
+    if (waves == 8)
+      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<8>), fgrid, dim3(64), ctx->cur.fine_lds, a);
+    else if (waves == 1)
+      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<1>), fgrid, dim3(64), ctx->cur.fine_lds, a);
+    else
+      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<6>), fgrid, dim3(64), ctx->cur.fine_lds, a);
+  } else {
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), (const rzk::Status*)st,
               ctx->acc_run);
+  }
   return check_launch(ctx, "scene pass 1");
 }
 
@@ -2286,6 +2294,13 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
     p.scatter_log2 = v;
     st = consider(p);
   }
+  if (key.kind == 1)
+    for (int32_t v : {6, 8, 1}) {
+      if (st != RZ_OK) break;
+      LaunchParams p = best;
+      p.scene_waves = v;
+      st = consider(p);
+    }
   if (st == RZ_OK) st = sweep_depths();
   ctx->tune_forced = false;
   ctx->use_graphs = graphs;
@@ -2308,6 +2323,7 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
     res->bin_lds = best.bin_lds;
     res->fine_lds = best.fine_lds;
     res->scatter_log2 = best.scatter_log2;
+    res->scene_waves = best.scene_waves;
     res->trials = (uint32_t)seen.size();
     res->ms_default = ms_default;
     res->ms_tuned = best_ms;
@@ -2317,15 +2333,15 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
 
 // The tuning cache file: lines "<gpu> <kind> <mode> <px> <slots> <quad>
 // <cubic> <split_auto> <dq> <dc> <flat_lds> <bin_lds> <fine_lds> <scatter>
-// <ms_default> <ms_tuned>"; '#' starts a comment. Entries of other GPUs are
+// <scene_waves> <ms_default> <ms_tuned>"; '#' starts a comment. Entries of other GPUs are
 // kept by a save and skipped by a load.
 std::string tune_line(const std::string& gpu, const TuneEntry& t) {
   std::ostringstream o;
   o << gpu << ' ' << t.key.kind << ' ' << t.key.mode << ' ' << t.key.px_log2 << ' '
     << t.key.slots_log2 << ' ' << t.key.quad_log2 << ' ' << t.key.cubic_log2 << ' '
     << t.p.split_auto << ' ' << t.p.dq << ' ' << t.p.dc << ' ' << t.p.flat_lds << ' '
-    << t.p.bin_lds << ' ' << t.p.fine_lds << ' ' << t.p.scatter_log2 << ' ' << t.ms_default << ' '
-    << t.ms_tuned;
+    << t.p.bin_lds << ' ' << t.p.fine_lds << ' ' << t.p.scatter_log2 << ' ' << t.p.scene_waves << ' '
+    << t.ms_default << ' ' << t.ms_tuned;
   return o.str();
 }
 
@@ -2335,12 +2351,13 @@ bool parse_tune_line(const std::string& line, std::string* gpu, TuneEntry* t) {
   LaunchParams p;
   if (!(in >> *gpu >> t->key.kind >> t->key.mode >> t->key.px_log2 >> t->key.slots_log2 >>
         t->key.quad_log2 >> t->key.cubic_log2 >> p.split_auto >> p.dq >> p.dc >> p.flat_lds >>
-        p.bin_lds >> p.fine_lds >> p.scatter_log2 >> t->ms_default >> t->ms_tuned))
+        p.bin_lds >> p.fine_lds >> p.scatter_log2 >> p.scene_waves >> t->ms_default >> t->ms_tuned))
     return false;
   // only choices that rz_set_param could make
   if (p.split_auto < 0 || p.split_auto > 1 || p.dq < 0 || p.dq > 8 || p.dc < 0 || p.dc > 8 ||
       p.flat_lds < -1 || p.flat_lds > 32768 || p.bin_lds < 0 || p.bin_lds > 32768 ||
-      p.fine_lds < 0 || p.fine_lds > 32768 || p.scatter_log2 < 8 || p.scatter_log2 > 18)
+      p.fine_lds < 0 || p.fine_lds > 32768 || p.scatter_log2 < 8 || p.scatter_log2 > 18 ||
+      (p.scene_waves != 1 && p.scene_waves != 6 && p.scene_waves != 8))
     return false;
   t->p = p;
   return true;
@@ -2389,6 +2406,10 @@ rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   } else if (param == RZ_PARAM_SCATTER_WAVES) {
     RZ_TRY(range(8, 18));
     lp.scatter_log2 = value;
+  } else if (param == RZ_PARAM_SCENE_WAVES) {
+    if (value != 1 && value != 6 && value != 8)
+      return fail(ctx, RZ_E_ARG, "parameter must be 1, 6 or 8");
+    lp.scene_waves = value;
   } else if (param == RZ_PARAM_TUNED) {
     RZ_TRY(range(0, 1));
     ctx->use_tuned = value != 0;
@@ -2408,6 +2429,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   else if (param == RZ_PARAM_BIN_LDS) *value = lp.bin_lds / 1024;
   else if (param == RZ_PARAM_FINE_LDS) *value = lp.fine_lds / 1024;
   else if (param == RZ_PARAM_SCATTER_WAVES) *value = lp.scatter_log2;
+  else if (param == RZ_PARAM_SCENE_WAVES) *value = lp.scene_waves;
   else if (param == RZ_PARAM_TUNED) *value = ctx->use_tuned ? 1 : 0;
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
@@ -2499,7 +2521,7 @@ rz_status rz_tune_save(rz_ctx* ctx, const char* path) {
     std::ofstream out(tmp, std::ios::trunc);
     if (!out) return fail(ctx, RZ_E_ARG, std::string("cannot write ") + tmp);
     out << "# rasterize_amd launch tuning: gpu kind mode px slots quad cubic split_auto dq dc "
-           "flat_lds bin_lds fine_lds scatter ms_default ms_tuned\n";
+           "flat_lds bin_lds fine_lds scatter scene_waves ms_default ms_tuned\n";
     for (const std::string& l : keep) out << l << '\n';
     out.precision(6);
     for (const TuneEntry& t : ctx->tuned) out << tune_line(ctx->gpu_id, t) << '\n';

@@ -324,7 +324,10 @@ struct SceneFineArgs {
 // no records or events in the tile, a zero backdrop and no pixel at or right
 // of its right-edge column has zero winding on the whole tile (alpha 0
 // everywhere) and is skipped.
-__global__ __launch_bounds__(64) void k_scene_fine(SceneFineArgs a) {
+// MINW: waves per SIMD the registers must allow (1: no bound; 6: 80 VGPRs,
+// no spills; 8: 64 VGPRs with a few spilled)
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
   if (a.st->err || a.st->overflow) return;
   __shared__ TileLDS L;

@@ -83,7 +83,7 @@ def test_tuner_mask_entry_is_used_and_cached(tmp_path):
     assert ctx.tune_entries() == 1
     f = str(tmp_path / "tune.txt")
     with open(f, "w") as fh:
-        fh.write("# a comment\nothergpu/1cu 0 0 20 14 1 2 0 3 4 -1 0 0 15 1.0 0.9\nnot a line\n")
+        fh.write("# a comment\nothergpu/1cu 0 0 20 14 1 2 0 3 4 -1 0 0 15 6 1.0 0.9\nnot a line\n")
     ctx.tune_save(f)
     text = open(f).read()
     assert "othergpu/1cu" in text and "not a line" not in text
@@ -119,6 +119,14 @@ def test_tuner_scene_and_atlas_keep_their_output():
     _say(f"tuned scene: {r}")
     assert r["ms_tuned"] <= r["ms_default"]
     assert np.array_equal(render(), ref)
+    # the scene fine kernel's register budgets give the same image
+    ctx.set_param(_abi.RZ_PARAM_TUNED, 0)
+    for v in (1, 8, 6):
+        ctx.set_param(_abi.RZ_PARAM_SCENE_WAVES, v)
+        assert np.array_equal(render(), ref), f"scene waves {v}"
+    with pytest.raises(rz.RzError):
+        ctx.set_param(_abi.RZ_PARAM_SCENE_WAVES, 4)
+    ctx.set_param(_abi.RZ_PARAM_TUNED, 1)
 
     gs = rz.workload_glyphs(4, 2000, 2048)
     AW, AH = gs.atlas_size
