@@ -432,14 +432,18 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
  * fine stage (fewer waves per SIMD). RZ_PARAM_SCATTER_WAVES (8..18, default
  * 15): the scatter's grid holds at most 2^value waves. RZ_PARAM_SCENE_WAVES
  * (1, 6 or 8, default 6): the batched scene's fine kernel built for that many
- * waves per SIMD (1: registers unbounded). RZ_PARAM_TUNED (0 or
+ * waves per SIMD (1: registers unbounded). RZ_PARAM_SCENE_QUEUE (0 or 1,
+ * default 0): the batched scene's fine stage as a persistent grid taking
+ * canvas tiles from a device queue, the tiles covered by the most paths
+ * first. RZ_PARAM_TUNED (0 or
  * 1, default 1): a call whose workload has a tuned entry (rz_tune_*) uses that
  * entry's choices instead of the context's. Results do not depend on any of
  * them. */
 enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 2,
        RZ_PARAM_SCAN_DIRECT = 3, RZ_PARAM_GRAPH = 4, RZ_PARAM_GRAPH_REPLAYS = 5,
        RZ_PARAM_FLAT_LDS = 6, RZ_PARAM_BIN_LDS = 7, RZ_PARAM_FINE_LDS = 8,
-       RZ_PARAM_SCATTER_WAVES = 9, RZ_PARAM_TUNED = 10, RZ_PARAM_SCENE_WAVES = 11 };
+       RZ_PARAM_SCATTER_WAVES = 9, RZ_PARAM_TUNED = 10, RZ_PARAM_SCENE_WAVES = 11,
+       RZ_PARAM_SCENE_QUEUE = 12 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 
@@ -470,7 +474,7 @@ typedef struct rz_tune_result {
   int32_t flat_lds;      /* bytes of extra LDS per flatten wave, -1: the built-in choice */
   int32_t bin_lds, fine_lds;
   int32_t scatter_log2;
-  int32_t scene_waves;
+  int32_t scene_waves, scene_queue;
   uint32_t trials;       /* candidates timed */
   double ms_default;     /* per frame with the context's parameters */
   double ms_tuned;       /* per frame with the chosen ones */

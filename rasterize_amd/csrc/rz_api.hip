@@ -73,6 +73,7 @@ struct LaunchParams {
   int32_t fine_lds = 0;       // bytes
   int32_t scatter_log2 = 15;  // the scatter grid holds at most 2^v waves
   int32_t scene_waves = 6;    // k_scene_fine built for 1 (no bound), 6 or 8 waves per SIMD
+  int32_t scene_queue = 0;    // 1: k_scene_fine persistent over a queue of tiles, costliest first
 };
 bool same_params(const LaunchParams& a, const LaunchParams& b) {
   return std::memcmp(&a, &b, sizeof(a)) == 0;
@@ -136,6 +137,8 @@ struct rz_ctx {
   DBuf line_path;  // path of each line
   DBuf bw_w, bw_p, bw_tr;  // scene backward: per-(path, tile) winding / dL/dW, colour below, transform partials
   DBuf bw_stops;           // scene backward: dL/d(stop as read) per stop of the scene
+  DBuf sc_order;           // queued scene fine: tile order, queue head, per-tile path counts
+  int n_cus = 256;         // compute units of the device
   DBuf sc_pi, sc_counts, sc_rects, sc_rows, sc_rowcnt;
   uint32_t sc_cap_t = 0, sc_cap_b = 0, sc_cap_r = 0;  // per-tile / per-band / per-row slices
   DBuf list_buf;  // binning: record list (records, tiles) and event list
@@ -1389,9 +1392,26 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.lw = lw;
   a.lh = lh;
   a.tiles_x = ctx_x;
+  a.queue = nullptr;
+  a.order = nullptr;
+  a.n_tiles = (uint32_t)(ctx_x * cty);
   // no layer (the adjoint's own tile pass recomposites): the binned state only
   if (d_out) {
-    const dim3 fgrid((uint32_t)(ctx_x * cty));
+    dim3 fgrid((uint32_t)(ctx_x * cty));
+    if (ctx->cur.scene_queue) {
+      const size_t nt = (size_t)ctx_x * cty, nd = (size_t)(ctx_x + 1) * (cty + 1);
+      RZ_TRY(ensure(ctx, ctx->sc_order, (nt + nd + 64) * 4));
+      uint32_t* ord = (uint32_t*)ctx->sc_order.p;
+      uint32_t* q = ord + nt;
+      int* cnt = (int*)(q + 64);
+      RZ_LAUNCH(ctx, "k_scene_order", rzk::k_scene_order, dim3(1), dim3(rzk::kOrderBlock),
+                (const int4*)ctx->sc_rects.p, s->n, ctx_x, cty, cnt, ord, q, (const rzk::Status*)st);
+      a.queue = q;
+      a.order = ord;
+      // persistent: as many waves as the chip holds of this kernel
+      const uint32_t resident = (uint32_t)ctx->n_cus * 4u * (uint32_t)std::max(ctx->cur.scene_waves, 5);
+      fgrid = dim3(std::min<uint32_t>(a.n_tiles, resident));
+    }
     const int waves = ctx->cur.scene_waves;
     if (waves == 8)
       RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<8>), fgrid, dim3(64), ctx->cur.fine_lds, a);
@@ -2294,13 +2314,20 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
     p.scatter_log2 = v;
     st = consider(p);
   }
-  if (key.kind == 1)
+  if (key.kind == 1) {
     for (int32_t v : {6, 8, 1}) {
       if (st != RZ_OK) break;
       LaunchParams p = best;
       p.scene_waves = v;
       st = consider(p);
     }
+    for (int32_t v : {0, 1}) {
+      if (st != RZ_OK) break;
+      LaunchParams p = best;
+      p.scene_queue = v;
+      st = consider(p);
+    }
+  }
   if (st == RZ_OK) st = sweep_depths();
   ctx->tune_forced = false;
   ctx->use_graphs = graphs;
@@ -2324,6 +2351,7 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
     res->fine_lds = best.fine_lds;
     res->scatter_log2 = best.scatter_log2;
     res->scene_waves = best.scene_waves;
+    res->scene_queue = best.scene_queue;
     res->trials = (uint32_t)seen.size();
     res->ms_default = ms_default;
     res->ms_tuned = best_ms;
@@ -2333,7 +2361,7 @@ rz_status tune_search(rz_ctx* ctx, const TuneKey& key, int auto_dq, int auto_dc,
 
 // The tuning cache file: lines "<gpu> <kind> <mode> <px> <slots> <quad>
 // <cubic> <split_auto> <dq> <dc> <flat_lds> <bin_lds> <fine_lds> <scatter>
-// <scene_waves> <ms_default> <ms_tuned>"; '#' starts a comment. Entries of other GPUs are
+// <scene_waves> <scene_queue> <ms_default> <ms_tuned>"; '#' starts a comment. Entries of other GPUs are
 // kept by a save and skipped by a load.
 std::string tune_line(const std::string& gpu, const TuneEntry& t) {
   std::ostringstream o;
@@ -2341,7 +2369,7 @@ std::string tune_line(const std::string& gpu, const TuneEntry& t) {
     << t.key.slots_log2 << ' ' << t.key.quad_log2 << ' ' << t.key.cubic_log2 << ' '
     << t.p.split_auto << ' ' << t.p.dq << ' ' << t.p.dc << ' ' << t.p.flat_lds << ' '
     << t.p.bin_lds << ' ' << t.p.fine_lds << ' ' << t.p.scatter_log2 << ' ' << t.p.scene_waves << ' '
-    << t.ms_default << ' ' << t.ms_tuned;
+    << t.p.scene_queue << ' ' << t.ms_default << ' ' << t.ms_tuned;
   return o.str();
 }
 
@@ -2351,13 +2379,15 @@ bool parse_tune_line(const std::string& line, std::string* gpu, TuneEntry* t) {
   LaunchParams p;
   if (!(in >> *gpu >> t->key.kind >> t->key.mode >> t->key.px_log2 >> t->key.slots_log2 >>
         t->key.quad_log2 >> t->key.cubic_log2 >> p.split_auto >> p.dq >> p.dc >> p.flat_lds >>
-        p.bin_lds >> p.fine_lds >> p.scatter_log2 >> p.scene_waves >> t->ms_default >> t->ms_tuned))
+        p.bin_lds >> p.fine_lds >> p.scatter_log2 >> p.scene_waves >> p.scene_queue >>
+        t->ms_default >> t->ms_tuned))
     return false;
   // only choices that rz_set_param could make
   if (p.split_auto < 0 || p.split_auto > 1 || p.dq < 0 || p.dq > 8 || p.dc < 0 || p.dc > 8 ||
       p.flat_lds < -1 || p.flat_lds > 32768 || p.bin_lds < 0 || p.bin_lds > 32768 ||
       p.fine_lds < 0 || p.fine_lds > 32768 || p.scatter_log2 < 8 || p.scatter_log2 > 18 ||
-      (p.scene_waves != 1 && p.scene_waves != 6 && p.scene_waves != 8))
+      (p.scene_waves != 1 && p.scene_waves != 6 && p.scene_waves != 8) || p.scene_queue < 0 ||
+      p.scene_queue > 1)
     return false;
   t->p = p;
   return true;
@@ -2410,6 +2440,9 @@ rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
     if (value != 1 && value != 6 && value != 8)
       return fail(ctx, RZ_E_ARG, "parameter must be 1, 6 or 8");
     lp.scene_waves = value;
+  } else if (param == RZ_PARAM_SCENE_QUEUE) {
+    RZ_TRY(range(0, 1));
+    lp.scene_queue = value;
   } else if (param == RZ_PARAM_TUNED) {
     RZ_TRY(range(0, 1));
     ctx->use_tuned = value != 0;
@@ -2430,6 +2463,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   else if (param == RZ_PARAM_FINE_LDS) *value = lp.fine_lds / 1024;
   else if (param == RZ_PARAM_SCATTER_WAVES) *value = lp.scatter_log2;
   else if (param == RZ_PARAM_SCENE_WAVES) *value = lp.scene_waves;
+  else if (param == RZ_PARAM_SCENE_QUEUE) *value = lp.scene_queue;
   else if (param == RZ_PARAM_TUNED) *value = ctx->use_tuned ? 1 : 0;
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
@@ -2521,7 +2555,7 @@ rz_status rz_tune_save(rz_ctx* ctx, const char* path) {
     std::ofstream out(tmp, std::ios::trunc);
     if (!out) return fail(ctx, RZ_E_ARG, std::string("cannot write ") + tmp);
     out << "# rasterize_amd launch tuning: gpu kind mode px slots quad cubic split_auto dq dc "
-           "flat_lds bin_lds fine_lds scatter scene_waves ms_default ms_tuned\n";
+           "flat_lds bin_lds fine_lds scatter scene_waves scene_queue ms_default ms_tuned\n";
     for (const std::string& l : keep) out << l << '\n';
     out.precision(6);
     for (const TuneEntry& t : ctx->tuned) out << tune_line(ctx->gpu_id, t) << '\n';
@@ -2613,6 +2647,7 @@ rz_status rz_ctx_create(rz_ctx** out, int device) {
       std::string arch = prop.gcnArchName;
       arch = arch.substr(0, arch.find(':'));
       ctx->gpu_id = arch + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+      ctx->n_cus = std::max(1, prop.multiProcessorCount);
     }
   }
   if (e != hipSuccess) {
@@ -2636,6 +2671,7 @@ rz_status rz_ctx_destroy(rz_ctx* ctx) {
                   &ctx->recs,      &ctx->events,    &ctx->out,       &ctx->lines,
                   &ctx->counts,    &ctx->stops,     &ctx->line_path,
                   &ctx->bw_w,      &ctx->bw_p,      &ctx->bw_tr,     &ctx->bw_stops,
+                  &ctx->sc_order,
                   &ctx->sc_pi,     &ctx->sc_counts,
                   &ctx->sc_rects,  &ctx->sc_rows,   &ctx->sc_rowcnt, &ctx->list_buf,
                   &ctx->wave_lines, &ctx->atlas_tiles};

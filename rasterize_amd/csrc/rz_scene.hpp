@@ -309,6 +309,9 @@ struct SceneFineArgs {
   size_t row_stride;  // pixels
   int lw, lh;         // layer size
   int tiles_x;        // canvas tiles per row
+  uint32_t* queue;    // queued form: the next entry of `order` to take, or null
+  const uint32_t* order;  // queued form: canvas tiles, costliest first
+  uint32_t n_tiles;
 };
 
 // One canvas tile per wave. Each lane owns 4 pixels (rows 4k + lane/16,
@@ -326,13 +329,9 @@ struct SceneFineArgs {
 // everywhere) and is skipped.
 // MINW: waves per SIMD the registers must allow (1: no bound; 6: 80 VGPRs,
 // no spills; 8: 64 VGPRs with a few spilled)
-template <int MINW>
-__global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a) {
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
-  if (a.st->err || a.st->overflow) return;
-  __shared__ TileLDS L;
+__device__ inline void scene_tile(const SceneFineArgs& a, TileLDS& L, uint32_t tile) {
   const int lane = threadIdx.x;
-  const int TX = (int)(blockIdx.x % (uint32_t)a.tiles_x), TY = (int)(blockIdx.x / (uint32_t)a.tiles_x);
+  const int TX = (int)(tile % (uint32_t)a.tiles_x), TY = (int)(tile / (uint32_t)a.tiles_x);
   const int col = lane & 15;
   rzp::C4 c[4];
 #pragma unroll
@@ -408,6 +407,92 @@ __global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a) {
     const int y = TY * 16 + 4 * k + (lane >> 4);
     if (x < a.lw && y < a.lh) a.out[(size_t)y * a.row_stride + x] = c[k];
   }
+}
+
+// One wave per workgroup. Without a queue, workgroup i takes canvas tile i.
+// With one (a.queue), the grid is persistent (a few waves per SIMD) and each
+// wave takes tiles from the front of a.order, costliest first, until none is
+// left: the tiles with the most paths start first instead of last.
+// MINW: waves per SIMD the registers must allow (1: no bound; 6: 80 VGPRs,
+// no spills; 8: 64 VGPRs with a few spilled).
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
+  if (a.st->err || a.st->overflow) return;
+  __shared__ TileLDS L;
+  if (!a.queue) {
+    scene_tile(a, L, blockIdx.x);
+    return;
+  }
+  for (;;) {
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(a.queue, 1u);
+    t = (uint32_t)__shfl((int)t, 0, 64);
+    if (t >= a.n_tiles) break;
+    __syncthreads();  // the previous tile's LDS reads are done
+    scene_tile(a, L, a.order[t]);
+  }
+}
+
+// The tile order of the queued form, in one workgroup of kOrderBlock
+// threads: each tile's count of paths whose tile rectangle covers it (a 2D
+// difference array over the rectangles, then prefix sums along rows and
+// columns), tiles sorted by that count, highest first (a counting sort over
+// min(count, 1023)), and the queue's head reset.
+constexpr int kOrderBlock = 1024;
+constexpr int kOrderBuckets = 1024;
+__global__ __launch_bounds__(kOrderBlock) void k_scene_order(const int4* rects, uint32_t n_paths,
+                                                             int tiles_x, int tiles_y, int* cnt,
+                                                             uint32_t* order, uint32_t* queue,
+                                                             const Status* st) {
+  __shared__ uint32_t hist[kOrderBuckets];
+  const int tid = threadIdx.x;
+  const uint32_t nt = (uint32_t)tiles_x * (uint32_t)tiles_y;
+  if (tid == 0) *queue = 0;
+  if (st->err || st->overflow) return;
+  // (tiles_x + 1) x (tiles_y + 1) differences
+  const uint32_t nd = (uint32_t)(tiles_x + 1) * (uint32_t)(tiles_y + 1);
+  for (uint32_t i = tid; i < nd; i += kOrderBlock) cnt[i] = 0;
+  for (int i = tid; i < kOrderBuckets; i += kOrderBlock) hist[i] = 0;
+  __syncthreads();
+  const int W1 = tiles_x + 1;
+  for (uint32_t p = tid; p < n_paths; p += kOrderBlock) {
+    const int4 r = rects[p];  // [x0, x1) x [y0, y1) in canvas tiles
+    const int x0 = min(max(r.x, 0), tiles_x), x1 = min(max(r.z, 0), tiles_x);
+    const int y0 = min(max(r.y, 0), tiles_y), y1 = min(max(r.w, 0), tiles_y);
+    if (x0 >= x1 || y0 >= y1) continue;
+    atomicAdd(&cnt[y0 * W1 + x0], 1);
+    atomicAdd(&cnt[y0 * W1 + x1], -1);
+    atomicAdd(&cnt[y1 * W1 + x0], -1);
+    atomicAdd(&cnt[y1 * W1 + x1], 1);
+  }
+  __syncthreads();
+  for (int y = tid; y < tiles_y; y += kOrderBlock) {
+    int run = 0;
+    for (int x = 0; x < tiles_x; ++x) cnt[y * W1 + x] = run += cnt[y * W1 + x];
+  }
+  __syncthreads();
+  for (int x = tid; x < tiles_x; x += kOrderBlock) {
+    int run = 0;
+    for (int y = 0; y < tiles_y; ++y) cnt[y * W1 + x] = run += cnt[y * W1 + x];
+  }
+  __syncthreads();
+  auto bucket = [&](uint32_t t) {
+    const int c = cnt[(int)(t / (uint32_t)tiles_x) * W1 + (int)(t % (uint32_t)tiles_x)];
+    return (uint32_t)(kOrderBuckets - 1 - min(max(c, 0), kOrderBuckets - 1));  // high counts first
+  };
+  for (uint32_t t = tid; t < nt; t += kOrderBlock) atomicAdd(&hist[bucket(t)], 1u);
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < kOrderBuckets; ++b) {
+      const uint32_t v = hist[b];
+      hist[b] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (uint32_t t = tid; t < nt; t += kOrderBlock) order[atomicAdd(&hist[bucket(t)], 1u)] = t;
 }
 
 }  // namespace rzk

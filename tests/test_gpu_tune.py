@@ -83,7 +83,7 @@ def test_tuner_mask_entry_is_used_and_cached(tmp_path):
     assert ctx.tune_entries() == 1
     f = str(tmp_path / "tune.txt")
     with open(f, "w") as fh:
-        fh.write("# a comment\nothergpu/1cu 0 0 20 14 1 2 0 3 4 -1 0 0 15 6 1.0 0.9\nnot a line\n")
+        fh.write("# a comment\nothergpu/1cu 0 0 20 14 1 2 0 3 4 -1 0 0 15 6 0 1.0 0.9\nnot a line\n")
     ctx.tune_save(f)
     text = open(f).read()
     assert "othergpu/1cu" in text and "not a line" not in text
@@ -126,6 +126,12 @@ def test_tuner_scene_and_atlas_keep_their_output():
         assert np.array_equal(render(), ref), f"scene waves {v}"
     with pytest.raises(rz.RzError):
         ctx.set_param(_abi.RZ_PARAM_SCENE_WAVES, 4)
+    # the queued (persistent, costliest tiles first) fine stage too
+    for v in (8, 6):
+        ctx.set_param(_abi.RZ_PARAM_SCENE_WAVES, v)
+        ctx.set_param(_abi.RZ_PARAM_SCENE_QUEUE, 1)
+        assert np.array_equal(render(), ref), f"queued, scene waves {v}"
+    ctx.set_param(_abi.RZ_PARAM_SCENE_QUEUE, 0)
     ctx.set_param(_abi.RZ_PARAM_TUNED, 1)
 
     gs = rz.workload_glyphs(4, 2000, 2048)
