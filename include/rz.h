@@ -426,7 +426,8 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
  * HIP graph captured after the same call completed once without growing the
  * context's buffers (the device data they read, e.g. updated scene points,
  * may change between frames). RZ_PARAM_GRAPH_REPLAYS (read only): frames
- * replayed so far. RZ_PARAM_FLAT_LDS (-1..32, default -1: 4 for one path, 0
+ * replayed so far. RZ_PARAM_GRAPH_GEN (read only): the context's buffer and
+ * parameter generation (a frame captured under another one is not replayed). RZ_PARAM_FLAT_LDS (-1..32, default -1: 4 for one path, 0
  * for scenes and atlases), RZ_PARAM_BIN_LDS, RZ_PARAM_FINE_LDS (0..32,
  * default 0): KB of extra LDS per wave of the flatten, the count pass and the
  * fine stage (fewer waves per SIMD). RZ_PARAM_SCATTER_WAVES (8..18, default
@@ -443,7 +444,7 @@ enum { RZ_PARAM_SPLIT_QUAD = 0, RZ_PARAM_SPLIT_CUBIC = 1, RZ_PARAM_SPLIT_AUTO = 
        RZ_PARAM_SCAN_DIRECT = 3, RZ_PARAM_GRAPH = 4, RZ_PARAM_GRAPH_REPLAYS = 5,
        RZ_PARAM_FLAT_LDS = 6, RZ_PARAM_BIN_LDS = 7, RZ_PARAM_FINE_LDS = 8,
        RZ_PARAM_SCATTER_WAVES = 9, RZ_PARAM_TUNED = 10, RZ_PARAM_SCENE_WAVES = 11,
-       RZ_PARAM_SCENE_QUEUE = 12 };
+       RZ_PARAM_SCENE_QUEUE = 12, RZ_PARAM_GRAPH_GEN = 13 };
 rz_status rz_set_param(rz_ctx* ctx, int param, int value);
 rz_status rz_get_param(rz_ctx* ctx, int param, int* value);
 

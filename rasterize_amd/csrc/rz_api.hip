@@ -74,7 +74,9 @@ struct LaunchParams {
   int32_t scatter_log2 = 15;  // the scatter grid holds at most 2^v waves
   int32_t scene_waves = 6;    // k_scene_fine built for 1 (no bound), 6 or 8 waves per SIMD
   int32_t scene_queue = 0;    // 1: k_scene_fine persistent over a queue of tiles, costliest first
+  int32_t reserved = 0;       // keeps the size a multiple of 8: GraphKey, compared bytewise, has no padding
 };
+static_assert(sizeof(LaunchParams) == 10 * sizeof(int32_t), "LaunchParams has padding");
 bool same_params(const LaunchParams& a, const LaunchParams& b) {
   return std::memcmp(&a, &b, sizeof(a)) == 0;
 }
@@ -106,6 +108,11 @@ struct GraphKey {
   const void* ptr[5];       // device outputs / inputs
   LaunchParams lp;          // the launch choices the frame was enqueued with
 };
+// keys are compared with memcmp and copied member by member: no padding bytes
+static_assert(sizeof(GraphKey) == 4 * sizeof(int) + 2 * sizeof(uint64_t) + 11 * sizeof(double) +
+                                      4 * sizeof(float) + 4 * sizeof(size_t) + 5 * sizeof(void*) +
+                                      sizeof(LaunchParams),
+              "GraphKey has padding");
 struct GraphEntry {
   GraphKey key;
   hipGraphExec_t exec;
@@ -2399,7 +2406,8 @@ extern "C" {
 
 rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   if (!ctx) return RZ_E_ARG;
-  if (param == RZ_PARAM_GRAPH_REPLAYS) return fail(ctx, RZ_E_ARG, "read-only parameter");
+  if (param == RZ_PARAM_GRAPH_REPLAYS || param == RZ_PARAM_GRAPH_GEN)
+    return fail(ctx, RZ_E_ARG, "read-only parameter");
   ++ctx->gen;  // frames captured with the old value no longer match
   if (param == RZ_PARAM_GRAPH) {
     if (value != 0 && value != 1) return fail(ctx, RZ_E_ARG, "parameter out of range 0..1");
@@ -2468,6 +2476,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
   else if (param == RZ_PARAM_GRAPH_REPLAYS) *value = (int)std::min<uint64_t>(ctx->replays, INT32_MAX);
+  else if (param == RZ_PARAM_GRAPH_GEN) *value = (int)(ctx->gen & 0x7fffffff);
   else return fail(ctx, RZ_E_ARG, "unknown parameter");
   return RZ_OK;
 }

@@ -1278,8 +1278,12 @@ __device__ inline void flush_row0(int* rowd, uint32_t r0, int lane) {
   if (lane == 0 && t) atomicAdd(&rowd[0], (int)t);
 }
 
+// The tile's accumulators are rows of kAccStride ints: with 17, the same
+// column of different rows falls in different LDS banks (neighbouring lanes
+// often deposit into neighbouring rows of one steep record).
+constexpr int kAccStride = 17;
 __device__ inline void dep(int* acc, int row, int col, float v) {
-  if ((unsigned)col < 16u) atomicAdd(&acc[row * 16 + col], fix(v));
+  if ((unsigned)col < 16u) atomicAdd(&acc[row * kAccStride + col], fix(v));
 }
 
 // signed_difference_line (src/rasterize.rs:395-469) for the cell records of
@@ -1400,7 +1404,7 @@ __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re
 
 // The LDS of one tile's coverage (2^-20 fixed point, int32).
 struct TileLDS {
-  int acc[256];
+  int acc[16 * kAccStride];
   int rowd[18];
   RowStage rs[64];
   int marks[64];
@@ -1424,7 +1428,7 @@ __device__ inline bool tile_narrow(uint32_t n_records, uint32_t n_events) {
 __device__ inline void chunk_build(TileLDS& L, const float4* recs, uint32_t rb, uint32_t re,
                                    const int4* events, uint32_t eb, uint32_t ee, int tx, int lane) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) L.acc[lane + 64 * k] = 0;
+  for (int k = lane; k < 16 * kAccStride; k += 64) L.acc[k] = 0;
   if (lane < 18) L.rowd[lane] = 0;
   __syncthreads();
   uint32_t r0 = 0;
@@ -1456,7 +1460,7 @@ __device__ inline void tile_coverage(TileLDS& L, const float4* recs, uint32_t rb
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int row = 4 * k + (lane >> 4);
-      w[k] += (int64_t)(int32_t)(L.rowd[row] + row_scan16(L.acc[row * 16 + col]));
+      w[k] += (int64_t)(int32_t)(L.rowd[row] + row_scan16(L.acc[row * kAccStride + col]));
     }
     r0 = r1;
     e0 = e1;
