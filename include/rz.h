@@ -152,6 +152,29 @@ rz_status rz_svg_transform_parse(const char* text, size_t len, rz_transform* out
 rz_status rz_path_buf_view(const rz_path_buf* path, rz_path* view);
 rz_status rz_path_buf_free(rz_path_buf* path);
 
+/* ---- stroking --------------------------------------------------------- */
+
+/* LineJoin and LineCap (src/path.rs:70-116) */
+enum { RZ_JOIN_MITER = 0, RZ_JOIN_BEVEL = 1, RZ_JOIN_ROUND = 2 };
+enum { RZ_CAP_BUTT = 0, RZ_CAP_SQUARE = 1, RZ_CAP_ROUND = 2 };
+/* StrokeStyle (src/path.rs:118-136). miter_limit is LineJoin::Miter's
+ * argument (the reference's default style is Miter(4.0), Butt). */
+typedef struct rz_stroke_style {
+  double width;
+  double miter_limit;
+  int32_t line_join;
+  int32_t line_cap;
+} rz_stroke_style;
+/* Path::stroke (src/path.rs:370-415): the outline of `path` offset by
+ * width / 2 on both sides, joined and capped as `style` says, as a new owned
+ * path whose subpaths are all closed (free it with rz_path_buf_free). The
+ * offsetting, joins and caps run on the GPU, one work item per segment and
+ * direction; the input is copied from and the result to host memory, so the
+ * call synchronises the context's stream. RZ_E_ARG for an unknown join or
+ * cap. */
+rz_status rz_path_stroke(rz_ctx* ctx, const rz_path* path, rz_stroke_style style,
+                         rz_path_buf** out);
+
 /* ---- stage hooks (bit-exact parity surfaces) ------------------------- */
 
 /* Path::flatten(tr, flatness, close) (src/path.rs:418-425, 734-796): lines as
@@ -282,11 +305,14 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
 /* A scene tree (SceneInner, src/scene.rs:19-63) as a node array: Fill (path,
  * paint, fill_rule), Group (children[first .. first + count)), Transform
  * (tr, child), Opacity (opacity, used as given: Scene::opacity's clamp to
- * [0, 1] is the caller's, as in the reference's deserialized scenes; child) and Clip (the clip
+ * [0, 1] is the caller's, as in the reference's deserialized scenes; child), Clip (the clip
  * path, fill_rule, units: RZ_UNITS_USER_SPACE or RZ_UNITS_BBOX of the child,
- * child). Indices refer to the paths / paints / nodes / children arrays. */
+ * child) and Stroke (path, paint, and the StrokeStyle in stroke_width,
+ * miter_limit, line_join, line_cap: the path stroked by rz_path_stroke and
+ * filled with the non-zero rule, src/scene.rs:297-306). Indices refer to the
+ * paths / paints / nodes / children arrays. */
 enum { RZ_NODE_FILL = 0, RZ_NODE_GROUP = 1, RZ_NODE_TRANSFORM = 2, RZ_NODE_OPACITY = 3,
-       RZ_NODE_CLIP = 4 };
+       RZ_NODE_CLIP = 4, RZ_NODE_STROKE = 5 };
 typedef struct rz_scene_node {
   int32_t kind;
   int32_t fill_rule;
@@ -297,6 +323,10 @@ typedef struct rz_scene_node {
   uint32_t first, count;
   double tr[6];
   double opacity;
+  double stroke_width;
+  double miter_limit;
+  int32_t line_join;
+  int32_t line_cap;
 } rz_scene_node;
 
 /* Blocking Scene::render(rasterizer, tr, view, bg) (src/scene.rs:186-199) of

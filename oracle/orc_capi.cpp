@@ -12,6 +12,7 @@
 #include "orc_exact.hpp"
 #include "orc_svg.hpp"
 #include "orc_paint.hpp"
+#include "orc_stroke.hpp"
 
 using namespace orc;
 
@@ -83,6 +84,34 @@ static Paint to_paint(const orc_paint* p) {
   return Paint::radial(stops, (Units)p->units, p->linear_colors != 0, (Spread)p->spread,
                        to_tr(p->tr), P(p->center[0], p->center[1]), p->radius,
                        P(p->fcenter[0], p->fcenter[1]), p->fradius);
+}
+
+// A Path into caller buffers (kinds, point offsets, x/y points, subpath
+// offsets, closed flags); sizes always to *n_seg / *n_pts / *n_sub.
+static int put_path(const Path& path, uint8_t* kinds, uint32_t* offs, double* pts, uint32_t* subs,
+                    uint8_t* closed, size_t cap_seg, size_t cap_pts, size_t cap_sub, size_t* n_seg,
+                    size_t* n_pts, size_t* n_sub) {
+  size_t np = 0;
+  for (const auto& sg : path.segments) np += (size_t)sg.npts();
+  *n_seg = path.segments.size();
+  *n_pts = np;
+  *n_sub = path.nsubpaths();
+  if (!kinds) return ORC_OK;
+  if (cap_seg < *n_seg || cap_pts < np || cap_sub < path.subpaths.size()) return ORC_E_CAP;
+  size_t o = 0;
+  offs[0] = 0;
+  for (size_t k = 0; k < path.segments.size(); ++k) {
+    const Segment& sg = path.segments[k];
+    kinds[k] = (uint8_t)sg.kind;
+    for (int i = 0; i < sg.npts(); ++i, ++o) {
+      pts[2 * o] = sg.p[i].x;
+      pts[2 * o + 1] = sg.p[i].y;
+    }
+    offs[k + 1] = (uint32_t)o;
+  }
+  for (size_t i = 0; i < path.subpaths.size(); ++i) subs[i] = (uint32_t)path.subpaths[i];
+  for (size_t i = 0; i < path.closed.size(); ++i) closed[i] = path.closed[i];
+  return ORC_OK;
 }
 
 extern "C" {
@@ -381,27 +410,42 @@ int orc_svg_parse(const char* text, size_t len, uint8_t* kinds, uint32_t* offs, 
     *n_seg = e.offset;
     return ORC_E_ARG;
   }
-  size_t np = 0;
-  for (const auto& sg : path.segments) np += (size_t)sg.npts();
-  *n_seg = path.segments.size();
-  *n_pts = np;
-  *n_sub = path.nsubpaths();
-  if (!kinds) return ORC_OK;
-  if (cap_seg < *n_seg || cap_pts < np || cap_sub < path.subpaths.size()) return ORC_E_CAP;
-  size_t o = 0;
-  offs[0] = 0;
-  for (size_t k = 0; k < path.segments.size(); ++k) {
-    const Segment& sg = path.segments[k];
-    kinds[k] = (uint8_t)sg.kind;
-    for (int i = 0; i < sg.npts(); ++i, ++o) {
-      pts[2 * o] = sg.p[i].x;
-      pts[2 * o + 1] = sg.p[i].y;
-    }
-    offs[k + 1] = (uint32_t)o;
-  }
-  for (size_t i = 0; i < path.subpaths.size(); ++i) subs[i] = (uint32_t)path.subpaths[i];
-  for (size_t i = 0; i < path.closed.size(); ++i) closed[i] = path.closed[i];
-  return ORC_OK;
+  return put_path(path, kinds, offs, pts, subs, closed, cap_seg, cap_pts, cap_sub, n_seg, n_pts,
+                  n_sub);
+}
+
+// Path::stroke (src/path.rs:370-415) of the path, written like orc_svg_parse's
+// output. join: 0 miter (miter_limit), 1 bevel, 2 round; cap: 0 butt, 1 square,
+// 2 round.
+int orc_path_stroke(const orc_path* p, double width, int join, double miter_limit, int cap,
+                    uint8_t* kinds, uint32_t* offs, double* pts, uint32_t* subs, uint8_t* closed,
+                    size_t cap_seg, size_t cap_pts, size_t cap_sub, size_t* n_seg, size_t* n_pts,
+                    size_t* n_sub) {
+  StrokeStyle st;
+  st.width = width;
+  st.line_join = (LineJoin)join;
+  st.miter_limit = miter_limit;
+  st.line_cap = (LineCap)cap;
+  const Path out = stroke(to_path(p), st);
+  return put_path(out, kinds, offs, pts, subs, closed, cap_seg, cap_pts, cap_sub, n_seg, n_pts,
+                  n_sub);
+}
+
+// Time Path::stroke: seconds per call over `reps` calls.
+double orc_time_stroke(const orc_path* p, double width, int join, double miter_limit, int cap,
+                       int reps, size_t* n_seg) {
+  StrokeStyle st;
+  st.width = width;
+  st.line_join = (LineJoin)join;
+  st.miter_limit = miter_limit;
+  st.line_cap = (LineCap)cap;
+  const Path in = to_path(p);
+  size_t n = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < reps; ++r) n = stroke(in, st).segments.size();
+  const auto t1 = std::chrono::steady_clock::now();
+  if (n_seg) *n_seg = n;
+  return std::chrono::duration<double>(t1 - t0).count() / (reps > 0 ? reps : 1);
 }
 
 // FromStr for Transform (src/svg.rs:592-601); ORC_E_ARG on a syntax error.
@@ -416,12 +460,15 @@ int orc_svg_transform(const char* text, size_t len, double* out6) {
 }
 
 // Scene::render of a scene tree given as rz_scene_node's layout (kind,
-// fill_rule, units, path, paint, child, first, count, tr[6], opacity).
+// fill_rule, units, path, paint, child, first, count, tr[6], opacity,
+// stroke_width, miter_limit, line_join, line_cap).
 typedef struct {
   int32_t kind, fill_rule, units;
   uint32_t path, paint, child, first, count;
   double tr[6];
   double opacity;
+  double stroke_width, miter_limit;
+  int32_t line_join, line_cap;
 } orc_scene_node;
 
 static SceneRef tree_node(const orc_scene_node* nodes, const uint32_t* children, uint32_t id,
@@ -452,6 +499,22 @@ static SceneRef tree_node(const orc_scene_node* nodes, const uint32_t* children,
       s->opacity = n.opacity;  // as given (SceneInner::Opacity); Scene::opacity clamps
       s->child = tree_node(nodes, children, n.child, paths, paints, depth + 1);
       break;
+    case 5: {
+      s->kind = SceneNode::STROKE;
+      s->path = paths[n.path];
+      StrokeStyle st;
+      st.width = n.stroke_width;
+      st.miter_limit = n.miter_limit;
+      st.line_join = (LineJoin)n.line_join;
+      st.line_cap = (LineCap)n.line_cap;
+      auto f = std::make_shared<SceneNode>();
+      f->kind = SceneNode::FILL;
+      f->path = std::make_shared<Path>(stroke(*s->path, st));
+      f->paint = paints[n.paint];
+      f->rule = NONZERO;
+      s->child = f;
+      break;
+    }
     default:
       s->kind = SceneNode::CLIP;
       s->path = paths[n.path];

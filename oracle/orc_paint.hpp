@@ -285,14 +285,17 @@ inline void sd_fill(const Path& path, const Transform& tr, double flatness, Fill
 }
 
 // ---------------------------------------------------------------------------
-// Scene: Fill / Group / Transform / Opacity / Clip (src/scene.rs:19-63,
-// 87-200, 256-566). Stroke nodes are not part of this checker.
+// Scene: Fill / Group / Transform / Opacity / Clip / Stroke (src/scene.rs:19-63,
+// 87-200, 256-566). A Stroke node holds, as its child, the Fill of its
+// stroked path (Path::stroke, non-zero rule) that Pipeline::build_rec makes
+// of it (src/scene.rs:297-306); Scene::bbox takes the unstroked path
+// (src/scene.rs:158).
 // ---------------------------------------------------------------------------
 struct SceneNode;
 typedef std::shared_ptr<SceneNode> SceneRef;
 struct SceneNode {
-  enum Kind { FILL, GROUP, TRANSFORM, OPACITY, CLIP } kind;
-  std::shared_ptr<Path> path;  // fill; the clip path of a clip
+  enum Kind { FILL, GROUP, TRANSFORM, OPACITY, CLIP, STROKE } kind;
+  std::shared_ptr<Path> path;  // fill; the clip path of a clip; the unstroked path of a stroke
   std::shared_ptr<Paint> paint;
   FillRule rule = NONZERO;
   std::vector<SceneRef> children;
@@ -321,6 +324,8 @@ inline bool scene_bbox(const SceneNode* s, const Transform& tr, BBox* out) {
       return scene_bbox(s->child.get(), tr * s->tr, out);
     case SceneNode::OPACITY:
       return scene_bbox(s->child.get(), tr, out);
+    case SceneNode::STROKE:
+      return s->path->bbox(tr, out);
     default: {
       Transform ctr = tr;
       if (s->units == BOUNDING_BOX) {
@@ -393,6 +398,7 @@ struct Pipeline {
       *id = nodes.size() - 1;
       return true;
     }
+    if (s->kind == SceneNode::STROKE) return build_rec(s->child.get(), view, tr, id);
     if (s->kind == SceneNode::GROUP) {
       std::vector<size_t> ch;
       for (auto& c : s->children) {

@@ -42,6 +42,41 @@ class FillRule:
     EvenOdd = _abi.RZ_EVENODD
 
 
+class LineJoin:
+    """LineJoin (src/path.rs:70-93): Miter(limit) is LineJoin.Miter with
+    StrokeStyle.miter_limit."""
+    Miter = _abi.RZ_JOIN_MITER
+    Bevel = _abi.RZ_JOIN_BEVEL
+    Round = _abi.RZ_JOIN_ROUND
+
+
+class LineCap:
+    """LineCap (src/path.rs:95-116)."""
+    Butt = _abi.RZ_CAP_BUTT
+    Square = _abi.RZ_CAP_SQUARE
+    Round = _abi.RZ_CAP_ROUND
+
+
+class StrokeStyle:
+    """StrokeStyle (src/path.rs:118-136); the default join is Miter(4.0), the
+    default cap Butt."""
+
+    __slots__ = ("width", "line_join", "miter_limit", "line_cap")
+
+    def __init__(self, width, line_join=LineJoin.Miter, line_cap=LineCap.Butt, miter_limit=4.0):
+        self.width = float(width)
+        self.line_join = int(line_join)
+        self.line_cap = int(line_cap)
+        self.miter_limit = float(miter_limit)
+
+    def abi(self):
+        return _abi.rz_stroke_style(self.width, self.miter_limit, self.line_join, self.line_cap)
+
+    def __repr__(self):
+        return (f"StrokeStyle(width={self.width}, line_join={self.line_join}, "
+                f"line_cap={self.line_cap}, miter_limit={self.miter_limit})")
+
+
 class Transform:
     """2D affine [m00, m01, m02, m10, m11, m12] (src/geometry.rs:316-539)."""
 
@@ -196,6 +231,25 @@ class Path:
 
     def segments_count(self):
         return int(self.kinds.size)
+
+    def stroke(self, style, ctx=None):
+        """Path::stroke (src/path.rs:370-415) on the GPU (rz_path_stroke): a
+        new path, the outline of this one offset by width / 2 on both sides,
+        joined and capped as `style` (a StrokeStyle, or a width) says."""
+        if not isinstance(style, StrokeStyle):
+            style = StrokeStyle(style)
+        ctx = ctx or default_context()
+        h = C.c_void_p()
+        L = lib()
+        ctx.check(L.rz_path_stroke(ctx.h, C.byref(self.abi()), style.abi(), C.byref(h)))
+        try:
+            v = _abi.rz_path()
+            L.rz_path_buf_view(h, C.byref(v))
+            return Path.from_abi(v) if v.n_segments else Path(
+                np.zeros(0, np.uint8), np.zeros(1, np.uint32), np.zeros(0), np.zeros(0, np.uint32),
+                np.zeros(0, np.uint8))
+        finally:
+            L.rz_path_buf_free(h)
 
     def __len__(self):
         return max(0, int(self.subpaths.size) - 1)
@@ -748,16 +802,17 @@ class Layer:
 
 
 class Scene:
-    """A scene tree (SceneInner, src/scene.rs:19-63): Fill, Group, Transform,
-    Opacity and Clip nodes, built with the constructors of src/scene.rs:87-151
-    or read from the reference's JSON scene format (Scene.from_json)."""
+    """A scene tree (SceneInner, src/scene.rs:19-63): Fill, Stroke, Group,
+    Transform, Opacity and Clip nodes, built with the constructors of
+    src/scene.rs:87-151 or read from the reference's JSON scene format
+    (Scene.from_json)."""
 
-    FILL, GROUP, TRANSFORM, OPACITY, CLIP = (_abi.RZ_NODE_FILL, _abi.RZ_NODE_GROUP,
-                                             _abi.RZ_NODE_TRANSFORM, _abi.RZ_NODE_OPACITY,
-                                             _abi.RZ_NODE_CLIP)
+    FILL, GROUP, TRANSFORM, OPACITY, CLIP, STROKE = (
+        _abi.RZ_NODE_FILL, _abi.RZ_NODE_GROUP, _abi.RZ_NODE_TRANSFORM, _abi.RZ_NODE_OPACITY,
+        _abi.RZ_NODE_CLIP, _abi.RZ_NODE_STROKE)
 
     def __init__(self, kind, path=None, paint=None, fill_rule=FillRule.NonZero, children=(),
-                 child=None, tr=None, opacity=1.0, units=0):
+                 child=None, tr=None, opacity=1.0, units=0, style=None):
         self.kind = kind
         self.path, self.paint, self.fill_rule = path, paint, int(fill_rule)
         self.children = list(children)
@@ -765,10 +820,19 @@ class Scene:
         self.tr = tr
         self.opacity = float(opacity)
         self.units = int(units)
+        self.style = style
 
     @staticmethod
     def fill(path, paint, fill_rule=FillRule.NonZero):
         return Scene(Scene.FILL, path=path, paint=paint, fill_rule=fill_rule)
+
+    @staticmethod
+    def stroke(path, paint, style):
+        """Scene::stroke (src/scene.rs:98-101): the path stroked with `style`
+        (a StrokeStyle, or a width) and filled with the non-zero rule."""
+        if not isinstance(style, StrokeStyle):
+            style = StrokeStyle(style)
+        return Scene(Scene.STROKE, path=path, paint=paint, style=style)
 
     @staticmethod
     def group(children):
@@ -808,8 +872,10 @@ class Scene:
         (or its parsed value) of {"type": "fill" | "group" | "transform" |
         "opacity" | "clip", ...} nodes. Paths and transforms are SVG strings,
         colours "#rrggbb[aa][/alpha]" or names (`colors` adds names),
-        gradients {"type": "linear-gradient" | "radial-gradient", ...}.
-        Stroke nodes are refused (NotImplementedError)."""
+        gradients {"type": "linear-gradient" | "radial-gradient", ...}; a
+        "stroke" node carries its StrokeStyle flattened: "width", "line_join"
+        ({"miter": limit} | "bevel" | "round", default {"miter": 4}),
+        "line_cap" ("butt" | "square" | "round", default "butt")."""
         import json
         if isinstance(doc, (str, bytes)):
             doc = json.loads(doc)
@@ -871,6 +937,10 @@ class Scene:
             n.opacity = sc.opacity
             if sc.kind == Scene.FILL:
                 n.path, n.paint = path_id(sc.path), paint_id(sc.paint)
+            elif sc.kind == Scene.STROKE:
+                n.path, n.paint = path_id(sc.path), paint_id(sc.paint)
+                n.stroke_width, n.miter_limit = sc.style.width, sc.style.miter_limit
+                n.line_join, n.line_cap = sc.style.line_join, sc.style.line_cap
             elif sc.kind == Scene.GROUP:
                 ids = [visit(c, depth + 1) for c in sc.children]
                 n.first, n.count = len(children), len(ids)
@@ -998,8 +1068,28 @@ def _scene_of_json(v, colors, depth=0):
         return Scene(Scene.CLIP, path=Path.from_svg(v["clip"]), fill_rule=_json_rule(v.get("fill_rule")),
                      units=_json_units(v.get("units")), child=_scene_of_json(v["child"], colors, depth + 1))
     if t == "stroke":
-        raise NotImplementedError("stroke nodes are not supported")
+        return Scene(Scene.STROKE, path=Path.from_svg(v["path"]), paint=_json_paint(v["paint"], colors),
+                     style=_json_stroke_style(v))
     raise ValueError(f"unknown scene node type: {t!r}")
+
+
+def _json_stroke_style(v):
+    """StrokeStyle as serde reads it (src/path.rs:70-136, flattened into the
+    Stroke node): LineJoin externally tagged, lower case."""
+    j = v.get("line_join", {"miter": 4.0})
+    if isinstance(j, dict) and list(j) == ["miter"]:
+        join, limit = LineJoin.Miter, float(j["miter"])
+    elif j == "bevel":
+        join, limit = LineJoin.Bevel, 4.0
+    elif j == "round":
+        join, limit = LineJoin.Round, 4.0
+    else:
+        raise ValueError(f"unknown line join: {j!r}")
+    caps = {"butt": LineCap.Butt, "square": LineCap.Square, "round": LineCap.Round}
+    c = v.get("line_cap", "butt")
+    if c not in caps:
+        raise ValueError(f"unknown line cap: {c!r}")
+    return StrokeStyle(float(v["width"]), join, caps[c], limit)
 
 
 def _tune_dict(r):

@@ -22,6 +22,8 @@ RZ_PARAM_FLAT_LDS, RZ_PARAM_BIN_LDS, RZ_PARAM_FINE_LDS, RZ_PARAM_SCATTER_WAVES, 
 RZ_PARAM_SCENE_WAVES, RZ_PARAM_SCENE_QUEUE, RZ_PARAM_GRAPH_GEN = 11, 12, 13
 RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
 RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
+RZ_JOIN_MITER, RZ_JOIN_BEVEL, RZ_JOIN_ROUND = 0, 1, 2
+RZ_CAP_BUTT, RZ_CAP_SQUARE, RZ_CAP_ROUND = 0, 1, 2
 RZ_PAINT_SOLID, RZ_PAINT_LINEAR, RZ_PAINT_RADIAL = 0, 1, 2
 
 
@@ -38,6 +40,15 @@ class rz_path(C.Structure):
         ("points", C.POINTER(C.c_double)),
         ("subpaths", C.POINTER(C.c_uint32)),
         ("closed", C.POINTER(C.c_uint8)),
+    ]
+
+
+class rz_stroke_style(C.Structure):
+    _fields_ = [
+        ("width", C.c_double),
+        ("miter_limit", C.c_double),
+        ("line_join", C.c_int32),
+        ("line_cap", C.c_int32),
     ]
 
 
@@ -77,13 +88,15 @@ class rz_atlas_slot(C.Structure):
 
 
 RZ_NODE_FILL, RZ_NODE_GROUP, RZ_NODE_TRANSFORM, RZ_NODE_OPACITY, RZ_NODE_CLIP = 0, 1, 2, 3, 4
+RZ_NODE_STROKE = 5
 
 
 class rz_scene_node(C.Structure):
     _fields_ = [("kind", C.c_int32), ("fill_rule", C.c_int32), ("units", C.c_int32),
                 ("path", C.c_uint32), ("paint", C.c_uint32), ("child", C.c_uint32),
                 ("first", C.c_uint32), ("count", C.c_uint32), ("tr", C.c_double * 6),
-                ("opacity", C.c_double)]
+                ("opacity", C.c_double), ("stroke_width", C.c_double),
+                ("miter_limit", C.c_double), ("line_join", C.c_int32), ("line_cap", C.c_int32)]
 
 
 class rz_tune_result(C.Structure):
@@ -116,6 +129,7 @@ PROTOTYPES = [
     ("rz_stream_wait_ctx", C.c_int, [c_ctx, C.c_void_p]),
     ("rz_svg_path_parse", C.c_int, [C.c_char_p, C.c_size_t, P(C.c_void_p), P(C.c_size_t)]),
     ("rz_path_buf_view", C.c_int, [C.c_void_p, P(rz_path)]),
+    ("rz_path_stroke", C.c_int, [C.c_void_p, P(rz_path), rz_stroke_style, P(C.c_void_p)]),
     ("rz_svg_transform_parse", C.c_int, [C.c_char_p, C.c_size_t, P(rz_transform), P(C.c_size_t)]),
     ("rz_path_buf_free", C.c_int, [C.c_void_p]),
     ("rz_flatten", C.c_int, [c_ctx, P(rz_path), rz_transform, C.c_double, C.c_int,

@@ -21,6 +21,8 @@
 #include "rz_backward.hpp"
 #include "rz_encode.hpp"
 #include "rz_tree.hpp"
+#include "rz_stroke_kernels.hpp"
+#include "../../include/rasterize.hpp"
 
 namespace {
 
@@ -151,6 +153,7 @@ struct rz_ctx {
   DBuf list_buf;  // binning: record list (records, tiles) and event list
   DBuf wave_lines;  // per flatten wave: its lines' first index and count
   DBuf atlas_tiles;  // per atlas tile: what the fine stage needs (AtlasTile)
+  DBuf sk_work, sk_out;  // rz_path_stroke: input path and work arrays, output path
   uint32_t rec_cap = 0, ev_cap = 0;
   uint32_t part_cap = 0;  // overflow record-list entries per part
   rzk::Status* host_status = nullptr;  // pinned, two blocks (the copies of acc[0], acc[1])
@@ -1399,7 +1402,7 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   a.lw = lw;
   a.lh = lh;
   a.tiles_x = ctx_x;
-  a.queue = nullptr;
+  uint32_t* queue = nullptr;
   a.order = nullptr;
   a.n_tiles = (uint32_t)(ctx_x * cty);
   // no layer (the adjoint's own tile pass recomposites): the binned state only
@@ -1413,19 +1416,26 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
       int* cnt = (int*)(q + 64);
       RZ_LAUNCH(ctx, "k_scene_order", rzk::k_scene_order, dim3(1), dim3(rzk::kOrderBlock),
                 (const int4*)ctx->sc_rects.p, s->n, ctx_x, cty, cnt, ord, q, (const rzk::Status*)st);
-      a.queue = q;
+      queue = q;
       a.order = ord;
       // persistent: as many waves as the chip holds of this kernel
       const uint32_t resident = (uint32_t)ctx->n_cus * 4u * (uint32_t)std::max(ctx->cur.scene_waves, 5);
       fgrid = dim3(std::min<uint32_t>(a.n_tiles, resident));
     }
     const int waves = ctx->cur.scene_waves;
-    if (waves == 8)
-      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<8>), fgrid, dim3(64), ctx->cur.fine_lds, a);
-    else if (waves == 1)
-      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<1>), fgrid, dim3(64), ctx->cur.fine_lds, a);
-    else
-      RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<6>), fgrid, dim3(64), ctx->cur.fine_lds, a);
+#define RZ_SCENE_FINE(W, Q)                                                                \
+  RZ_LAUNCH_LDS(ctx, "k_scene_fine", (rzk::k_scene_fine<W, Q>), fgrid, dim3(64), ctx->cur.fine_lds, \
+                a, queue)
+    if (queue) {
+      if (waves == 8) RZ_SCENE_FINE(8, true);
+      else if (waves == 1) RZ_SCENE_FINE(1, true);
+      else RZ_SCENE_FINE(6, true);
+    } else {
+      if (waves == 8) RZ_SCENE_FINE(8, false);
+      else if (waves == 1) RZ_SCENE_FINE(1, false);
+      else RZ_SCENE_FINE(6, false);
+    }
+#undef RZ_SCENE_FINE
   } else {
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64), (const rzk::Status*)st,
               ctx->acc_run);
@@ -1873,6 +1883,7 @@ struct TreeBuild {
   std::vector<TreeNode> out;
   int depth = 0;
   bool bad = false;
+  const std::vector<uint32_t>* stroked = nullptr;  // Stroke node -> its stroked path's index
 
   const rz_scene_node* node(uint32_t id) {
     if (id >= n_nodes || depth > 256) {
@@ -1913,6 +1924,9 @@ struct TreeBuild {
       case RZ_NODE_OPACITY:
         have = scene_bbox(n->child, tr, res);
         break;
+      case RZ_NODE_STROKE:  // the unstroked path (src/scene.rs:158)
+        have = n->path < n_paths && rzh::path_bbox(paths[n->path], tr, res);
+        break;
       case RZ_NODE_CLIP: {
         rzh::Tr ctr = tr;
         if (n->units == RZ_UNITS_BBOX) {
@@ -1951,16 +1965,22 @@ struct TreeBuild {
     t.has_view = view != nullptr;
     if (view) t.view = *view;
     switch (n->kind) {
-      case RZ_NODE_FILL: {
+      case RZ_NODE_FILL:
+      case RZ_NODE_STROKE: {
+        // a Stroke is the Fill of its stroked path, non-zero rule
+        // (src/scene.rs:297-306)
+        const bool stroke = n->kind == RZ_NODE_STROKE;
+        const uint32_t path = stroke ? (stroked ? (*stroked)[id] : ~0u) : n->path;
         rzh::BBox pb;
-        if (n->path >= n_paths || n->paint >= n_paints) {
+        if (path >= n_paths || n->paint >= n_paints) {
           bad = true;
           break;
         }
-        if (!rzh::path_bbox(paths[n->path], tr, &pb) || !view_apply(pb, &t.bbox)) break;
-        t.path = n->path;
+        if (!rzh::path_bbox(paths[path], tr, &pb) || !view_apply(pb, &t.bbox)) break;
+        t.kind = RZ_NODE_FILL;
+        t.path = path;
         t.paint = n->paint;
-        t.rule = n->fill_rule;
+        t.rule = stroke ? RZ_NONZERO : n->fill_rule;
         out.push_back(t);
         r = (int)out.size() - 1;
         break;
@@ -2795,6 +2815,175 @@ rz_status rz_flatten(rz_ctx* ctx, const rz_path* path, rz_transform tr, double f
   return RZ_OK;
 }
 
+}  // extern "C"
+
+// Owned path buffers (rz_svg.cpp)
+rz_path_buf* rz_path_buf_adopt(rz::Path&& p);
+
+namespace {
+// Consecutive sub-buffers of one device buffer, 256-byte aligned.
+struct Carve {
+  char* base;
+  size_t at = 0;
+  template <typename T>
+  T* take(size_t n) {
+    T* p = (T*)(base + at);
+    at += (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return p;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+rz_status rz_path_stroke(rz_ctx* ctx, const rz_path* path, rz_stroke_style style,
+                         rz_path_buf** out) {
+  if (!ctx || !out) return RZ_E_ARG;
+  *out = nullptr;
+  RZ_TRY(validate_path(ctx, path));
+  if (style.line_join < RZ_JOIN_MITER || style.line_join > RZ_JOIN_ROUND ||
+      style.line_cap < RZ_CAP_BUTT || style.line_cap > RZ_CAP_ROUND)
+    return fail(ctx, RZ_E_ARG, "unknown line join or line cap");
+  const uint32_t ns = path->n_segments, nsub = path->n_subpaths;
+  if ((uint64_t)ns * 2 >= 0x7fffffffu) return fail(ctx, RZ_E_ARG, "too many segments to stroke");
+  rz::Path res;
+  if (ns == 0) {
+    *out = rz_path_buf_adopt(std::move(res));
+    return RZ_OK;
+  }
+  const uint32_t npts = path->point_offsets[ns];
+  const uint32_t n_items = 2 * ns, n_runs = 2 * nsub;
+  const uint32_t nblk = (n_items + rzk::kStrokeBlock - 1) / rzk::kStrokeBlock;
+  std::vector<uint32_t> sub_of(ns);
+  for (uint32_t i = 0; i < nsub; ++i)
+    for (uint32_t k = path->subpaths[i]; k < path->subpaths[i + 1]; ++k) sub_of[k] = i;
+  // device layout: the input path, then the work arrays
+  size_t need = 0;
+  {
+    Carve c{nullptr};
+    c.take<uint8_t>(ns);
+    c.take<uint32_t>(ns + 1);
+    c.take<double>(2 * (size_t)npts);
+    c.take<uint32_t>(nsub + 1);
+    c.take<uint8_t>(nsub);
+    c.take<uint32_t>(ns);
+    c.take<rzs::Seg>(n_items);
+    c.take<rzs::Seg>(n_items);
+    for (int k = 0; k < 3; ++k) c.take<uint32_t>(n_items);  // pieces, piece_pts, incl
+    c.take<int32_t>(nblk);
+    c.take<int32_t>(n_runs);
+    c.take<uint32_t>(n_runs);
+    for (int k = 0; k < 4; ++k) c.take<uint32_t>(n_items + 1);  // counts and their sums
+    c.take<uint32_t>(n_runs + 1);
+    c.take<uint32_t>(4);
+    need = c.at;
+  }
+  RZ_TRY(ensure(ctx, ctx->sk_work, need));
+  Carve c{(char*)ctx->sk_work.p};
+  rzs::Input in;
+  uint8_t* d_kinds = c.take<uint8_t>(ns);
+  uint32_t* d_off = c.take<uint32_t>(ns + 1);
+  double* d_pts = c.take<double>(2 * (size_t)npts);
+  uint32_t* d_subs = c.take<uint32_t>(nsub + 1);
+  uint8_t* d_closed = c.take<uint8_t>(nsub);
+  uint32_t* d_sub_of = c.take<uint32_t>(ns);
+  hipStream_t s = ctx->stream;
+  RZ_HIP(ctx, hipMemcpyAsync(d_kinds, path->kinds, ns, hipMemcpyHostToDevice, s));
+  RZ_HIP(ctx, hipMemcpyAsync(d_off, path->point_offsets, (ns + 1) * 4, hipMemcpyHostToDevice, s));
+  RZ_HIP(ctx, hipMemcpyAsync(d_pts, path->points, 16 * (size_t)npts, hipMemcpyHostToDevice, s));
+  RZ_HIP(ctx, hipMemcpyAsync(d_subs, path->subpaths, (nsub + 1) * 4, hipMemcpyHostToDevice, s));
+  RZ_HIP(ctx, hipMemcpyAsync(d_closed, path->closed, nsub, hipMemcpyHostToDevice, s));
+  RZ_HIP(ctx, hipMemcpyAsync(d_sub_of, sub_of.data(), ns * 4, hipMemcpyHostToDevice, s));
+  in.kinds = d_kinds;
+  in.pt_off = d_off;
+  in.pts = d_pts;
+  in.subs = d_subs;
+  in.closed = d_closed;
+  in.sub_of = d_sub_of;
+  in.n_items = n_items;
+  in.st.width = style.width;
+  in.st.miter_limit = style.miter_limit;
+  in.st.join = style.line_join;
+  in.st.cap = style.line_cap;
+  rzk::StrokeBufs b;
+  b.first = c.take<rzs::Seg>(n_items);
+  b.last = c.take<rzs::Seg>(n_items);
+  b.pieces = c.take<uint32_t>(n_items);
+  b.piece_pts = c.take<uint32_t>(n_items);
+  b.incl = c.take<int32_t>(n_items);
+  b.blk = c.take<int32_t>(nblk);
+  b.run_first = c.take<int32_t>(n_runs);
+  b.run_has = c.take<uint32_t>(n_runs);
+  b.cnt_seg = c.take<uint32_t>(n_items + 1);
+  b.cnt_pts = c.take<uint32_t>(n_items + 1);
+  uint32_t* seg_off = c.take<uint32_t>(n_items + 1);
+  uint32_t* pt_off = c.take<uint32_t>(n_items + 1);
+  uint32_t* run_idx = c.take<uint32_t>(n_runs + 1);
+  uint32_t* tot = c.take<uint32_t>(4);
+  const dim3 grid(nblk), blk(rzk::kStrokeBlock);
+  RZ_LAUNCH(ctx, "k_stroke_pieces", rzk::k_stroke_pieces, grid, blk, in, b);
+  RZ_LAUNCH(ctx, "k_stroke_blocks", rzk::k_stroke_blocks, dim3(1), dim3(rzk::kBlock), b.blk, nblk);
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3(blocks_for(n_runs)), dim3(rzk::kBlock),
+            (uint32_t*)b.run_first, (size_t)n_runs, ~0u);
+  RZ_LAUNCH(ctx, "k_fill_u32", rzk::k_fill_u32, dim3(blocks_for(n_runs)), dim3(rzk::kBlock),
+            b.run_has, (size_t)n_runs, 0u);
+  RZ_LAUNCH(ctx, "k_stroke_runs", rzk::k_stroke_runs, grid, blk, in, b);
+  RZ_LAUNCH(ctx, "k_stroke_count", rzk::k_stroke_count, grid, blk, in, b);
+  RZ_TRY(check_launch(ctx, "stroke count"));
+  RZ_TRY(scan_u32(ctx, b.cnt_seg, seg_off, n_items, nullptr, tot + 0));
+  RZ_TRY(scan_u32(ctx, b.cnt_pts, pt_off, n_items, nullptr, tot + 1));
+  RZ_TRY(scan_u32(ctx, b.run_has, run_idx, n_runs, nullptr, tot + 2));
+  uint32_t h_tot[4] = {0, 0, 0, 0};
+  RZ_HIP(ctx, hipMemcpyAsync(h_tot, tot, 12, hipMemcpyDeviceToHost, s));
+  RZ_HIP(ctx, hipStreamSynchronize(s));
+  const uint32_t o_seg = h_tot[0], o_pts = h_tot[1], o_sub = h_tot[2];
+  size_t need_out = 0;
+  {
+    Carve co{nullptr};
+    co.take<uint8_t>(o_seg);
+    co.take<uint32_t>(o_seg + 1);
+    co.take<double>(2 * (size_t)o_pts);
+    co.take<uint32_t>(o_sub + 1);
+    co.take<uint8_t>(o_sub);
+    need_out = co.at;
+  }
+  RZ_TRY(ensure(ctx, ctx->sk_out, need_out));
+  Carve co{(char*)ctx->sk_out.p};
+  rzk::StrokeOut o;
+  o.seg_off = seg_off;
+  o.pt_off = pt_off;
+  o.run_idx = run_idx;
+  o.kinds = co.take<uint8_t>(o_seg);
+  o.seg_pts = co.take<uint32_t>(o_seg + 1);
+  o.pts = co.take<double>(2 * (size_t)o_pts);
+  o.subs = co.take<uint32_t>(o_sub + 1);
+  o.closed = co.take<uint8_t>(o_sub);
+  RZ_LAUNCH(ctx, "k_stroke_emit", rzk::k_stroke_emit, grid, blk, in, b, o);
+  RZ_TRY(check_launch(ctx, "stroke emit"));
+  res.kinds.resize(o_seg);
+  res.point_offsets.resize((size_t)o_seg + 1);
+  res.points.resize(2 * (size_t)o_pts);
+  res.subpaths.resize(o_sub ? (size_t)o_sub + 1 : 0);
+  res.closed.resize(o_sub);
+  if (o_seg) {
+    RZ_HIP(ctx, hipMemcpyAsync(res.kinds.data(), o.kinds, o_seg, hipMemcpyDeviceToHost, s));
+    RZ_HIP(ctx, hipMemcpyAsync(res.point_offsets.data(), o.seg_pts, (size_t)o_seg * 4,
+                               hipMemcpyDeviceToHost, s));
+    RZ_HIP(ctx, hipMemcpyAsync(res.points.data(), o.pts, 16 * (size_t)o_pts, hipMemcpyDeviceToHost,
+                               s));
+  }
+  if (o_sub) {
+    RZ_HIP(ctx, hipMemcpyAsync(res.subpaths.data() + 1, o.subs + 1, (size_t)o_sub * 4,
+                               hipMemcpyDeviceToHost, s));
+    RZ_HIP(ctx, hipMemcpyAsync(res.closed.data(), o.closed, o_sub, hipMemcpyDeviceToHost, s));
+  }
+  RZ_HIP(ctx, hipStreamSynchronize(s));
+  res.point_offsets[o_seg] = o_pts;
+  if (o_sub) res.subpaths[0] = 0;
+  *out = rz_path_buf_adopt(std::move(res));
+  return RZ_OK;
+}
+
 rz_status rz_mask_device(rz_ctx* ctx, const rz_dpath* d, rz_transform tr, double flatness,
                          int fill_rule, int edge_mode, size_t width, size_t height, float* d_out,
                          size_t row_stride) {
@@ -3175,7 +3364,35 @@ rz_status rz_render_tree(rz_ctx* ctx, const rz_path* const* paths, size_t n_path
   if (edge_mode < RZ_EDGE_MASK || edge_mode > RZ_EDGE_NONE) return fail(ctx, RZ_E_ARG, "bad edge mode");
   for (size_t i = 0; i < n_paths; ++i) RZ_TRY(validate_path(ctx, paths ? paths[i] : nullptr));
   for (int k = 0; k < 4; ++k) layer_xywh[k] = 0;
-  TreeBuild tb{nodes, n_nodes, children, paths, n_paths, n_paints, {}};
+  // Stroke nodes: their paths stroked on the GPU, appended to the path list
+  std::vector<const rz_path*> all(paths, paths + n_paths);
+  std::vector<uint32_t> stroked(n_nodes, ~0u);
+  struct Owned {
+    std::vector<rz_path_buf*> bufs;
+    ~Owned() {
+      for (rz_path_buf* b : bufs) rz_path_buf_free(b);
+    }
+  } owned;
+  std::vector<rz_path> views;
+  size_t n_strokes = 0;
+  for (size_t i = 0; i < n_nodes; ++i) n_strokes += nodes[i].kind == RZ_NODE_STROKE;
+  views.reserve(n_strokes);  // `all` points into it
+  for (size_t i = 0; i < n_nodes; ++i) {
+    if (nodes[i].kind != RZ_NODE_STROKE) continue;
+    if (nodes[i].path >= n_paths) return fail(ctx, RZ_E_ARG, "malformed scene tree (stroke path index)");
+    const rz_stroke_style st{nodes[i].stroke_width, nodes[i].miter_limit, nodes[i].line_join,
+                             nodes[i].line_cap};
+    rz_path_buf* b = nullptr;
+    RZ_TRY(rz_path_stroke(ctx, paths[nodes[i].path], st, &b));
+    owned.bufs.push_back(b);
+    rz_path v;
+    rz_path_buf_view(b, &v);
+    views.push_back(v);
+    all.push_back(&views.back());
+    stroked[i] = (uint32_t)all.size() - 1;
+  }
+  TreeBuild tb{nodes, n_nodes, children, all.data(), all.size(), n_paints, {}};
+  tb.stroked = &stroked;
   rzh::BBox vb;
   if (view) {
     for (int k = 0; k < 4; ++k)

@@ -118,13 +118,13 @@ __global__ __launch_bounds__(64) void k_scene_bwd_tiles(BwdTileArgs b) {
       const uint32_t ee = (uint32_t)__builtin_amdgcn_readlane((int)lee, src);
       const bool local = rb != re || eb != ee;
       const int64_t bd = (int64_t)a.bd[t] << rzg::kFixShift;
-      int64_t wl[4] = {bd, bd, bd, bd};
-      if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, txl, lane, wl);
+      if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, txl, lane);
       const int xs = txl * 16 - P.g.phx + col;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * k + (lane >> 4);
-        int64_t w = wl[k];
+        int64_t w = bd;
+        if (local) w += tile_local(L, k, lane);
         const int ys = tyl * 16 - P.g.phy + r;
         float W = 0.0f;
         const size_t e = (size_t)t * 256 + (size_t)(r * 16 + col);

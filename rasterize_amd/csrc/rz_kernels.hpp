@@ -1402,12 +1402,14 @@ __device__ inline void tile_records(const float4* recs, uint32_t rb, uint32_t re
   }
 }
 
-// The LDS of one tile's coverage (2^-20 fixed point, int32).
+// The LDS of one tile's coverage (2^-20 fixed point, int32), and the int64
+// per-pixel sums of a tile accumulated in chunks (lane-major: [k][lane]).
 struct TileLDS {
   int acc[16 * kAccStride];
   int rowd[18];
   RowStage rs[64];
   int marks[64];
+  long long w64[4 * 64];
 };
 
 // The int32 accumulation is exact while a tile's local winding (its records'
@@ -1445,14 +1447,18 @@ __device__ inline void chunk_build(TileLDS& L, const float4* recs, uint32_t rb, 
 }
 
 // The local winding of one tile (records [rb, re), the band's events [eb, ee))
-// at the lane's four pixels, rows 4k + lane / 16, column lane % 16, into
-// w[0..3] (added). Every lane of the wave takes part (the row scans are DPP).
+// at each lane's four pixels (rows 4k + lane / 16, column lane % 16) into
+// L.w64[k * 64 + lane], which tile_local reads back: the per-pixel sums stay in
+// LDS, not in registers, while records are deposited. A narrow tile is one
+// int32 chunk; a larger one is summed chunk by chunk. Every lane of the wave
+// takes part (the row scans are DPP).
 __device__ inline void tile_coverage(TileLDS& L, const float4* recs, uint32_t rb, uint32_t re,
                                      const int4* events, uint32_t eb, uint32_t ee, int tx,
-                                     int lane, int64_t w[4]) {
+                                     int lane) {
   const int col = lane & 15;
   const bool narrow = tile_narrow(re - rb, ee - eb);
   uint32_t r0 = rb, e0 = eb;
+  bool first = true;
   do {
     const uint32_t r1 = narrow ? re : min(re, r0 + kChunkRecords);
     const uint32_t e1 = narrow ? ee : min(ee, e0 + kChunkEvents);
@@ -1460,13 +1466,22 @@ __device__ inline void tile_coverage(TileLDS& L, const float4* recs, uint32_t rb
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const int row = 4 * k + (lane >> 4);
-      w[k] += (int64_t)(int32_t)(L.rowd[row] + row_scan16(L.acc[row * kAccStride + col]));
+      const long long v = (int32_t)(L.rowd[row] + row_scan16(L.acc[row * kAccStride + col]));
+      L.w64[k * 64 + lane] = first ? v : L.w64[k * 64 + lane] + v;
     }
+    first = false;
     r0 = r1;
     e0 = e1;
-    if (r0 < re || e0 < ee) __syncthreads();  // the next chunk clears the LDS
+    // the next chunk clears the LDS; after the last, the barrier keeps the
+    // caller's later uniform global loads scalar (the compiler takes an LDS
+    // store not followed by a barrier as a possible clobber of them)
+    __syncthreads();
   } while (r0 < re || e0 < ee);
 }
+
+// The local winding at the lane's pixel k after tile_coverage (each lane reads
+// the slots it wrote: no barrier).
+__device__ inline int64_t tile_local(const TileLDS& L, int k, int lane) { return L.w64[k * 64 + lane]; }
 
 struct FineArgs {
   const uint32_t* tile_off;
@@ -1531,10 +1546,7 @@ struct FillOut {
 // four passes of 16-lane DPP row scans (one pixel row per 16 lanes) and
 // coalesced stores.
 template <int RULE, class Out>
-__global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
-  if (a.st->err || a.st->overflow) return;
-  __shared__ TileLDS L;
+__device__ inline void fine_tile(const FineArgs& a, const Out& o, TileLDS& L) {
   const int lane = threadIdx.x;
   const uint32_t tile = blockIdx.x;
   const int tx = (int)(tile % (uint32_t)a.tiles_x), ty = (int)(tile / (uint32_t)a.tiles_x);
@@ -1544,14 +1556,14 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
   // backdrop (plus the right-edge column) everywhere: no LDS work
   const bool local = rb != re || eb != ee;
   const int64_t bd = (int64_t)a.bd[tile] << rzg::kFixShift;
-  int64_t wl[4] = {bd, bd, bd, bd};
-  if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, tx, lane, wl);
+  if (local) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, tx, lane);
   const int col = lane & 15;
   const int x = tx * 16 + col;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int row = 4 * k + (lane >> 4);
-    int64_t w = wl[k];
+    int64_t w = bd;
+    if (local) w += tile_local(L, k, lane);
     const int y = ty * 16 + row;
     if (x < a.W && y < a.H) {
       if (x >= a.redge_col) {
@@ -1561,6 +1573,16 @@ __global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
       o.put(x, y, alpha_fix<RULE>(w));
     }
   }
+}
+
+// The status merge (one thread of block 0) comes after the tile's work: a
+// global store ahead of the loads would keep the compiler from reading the
+// tile's uniform data with scalar loads.
+template <int RULE, class Out>
+__global__ __launch_bounds__(kFineWave) void k_fine(FineArgs a, Out o) {
+  __shared__ TileLDS L;
+  if (!(a.st->err || a.st->overflow)) fine_tile<RULE>(a, o, L);
+  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
 }
 
 }  // namespace rzk

@@ -309,7 +309,6 @@ struct SceneFineArgs {
   size_t row_stride;  // pixels
   int lw, lh;         // layer size
   int tiles_x;        // canvas tiles per row
-  uint32_t* queue;    // queued form: the next entry of `order` to take, or null
   const uint32_t* order;  // queued form: canvas tiles, costliest first
   uint32_t n_tiles;
 };
@@ -378,13 +377,13 @@ __device__ inline void scene_tile(const SceneFineArgs& a, TileLDS& L, uint32_t t
       const PathInfo& P = a.pi[q];
       const bool local = qrb != qre || qeb != qee;
       const int64_t bd = (int64_t)qbd << rzg::kFixShift;
-      int64_t wl[4] = {bd, bd, bd, bd};
-      if (local) tile_coverage(L, a.recs, qrb, qre, a.events, qeb, qee, qtxl, lane, wl);
+      if (local) tile_coverage(L, a.recs, qrb, qre, a.events, qeb, qee, qtxl, lane);
       const int xs = qtxl * 16 - P.g.phx + col;  // sub-view pixel column
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * k + (lane >> 4);
-        int64_t w = wl[k];
+        int64_t w = bd;
+        if (local) w += tile_local(L, k, lane);
         const int ys = qtyl * 16 - P.g.phy + r;
         if (xs >= 0 && xs < P.g.W && ys >= 0 && ys < P.g.H) {
           if (xs >= P.redge_col) {
@@ -409,29 +408,35 @@ __device__ inline void scene_tile(const SceneFineArgs& a, TileLDS& L, uint32_t t
   }
 }
 
-// One wave per workgroup. Without a queue, workgroup i takes canvas tile i.
-// With one (a.queue), the grid is persistent (a few waves per SIMD) and each
-// wave takes tiles from the front of a.order, costliest first, until none is
-// left: the tiles with the most paths start first instead of last.
-// MINW: waves per SIMD the registers must allow (1: no bound; 6: 80 VGPRs,
-// no spills; 8: 64 VGPRs with a few spilled).
-template <int MINW>
-__global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a) {
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
-  if (a.st->err || a.st->overflow) return;
+// One wave per workgroup. Without a queue (QUEUE false), workgroup i takes
+// canvas tile i. With one, the grid is persistent (a few waves per SIMD) and
+// each wave takes tiles from the front of a.order, costliest first, until
+// none is left: the tiles with the most paths start first instead of last.
+// The queue head comes as its own restrict parameter: the compiler then
+// knows its atomics touch nothing the tiles read, and the tiles' uniform
+// loads stay scalar.
+// MINW: waves per SIMD the registers must allow (1: no bound; 6: 80 VGPRs;
+// 8: 64 VGPRs with some spilled).
+template <int MINW, bool QUEUE>
+__global__ __launch_bounds__(64, MINW) void k_scene_fine(SceneFineArgs a,
+                                                         uint32_t* __restrict__ queue) {
   __shared__ TileLDS L;
-  if (!a.queue) {
+  if (a.st->err || a.st->overflow) {
+  } else if (!QUEUE) {
     scene_tile(a, L, blockIdx.x);
-    return;
+  } else {
+    for (;;) {
+      uint32_t t = 0;
+      if (threadIdx.x == 0) t = atomicAdd(queue, 1u);
+      t = (uint32_t)__shfl((int)t, 0, 64);
+      if (t >= a.n_tiles) break;
+      __syncthreads();  // the previous tile's LDS reads are done
+      scene_tile(a, L, a.order[t]);
+    }
   }
-  for (;;) {
-    uint32_t t = 0;
-    if (threadIdx.x == 0) t = atomicAdd(a.queue, 1u);
-    t = (uint32_t)__shfl((int)t, 0, 64);
-    if (t >= a.n_tiles) break;
-    __syncthreads();  // the previous tile's LDS reads are done
-    scene_tile(a, L, a.order[t]);
-  }
+  // after the work: a global store ahead of the loads would keep the
+  // compiler from reading uniform path data with scalar loads
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.acc) merge_status(a.st, a.acc);
 }
 
 // The tile order of the queued form, in one workgroup of kOrderBlock
@@ -606,10 +611,7 @@ struct AtlasFineArgs {
 // One tile of one glyph per wave (from its AtlasTile); coverage as in k_fine,
 // written into the glyph's atlas view.
 template <int RULE>
-__global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
-  if (a.st->err || a.st->overflow) return;
-  __shared__ TileLDS L;
+__device__ inline void atlas_tile(const AtlasFineArgs& a, TileLDS& L) {
   const int lane = threadIdx.x;
   const uint32_t t = blockIdx.x;
   const AtlasTile T = a.tiles[t];
@@ -617,13 +619,13 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
   const uint32_t rb = T.rb, re = T.re, eb = T.eb, ee = T.ee;
   const int64_t bd = (int64_t)T.bd << rzg::kFixShift;
   const bool local_work = rb != re || eb != ee;
-  int64_t wl[4] = {bd, bd, bd, bd};
-  if (local_work) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, T.txl, lane, wl);
+  if (local_work) tile_coverage(L, a.recs, rb, re, a.events, eb, ee, T.txl, lane);
   const int col = lane & 15;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int r = 4 * k + (lane >> 4);
-    int64_t w = wl[k];
+    int64_t w = bd;
+    if (local_work) w += tile_local(L, k, lane);
     if (col < T.w && r < T.h) {
       if (col >= T.rcol) {
         const int64_t rr = a.redge[T.redge + (uint32_t)r];
@@ -632,6 +634,14 @@ __global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
       a.out[(size_t)(T.y + r) * a.row_stride + (size_t)(T.x + col)] = alpha_fix<RULE>(w);
     }
   }
+}
+
+// The status merge comes after the tile's work (see k_scene_fine).
+template <int RULE>
+__global__ __launch_bounds__(64) void k_atlas_fine(AtlasFineArgs a) {
+  __shared__ TileLDS L;
+  if (!(a.st->err || a.st->overflow)) atlas_tile<RULE>(a, L);
+  if (blockIdx.x == 0 && threadIdx.x == 0) merge_status(a.st, a.acc);
 }
 
 }  // namespace rzk

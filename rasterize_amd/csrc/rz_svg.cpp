@@ -340,6 +340,12 @@ rz_status rz_svg_transform_parse(const char* text, size_t len, rz_transform* out
   return RZ_OK;
 }
 
+}  // extern "C"
+
+rz_path_buf* rz_path_buf_adopt(rz::Path&& p) { return new rz_path_buf{std::move(p)}; }
+
+extern "C" {
+
 rz_status rz_path_buf_view(const rz_path_buf* p, rz_path* view) {
   if (!p || !view) return RZ_E_ARG;
   *view = p->path.view();

@@ -85,6 +85,9 @@ def lib():
             "orc_svg_transform": (i, [C.c_char_p, sz, P(d)]),
             "orc_svg_parse": (i, [C.c_char_p, sz, P(C.c_uint8), P(C.c_uint32), P(d), P(C.c_uint32),
                                   P(C.c_uint8), sz, sz, sz, P(sz), P(sz), P(sz)]),
+            "orc_path_stroke": (i, [P(orc_path), d, i, d, i, P(C.c_uint8), P(C.c_uint32), P(d),
+                                    P(C.c_uint32), P(C.c_uint8), sz, sz, sz, P(sz), P(sz), P(sz)]),
+            "orc_time_stroke": (d, [P(orc_path), d, i, d, i, i, P(sz)]),
             "orc_bin_replay": (i, [P(orc_path), P(d), d, i, sz, sz, P(C.c_uint32), P(C.c_float),
                                    sz, P(C.c_int32), P(C.c_float), sz, P(C.c_int32), P(sz),
                                    P(sz), i, i]),
@@ -381,6 +384,40 @@ def svg_parse(text):
                            ns.value, npt.value, len(subs), C.byref(ns), C.byref(npt),
                            C.byref(nsub)))
     return kinds, offs, pts, subs, closed
+
+
+JOINS = {"miter": 0, "bevel": 1, "round": 2}
+CAPS = {"butt": 0, "square": 1, "round": 2}
+
+
+def path_stroke(path, width, join="miter", miter_limit=4.0, cap="butt"):
+    """Path::stroke (src/path.rs:370-415) as the oracle restates it:
+    (kinds, point_offsets, points, subpaths, closed) arrays."""
+    L = lib()
+    op = opath(path)
+    j, c = JOINS[join], CAPS[cap]
+    ns, npt, nsub = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    _check(L.orc_path_stroke(C.byref(op), width, j, miter_limit, c, None, None, None, None, None,
+                             0, 0, 0, C.byref(ns), C.byref(npt), C.byref(nsub)))
+    kinds = np.zeros(ns.value, np.uint8)
+    offs = np.zeros(ns.value + 1, np.uint32)
+    pts = np.zeros(2 * npt.value)
+    subs = np.zeros(nsub.value + 1 if nsub.value else 0, np.uint32)
+    closed = np.zeros(nsub.value, np.uint8)
+    u8, u32 = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)
+    _check(L.orc_path_stroke(C.byref(op), width, j, miter_limit, c, kinds.ctypes.data_as(u8),
+                             offs.ctypes.data_as(u32), _dp(pts), subs.ctypes.data_as(u32),
+                             closed.ctypes.data_as(u8), ns.value, npt.value, len(subs),
+                             C.byref(ns), C.byref(npt), C.byref(nsub)))
+    return kinds, offs, pts, subs, closed
+
+
+def time_stroke(path, width, join="miter", miter_limit=4.0, cap="butt", reps=1):
+    """Seconds per Path::stroke call of the oracle, and the output's segment count."""
+    n = C.c_size_t()
+    t = lib().orc_time_stroke(C.byref(opath(path)), width, JOINS[join], miter_limit, CAPS[cap],
+                              reps, C.byref(n))
+    return t, n.value
 
 
 def svg_transform(text):

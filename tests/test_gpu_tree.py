@@ -158,3 +158,22 @@ def test_active_edge_rasterizer():
     rng = np.random.default_rng(11)
     sc = Scene(Scene.GROUP, children=[_random_tree(rng, 4, 192.0) for _ in range(3)])
     check(sc, I, (0.0, 0.0, 192.0, 192.0), what="generated active-edge", rast=ActiveEdgeRasterizer())
+
+
+def test_stroke_nodes():
+    """Stroke nodes (src/scene.rs:297-306): each path stroked by rz_path_stroke
+    and filled non-zero, inside transforms, opacities and a bounding-box clip
+    (whose box is the unstroked path's, src/scene.rs:158)."""
+    from rasterize_amd import LineCap, LineJoin, StrokeStyle
+    a = PathBuilder().move_to((6, 6)).cubic_to((60, 2), (64, 60), (10, 56)).line_to((30, 30)).build()
+    b = PathBuilder().move_to((20, 70)).quad_to((50, 40), (90, 80)).line_to((70, 20)).close().build()
+    sa = Scene.stroke(a, LinColor(0.8, 0.2, 0.1, 1.0), StrokeStyle(4.0, LineJoin.Round, LineCap.Round))
+    sb = Scene.stroke(b, GradLinear([(0.0, LinColor(0, 0, 1, 1)), (1.0, LinColor(0, 1, 0, 1))],
+                                    start=(20, 20), end=(90, 80)),
+                      StrokeStyle(2.5, LineJoin.Miter, LineCap.Square, miter_limit=10.0))
+    fill = Scene.fill(rect(30, 30, 70, 70), LinColor(0.3, 0.3, 0.3, 0.5))
+    tree = Scene.group([fill, sa.with_opacity(0.6), sb.with_transform(Transform.new_rotate(0.1))])
+    check(tree, Transform.new_scale(2.0, 2.0), what="strokes")
+    check(sa.with_clip(rect(0.1, 0.1, 0.7, 0.9), Units.BoundingBox), I, what="stroke bbox clip")
+    check(Scene.stroke(a, LinColor(1, 1, 1, 1), 3.0), I, (0.0, 0.0, 50.0, 40.0), (0, 0, 0, 1),
+          what="stroke view")

@@ -1,6 +1,6 @@
 """Scene trees on the CPU: the JSON scene format (Deserialize for Scene,
 src/scene.rs:13-63, 587-640; colours src/color.rs:96-141), and pins of the
-oracle's Transform / Opacity / Clip nodes (src/scene.rs:268-357, 397-459)
+oracle's Transform / Opacity / Clip / Stroke nodes (src/scene.rs:268-357, 397-459)
 against renders of flat scenes and masks composed independently here."""
 import json
 import os
@@ -10,8 +10,8 @@ import pytest
 
 import _oracle as O
 from _paths import rect
-from rasterize_amd import (FillRule, GradLinear, GradRadial, LinColor, Path, PathBuilder, Scene,
-                           Transform, Units)
+from rasterize_amd import (FillRule, GradLinear, GradRadial, LineCap, LineJoin, LinColor, Path,
+                           PathBuilder, Scene, StrokeStyle, Transform, Units)
 from rasterize_amd.color import SVG_COLORS, ColorError, parse_rgba
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -106,9 +106,26 @@ def test_json_scene_shapes():
     assert lc.children[2].child.paint.linear_colors is True
 
 
+def test_json_stroke_nodes():
+    # StrokeStyle flattened into the node (serde(flatten), src/scene.rs:30-37)
+    s = Scene.from_json({"type": "stroke", "width": 1.5, "paint": "#000000", "path": "M0,0 1,1"})
+    assert s.kind == Scene.STROKE and s.style.width == 1.5
+    assert s.style.line_join == LineJoin.Miter and s.style.miter_limit == 4.0
+    assert s.style.line_cap == LineCap.Butt
+    s = Scene.from_json({"type": "stroke", "width": 2, "paint": "#000000", "path": "M0,0 1,1",
+                         "line_join": {"miter": 7.5}, "line_cap": "square"})
+    assert s.style.miter_limit == 7.5 and s.style.line_cap == LineCap.Square
+    s = Scene.from_json({"type": "stroke", "width": 2, "paint": "#000000", "path": "M0,0 1,1",
+                         "line_join": "round", "line_cap": "round"})
+    assert s.style.line_join == LineJoin.Round and s.style.line_cap == LineCap.Round
+    with pytest.raises(ValueError):
+        Scene.from_json({"type": "stroke", "width": 1, "paint": "#000000", "path": "M0,0 1,1",
+                         "line_join": "sharp"})
+    with pytest.raises(ValueError):
+        Scene.from_json({"type": "stroke", "paint": "#000000", "path": "M0,0 1,1"})
+
+
 def test_json_scene_errors():
-    with pytest.raises(NotImplementedError):
-        Scene.from_json({"type": "stroke", "width": 1, "paint": "#000000", "path": "M0,0 1,1"})
     with pytest.raises(ValueError):
         Scene.from_json({"type": "fill", "paint": {"type": "conic"}, "path": "M0,0 1,1"})
     with pytest.raises(ValueError):
@@ -206,6 +223,28 @@ def test_oracle_clip_units_bbox_is_the_unit_square_of_the_child():
     # its right edge (the layer's last) empty, as the reference does
     assert a.shape[:2] == (16, 16) and oa == (16, 16)
     assert a[:, :15, 3].min() > 0.99 and a[:, 15, 3].max() == 0.0
+
+
+def test_oracle_stroke_node_is_the_fill_of_the_stroked_path():
+    """Stroke(path, paint, style) renders as the non-zero Fill of
+    Path::stroke(style) (src/scene.rs:297-306), while Scene::bbox takes the
+    unstroked path (src/scene.rs:158): a bounding-box clip over a stroke maps
+    onto the unstroked path's box."""
+    path = PathBuilder().move_to((6, 6)).cubic_to((40, 2), (44, 40), (10, 36)).line_to((20, 20)).build()
+    col = LinColor(0.2, 0.7, 0.4, 1.0)
+    style = StrokeStyle(3.0, LineJoin.Round, LineCap.Square)
+    stroked = Path(*O.path_stroke(path, 3.0, "round", 4.0, "square"))
+    got, org = O.scene_tree_render(Scene.stroke(path, col, style), I)
+    ref, rorg = O.scene_render([(stroked, col.abi(), 0)], I)
+    assert org == rorg and got.shape == ref.shape and got.size
+    assert np.abs(got - ref).max() <= 1e-7
+    # the bbox-units clip's unit square spans the unstroked path's box
+    bx0, by0, bx1, by1 = O.path_bbox(path, I)
+    unit = rect(0.0, 0.0, 1.0, 1.0)
+    user = rect(bx0, by0, bx1, by1)
+    a, oa = O.scene_tree_render(Scene.stroke(path, col, style).with_clip(unit, Units.BoundingBox), I)
+    b, ob = O.scene_tree_render(Scene.stroke(path, col, style).with_clip(user), I)
+    assert oa == ob and a.shape == b.shape and np.abs(a - b).max() <= 1e-6
 
 
 def test_oracle_disjoint_clip_is_the_empty_layer():
