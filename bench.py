@@ -31,7 +31,7 @@ def parse():
     ap.add_argument("--rule", choices=["nonzero", "evenodd"], default="nonzero")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scene", action="store_true", help="skip the scene measurements")
-    ap.add_argument("--workload", choices=["scene", "circles", "atlas", "diff"],
+    ap.add_argument("--workload", choices=["scene", "circles", "atlas", "diff", "stroke"],
                     help="only this measurement, printed as one JSON line (for per-kernel "
                          "profiles of one workload)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -201,6 +201,61 @@ def atlas_measure(torch, rz, lib, _abi, frames):
     }, gs
 
 
+def stroke_measure(torch, rz, lib, _abi, reps, cpu=True):
+    """Path::stroke of the 200k-segment random walk (the mask benchmark's path;
+    width 2, round joins and caps) on the GPU, host path in and out as
+    rz_path_stroke takes them, and the 4096x4096 mask of the stroked path:
+    the reference's "stroke" and "mask stroked" benches
+    (benches/rasterize_bench.rs:35-74) on that path."""
+    import time
+    ws = rz.workload_random_walk(2, 200_000, 4096.0)
+    path, tr = ws.paths[0], ws.transform
+    ctx = rz.Context(0)
+    style = rz.StrokeStyle(2.0, rz.LineJoin.Round, rz.LineCap.Round)
+    stroked = path.stroke(style, ctx)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        stroked = path.stroke(style, ctx)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    k = kernel_stats(lib, _abi, ctx, lambda: path.stroke(style, ctx), 2)
+    W = H = 4096
+    dp = rz.DevicePath(stroked, ctx)
+    out = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+
+    def run():
+        dp.mask_device(tr, W, H, out.data_ptr(), W)
+    settle(lib, _abi, ctx, run)
+    stream = torch.cuda.ExternalStream(lib.rz_stream(ctx.h))
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        run()
+    b.record(stream)
+    ctx.check(lib.rz_sync(ctx.h))
+    torch.cuda.synchronize()
+    mask_ms = a.elapsed_time(b) / reps
+    res = {
+        "workload": "Path::stroke of the 200k-segment random walk (width 2, round joins and caps), "
+                    "then its 4096x4096 mask (NonZero)",
+        "segments": path.segments_count(),
+        "stroked_segments": stroked.segments_count(),
+        "stroke_ms": round(ms, 3),
+        "stroke_msegs_s": round(path.segments_count() / ms / 1e3, 1),
+        "stroke_kernels_ms": {n: round(v[1], 4) for n, v in sorted(k.items())},
+        "mask_stroked_ms": round(mask_ms, 4),
+        "mask_stroked_mpix_s": round(W * H / mask_ms / 1e3, 1),
+    }
+    dp.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        t, n = O.time_stroke(path, 2.0, "round", 4.0, "round", reps=1)
+        res["cpu"] = {"stroke_ms": round(t * 1e3, 2), "cores": 1, "kind": "port",
+                      "sample": "the oracle's Path::stroke of the same path, one thread, once",
+                      "stroked_segments": n}
+    return res
+
+
 def diff_measure(torch, rz, steps):
     """Differentiable render: 256 closed paths x 32 cubics, 1024x1024 RGBA,
     L2 loss against a perturbed target, forward + adjoint backward + Adam;
@@ -302,6 +357,10 @@ def main():
         return
     if args.workload == "diff":
         print(json.dumps(diff_measure(torch, rz, args.diff_steps)), flush=True)
+        return
+    if args.workload == "stroke":
+        print(json.dumps(stroke_measure(torch, rz, lib, _abi, max(1, min(args.steps, 20)),
+                                        not args.no_cpu)), flush=True)
         return
 
     W = H = 4096

@@ -232,6 +232,22 @@ class Path:
     def segments_count(self):
         return int(self.kinds.size)
 
+    def extend(self, other):
+        """Extend<SubPath> for Path (src/path.rs:683-689): append the
+        subpaths of `other` (in place; returns self)."""
+        if other.kinds.size == 0:
+            return self
+        base_seg, base_pt = self.kinds.size, int(self.point_offsets[-1]) if self.kinds.size else 0
+        subs = self.subpaths if self.subpaths.size else np.zeros(1, np.uint32)
+        self.kinds = np.concatenate([self.kinds, other.kinds]).astype(np.uint8)
+        self.point_offsets = np.concatenate(
+            [self.point_offsets[:base_seg], other.point_offsets + base_pt]).astype(np.uint32)
+        self.points = np.concatenate([self.points[:2 * base_pt], other.points])
+        self.subpaths = np.concatenate([subs, other.subpaths[1:] + base_seg]).astype(np.uint32)
+        self.closed = np.concatenate([self.closed, other.closed]).astype(np.uint8)
+        self._abi = None
+        return self
+
     def stroke(self, style, ctx=None):
         """Path::stroke (src/path.rs:370-415) on the GPU (rz_path_stroke): a
         new path, the outline of this one offset by width / 2 on both sides,

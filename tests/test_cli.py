@@ -24,10 +24,42 @@ def test_arguments():
     assert (a["input"], a["output"]) == ("in.path", "out.png")
     assert cli.parse_args(["in.path"]) is None  # usage
     for bad in (["-w"], ["-f", "gif", "a", "b"], ["-rf", "0", "a", "b"], ["a", "b", "c"],
-                ["-s", "2", "a", "b"], ["-v", "1 2 3", "a", "b"]):
+                ["-s", "wide", "a", "b"], ["-v", "1 2 3", "a", "b"]):
         with pytest.raises((cli.ArgError, ValueError)):
             cli.parse_args(bad)
-    assert cli.main(["-s", "1", "a", "b"]) == 1
+    assert cli.main(["-s", "wide", "a", "b"]) == 1
+    a = cli.parse_args(["-s", "2.5", "-ro", "in.path", "out.bmp"])
+    assert a["stroke"] == 2.5 and a["outline"]
+
+
+def oracle_stroke(path, style):
+    join = {rz.LineJoin.Miter: "miter", rz.LineJoin.Bevel: "bevel", rz.LineJoin.Round: "round"}
+    cap = {rz.LineCap.Butt: "butt", rz.LineCap.Square: "square", rz.LineCap.Round: "round"}
+    return rz.Path(*O.path_stroke(path, style.width, join[style.line_join], style.miter_limit,
+                                  cap[style.line_cap]))
+
+
+def test_stroke_and_outline_scene():
+    """-s strokes the path (round joins and caps) before fitting; -ro adds the
+    outline fill: the stroked path, a disc per segment start, the stroked
+    control polygons of quads and cubics, and a disc at an open end
+    (examples/rasterize.rs:167-228, 248-262, 283-296)."""
+    path = rz.Path.from_svg("M2,2 L8,2 Q11,2 11,5 C11,8 8,8 5,8")
+    a = cli.parse_args(["-w", "64", "-s", "1.5", "in", "out"])
+    (w, h), scene, bg = cli.build_scene(path, a, stroke=oracle_stroke)
+    stroked = oracle_stroke(path, rz.StrokeStyle(1.5, rz.LineJoin.Round, rz.LineCap.Round))
+    fill = scene.child if scene.kind == rz.Scene.TRANSFORM else scene.children[-1].child
+    assert fill.kind == rz.Scene.FILL and np.array_equal(fill.path.points, stroked.points)
+    a = cli.parse_args(["-w", "64", "-ro", "in", "out"])
+    (w, h), scene, bg = cli.build_scene(path, a, stroke=oracle_stroke)
+    top = scene.children[-1]
+    assert top.kind == rz.Scene.GROUP and len(top.children) == 2
+    assert top.children[0].child.paint.abi().color[3] == pytest.approx(0.6)
+    out = top.children[1].path
+    # 1 stroked subpath, 3 start discs, the quad's control polygon (2 open
+    # legs + a ring: 3 strokes -> 1 + 1 + 2 = 4 subpaths), the cubic's (two
+    # legs, two rings: 2 + 4 = 6), and the end disc
+    assert len(out) == 1 + 3 + 4 + 6 + 1
 
 
 @pytest.mark.parametrize("w,h", [(0, 0), (64, 0), (0, 48), (100, 40)])
@@ -61,7 +93,8 @@ def test_checkerboard_cells():
 @pytest.mark.gpu
 @pytest.mark.parametrize("args", [["-w", "96"], ["-h", "80", "-bg", "#203040", "-fg", "#ffcc00/0.5"],
                                   ["-w", "64", "-t", "rotate(30) scale(1, 0.5)", "-ra"],
-                                  ["-w", "72", "-v", "2 0 12 14"]])
+                                  ["-w", "72", "-v", "2 0 12 14"], ["-w", "96", "-s", "0.4"],
+                                  ["-w", "120", "-ro"]])
 def test_rendered_file_matches_oracle(tmp_path, args):
     src = os.path.join(GOLDEN, "squirrel.path")
     out = str(tmp_path / "out.rgba")
@@ -71,7 +104,7 @@ def test_rendered_file_matches_oracle(tmp_path, args):
     got = np.frombuffer(data[8:], np.uint8).reshape(h, w, 4)
     a = cli.parse_args(args + [src, out])
     path = rz.Path.from_svg(open(src).read())
-    (W, H), scene, bg = cli.build_scene(path, a)
+    (W, H), scene, bg = cli.build_scene(path, a, stroke=oracle_stroke)
     assert (W, H) == (w, h)
     img, org = O.scene_tree_render(scene, rz.Transform.identity(), (0.0, 0.0, float(W), float(H)),
                                    tuple(bg), flatness=a["flatness"], active_edge=a["active_edge"])
