@@ -20,6 +20,7 @@ RZ_PARAM_SPLIT_QUAD, RZ_PARAM_SPLIT_CUBIC, RZ_PARAM_SPLIT_AUTO, RZ_PARAM_SCAN_DI
 RZ_PARAM_GRAPH, RZ_PARAM_GRAPH_REPLAYS = 4, 5
 RZ_PARAM_FLAT_LDS, RZ_PARAM_BIN_LDS, RZ_PARAM_FINE_LDS, RZ_PARAM_SCATTER_WAVES, RZ_PARAM_TUNED = 6, 7, 8, 9, 10
 RZ_PARAM_SCENE_WAVES, RZ_PARAM_SCENE_QUEUE, RZ_PARAM_GRAPH_GEN = 11, 12, 13
+RZ_PARAM_FUSED = 14
 RZ_EDGE_MASK, RZ_EDGE_ITER, RZ_EDGE_NONE = 0, 1, 2
 RZ_LINE, RZ_QUAD, RZ_CUBIC = 0, 1, 2
 RZ_JOIN_MITER, RZ_JOIN_BEVEL, RZ_JOIN_ROUND = 0, 1, 2

@@ -76,7 +76,8 @@ struct LaunchParams {
   int32_t scatter_log2 = 15;  // the scatter grid holds at most 2^v waves
   int32_t scene_waves = 6;    // k_scene_fine built for 1 (no bound), 6 or 8 waves per SIMD
   int32_t scene_queue = 0;    // 1: k_scene_fine persistent over a queue of tiles, costliest first
-  int32_t reserved = 0;       // keeps the size a multiple of 8: GraphKey, compared bytewise, has no padding
+  int32_t fused = 0;          // 1: flatten and count pass as one kernel (k_flat_bin); not tuned
+                              // (also keeps the size a multiple of 8: GraphKey has no padding)
 };
 static_assert(sizeof(LaunchParams) == 10 * sizeof(int32_t), "LaunchParams has padding");
 bool same_params(const LaunchParams& a, const LaunchParams& b) {
@@ -381,6 +382,7 @@ void select_params(rz_ctx* ctx, const TuneKey& key) {
   for (const TuneEntry& t : ctx->tuned)
     if (same_tune_key(t.key, key)) {
       ctx->cur = t.p;
+      ctx->cur.fused = ctx->lp.fused;  // the context's choice, not the tuner's
       return;
     }
 }
@@ -646,7 +648,7 @@ struct Pipe {
 rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, double fl,
                           const rzk::PathInfo* pi, const uint32_t* slot_path, rzk::Status* st,
                           uint32_t* line_fill, int dq, int dc, int mode, double H,
-                          rzk::LineBufs* lb) {
+                          const rzg::WrapClip& wrap, rzk::LineBufs* lb, bool launch = true) {
   const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq,
                  bc = d->view.blocks_cubic << dc;
   const uint32_t nblk = bl + bq + bc;
@@ -657,10 +659,11 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
   lb->st = st;
   lb->n_waves = nblk * (rzk::kBlock / 64);
   lb->ae_h = H;
+  lb->wrap = wrap;
   RZ_TRY(ensure(ctx, ctx->wave_lines, (size_t)std::max<uint32_t>(lb->n_waves, 1) * sizeof(uint2)));
   lb->wave_lines = (uint2*)ctx->wave_lines.p;
   const uint32_t fsub = rzk::kBlock / rzk::kFlatBlock;
-  if (nblk) {
+  if (nblk && launch) {
     // By default a single path's flatten runs at three waves per SIMD rather
     // than the four its registers allow: 4 KB more LDS per wave. Measured on
     // the 200k-segment path: 169 us at three, 181 us at four (the batched
@@ -698,7 +701,7 @@ uint32_t line_cap_for(size_t lines) {
 template <bool MULTI>
 rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
                             const rzk::PathInfo* pi, const rzk::GapArgs& ga, uint32_t* fill,
-                            rzk::BinBufs* b) {
+                            rzk::BinBufs* b, bool launch = true) {
   // one count-pass wave per flatten wave; the record list holds two entries
   // per line slot (records are usually as many as lines or fewer), the
   // overflow list takes the rest
@@ -724,13 +727,34 @@ rz_status enqueue_bin_count(rz_ctx* ctx, const rzk::LineBufs& lb, const rzg::Bin
   b->list_regions = nreg;
   b->part_cap = ctx->part_cap;
   b->fill = fill;
-  if (nw == 0) return RZ_OK;
+  if (nw == 0 || !launch) return RZ_OK;
   // then the gap slots, 64 per wave
   const uint32_t gap_blocks = (ga.p.n_gaps + rzk::kBinBlock - 1) / rzk::kBinBlock;
   RZ_LAUNCH_LDS(ctx, "k_bin_lines", (rzk::k_bin_lines<MULTI>), dim3(nw / wpb + gap_blocks),
             dim3(rzk::kBinBlock), ctx->cur.bin_lds, (const double4*)lb.lines, (const uint32_t*)lb.line_path, g0, pi,
             *b, ga);
   return RZ_OK;
+}
+
+// Flatten and count pass as one kernel (k_flat_bin), with `lb` and `b` as
+// enqueue_flatten and enqueue_bin_count set them up (launch = false).
+template <bool MULTI>
+rz_status enqueue_flat_bin(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, double fl,
+                           const rzk::PathInfo* pi, const uint32_t* slot_path, int dq, int dc,
+                           int mode, const rzk::LineBufs& lb, const rzg::BinGrid& g0,
+                           const rzk::BinBufs& b, const rzk::GapArgs& ga) {
+  const uint32_t bl = d->view.blocks_line, bq = d->view.blocks_quad << dq;
+  const uint32_t gap_waves = (ga.p.n_gaps + 63) / 64;
+  const uint32_t nw = lb.n_waves + gap_waves;
+  if (nw == 0) return RZ_OK;
+  const dim3 grid(nw), block(64);
+  if (mode == RZ_EDGE_NONE)
+    RZ_LAUNCH_LDS(ctx, "k_flat_bin", (rzk::k_flat_bin<MULTI, true>), grid, block, ctx->cur.bin_lds,
+                  d->view, x, fl, bl, bq, dq, dc, pi, slot_path, lb, g0, b, ga);
+  else
+    RZ_LAUNCH_LDS(ctx, "k_flat_bin", (rzk::k_flat_bin<MULTI, false>), grid, block, ctx->cur.bin_lds,
+                  d->view, x, fl, bl, bq, dq, dc, pi, slot_path, lb, g0, b, ga);
+  return check_launch(ctx, "flatten + count pass");
 }
 
 rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* line_fill) {
@@ -749,8 +773,10 @@ rz_status enqueue_bin_place(rz_ctx* ctx, const rzk::BinBufs& b, const uint32_t* 
 // Enqueue flatten, count pass, scans and scatter pass. Lines, records and
 // events land in ctx->lines / ctx->recs / ctx->events (capacity-checked on the
 // device; finish() grows them and asks for a re-run).
+// A mask's lines kept beyond the right clip (rzg::wrap_lines) deposit into an
+// image with rows of mask_stride pixels (0: W).
 rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
-                      int mode, size_t W, size_t H, Pipe* pp) {
+                      int mode, size_t W, size_t H, Pipe* pp, size_t mask_stride = 0) {
   select_params(ctx, mask_tune_key(d, tr, flatness, mode, W, H));
   Pipe& P = *pp;
   P.grid = make_grid(mode, W, H);
@@ -793,8 +819,14 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
   int dq, dc;
   split_depths(ctx, d->est, std::fabs(x.m[0] * x.m[4] - x.m[1] * x.m[3]), flatness, &dq, &dc);
+  const bool fused = ctx->cur.fused != 0;
+  rzg::WrapClip wrap{-1.0, 1.0, (int)W, (int)H};
+  if (mode == RZ_EDGE_MASK)
+    wrap = rzg::WrapClip{(double)W - 1.0, (double)(mask_stride ? mask_stride : W), (int)W, (int)H};
+  else if (mode == RZ_EDGE_ITER)
+    wrap = rzg::WrapClip{(double)W, (double)W + 1.0, (int)W, (int)H};
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, nullptr, nullptr, P.st, bin_fill + rzk::kParts + 1, dq, dc,
-                         mode, (double)H, &lb));
+                         mode, (double)H, wrap, &lb, !fused));
   rzk::BinBufs b;
   b.tile_count = P.tile_count;
   b.tile_off = P.tile_off;
@@ -806,8 +838,11 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
   b.ev_cap = ctx->ev_cap;
   b.redge = P.redge;
   b.st = P.st;
-  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr, rzk::GapArgs{d->view, x, nullptr},
-                                   bin_fill, &b));
+  const rzk::GapArgs ga{d->view, x, nullptr};
+  RZ_TRY(enqueue_bin_count<false>(ctx, lb, P.grid.g, nullptr, ga, bin_fill, &b, !fused));
+  if (fused)
+    RZ_TRY(enqueue_flat_bin<false>(ctx, d, x, fl, nullptr, nullptr, dq, dc, mode, lb, P.grid.g, b,
+                                   ga));
   RZ_TRY(check_launch(ctx, "bin pass 0"));
   // with the backdrop differences along each tile row and the right-edge
   // row differences
@@ -824,10 +859,10 @@ rz_status enqueue_bin(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, do
 template <class Out>
 rz_status enqueue_fine(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
                        int rule, int mode, size_t W, size_t H, const Out& o, const char* name0,
-                       const char* name1) {
+                       const char* name1, size_t mask_stride = 0) {
   if (W == 0 || H == 0) return RZ_OK;
   Pipe P;
-  RZ_TRY(enqueue_bin(ctx, d, tr, flatness, mode, W, H, &P));
+  RZ_TRY(enqueue_bin(ctx, d, tr, flatness, mode, W, H, &P, mask_stride));
   rzk::FineArgs a;
   a.tile_off = P.tile_off;
   a.recs = (const float4*)ctx->recs.p;
@@ -864,7 +899,8 @@ rz_status enqueue_mask(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, d
   rzk::MaskOut o;
   o.out = d_out;
   o.row_stride = row_stride;
-  return enqueue_fine(ctx, d, tr, flatness, rule, mode, W, H, o, "k_fine_mask<0>", "k_fine_mask<1>");
+  return enqueue_fine(ctx, d, tr, flatness, rule, mode, W, H, o, "k_fine_mask<0>", "k_fine_mask<1>",
+                      row_stride);
 }
 
 // Resolve the paint on the host and stage its stops in ctx->stops.
@@ -971,10 +1007,16 @@ rz_status finish(rz_ctx* ctx) {
   return RZ_OK;
 }
 
-// Blocking mask into a device buffer, re-running once after a capacity miss.
+// Runs of a blocking call at most: each re-run after a capacity miss has grown
+// at least one buffer (lines first, as the count pass is skipped then; records,
+// events and the record list next; records once more when a full list hid
+// some of them from the count).
+constexpr int kRuns = 5;
+
+// Blocking mask into a device buffer, re-running after a capacity miss.
 rz_status mask_blocking(rz_ctx* ctx, const rz_dpath* d, const rz_transform& tr, double flatness,
                         int rule, int mode, size_t W, size_t H, float* d_out, size_t stride) {
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     RZ_TRY(enqueue_mask(ctx, d, tr, flatness, rule, mode, W, H, d_out, stride));
     rz_status s = finish(ctx);
     if (s != RZ_E_CAPACITY) return s;
@@ -1356,8 +1398,9 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
   int dq, dc;
   split_depths(ctx, d->est, std::fabs(tr.m[0] * tr.m[4] - tr.m[1] * tr.m[3]), flatness, &dq, &dc);
+  const bool fused = ctx->cur.fused != 0;
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
-                         mode, 0.0, &lb));
+                         mode, 0.0, rzg::WrapClip{-1.0, 1.0, 0, 0}, &lb, !fused));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -1370,8 +1413,10 @@ rz_status enqueue_scene_batched(rz_ctx* ctx, const rz_dscene* s, const rz_transf
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, rzk::GapArgs{d->view, x, s->slot_path},
-                                  bin_fill, &b));
+  const rzk::GapArgs ga{d->view, x, s->slot_path};
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, ga, bin_fill, &b, !fused));
+  if (fused)
+    RZ_TRY(enqueue_flat_bin<true>(ctx, d, x, fl, pi, s->slot_path, dq, dc, mode, lb, g0, b, ga));
   RZ_TRY(check_launch(ctx, "scene pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
@@ -1538,8 +1583,9 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   uint32_t* bin_fill = meta + meta_u32 - (rzk::kParts + 1 + rzk::kLineParts);
   int dq, dc;
   split_depths(ctx, s->atlas_est, 1.0, flatness, &dq, &dc);
+  const bool fused = ctx->cur.fused != 0;
   RZ_TRY(enqueue_flatten(ctx, d, x, fl, pi, s->slot_path, st, bin_fill + rzk::kParts + 1, dq, dc,
-                         mode, 0.0, &lb));
+                         mode, 0.0, rzg::WrapClip{-1.0, 1.0, 0, 0}, &lb, !fused));
   rzk::BinBufs b;
   b.tile_count = tile_count;
   b.tile_off = tile_off;
@@ -1552,8 +1598,10 @@ rz_status enqueue_atlas(rz_ctx* ctx, const rz_dscene* s, int rule, int mode, dou
   b.redge = redge;
   b.st = st;
   rzg::BinGrid g0{};
-  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, rzk::GapArgs{d->view, x, s->slot_path},
-                                  bin_fill, &b));
+  const rzk::GapArgs ga{d->view, x, s->slot_path};
+  RZ_TRY(enqueue_bin_count<true>(ctx, lb, g0, pi, ga, bin_fill, &b, !fused));
+  if (fused)
+    RZ_TRY(enqueue_flat_bin<true>(ctx, d, x, fl, pi, s->slot_path, dq, dc, mode, lb, g0, b, ga));
   RZ_TRY(check_launch(ctx, "atlas pass 0"));
   RZ_TRY(scan_tiles_bands(ctx, tile_count, tile_off, ct, &st->n_records, band_ev, ev_off, cb,
                           &st->n_events));
@@ -2471,6 +2519,9 @@ rz_status rz_set_param(rz_ctx* ctx, int param, int value) {
   } else if (param == RZ_PARAM_SCENE_QUEUE) {
     RZ_TRY(range(0, 1));
     lp.scene_queue = value;
+  } else if (param == RZ_PARAM_FUSED) {
+    RZ_TRY(range(0, 1));
+    lp.fused = value;
   } else if (param == RZ_PARAM_TUNED) {
     RZ_TRY(range(0, 1));
     ctx->use_tuned = value != 0;
@@ -2492,6 +2543,7 @@ rz_status rz_get_param(rz_ctx* ctx, int param, int* value) {
   else if (param == RZ_PARAM_SCATTER_WAVES) *value = lp.scatter_log2;
   else if (param == RZ_PARAM_SCENE_WAVES) *value = lp.scene_waves;
   else if (param == RZ_PARAM_SCENE_QUEUE) *value = lp.scene_queue;
+  else if (param == RZ_PARAM_FUSED) *value = lp.fused;
   else if (param == RZ_PARAM_TUNED) *value = ctx->use_tuned ? 1 : 0;
   else if (param == RZ_PARAM_SCAN_DIRECT) *value = 31 - __builtin_clz(ctx->scan_direct);
   else if (param == RZ_PARAM_GRAPH) *value = ctx->use_graphs ? 1 : 0;
@@ -3135,7 +3187,7 @@ rz_status rz_fill(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flat
   RZ_TRY(ensure(ctx, ctx->out, bytes));
   RZ_HIP(ctx, hipMemcpy(ctx->out.p, dense.data(), bytes, hipMemcpyHostToDevice));
   rz_status st = RZ_OK;
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     st = enqueue_fill(ctx, d, tr, flatness, fill_rule, edge_mode, paint, W, H, (float*)ctx->out.p, W);
     if (st != RZ_OK) return st;
     st = finish(ctx);
@@ -3208,7 +3260,7 @@ rz_status rz_bin(rz_ctx* ctx, const rz_path* path, rz_transform tr, double flatn
   } guard{ctx, d};
   Pipe P;
   rz_status st = RZ_OK;
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     RZ_TRY(enqueue_bin(ctx, d, tr, flatness, edge_mode, width, height, &P));
     RZ_LAUNCH(ctx, "k_status_merge", rzk::k_status_merge, dim3(1), dim3(64),
               (const rzk::Status*)P.st, ctx->acc_run);
@@ -3342,7 +3394,7 @@ rz_status rz_render(rz_ctx* ctx, const rz_path* const* paths, const rz_paint* pa
   } guard{ctx, s};
   RZ_TRY(ensure(ctx, ctx->out, std::max<size_t>(px, 1) * 16));
   rz_status st = RZ_OK;
-  for (int attempt = 0; attempt < 4; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     st = rz_render_device(ctx, s, tr, flatness, edge_mode, v, bg, batched, (float*)ctx->out.p,
                           (size_t)lw);
     if (st != RZ_OK) return st;
@@ -3414,7 +3466,7 @@ rz_status rz_render_tree(rz_ctx* ctx, const rz_path* const* paths, size_t n_path
   if (!rgba || cap_px < (size_t)lw * lh || !lw || !lh) return RZ_OK;
   const rzp::C4 c = bg ? rzp::c4(bg[0], bg[1], bg[2], bg[3]) : rzp::splat(0.0f);
   rz_status st = RZ_OK;
-  for (int attempt = 0; attempt < 4; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     TreeRender R{ctx, &tb, paints, edge_mode, flatness, {}, {}, {}, {}};
     DLayer L;
     st = R.new_layer(lb, c, &L);
@@ -3474,7 +3526,7 @@ rz_status rz_mask_atlas(rz_ctx* ctx, const rz_path* const* paths, const rz_atlas
   // pixels outside every glyph view keep the caller's values
   RZ_HIP(ctx, hipMemcpy(ctx->out.p, atlas, bytes, hipMemcpyHostToDevice));
   rz_status st = RZ_OK;
-  for (int attempt = 0; attempt < 4; ++attempt) {
+  for (int attempt = 0; attempt < kRuns; ++attempt) {
     st = enqueue_atlas(ctx, s, fill_rule, edge_mode, flatness, (float*)ctx->out.p, atlas_width);
     if (st != RZ_OK) return st;
     st = finish(ctx);

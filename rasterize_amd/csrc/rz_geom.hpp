@@ -245,11 +245,8 @@ RZ_HD Pt at_x(Pt a, Pt b, double X) {
 // (src/rasterize.rs:389-393, 922-937) whose x<0 part becomes a vertical line on
 // x = 0. Produces up to two pieces in `out` (returns the count) and reports
 // which endpoints of the effective polyline were created by the right clip.
-//
-// Deliberate deviation: for a line starting exactly on x = wc and ending to
-// its right the reference's `p0.x() < width` test keeps the part beyond the
-// clip line (and then indexes past the row); here that part is dropped, like
-// every other part beyond the clip.
+// A line starting exactly on x = wc and ending to its right never gets here:
+// the flatten replaces it by wrap_lines (below).
 struct EffLine {
   Pt p[3];   // polyline vertices p[0] -> ... -> p[n]
   int n;     // number of pieces (0, 1 or 2)
@@ -260,6 +257,79 @@ struct EffLine {
 // Abscissa of the vertical line that replaces a line lying entirely beyond the
 // right clip (src/rasterize.rs:372-375).
 RZ_HD double redge_x(const BinGrid& g) { return g.wc - 0.001; }
+
+// A line that starts exactly on the right clip abscissa wc and ends to its
+// right. signed_difference_line keeps such a line whole: its `p0.x() < width`
+// test fails at equality, so the part it keeps is the one beyond the clip
+// (src/rasterize.rs:373-384). Every row's deposit pair then lands at columns
+// >= wc, and those past the end of the accumulation buffer's row (of `stride`
+// entries: W + 1 for mask_iter, W for a dense mask, the row stride for a mask
+// into a wider image) continue in the next row: with stride wc + 1, a deposit
+// at column wc + 1 is the next row's first pixel.
+struct WrapClip {
+  double wc;      // the clip abscissa; negative: no such lines (ActiveEdge, atlases)
+  double stride;  // entries per buffer row
+  int W, H;       // visible columns and rows of the buffer
+};
+RZ_HD bool wraps(Pt p0, Pt p1, const WrapClip& q) {
+  return q.wc >= 0.0 && p0.x == q.wc && p1.x > q.wc;
+}
+
+// The flatten stores a line of wrap_lines (below) with this NaN as its x1: a
+// payload no arithmetic makes, so the count pass knows the line (vertical on
+// its x0) and gives both its ends events, as for the ends the right clip
+// makes. These lines are not part of a closed chain.
+constexpr uint64_t kWrapTag = 0x7ffc0ffee0000001ull;
+RZ_HD double wrap_tag() { return __builtin_bit_cast(double, kWrapTag); }
+RZ_HD bool is_wrap_tag(double x) { return __builtin_bit_cast(uint64_t, x) == kWrapTag; }
+
+// The deposits of such a line (the reference's row loop, src/rasterize.rs:
+// 398-444, whose single-pixel case is the only one here: x1 is clamped to wc
+// and x0 >= wc), each made by a vertical line on its pixel's left edge:
+// f(a, b) with a = (c, r) and b = (c, r + v) (reversed when v < 0), which
+// deposits v at column c of row r (|v| > 1: several such lines). Deposits
+// outside the visible columns (the dead column of mask_iter, c = W, or the
+// rest of a wider image's row) or past the last row are dropped.
+template <typename F>
+RZ_HD void wrap_lines(Pt p0, Pt p1, const WrapClip& q, F&& f) {
+  if (fabs(p0.y - p1.y) < kEpsilon) return;
+  double dir = 1.0;
+  if (p0.y > p1.y) {
+    dir = -1.0;
+    const Pt t = p0;
+    p0 = p1;
+    p1 = t;
+  }
+  auto put = [&](double y, double c, double v) {
+    const double k = floor(c / q.stride);
+    const double r = y + k, cc = c - k * q.stride;
+    if (r >= (double)q.H || cc >= (double)q.W || v == 0.0) return;
+    // a row's deposit can exceed 1 in size (xmf > 1 when the line crosses
+    // several columns within the row): then n lines of |v| / n each
+    const double n = ceil(fabs(v)), h = fabs(v) / n;
+    const Pt a = mkpt(cc, r), b = mkpt(cc, r + h);
+    for (double i = 0.0; i < n; i += 1.0) {
+      if (v > 0.0) f(a, b);
+      else f(b, a);
+    }
+  };
+  const double dxdy = (p1.x - p0.x) / (p1.y - p0.y);
+  double y = floor(fmax(p0.y, 0.0));
+  double x = p0.y < 0.0 ? p0.x - p0.y * dxdy : p0.x;
+  double x_next = x;
+  const double yend = fmin((double)q.H, fmax(ceil(p1.y), 0.0));
+  for (; y < yend; y += 1.0) {
+    x = x_next;
+    const double dy = fmin(y + 1.0, p1.y) - fmax(y, p0.y);
+    const double d = dir * dy;
+    x_next = x + dxdy * dy;
+    const double x0 = x < x_next ? x : x_next;
+    const double x0f = fmax(floor(x0), 0.0);
+    const double xmf = 0.5 * (x + x_next) - x0f;
+    put(y, x0f, d * (1.0 - xmf));
+    put(y, x0f + 1.0, d * xmf);
+  }
+}
 
 RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {
   EffLine e;

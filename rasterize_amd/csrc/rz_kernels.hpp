@@ -206,8 +206,24 @@ struct LineBufs {
   uint2* wave_lines;    // per flatten wave: first line, line count
   uint32_t n_waves;     // flatten waves
   double ae_h;          // ActiveEdgeRasterizer: image rows of a single path (rzg::ae_chain)
+  rzg::WrapClip wrap;   // a single path's right clip for rzg::wrap_lines
   Status* st;
 };
+
+// The right clip for rzg::wrap_lines of a line of path pid: the single path's,
+// or a batched path's sub-view when it is drawn as mask_iter (scenes; the
+// atlas's masks leave lines beyond their slot out).
+template <bool MULTI>
+__device__ inline rzg::WrapClip wrap_clip(const LineBufs& lb, const PathInfo* pi, uint32_t pid,
+                                          bool active) {
+  if (!MULTI) return lb.wrap;
+  rzg::WrapClip q{-1.0, 1.0, 0, 0};
+  if (active && pi[pid].g.mode == rzg::EDGE_ITER) {
+    const rzg::BinGrid& g = pi[pid].g;
+    q = rzg::WrapClip{g.wc, g.wc + 1.0, g.W, g.H};
+  }
+  return q;
+}
 
 // Walk from the slot's root along j for D0 levels. Returns the subtree root
 // in *n and whether a traversal from it is needed; *leaf is set when a leaf
@@ -267,10 +283,20 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
   double ae_h = 0.0;
   if (AE) ae_h = MULTI ? (double)(active ? pi[pid].g.H : 0) : lb.ae_h;
-  // each line of the subtree, as it is or as its chain
+  const rzg::WrapClip wq = wrap_clip<MULTI>(lb, pi, pid, active);
+  // each line of the subtree, as it is, as its chain or as the deposits of a
+  // line kept beyond the right clip (then the lines are not a chain, and the
+  // kept points below cannot stand for them)
+  bool chain = true;
   auto lines_of = [&](Pt a, Pt c, auto&& f) {
-    if (AE) rzg::ae_chain(a, c, ae_h, f);
-    else f(a, c);
+    if (AE) {
+      rzg::ae_chain(a, c, ae_h, f);
+    } else if (rzg::wraps(a, c, wq)) {
+      chain = false;
+      rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); });
+    } else {
+      f(a, c);
+    }
   };
   uint32_t cnt = 0;
   auto count = [&](Pt a, Pt c) {
@@ -308,7 +334,7 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   }
   if (!cnt) return;
   size_t o = (size_t)part * lb.part_cap + base + incl - cnt;
-  if (cnt <= (uint32_t)kLineKeep) {
+  if (cnt <= (uint32_t)kLineKeep && chain) {
     for (uint32_t k = 0; k < cnt; ++k, ++o) {
       const double2 a = keep[k], c = keep[k + 1];
       lb.lines[o] = make_double4(a.x, a.y, c.x, c.y);
@@ -321,6 +347,92 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
     if (MULTI) lb.line_path[o] = pid;
     lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y);
   };
+  if (leaf) {
+    lines_of(n.p[0], n.p[NP - 1], put);
+    return;
+  }
+  rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, put); });
+}
+
+// The flatten half of k_flat_bin: flat_item's count and placement, but the
+// lines of a thread with at most kLineKeep of them stay in `keep` (LDS) for
+// the count pass of the same wave; only longer runs go to the line buffer.
+struct FlatRes {
+  uint32_t cnt;    // this lane's lines
+  uint32_t pre;    // lines of the lanes before it
+  uint32_t total;  // the wave's lines (0 when none, or they did not fit)
+  uint32_t line0;  // the wave's first line index
+  uint32_t pid;    // batched: the lane's path
+  bool kept;       // the lane's lines are in its LDS keep
+};
+template <int NP, bool MULTI, bool AE>
+__device__ void flat_keep(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
+                          const Xform& x, double fl, const PathInfo* pi, const uint32_t* slot_path,
+                          const LineBufs& lb, double2* keep, uint32_t part, uint32_t wave,
+                          FlatRes* fr) {
+  rzg::Node<NP> n;
+  bool walk, leaf;
+  int err;
+  uint32_t pid;
+  flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
+  double ae_h = 0.0;
+  if (AE) ae_h = MULTI ? (double)(active ? pi[pid].g.H : 0) : lb.ae_h;
+  const rzg::WrapClip wq = wrap_clip<MULTI>(lb, pi, pid, active);
+  bool chain = true;  // as in flat_item
+  auto lines_of = [&](Pt a, Pt c, auto&& f) {
+    if (AE) {
+      rzg::ae_chain(a, c, ae_h, f);
+    } else if (rzg::wraps(a, c, wq)) {
+      chain = false;
+      rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); });
+    } else {
+      f(a, c);
+    }
+  };
+  uint32_t cnt = 0;
+  auto count = [&](Pt a, Pt c) {
+    if (cnt < (uint32_t)kLineKeep) {
+      if (cnt == 0) keep[0] = make_double2(a.x, a.y);
+      keep[cnt + 1] = make_double2(c.x, c.y);
+    }
+    ++cnt;
+  };
+  if (leaf) lines_of(n.p[0], n.p[NP - 1], count);
+  if (walk) {
+    const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, count); });
+    if (r != rzg::ST_OK) err = r;
+  }
+  if (err != rzg::ST_OK) {
+    set_err(lb.st, (uint32_t)err);
+    cnt = 0;
+  }
+  const int lane = threadIdx.x & 63;
+  const uint32_t incl = (uint32_t)dpp_add_scan((int)cnt);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+  uint32_t base = 0;
+  if (lane == 0 && total) base = atomicAdd(&lb.fill[part], total);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+  const bool fits = base + total <= lb.part_cap;
+  fr->cnt = cnt;
+  fr->kept = cnt <= (uint32_t)kLineKeep && chain;
+  fr->pre = incl - cnt;
+  fr->total = fits ? total : 0u;
+  fr->line0 = part * lb.part_cap + base;
+  fr->pid = pid;
+  // the wave's line range, as k_flat_lines records it (the scatter reads it)
+  if (lane == 0) lb.wave_lines[wave] = make_uint2(fr->line0, fr->total);
+  if (total == 0) return;
+  if (!fits) {
+    if (lane == 0) {
+      atomicOr(&lb.st->overflow, 4u);
+      atomicMax(&lb.st->n_line_part, base + total);
+    }
+    return;
+  }
+  if (fr->kept) return;
+  // more lines than were kept: walk the subtree again into the line buffer
+  size_t o = (size_t)fr->line0 + incl - cnt;
+  auto put = [&](Pt a, Pt c) { lb.lines[o++] = make_double4(a.x, a.y, c.x, c.y); };
   if (leaf) {
     lines_of(n.p[0], n.p[NP - 1], put);
     return;
@@ -625,32 +737,29 @@ __device__ void gap_events(const GapArgs& ga, const rzg::BinGrid& g0, const BinB
 // work varies with their lines, so one wave per workgroup.
 constexpr uint32_t kBinBlock = 64;
 
-template <bool MULTI>
-__global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
-                                                          const uint32_t* line_path, rzg::BinGrid g0,
-                                                          const PathInfo* pi, BinBufs b, GapArgs ga) {
-  __shared__ LineStage lstage[kBinBlock];
-  __shared__ BandStage bstage[kBinBlock];
-  __shared__ uint32_t mkey[kBinBlock / 64][kMap];
-  __shared__ uint32_t mcnt[kBinBlock / 64][kMap];
-  __shared__ int mark_all[kBinBlock / 64][64];
-  if (b.st->err || b.st->overflow) return;
+// The LDS of one count-pass wave.
+struct BinLDS {
+  LineStage lstage[64];
+  BandStage bstage[64];
+  uint32_t keys[kMap];
+  uint32_t cnts[kMap];
+  int marks[64];
+};
+
+// The count pass of one wave over its lines: wl_range = (first line index,
+// line count), `load(c, &line, &path)` sets this lane's line c + lane (and,
+// batched, its path) where c + lane < the count and leaves them otherwise;
+// it is called for c = 0, 64, 128, ... in that order, by every lane.
+template <bool MULTI, class Load>
+__device__ inline void bin_wave(BinLDS& lds, uint32_t wave, uint2 wl_range, const rzg::BinGrid& g0,
+                                const PathInfo* pi, const BinBufs& b, Load&& load) {
+  LineStage* wl = lds.lstage;
+  BandStage* wb = lds.bstage;
+  uint32_t* keys = lds.keys;
+  uint32_t* cnts = lds.cnts;
+  int* marks = lds.marks;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
-  const uint32_t wave = blockIdx.x * (kBinBlock / 64) + (threadIdx.x >> 6);
-  if (wave >= b.n_waves) {
-    // the waves past the lines' take the gap slots, 64 each
-    const uint32_t i = (wave - b.n_waves) * 64 + (uint32_t)lane;
-    gap_events<MULTI>(ga, g0, b, pi, i < ga.p.n_gaps ? i : ~0u, lane);
-    return;
-  }
-  const uint2 wl_range = b.wave_lines[wave];
-  if (wl_range.y == 0) return;  // the scatter skips the same regions
-  LineStage* wl = lstage + (threadIdx.x & ~63u);
-  BandStage* wb = bstage + (threadIdx.x & ~63u);
-  uint32_t* keys = mkey[threadIdx.x >> 6];
-  uint32_t* cnts = mcnt[threadIdx.x >> 6];
-  int* marks = mark_all[threadIdx.x >> 6];
   for (uint32_t q = (uint32_t)lane; q < kMap; q += 64) {
     keys[q] = kEmpty;
     cnts[q] = 0;
@@ -720,13 +829,7 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
   // each group's line (and path) is loaded while the previous group is binned
   double4 fL = make_double4(0, 0, 0, 0);
   uint32_t fpid = 0;
-  auto fetch = [&](uint32_t c) {
-    const size_t i = (size_t)wl_range.x + c + (uint32_t)lane;
-    if (c + (uint32_t)lane < wl_range.y) {
-      fL = lines[i];
-      if (MULTI) fpid = line_path[i];
-    }
-  };
+  auto fetch = [&](uint32_t c) { load(c, &fL, &fpid); };
   // batched forms: the group's path data, loaded during the previous group
   rzg::BinGrid fg = g0;
   uint32_t ftb = 0, fbb = 0, frb0 = 0;
@@ -755,7 +858,9 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
       bb = fbb;
       rb0 = frb0;
     }
-    const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(L.z, L.w);
+    // a line of rzg::wrap_lines: vertical on L.x, both ends free
+    const bool wrapv = have && rzg::is_wrap_tag(L.z);
+    const Pt p0 = rzg::mkpt(L.x, L.y), p1 = rzg::mkpt(wrapv ? L.x : L.z, L.w);
     if (have && !(isfinite(p0.x) && isfinite(p0.y) && isfinite(p1.x) && isfinite(p1.y))) {
       set_err(b.st, rzg::ST_NAN);
       have = false;
@@ -782,7 +887,8 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
     uint32_t evb0 = 0, evb1 = 0;
     int nb0 = 0, nb1 = 0;
     if (have && !proj) {
-      const rzg::EffLine e = rzg::effective_line(p0, p1, g);
+      rzg::EffLine e = rzg::effective_line(p0, p1, g);
+      if (wrapv) e.clip0 = e.clip1 = true;
       rzg::Event ev;
       if (e.clip0 && rzg::make_event(e.p[0], -1.0f, g, &ev)) {
         ev0 = event_bits(ev);
@@ -886,6 +992,98 @@ __global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
     if (t != kEmpty) b.wave_map[(size_t)wave * kMap + q] = atomicAdd(&b.tile_count[t], cnts[q]);
   }
   if (lane == 0) b.wave_used[wave] = used;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBinBlock) void k_bin_lines(const double4* lines,
+                                                          const uint32_t* line_path, rzg::BinGrid g0,
+                                                          const PathInfo* pi, BinBufs b, GapArgs ga) {
+  __shared__ BinLDS S;
+  if (b.st->err || b.st->overflow) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = blockIdx.x * (kBinBlock / 64) + (threadIdx.x >> 6);
+  if (wave >= b.n_waves) {
+    // the waves past the lines' take the gap slots, 64 each
+    const uint32_t i = (wave - b.n_waves) * 64 + (uint32_t)lane;
+    gap_events<MULTI>(ga, g0, b, pi, i < ga.p.n_gaps ? i : ~0u, lane);
+    return;
+  }
+  const uint2 wl_range = b.wave_lines[wave];
+  if (wl_range.y == 0) return;  // the scatter skips the same regions
+  bin_wave<MULTI>(S, wave, wl_range, g0, pi, b, [&](uint32_t c, double4* L, uint32_t* pid) {
+    const size_t i = (size_t)wl_range.x + c + (uint32_t)lane;
+    if (c + (uint32_t)lane < wl_range.y) {
+      *L = lines[i];
+      if (MULTI) *pid = line_path[i];
+    }
+  });
+}
+
+// Flatten and count pass in one kernel: each wave flattens its items (as
+// k_flat_lines) and bins the lines it made (as k_bin_lines), reading them
+// from its own LDS rather than back from HBM; a thread's run of more than
+// kLineKeep lines goes through the line buffer as before. Waves past the
+// flatten waves take the gap slots. Bit for bit the same records, events,
+// backdrops and right-edge rows as the two kernels.
+template <bool MULTI, bool AE>
+__global__ __launch_bounds__(64) void k_flat_bin(DPath p, Xform x, double fl, uint32_t bl,
+                                                 uint32_t bq, int dq, int dc, const PathInfo* pi,
+                                                 const uint32_t* slot_path, LineBufs lb,
+                                                 rzg::BinGrid g0, BinBufs b, GapArgs ga) {
+  __shared__ double2 keep_all[64 * (kLineKeep + 1)];
+  __shared__ BinLDS S;
+  if (b.st->err || b.st->overflow) return;
+  const int lane = threadIdx.x;
+  const uint32_t wave = blockIdx.x;
+  if (wave >= lb.n_waves) {
+    const uint32_t i = (wave - lb.n_waves) * 64 + (uint32_t)lane;
+    gap_events<MULTI>(ga, g0, b, pi, i < ga.p.n_gaps ? i : ~0u, lane);
+    return;
+  }
+  const uint32_t part = wave % kLineParts;
+  FlatRes fr;
+  {
+    double2* keep = keep_all + lane * (kLineKeep + 1);
+    const uint32_t blk = wave / (kBlock / 64), t = (wave % (kBlock / 64)) * 64 + (uint32_t)lane;
+    if (blk < bl) {
+      const uint32_t slot = p.order[blk * kBlock + t];
+      flat_keep<2, MULTI, AE>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, lb, keep, part,
+                              wave, &fr);
+    } else if (blk < bl + bq) {
+      const uint32_t v = (blk - bl) * kBlock + t;
+      const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];
+      flat_keep<3, MULTI, AE>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi,
+                              slot_path, lb, keep, part, wave, &fr);
+    } else {
+      const uint32_t v = (blk - bl - bq) * kBlock + t;
+      const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];
+      flat_keep<4, MULTI, AE>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi,
+                              slot_path, lb, keep, part, wave, &fr);
+    }
+  }
+  if (fr.total == 0) return;
+  // the kept lines (LDS) and the long runs (line buffer) written by other
+  // lanes are read below
+  __syncthreads();
+  int carry = -1;  // owner lane of the line before the current group
+  bin_wave<MULTI>(S, wave, make_uint2(fr.line0, fr.total), g0, pi, b,
+                  [&](uint32_t c, double4* L, uint32_t* pid) {
+    const int o = task_owner(S.marks, fr.pre, fr.cnt, c, lane, &carry);
+    const uint32_t opre = (uint32_t)__shfl((int)fr.pre, o, 64);
+    const uint32_t opid = (uint32_t)__shfl((int)fr.pid, o, 64);
+    const bool okept = __shfl((int)fr.kept, o, 64) != 0;
+    const uint32_t i = c + (uint32_t)lane;
+    if (i < fr.total) {
+      if (okept) {
+        const double2* ok = keep_all + o * (kLineKeep + 1) + (i - opre);
+        const double2 a = ok[0], e = ok[1];
+        *L = make_double4(a.x, a.y, e.x, e.y);
+      } else {
+        *L = lb.lines[(size_t)fr.line0 + i];
+      }
+      if (MULTI) *pid = opid;
+    }
+  });
 }
 
 // Write one listed record to its slot: tile offset + the wave's offset in the

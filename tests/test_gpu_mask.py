@@ -181,6 +181,33 @@ def test_bin_split_depths_agree(dq, dc, c2):
     assert np.array_equal(_sort_events(ge), _sort_events(re))
 
 
+@pytest.mark.parametrize("mode", [rz._abi.RZ_EDGE_MASK, rz._abi.RZ_EDGE_NONE])
+@pytest.mark.parametrize("dq,dc", [(None, None), (0, 0), (6, 6)])
+def test_bin_fused_and_two_kernels_agree(mode, dq, dc, c2):
+    """Flatten and count pass as one kernel (lines from LDS, RZ_PARAM_FUSED 1)
+    and as two (lines through HBM, the default): bit-identical bins, with
+    subtrees of few lines (all kept in LDS) and of many (through the line
+    buffer), and with ActiveEdge chains."""
+    path, tr = c2
+    out = []
+    for fused in (0, 1):
+        ctx = rz.Context(0)
+        try:
+            L = rz._abi.lib()
+            ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_FUSED, fused))
+            if dq is not None:
+                ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_SPLIT_QUAD, dq))
+                ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_SPLIT_CUBIC, dc))
+            _say(f"bin c2 fused={fused} mode={mode} dq={dq} dc={dc}")
+            out.append(rz.bin_stage(path, tr, 4096, 4096, edge_mode=mode, ctx=ctx))
+        finally:
+            ctx.close()
+    (rc, rr, rb, re, rx), (gc, gr, gb, ge, gx) = out
+    assert np.array_equal(gc, rc) and np.array_equal(gb, rb) and np.array_equal(gx, rx)
+    assert np.array_equal(_sort_tiles(gc, gr), _sort_tiles(rc, rr))
+    assert np.array_equal(_sort_events(ge), _sort_events(re))
+
+
 @pytest.mark.parametrize("direct_log2", [0, 3])
 def test_bin_scan_forms_agree(direct_log2, c2):
     """The tile scan's second pass sums the block totals itself up to
