@@ -167,3 +167,29 @@ def test_scene_view_layer_geometry(batched):
     assert (x, y) == (6, 4)
     d = np.abs(lay.image.astype(np.float64) - ref)
     assert d.max() <= 2e-3
+
+
+@pytest.mark.parametrize("mode", ["scene", "atlas"])
+def test_fused_flatten_and_count_batched_forms(small_scene, mode):
+    """RZ_PARAM_FUSED 1 (flatten and count pass as one kernel) against the two
+    kernels for the batched forms: the same pixels, bit for bit."""
+    from rasterize_amd import mask_atlas, workload_glyphs
+    L = rz._abi.lib()
+    out = []
+    gs = workload_glyphs(4, 300, 1024) if mode == "atlas" else None
+    for fused in (0, 1):
+        ctx = rz.Context(0)
+        try:
+            ctx.check(L.rz_set_param(ctx.h, rz._abi.RZ_PARAM_FUSED, fused))
+            _say(f"fused={fused} {mode}")
+            if mode == "scene":
+                lay = small_scene.scene().render(SignedDifferenceRasterizer(ctx=ctx), S,
+                                                 (0.0, 0.0, 256.0, 256.0))
+                out.append(np.array(lay.image))
+            else:
+                W, H = gs.atlas_size
+                atlas = np.zeros((H, W), np.float32)
+                out.append(mask_atlas(gs.paths, gs.slots, atlas, ctx=ctx))
+        finally:
+            ctx.close()
+    assert out[0].shape == out[1].shape and np.array_equal(out[0], out[1])
