@@ -672,17 +672,22 @@ rz_status enqueue_flatten(rz_ctx* ctx, const rz_dpath* d, const rzk::Xform& x, d
     const int32_t lds = ctx->cur.flat_lds >= 0 ? ctx->cur.flat_lds : (pi ? 0 : 4096);
     const bool ae = mode == RZ_EDGE_NONE;
     const dim3 grid(nblk * fsub), block(rzk::kFlatBlock);
+    // lines kept beyond the right clip: single masks and mask_iter, and
+    // batched scenes drawn as mask_iter (not the atlas, not ActiveEdge)
     if (pi && ae) {
-      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, true>), grid, block, lds, d->view,
-                    x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, true, false>), grid, block, lds,
+                    d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
+    } else if (pi && mode == RZ_EDGE_ITER) {
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, false, true>), grid, block, lds,
+                    d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     } else if (pi) {
-      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, false>), grid, block, lds,
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<true, false, false>), grid, block, lds,
                     d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     } else if (ae) {
-      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, true>), grid, block, lds,
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, true, false>), grid, block, lds,
                     d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     } else {
-      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, false>), grid, block, lds,
+      RZ_LAUNCH_LDS(ctx, "k_flat_lines", (rzk::k_flat_lines<false, false, true>), grid, block, lds,
                     d->view, x, fl, bl, bq, dq, dc, pi, slot_path, *lb);
     }
   }

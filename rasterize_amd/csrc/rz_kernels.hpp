@@ -272,7 +272,10 @@ __device__ inline void flat_subtree(const DPath& p, uint32_t slot, uint32_t j, i
 // write. Every lane of the wave takes part.
 // AE: ActiveEdgeRasterizer, each leaf line becomes the chain rzg::ae_chain
 // makes of it (the line itself unless it comes near x = 0).
-template <int NP, bool MULTI, bool AE>
+// WRAP: look for lines kept beyond the right clip (rzg::wraps); kernels for
+// forms that never have them are built without it, which keeps the walk's
+// registers down (the batched flatten then runs at four waves per SIMD).
+template <int NP, bool MULTI, bool AE, bool WRAP>
 __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, bool active,
                           const Xform& x, double fl, const PathInfo* pi, const uint32_t* slot_path,
                           const LineBufs& lb, double2* keep, uint32_t part, uint32_t wave) {
@@ -283,7 +286,8 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   flat_subtree<NP, MULTI>(p, slot, j, D0, active, x, fl, pi, slot_path, &n, &walk, &leaf, &err, &pid);
   double ae_h = 0.0;
   if (AE) ae_h = MULTI ? (double)(active ? pi[pid].g.H : 0) : lb.ae_h;
-  const rzg::WrapClip wq = wrap_clip<MULTI>(lb, pi, pid, active);
+  const rzg::WrapClip wq =
+      WRAP ? wrap_clip<MULTI>(lb, pi, pid, active) : rzg::WrapClip{-1.0, 1.0, 0, 0};
   // each line of the subtree, as it is, as its chain or as the deposits of a
   // line kept beyond the right clip (then the lines are not a chain, and the
   // kept points below cannot stand for them)
@@ -447,16 +451,16 @@ __device__ void flat_keep(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
                  t = (blockIdx.x % (kBlock / kFlatBlock)) * kFlatBlock + threadIdx.x;      \
   if (blk < bl) {                                                                          \
     const uint32_t slot = p.order[blk * kBlock + t];                                       \
-    ITEM<2, MULTI, AE>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);     \
+    ITEM<2, MULTI, AE, WRAP>(p, slot, 0, 0, slot != ~0u, x, fl, pi, slot_path, __VA_ARGS__);     \
   } else if (blk < bl + bq) {                                                              \
     const uint32_t v = (blk - bl) * kBlock + t;                                            \
     const uint32_t slot = p.order[p.blocks_line * kBlock + (v >> dq)];                     \
-    ITEM<3, MULTI, AE>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi,        \
+    ITEM<3, MULTI, AE, WRAP>(p, slot, v & ((1u << dq) - 1u), dq, slot != ~0u, x, fl, pi,        \
                        slot_path, __VA_ARGS__);                                            \
   } else {                                                                                 \
     const uint32_t v = (blk - bl - bq) * kBlock + t;                                       \
     const uint32_t slot = p.order[(p.blocks_line + p.blocks_quad) * kBlock + (v >> dc)];   \
-    ITEM<4, MULTI, AE>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi,        \
+    ITEM<4, MULTI, AE, WRAP>(p, slot, v & ((1u << dc) - 1u), dc, slot != ~0u, x, fl, pi,        \
                        slot_path, __VA_ARGS__);                                            \
   }
 
@@ -465,7 +469,7 @@ __device__ void flat_keep(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
 // with long subtrees then holds only its own LDS and wave slot.
 constexpr uint32_t kFlatBlock = 64;
 
-template <bool MULTI, bool AE>
+template <bool MULTI, bool AE, bool WRAP>
 __global__ __launch_bounds__(kFlatBlock) void k_flat_lines(DPath p, Xform x, double fl, uint32_t bl,
                                                             uint32_t bq, int dq, int dc,
                                                             const PathInfo* pi,
