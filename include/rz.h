@@ -37,7 +37,8 @@ typedef enum rz_status {
   RZ_E_NAN = 1,      /* "cannot flatten segment with NaN" (src/path.rs:765-767 panics) */
   RZ_E_ARG = 2,      /* invalid argument (empty subpath, flatness <= 0, bad shape, ...) */
   RZ_E_HIP = 3,      /* HIP runtime error */
-  RZ_E_DEPTH = 4,    /* subdivision deeper than 62 levels (non-finite or astronomically large input) */
+  RZ_E_DEPTH = 4,    /* subdivision deeper than 62 levels (non-finite or astronomically large input),
+                        or a line starting on the right clip that runs over ~512 px right within one row */
   RZ_E_NOMEM = 5,    /* device allocation failed */
   RZ_E_CAPACITY = 6  /* an async call outgrew the context's buffers; re-run (buffers were grown) */
 } rz_status;

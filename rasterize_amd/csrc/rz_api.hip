@@ -960,6 +960,9 @@ rz_status finish(rz_ctx* ctx) {
   const uint32_t err = s.err ? s.err : sn.err;
   if (err == rzg::ST_NAN) return fail(ctx, RZ_E_NAN, "cannot flatten segment with NaN");
   if (err == rzg::ST_DEPTH) return fail(ctx, RZ_E_DEPTH, "subdivision depth limit exceeded");
+  if (err == rzg::ST_WRAP)
+    return fail(ctx, RZ_E_DEPTH, "a line starting on the right clip runs too far right within a row "
+                                 "(its deposits would need more than 256 lines per pixel)");
   // a run that completed with its buffers as they are can be captured next time
   if (ctx->pending_valid && !s.overflow && ctx->graph_pending.gen == ctx->gen) {
     auto& st = ctx->graph_settled;

@@ -35,7 +35,7 @@ static constexpr int kMaxDepth = 62;
 
 enum SegKind { KIND_LINE = 0, KIND_QUAD = 1, KIND_CUBIC = 2 };
 enum EdgeMode { EDGE_MASK = 0, EDGE_ITER = 1, EDGE_NONE = 2 };
-enum Status { ST_OK = 0, ST_NAN = 1, ST_DEPTH = 2 };
+enum Status { ST_OK = 0, ST_NAN = 1, ST_DEPTH = 2, ST_WRAP = 3 };
 
 struct Pt {
   double x, y;
@@ -289,10 +289,13 @@ RZ_HD bool is_wrap_tag(double x) { return __builtin_bit_cast(uint64_t, x) == kWr
 // f(a, b) with a = (c, r) and b = (c, r + v) (reversed when v < 0), which
 // deposits v at column c of row r (|v| > 1: several such lines). Deposits
 // outside the visible columns (the dead column of mask_iter, c = W, or the
-// rest of a wider image's row) or past the last row are dropped.
+// rest of a wider image's row) or past the last row are dropped. A deposit
+// above kWrapSplit in size (a line running more than about 2 kWrapSplit
+// pixels right within one row) is refused: ST_WRAP, and nothing past it.
+constexpr double kWrapSplit = 256.0;
 template <typename F>
-RZ_HD void wrap_lines(Pt p0, Pt p1, const WrapClip& q, F&& f) {
-  if (fabs(p0.y - p1.y) < kEpsilon) return;
+RZ_HD int wrap_lines(Pt p0, Pt p1, const WrapClip& q, F&& f) {
+  if (fabs(p0.y - p1.y) < kEpsilon) return ST_OK;
   double dir = 1.0;
   if (p0.y > p1.y) {
     dir = -1.0;
@@ -303,15 +306,17 @@ RZ_HD void wrap_lines(Pt p0, Pt p1, const WrapClip& q, F&& f) {
   auto put = [&](double y, double c, double v) {
     const double k = floor(c / q.stride);
     const double r = y + k, cc = c - k * q.stride;
-    if (r >= (double)q.H || cc >= (double)q.W || v == 0.0) return;
+    if (r >= (double)q.H || cc >= (double)q.W || v == 0.0) return true;
     // a row's deposit can exceed 1 in size (xmf > 1 when the line crosses
     // several columns within the row): then n lines of |v| / n each
     const double n = ceil(fabs(v)), h = fabs(v) / n;
+    if (!(n <= kWrapSplit)) return false;
     const Pt a = mkpt(cc, r), b = mkpt(cc, r + h);
     for (double i = 0.0; i < n; i += 1.0) {
       if (v > 0.0) f(a, b);
       else f(b, a);
     }
+    return true;
   };
   const double dxdy = (p1.x - p0.x) / (p1.y - p0.y);
   double y = floor(fmax(p0.y, 0.0));
@@ -326,9 +331,9 @@ RZ_HD void wrap_lines(Pt p0, Pt p1, const WrapClip& q, F&& f) {
     const double x0 = x < x_next ? x : x_next;
     const double x0f = fmax(floor(x0), 0.0);
     const double xmf = 0.5 * (x + x_next) - x0f;
-    put(y, x0f, d * (1.0 - xmf));
-    put(y, x0f + 1.0, d * xmf);
+    if (!put(y, x0f, d * (1.0 - xmf)) || !put(y, x0f + 1.0, d * xmf)) return ST_WRAP;
   }
+  return ST_OK;
 }
 
 RZ_HD EffLine effective_line(Pt p0, Pt p1, const BinGrid& g) {

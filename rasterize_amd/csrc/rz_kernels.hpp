@@ -292,12 +292,14 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   // line kept beyond the right clip (then the lines are not a chain, and the
   // kept points below cannot stand for them)
   bool chain = true;
+  int werr = rzg::ST_OK;
   auto lines_of = [&](Pt a, Pt c, auto&& f) {
     if (AE) {
       rzg::ae_chain(a, c, ae_h, f);
     } else if (rzg::wraps(a, c, wq)) {
       chain = false;
-      rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); });
+      if (rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); }))
+        werr = rzg::ST_WRAP;
     } else {
       f(a, c);
     }
@@ -315,6 +317,7 @@ __device__ void flat_item(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
     const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, count); });
     if (r != rzg::ST_OK) err = r;
   }
+  if (err == rzg::ST_OK) err = werr;
   if (err != rzg::ST_OK) {
     set_err(lb.st, (uint32_t)err);
     cnt = 0;
@@ -383,12 +386,14 @@ __device__ void flat_keep(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
   if (AE) ae_h = MULTI ? (double)(active ? pi[pid].g.H : 0) : lb.ae_h;
   const rzg::WrapClip wq = wrap_clip<MULTI>(lb, pi, pid, active);
   bool chain = true;  // as in flat_item
+  int werr = rzg::ST_OK;
   auto lines_of = [&](Pt a, Pt c, auto&& f) {
     if (AE) {
       rzg::ae_chain(a, c, ae_h, f);
     } else if (rzg::wraps(a, c, wq)) {
       chain = false;
-      rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); });
+      if (rzg::wrap_lines(a, c, wq, [&](Pt u, Pt v) { f(u, rzg::mkpt(rzg::wrap_tag(), v.y)); }))
+        werr = rzg::ST_WRAP;
     } else {
       f(a, c);
     }
@@ -406,6 +411,7 @@ __device__ void flat_keep(const DPath& p, uint32_t slot, uint32_t j, int D0, boo
     const int r = rzg::traverse(n, fl, [&](Pt a, Pt c) { lines_of(a, c, count); });
     if (r != rzg::ST_OK) err = r;
   }
+  if (err == rzg::ST_OK) err = werr;
   if (err != rzg::ST_OK) {
     set_err(lb.st, (uint32_t)err);
     cnt = 0;

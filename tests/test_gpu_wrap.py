@@ -98,3 +98,23 @@ def test_wrap_scene_fill():
     img, org = O.scene_tree_render(sc, I, view)
     assert org == (0, 0)
     _check(lay.image, img, "scene fill")
+
+
+def test_wrap_too_far_is_refused():
+    # a line from the clip running 5000 px right within half a row would need
+    # deposits of ~1250 per pixel: refused with RZ_E_DEPTH, not expanded
+    W, H = 48, 48
+    path = PathBuilder().move_to((2.0, 2.0)).line_to((W - 1.0, 5.0)).line_to((W + 5000.0, 5.5)) \
+        .line_to((2.0, 30.0)).close().build()
+    ctx = rz.Context(0)
+    try:
+        _say("wrap refused")
+        with pytest.raises(rz.RzError) as e:
+            rz.SignedDifferenceRasterizer(ctx=ctx).mask_f32(path, I, W, H, FillRule.NonZero)
+        assert e.value.status == rz._abi.RZ_E_DEPTH
+        # the context stays usable
+        g = rz.SignedDifferenceRasterizer(ctx=ctx).mask_f32(_wrap_path(W - 1.0, H), I, W, H,
+                                                           FillRule.NonZero)
+        _check(g, O.sd_mask(_wrap_path(W - 1.0, H), I, W, H, 0), "after refusal")
+    finally:
+        ctx.close()
